@@ -1,0 +1,273 @@
+"""ctypes wrappers for the test-side checkers (TEST INFRASTRUCTURE ONLY).
+
+``Oracle(bd)`` loads the C restatement ``oracle/lib/liboracle{bd}.so``;
+``RefLib(bd)`` loads the compiled reference ``oracle/_ref/libuavs3e_ref{bd}.so``
+(present only where the reference tree was available at build time).
+Both expose the same numpy-level helpers so that tests can compare them.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+u8p = C.POINTER(C.c_uint8)
+PU_RES_DTYPE = np.dtype([("mv_int", "<i2", 2), ("mv_sub", "<i2", 2), ("cost_int", "<u8"),
+                         ("cost_sub", "<u8"), ("cost_final", "<u4"), ("valid", "<u4")])
+
+# The 29 AVS3 PU shapes in the order used by every frame-level API.
+SHAPES = [(4, 8), (8, 4), (4, 16), (16, 4), (4, 32), (32, 4), (8, 8), (8, 16), (16, 8), (8, 32),
+          (32, 8), (8, 64), (64, 8), (16, 16), (16, 32), (32, 16), (16, 64), (64, 16), (16, 128),
+          (128, 16), (32, 32), (32, 64), (64, 32), (32, 128), (128, 32), (64, 64), (64, 128),
+          (128, 64), (128, 128)]
+MASK_ALL = (1 << 29) - 1
+MASK_16x16 = 1 << SHAPES.index((16, 16))
+PAD = 144  # PIC_PAD_SIZE_L (com_define.h:293)
+
+
+def pel_dtype(bd):
+    return np.uint8 if bd == 8 else np.uint16
+
+
+def aligned_zeros(shape, dtype, align=64):
+    """Zeroed C-contiguous array whose data pointer is `align`-byte aligned.
+    The reference's SIMD kernels use aligned vector stores on prediction
+    buffers, which the encoder declares ALIGNED_32 (com_mc.c:142)."""
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    raw = np.zeros(n + align, np.uint8)
+    off = (-raw.ctypes.data) % align
+    return raw[off:off + n].view(dtype).reshape(shape)
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def ptr_at(a, off):
+    """Pointer to element `off` (flat index) of a C-contiguous array."""
+    return C.c_void_p(a.ctypes.data + int(off) * a.itemsize)
+
+
+class _Lib:
+    def __init__(self, path, bd):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        self.bd = bd
+        self.dt = pel_dtype(bd)
+
+
+class Oracle(_Lib):
+    def __init__(self, bd=8):
+        super().__init__(os.path.join(HERE, "lib", "liboracle%d.so" % bd), bd)
+        L = self.lib
+        L.orc_sad.restype = C.c_uint32
+        L.orc_satd.restype = C.c_uint32
+        L.orc_had.restype = C.c_uint32
+        L.orc_lambda_mv.restype = C.c_uint32
+        L.orc_luma_taps.restype = C.c_void_p
+        L.orc_chroma_taps.restype = C.c_void_p
+        L.orc_full_search.restype = C.c_uint64
+        L.orc_subpel.restype = C.c_uint64
+        L.orc_ipel_cost.restype = C.c_uint64
+        L.orc_final_cost.restype = C.c_uint32
+        L.orc_final_cost.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int]
+        L.orc_me_search_tz.restype = C.c_uint64
+        L.orc_subpel_ptr.restype = C.c_void_p
+        assert L.orc_bit_depth() == bd
+        assert L.orc_pu_res_size() == PU_RES_DTYPE.itemsize
+
+    # -- block costs --------------------------------------------------------
+    def sad(self, w, h, org, i_org, pred, i_pred, org_off=0, pred_off=0):
+        return self.lib.orc_sad(w, ptr_at(org, org_off), i_org, ptr_at(pred, pred_off), i_pred, h)
+
+    def satd(self, wl, hl, org, s_org, cur, s_cur, org_off=0, cur_off=0):
+        return self.lib.orc_satd(wl, hl, ptr_at(org, org_off), s_org, ptr_at(cur, cur_off), s_cur)
+
+    def had(self, w, h, org, s_org, cur, s_cur, org_off=0, cur_off=0):
+        return self.lib.orc_had(w, h, ptr_at(org, org_off), s_org, ptr_at(cur, cur_off), s_cur, self.bd)
+
+    # -- interpolation -----------------------------------------------------
+    def luma_taps(self, frac):
+        return np.ctypeslib.as_array(C.cast(self.lib.orc_luma_taps(frac), C.POINTER(C.c_int8)), (8,)).copy()
+
+    def chroma_taps(self, frac):
+        return np.ctypeslib.as_array(C.cast(self.lib.orc_chroma_taps(frac), C.POINTER(C.c_int8)), (4,)).copy()
+
+    def ipflt(self, typ, src, i_src, src_off, w, h, coef, max_val):
+        dst = aligned_zeros((h, w), self.dt)
+        coef = np.ascontiguousarray(coef, np.int8)
+        self.lib.orc_ipflt(typ, ptr_at(src, src_off), i_src, ptr(dst), w, w, h, ptr(coef), max_val)
+        return dst
+
+    def ipflt_ext(self, typ, src, i_src, src_off, w, h, cx, cy, max_val):
+        dst = aligned_zeros((h, w), self.dt)
+        cx = np.ascontiguousarray(cx, np.int8)
+        cy = np.ascontiguousarray(cy, np.int8)
+        self.lib.orc_ipflt_ext(typ, ptr_at(src, src_off), i_src, ptr(dst), w, w, h, ptr(cx), ptr(cy), max_val)
+        return dst
+
+    def pel_avrg(self, w, h, s1, s2):
+        dst = aligned_zeros((h, w), self.dt)
+        self.lib.orc_pel_avrg(w, ptr(dst), w, ptr(s1), ptr(s2), h)
+        return dst
+
+    def pad_plane(self, buf, stride, w, h, pad, ext):
+        self.lib.orc_pad_plane(ptr_at(buf, pad * stride + pad), stride, w, h, pad, ext)
+
+    def make_planes(self, luma, pad=PAD):
+        """Padded integer plane + 15 sub-pel planes, as (16, H+2pad, W+2pad)."""
+        h, w = luma.shape
+        stride = w + 2 * pad
+        planes = np.zeros((16, h + 2 * pad, stride), self.dt)
+        planes[0, pad:pad + h, pad:pad + w] = luma
+        self.pad_plane(planes[0], stride, w, h, pad, 0)
+        ptrs = (C.c_void_p * 16)(*[C.c_void_p(planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize)
+                                    for i in range(16)])
+        self.lib.orc_if_luma_frame(ptrs, stride, w, h, pad, self.bd)
+        return planes
+
+    # -- rate --------------------------------------------------------------
+    def mvbits(self, pic_w, pic_h):
+        off = self.lib.orc_mvbits_offset(pic_w, pic_h)
+        tab = np.zeros(2 * off + 1, np.uint8)
+        self.lib.orc_mvbits_fill(ptr(tab), off)
+        return off, tab
+
+    def lambda_mv(self, qp):
+        return self.lib.orc_lambda_mv(qp)
+
+    # -- frame -------------------------------------------------------------
+    def me_frame(self, cur, ref_planes_list, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, pad=PAD):
+        """Full search + sub-pel for CTUs [ctu0, ctu1); returns (res, best_ref)
+        with res shaped (n_ctu, npu, nref) of PU_RES_DTYPE."""
+        h, w = cur.shape
+        n_ctu = ((w + 127) // 128) * ((h + 127) // 128)
+        ctu1 = n_ctu if ctu1 is None else ctu1
+        npu = self.lib.orc_pus_per_ctu(C.c_uint32(mask))
+        nref = len(ref_planes_list)
+        stride = w + 2 * pad
+        ptrs = (C.c_void_p * (16 * nref))()
+        for r, planes in enumerate(ref_planes_list):
+            for i in range(16):
+                ptrs[16 * r + i] = planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize
+        res = np.zeros((n_ctu, npu, nref), PU_RES_DTYPE)
+        best = np.full((n_ctu, npu), -1, np.int8)
+        cur = np.ascontiguousarray(cur)
+        self.lib.orc_me_frame(w, h, self.bd, ptr(cur), w, nref, ptrs, stride, R, C.c_uint32(lambda_mv),
+                              cost_type, C.c_uint32(mask), ctu0, ctu1, ptr(res), ptr(best))
+        return res, best
+
+
+class RefLib(_Lib):
+    """The compiled reference (oracle/_ref).  level: 0 = C table, 2 = C+SSE+AVX2."""
+
+    @staticmethod
+    def available(bd=8):
+        return os.path.exists(os.path.join(HERE, "_ref", "libuavs3e_ref%d.so" % bd))
+
+    def __init__(self, bd=8, level=0):
+        super().__init__(os.path.join(HERE, "_ref", "libuavs3e_ref%d.so" % bd), bd)
+        L = self.lib
+        for n in ("refh_sad", "refh_satd", "refh_had"):
+            getattr(L, n).restype = C.c_uint32
+        L.refh_pic_create.restype = C.c_void_p
+        L.refh_pic_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_longlong]
+        L.refh_pic_plane.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.refh_pic_stride.argtypes = [C.c_void_p]
+        L.refh_pic_free.argtypes = [C.c_void_p]
+        L.refh_luma_coef.restype = C.c_void_p
+        L.refh_chroma_coef.restype = C.c_void_p
+        L.refh_me_search_tz.restype = C.c_uint64
+        L.refh_me_search_tz.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.refh_subpel.restype = C.c_uint64
+        L.refh_subpel.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.refh_me_frame.restype = C.c_double
+        L.refh_me_frame.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                    C.c_uint32, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]
+        self.set_level(level)
+
+    def set_level(self, level):
+        assert self.lib.refh_init(level) == self.bd
+        self.level = level
+
+    def sad(self, w, h, org, i_org, pred, i_pred, org_off=0, pred_off=0):
+        return self.lib.refh_sad(w, ptr_at(org, org_off), i_org, ptr_at(pred, pred_off), i_pred, h)
+
+    def sad_x4(self, w, h, org, i_org, pred, i_pred, offs):
+        out = np.zeros(4, np.uint32)
+        self.lib.refh_sad_x4(w, ptr(org), i_org, *[ptr_at(pred, o) for o in offs], i_pred, ptr(out), h)
+        return out
+
+    def sad_x3(self, w, h, org, i_org, pred, i_pred, offs):
+        out = np.zeros(3, np.uint32)
+        self.lib.refh_sad_x3(w, ptr(org), i_org, *[ptr_at(pred, o) for o in offs], i_pred, ptr(out), h)
+        return out
+
+    def satd(self, wl, hl, org, s_org, cur, s_cur, org_off=0, cur_off=0):
+        return self.lib.refh_satd(wl, hl, ptr_at(org, org_off), s_org, ptr_at(cur, cur_off), s_cur)
+
+    def had(self, w, h, org, s_org, cur, s_cur, org_off=0, cur_off=0):
+        return self.lib.refh_had(w, h, ptr_at(org, org_off), s_org, ptr_at(cur, cur_off), s_cur, self.bd)
+
+    def ipflt(self, typ, src, i_src, src_off, w, h, coef, max_val):
+        dst = aligned_zeros((h, w), self.dt)
+        coef = np.ascontiguousarray(coef, np.int8)
+        self.lib.refh_ipflt(typ, ptr_at(src, src_off), i_src, ptr(dst), w, w, h, ptr(coef), max_val)
+        return dst
+
+    def ipflt_ext(self, typ, src, i_src, src_off, w, h, cx, cy, max_val):
+        dst = aligned_zeros((h, w), self.dt)
+        cx = np.ascontiguousarray(cx, np.int8)
+        cy = np.ascontiguousarray(cy, np.int8)
+        self.lib.refh_ipflt_ext(typ, ptr_at(src, src_off), i_src, ptr(dst), w, w, h, ptr(cx), ptr(cy), max_val)
+        return dst
+
+    def pel_avrg(self, w, h, s1, s2):
+        dst = aligned_zeros((h, w), self.dt)
+        a1 = aligned_zeros((h, w), self.dt); a1[...] = s1
+        a2 = aligned_zeros((h, w), self.dt); a2[...] = s2
+        self.lib.refh_pel_avrg(w, ptr(dst), w, ptr(a1), ptr(a2), h)
+        return dst
+
+    def mvbits(self, pic_w, pic_h):
+        cap = ((max(pic_w, pic_h) << 2) + 3) * 2 + 1
+        tab = np.zeros(cap, np.uint8)
+        off = self.lib.refh_mvbits(pic_w, pic_h, ptr(tab), cap)
+        return off, tab[:2 * off + 1]
+
+    def pic_create(self, luma, poc=0):
+        luma = np.ascontiguousarray(luma, self.dt)
+        h, w = luma.shape
+        return self.lib.refh_pic_create(ptr(luma), w, h, poc)
+
+    def pic_planes(self, pic, w, h, pad=PAD):
+        stride = w + 2 * pad
+        assert self.lib.refh_pic_stride(pic) == stride
+        planes = np.zeros((16, h + 2 * pad, stride), self.dt)
+        for i in range(16):
+            self.lib.refh_pic_plane(pic, i, ptr(planes[i]))
+        return planes
+
+    def pic_free(self, pic):
+        self.lib.refh_pic_free(pic)
+
+    def me_frame(self, cur, pics, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, threads=1, ctu_ms=None):
+        h, w = cur.shape
+        n_ctu = ((w + 127) // 128) * ((h + 127) // 128)
+        ctu1 = n_ctu if ctu1 is None else ctu1
+        npu = sum((128 // sw) * (128 // sh) for s, (sw, sh) in enumerate(SHAPES) if mask >> s & 1)
+        nref = len(pics)
+        arr = (C.c_void_p * nref)(*pics)
+        res = np.zeros((n_ctu, npu, nref), PU_RES_DTYPE)
+        best = np.full((n_ctu, npu), -1, np.int8)
+        cur = np.ascontiguousarray(cur, self.dt)
+        ms = self.lib.refh_me_frame(w, h, ptr(cur), w, nref, arr, R, lambda_mv, cost_type, mask, ctu0, ctu1,
+                                    threads, ptr(res), ptr(best), ptr(ctu_ms) if ctu_ms is not None else None)
+        return res, best, ms

@@ -1,0 +1,414 @@
+/*
+ * ref_harness.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Thin C entry points around the *compiled reference* (uavs3e built from its
+ * own sources by oracle/ref.mk into oracle/_ref/).  Used to write golden
+ * vectors, to cross-check the oracle restatement, and as the timed CPU
+ * baseline ("reference AVX2 path") of bench.py.  Nothing in the product
+ * links it.
+ *
+ * me.c is compiled as part of this translation unit (it is #included from the
+ * reference tree, not copied) so that its static functions
+ * get_max_search_range and me_sub_pel_search (me.c:366-527) can be called.
+ */
+#define _GNU_SOURCE
+#include "define.h"
+#include "me.c"
+#include <pthread.h>
+#include <time.h>
+
+#define RH_EXPORT __attribute__((visibility("default")))
+
+static int g_level = -1;
+
+/* Fill the dispatch table the way uavs3e_create does (uavs3e.c:1332-1342):
+ * C first, then SSE, then AVX2 when the CPU has it.  level 0 = C only. */
+RH_EXPORT int refh_init(int level)
+{
+    uavs3e_funs_init_c();
+    if (level >= 1) uavs3e_funs_init_sse();
+    if (level >= 2 && uavs3e_simd_avx_level(NULL) >= 2) uavs3e_funs_init_avx2();
+    com_dct_coef_create();
+    g_level = level;
+    return BIT_DEPTH;
+}
+
+RH_EXPORT int refh_bit_depth(void) { return BIT_DEPTH; }
+
+static int wlog(int w) { return CONV_LOG2(w) - 2; }
+
+RH_EXPORT u32 refh_sad(int w, pel *org, int i_org, pel *pred, int i_pred, int h)
+{ return uavs3e_funs_handle.cost_sad[wlog(w)](org, i_org, pred, i_pred, h); }
+
+RH_EXPORT void refh_sad_x3(int w, pel *org, int i_org, pel *p0, pel *p1, pel *p2, int i_pred, u32 *sad, int h)
+{ uavs3e_funs_handle.cost_sad_x3[wlog(w)](org, i_org, p0, p1, p2, i_pred, sad, h); }
+
+RH_EXPORT void refh_sad_x4(int w, pel *org, int i_org, pel *p0, pel *p1, pel *p2, pel *p3, int i_pred, u32 *sad, int h)
+{ uavs3e_funs_handle.cost_sad_x4[wlog(w)](org, i_org, p0, p1, p2, p3, i_pred, sad, h); }
+
+RH_EXPORT u32 refh_satd(int wl, int hl, pel *org, int s_org, pel *cur, int s_cur)
+{ return uavs3e_funs_handle.cost_satd[wl][hl](org, s_org, cur, s_cur); }
+
+RH_EXPORT u32 refh_had(int w, int h, pel *org, int s_org, pel *cur, int s_cur, int bd)
+{ return com_had(w, h, org, s_org, cur, s_cur, bd); }
+
+RH_EXPORT void refh_ipcpy(const pel *src, int i_src, pel *dst, int i_dst, int w, int h)
+{ uavs3e_funs_handle.ipcpy[wlog(w)](src, i_src, dst, i_dst, w, h); }
+
+RH_EXPORT void refh_ipflt(int type, const pel *src, int i_src, pel *dst, int i_dst, int w, int h, const s8 *coef, int max_val)
+{ uavs3e_funs_handle.ipflt[type][wlog(w)](src, i_src, dst, i_dst, w, h, coef, max_val); }
+
+RH_EXPORT void refh_ipflt_ext(int type, const pel *src, int i_src, pel *dst, int i_dst, int w, int h,
+                              const s8 *cx, const s8 *cy, int max_val)
+{ uavs3e_funs_handle.ipflt_ext[type][wlog(w)](src, i_src, dst, i_dst, w, h, cx, cy, max_val); }
+
+RH_EXPORT void refh_pel_avrg(int w, pel *dst, int i_dst, pel *s1, pel *s2, int h)
+{ uavs3e_funs_handle.pel_avrg[wlog(w)](dst, i_dst, s1, s2, h); }
+
+RH_EXPORT const s8 *refh_luma_coef(int frac) { return com_tbl_mc_l_coeff[frac & 3]; }
+RH_EXPORT const s8 *refh_chroma_coef(int frac) { return com_tbl_mc_c_coeff[frac & 7]; }
+
+/* MV-bits table from inter_search_create (inter.c:2029-2058); copies
+ * 2*off+1 entries into out (out[off] is mv 0) and returns off. */
+RH_EXPORT int refh_mvbits(int pic_w, int pic_h, u8 *out, int cap)
+{
+    com_info_t *info = (com_info_t *)calloc(1, sizeof(com_info_t));
+    u8 *tab = NULL;
+    info->pic_width = pic_w;
+    info->pic_height = pic_h;
+    int off = inter_search_create(&tab, info);
+    if (2 * off + 1 <= cap) memcpy(out, tab - off, (size_t)(2 * off + 1));
+    inter_search_free(tab, off);
+    free(info);
+    return off;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Reference pictures with sub-pel planes                                   */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    com_pic_t *pic;
+    int w, h;
+} refh_pic_t;
+
+/* Build a reference picture the way the encoder does after reconstruction
+ * (uavs3e.c:1086-1092): copy the luma samples, edge-pad (com_img_padding ..,0),
+ * allocate the sub-pel images (com_subpel_create) and run com_if_luma_frame. */
+RH_EXPORT void *refh_pic_create(const pel *luma, int w, int h, long long poc)
+{
+    int err = 0;
+    refh_pic_t *r = (refh_pic_t *)calloc(1, sizeof(refh_pic_t));
+    r->w = w; r->h = h;
+    r->pic = com_pic_create(w, h, PIC_PAD_SIZE_L, PIC_PAD_SIZE_C, &err);
+    for (int y = 0; y < h; y++) memcpy(r->pic->y + y * r->pic->stride_luma, luma + y * w, sizeof(pel) * w);
+    com_img_padding(r->pic->img, 1, 0);
+    r->pic->subpel = com_subpel_create(w, h, PIC_PAD_SIZE_L, PIC_PAD_SIZE_C, &err);
+    r->pic->subpel->imgs[0][0] = r->pic->img;
+    r->pic->ptr = poc;
+    int ts = w + 16;
+    s16 *tmp[3];
+    /* The SIMD frame filters write whole vectors past the end of a row (the
+     * encoder carves the three buffers out of one allocation, uavs3e.c:122,
+     * 167-169), so give each buffer slack rows. */
+    for (int i = 0; i < 3; i++) tmp[i] = (s16 *)calloc((size_t)ts * (h + 16) + 8 * (size_t)ts + 256, sizeof(s16));
+    com_if_luma_frame(r->pic->subpel->imgs, tmp, BIT_DEPTH);
+    for (int i = 0; i < 3; i++) free(tmp[i]);
+    return r;
+}
+
+RH_EXPORT int refh_pic_stride(void *p) { return ((refh_pic_t *)p)->pic->stride_luma; }
+RH_EXPORT int refh_pic_pad(void *p) { return ((refh_pic_t *)p)->pic->padsize_luma; }
+
+/* Copy one whole padded plane (plane index dy*4+dx) into out. */
+RH_EXPORT void refh_pic_plane(void *p, int idx, pel *out)
+{
+    refh_pic_t *r = (refh_pic_t *)p;
+    com_img_t *img = r->pic->subpel->imgs[idx >> 2][idx & 3];
+    int s = img->stride[0] / (int)sizeof(pel), pad = img->pad[0];
+    const pel *base = (const pel *)img->planes[0] - pad * s - pad;
+    memcpy(out, base, sizeof(pel) * (size_t)s * (r->h + 2 * pad));
+}
+
+RH_EXPORT void refh_pic_free(void *p)
+{
+    refh_pic_t *r = (refh_pic_t *)p;
+    r->pic->subpel->imgs[0][0] = NULL;
+    com_subpel_free(r->pic->subpel);
+    r->pic->subpel = NULL;
+    com_pic_destroy(r->pic);
+    free(r);
+}
+
+/* ---------------------------------------------------------------------- */
+/* Search                                                                   */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    inter_search_t pi;
+    u8 *tab;
+} refh_search_t;
+
+static void search_setup(refh_search_t *s, int pic_w, int pic_h, u32 lambda_mv, int cost_type, int is_padding)
+{
+    com_info_t *info = (com_info_t *)calloc(1, sizeof(com_info_t));
+    info->pic_width = pic_w;
+    info->pic_height = pic_h;
+    info->bit_depth_internal = BIT_DEPTH;
+    info->gop_size = 16;
+    memset(&s->pi, 0, sizeof(s->pi));
+    int off = inter_search_create(&s->tab, info);
+    info->tab_mvbits = s->tab;
+    inter_search_init(&s->pi, info, is_padding);
+    s->pi.tab_mvbits_offset = off;
+    s->pi.lambda_mv = lambda_mv;
+    s->pi.subpel_cost_type = cost_type;
+    s->pi.curr_mvr = 0;
+    free(info);
+}
+
+static void search_free(refh_search_t *s) { inter_search_free(s->tab, s->pi.tab_mvbits_offset); }
+
+/* me_search_tz (me.c:529-568) exactly as the encoder calls it for uni
+ * prediction (inter.c:896), with the SATD caches cleared first as
+ * analyze_inter_cu does per CU (inter.c:1907-1908). */
+RH_EXPORT u64 refh_me_search_tz(void *ref, const pel *org, int i_org, int x, int y, int w, int h,
+                                int refi, int num_refp, long long cur_poc, int gop_size, int max_search_range,
+                                int mvr, int cost_type, int adaptive_raster_range, u32 lambda_mv,
+                                const s16 *mvp, s16 *mv_out, s16 *mv_ipel_out)
+{
+    refh_pic_t *r = (refh_pic_t *)ref;
+    refh_search_t s;
+    search_setup(&s, r->w, r->h, lambda_mv, cost_type, 1);
+    inter_search_t *pi = &s.pi;
+    pi->gop_size = gop_size;
+    pi->max_search_range = max_search_range;
+    pi->adaptive_raster_range = adaptive_raster_range;
+    pi->ptr = cur_poc;
+    pi->ref_pic = r->pic;
+    pi->num_refp = (u8)num_refp;
+    pi->curr_mvr = (u8)mvr;
+    pi->org = (pel *)org;
+    pi->i_org = i_org;
+    s16 mvp_l[2] = {mvp[0], mvp[1]}, mv[2] = {0, 0};
+    u64 c = me_search_tz(pi, x, y, w, h, r->w, r->h, (s8)refi, 0, mvp_l, mv, 0);
+    mv_out[0] = mv[0]; mv_out[1] = mv[1];
+    mv_ipel_out[0] = pi->mv_ipel[0][refi][0];
+    mv_ipel_out[1] = pi->mv_ipel[0][refi][1];
+    search_free(&s);
+    return c;
+}
+
+/* me_sub_pel_search (me.c:366-500) from a given integer MV (quarter-pel
+ * units in mv, refined on return), bi = 0, caches cleared. */
+RH_EXPORT u64 refh_subpel(void *ref, const pel *org, int i_org, int x, int y, int w, int h,
+                          int mvr, int cost_type, u32 lambda_mv, const s16 *mvp, s16 *mv)
+{
+    refh_pic_t *r = (refh_pic_t *)ref;
+    refh_search_t s;
+    search_setup(&s, r->w, r->h, lambda_mv, cost_type, 1);
+    s.pi.ref_pic = r->pic;
+    s.pi.curr_mvr = (u8)mvr;
+    s.pi.org = (pel *)org;
+    s.pi.i_org = i_org;
+    s16 mvp_l[2] = {mvp[0], mvp[1]}, m[2] = {mv[0], mv[1]};
+    u64 c = me_sub_pel_search(&s.pi, x, y, w, h, 0, 0, mvp_l, m, 0);
+    mv[0] = m[0]; mv[1] = m[1];
+    search_free(&s);
+    return c;
+}
+
+/* get_max_search_range (me.c:502-527); returns ok and writes the window. */
+RH_EXPORT int refh_window(int pic_w, int pic_h, int x, int y, int w, int h, int mvr, int is_padding,
+                          const s16 *mvp, s16 *min_mv, s16 *max_mv)
+{
+    refh_search_t s;
+    search_setup(&s, pic_w, pic_h, 0, 0, is_padding);
+    s.pi.curr_mvr = (u8)mvr;
+    int ok = get_max_search_range(&s.pi, mvp, x, y, w, h, pic_w, pic_h);
+    min_mv[0] = s.pi.min_mv[0]; min_mv[1] = s.pi.min_mv[1];
+    max_mv[0] = s.pi.max_mv[0]; max_mv[1] = s.pi.max_mv[1];
+    search_free(&s);
+    return ok;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Full-search frame driver (the timed "reference AVX2 path")             */
+/* ---------------------------------------------------------------------- */
+
+static const int SHAPES[29][2] = {
+    {4, 8}, {8, 4}, {4, 16}, {16, 4}, {4, 32}, {32, 4}, {8, 8}, {8, 16}, {16, 8}, {8, 32},
+    {32, 8}, {8, 64}, {64, 8}, {16, 16}, {16, 32}, {32, 16}, {16, 64}, {64, 16}, {16, 128},
+    {128, 16}, {32, 32}, {32, 64}, {64, 32}, {32, 128}, {128, 32}, {64, 64}, {64, 128},
+    {128, 64}, {128, 128}};
+
+typedef struct {
+    s16 mv_int[2];
+    s16 mv_sub[2];
+    u64 cost_int;
+    u64 cost_sub;
+    u32 cost_final;
+    u32 valid;
+} refh_pu_res_t;
+
+/* Step-1 raster over the window with the table's SAD x4 / x3 / x1 grouping
+ * of search_raster (me.c:150-188); strict '<' keeps the first minimum. */
+static u64 full_search(inter_search_t *pi, int x, int y, int w, int h, int R, s16 mv[2])
+{
+    s16 mvp[2] = {0, 0};
+    mv[0] = mv[1] = 0;
+    if (!get_max_search_range(pi, mvp, x, y, w, h, pi->ref_pic->width_luma, pi->ref_pic->height_luma))
+        return COM_UINT64_MAX;
+    int cx = COM_CLIP3(pi->min_mv[0], pi->max_mv[0], 0), cy = COM_CLIP3(pi->min_mv[1], pi->max_mv[1], 0);
+    int x0 = COM_CLIP3(pi->min_mv[0], pi->max_mv[0], cx - R), x1 = COM_CLIP3(pi->min_mv[0], pi->max_mv[0], cx + R);
+    int y0 = COM_CLIP3(pi->min_mv[1], pi->max_mv[1], cy - R), y1 = COM_CLIP3(pi->min_mv[1], pi->max_mv[1], cy + R);
+    int i_ref = pi->ref_pic->stride_luma;
+    pel *ref = pi->ref_pic->y + y * i_ref + x;
+    u8 *tab = pi->tab_mvbits;
+    u32 lambda = pi->lambda_mv;
+    int shift = 16 - (pi->bit_depth - 8);
+    int widx = CONV_LOG2(w) - 2;
+    u64 best = COM_UINT64_MAX;
+    for (int my = y0; my <= y1; my++) {
+        pel *p = ref + my * i_ref;
+        u64 cmy = (u64)(tab[my << 2] * lambda);
+        int mx = x0;
+        u32 sad[4];
+        for (; mx + 3 <= x1; mx += 4) {
+            uavs3e_funs_handle.cost_sad_x4[widx](pi->org, pi->i_org, p + mx, p + mx + 1, p + mx + 2, p + mx + 3, i_ref, sad, h);
+            for (int k = 0; k < 4; k++) {
+                u64 c = cmy + (u64)(tab[(mx + k) << 2] * lambda) + ((u64)sad[k] << shift);
+                if (c < best) { best = c; mv[0] = (s16)(mx + k); mv[1] = (s16)my; }
+            }
+        }
+        if (mx + 2 <= x1) {
+            uavs3e_funs_handle.cost_sad_x3[widx](pi->org, pi->i_org, p + mx, p + mx + 1, p + mx + 2, i_ref, sad, h);
+            for (int k = 0; k < 3; k++) {
+                u64 c = cmy + (u64)(tab[(mx + k) << 2] * lambda) + ((u64)sad[k] << shift);
+                if (c < best) { best = c; mv[0] = (s16)(mx + k); mv[1] = (s16)my; }
+            }
+            mx += 3;
+        }
+        for (; mx <= x1; mx++) {
+            u64 c = cmy + (u64)(tab[mx << 2] * lambda) +
+                    ((u64)uavs3e_funs_handle.cost_sad[widx](pi->org, pi->i_org, p + mx, i_ref, h) << shift);
+            if (c < best) { best = c; mv[0] = (s16)mx; mv[1] = (s16)my; }
+        }
+    }
+    return best;
+}
+
+typedef struct {
+    int pic_w, pic_h;
+    const pel *cur; int i_cur;
+    int num_refs; void **refs;
+    int R; u32 lambda_mv; int cost_type; u32 mask;
+    int ctu0, ctu1, npu;
+    refh_pu_res_t *res; s8 *best_ref;
+    double *ctu_ms;
+    int next;                 /* shared work counter */
+    pthread_mutex_t lock;
+} frame_job_t;
+
+static void run_ctu(frame_job_t *j, refh_search_t *s, int ctu)
+{
+    inter_search_t *pi = &s->pi;
+    int cols = (j->pic_w + 127) / 128;
+    int cx = (ctu % cols) * 128, cy = (ctu / cols) * 128;
+    int pu = 0;
+    for (int sh = 0; sh < 29; sh++) {
+        if (!(j->mask >> sh & 1)) continue;
+        int w = SHAPES[sh][0], h = SHAPES[sh][1];
+        for (int oy = 0; oy < 128; oy += h)
+            for (int ox = 0; ox < 128; ox += w, pu++) {
+                int x = cx + ox, y = cy + oy;
+                size_t base = (size_t)ctu * j->npu + pu;
+                int inside = x + w <= j->pic_w && y + h <= j->pic_h;
+                u32 bc = 0xFFFFFFFFu;
+                s8 br = -1;
+                for (int r = 0; r < j->num_refs; r++) {
+                    refh_pu_res_t *o = &j->res[base * j->num_refs + r];
+                    memset(o, 0, sizeof(*o));
+                    o->cost_int = o->cost_sub = COM_UINT64_MAX;
+                    o->cost_final = 0xFFFFFFFFu;
+                    if (!inside) continue;
+                    pi->ref_pic = ((refh_pic_t *)j->refs[r])->pic;
+                    pi->org = (pel *)j->cur + y * j->i_cur + x;
+                    pi->i_org = j->i_cur;
+                    memset(pi->hpel_satd, 0, sizeof(pi->hpel_satd));
+                    memset(pi->qpel_satd, 0, sizeof(pi->qpel_satd));
+                    memset(pi->hpel_start_mv, 0, sizeof(pi->hpel_start_mv));
+                    memset(pi->qpel_start_mv, 0, sizeof(pi->qpel_start_mv));
+                    s16 mv[2], mvp[2] = {0, 0};
+                    u64 ci = full_search(pi, x, y, w, h, j->R, mv);
+                    o->mv_int[0] = mv[0]; o->mv_int[1] = mv[1];
+                    o->cost_int = ci;
+                    if (ci == COM_UINT64_MAX) continue;
+                    mv[0] <<= 2; mv[1] <<= 2;
+                    u64 cs = me_sub_pel_search(pi, x, y, w, h, (s8)r, 0, mvp, mv, 0);
+                    o->mv_sub[0] = mv[0]; o->mv_sub[1] = mv[1];
+                    o->cost_sub = cs;
+                    int extra = tbl_refi_bits[j->num_refs][r] + 0 + 1 + 2;
+                    u64 cf = cs + j->lambda_mv * extra;
+                    o->cost_final = (u32)((cf + (1 << 15)) >> 16);
+                    o->valid = 1;
+                    if (o->cost_final < bc) { bc = o->cost_final; br = (s8)r; }
+                }
+                j->best_ref[base] = br;
+            }
+    }
+}
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+static void *frame_worker(void *arg)
+{
+    frame_job_t *j = (frame_job_t *)arg;
+    refh_search_t s;
+    search_setup(&s, j->pic_w, j->pic_h, j->lambda_mv, j->cost_type, 1);
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        int ctu = j->ctu0 + j->next++;
+        pthread_mutex_unlock(&j->lock);
+        if (ctu >= j->ctu1) break;
+        double t0 = now_ms();
+        run_ctu(j, &s, ctu);
+        if (j->ctu_ms) j->ctu_ms[ctu - j->ctu0] = now_ms() - t0;
+    }
+    search_free(&s);
+    return NULL;
+}
+
+/* Full search + me_sub_pel_search for CTUs [ctu0, ctu1) over `threads`
+ * pthreads (dynamic CTU distribution).  Results use the oracle's layout.
+ * ctu_ms (optional) receives each CTU's wall time.  Returns total ms. */
+RH_EXPORT double refh_me_frame(int pic_w, int pic_h, const pel *cur, int i_cur, int num_refs, void **refs,
+                               int R, u32 lambda_mv, int cost_type, u32 mask, int ctu0, int ctu1,
+                               int threads, refh_pu_res_t *res, s8 *best_ref, double *ctu_ms)
+{
+    frame_job_t j;
+    memset(&j, 0, sizeof(j));
+    j.pic_w = pic_w; j.pic_h = pic_h; j.cur = cur; j.i_cur = i_cur;
+    j.num_refs = num_refs; j.refs = refs; j.R = R; j.lambda_mv = lambda_mv;
+    j.cost_type = cost_type; j.mask = mask; j.ctu0 = ctu0; j.ctu1 = ctu1;
+    j.res = res; j.best_ref = best_ref; j.ctu_ms = ctu_ms;
+    int n = 0;
+    for (int sh = 0; sh < 29; sh++)
+        if (mask >> sh & 1) n += (128 / SHAPES[sh][0]) * (128 / SHAPES[sh][1]);
+    j.npu = n;
+    pthread_mutex_init(&j.lock, NULL);
+    if (threads < 1) threads = 1;
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    double t0 = now_ms();
+    for (int i = 0; i < threads; i++) pthread_create(&tid[i], NULL, frame_worker, &j);
+    for (int i = 0; i < threads; i++) pthread_join(tid[i], NULL);
+    double t = now_ms() - t0;
+    free(tid);
+    pthread_mutex_destroy(&j.lock);
+    return t;
+}
