@@ -1,0 +1,209 @@
+/*
+ * uavs3e_me_hip.h -- C ABI of the MI355X (gfx950) motion-estimation library.
+ *
+ * The library is built once per sample bit depth, exactly as uavs3e fixes
+ * `pel` at compile time (inc/com_api.h:46-52):
+ *   libuavs3e_me_hip8.so  (BIT_DEPTH 8,  me_pel = uint8_t)
+ *   libuavs3e_me_hip10.so (BIT_DEPTH 10, me_pel = uint16_t)
+ * Include this header with the same BIT_DEPTH as the library you link.
+ *
+ * Two layers:
+ *   1. Per-call drop-in entries.  me_funs_init_hip() overwrites the hot-path
+ *      slots of a dispatch table laid out like uavs3e's funs_handle_t
+ *      (inc/com_util.h:278-341), the way uavs3e_funs_init_avx2() does
+ *      (src/avx2/avx2.c:41).  Each entry takes host pointers and strides in
+ *      pels, runs on the GPU and returns only when its outputs are written,
+ *      like the reference's C/SIMD kernels.
+ *   2. Batched frame ME (no counterpart in the reference): device-resident
+ *      reference pictures with their 15 quarter-pel planes, and a per-frame
+ *      plan that runs integer full search + sub-pel refinement + reference
+ *      selection for every PU of a frame in one hipGraph launch.
+ *
+ * All functions returning int give 0 on success and a negative value on
+ * failure; me_hip_last_error() describes the failure.
+ */
+#ifndef UAVS3E_ME_HIP_H
+#define UAVS3E_ME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef BIT_DEPTH
+#define BIT_DEPTH 8
+#endif
+#if BIT_DEPTH == 8
+typedef uint8_t me_pel;
+#else
+typedef uint16_t me_pel;
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Library state                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Bit depth the library was built for (8 or 10). */
+int me_hip_bit_depth(void);
+
+/* Selects HIP device `device` for this process and checks that it can run
+ * the library's code objects.  Called implicitly by every entry point. */
+int me_hip_init(int device);
+
+/* Description of the most recent failure in this thread ("" if none). */
+const char *me_hip_last_error(void);
+
+/* Number of per-call entries that failed since load. */
+uint64_t me_hip_error_count(void);
+
+/* The per-call entries have no error channel (the reference slots return
+ * void/u32, com_util.h:300-330).  By default a failing entry prints the HIP
+ * error and aborts the process.  With enable=1 it instead calls the slot that
+ * me_funs_init_hip() replaced and counts the failure. */
+void me_hip_set_fallback(int enable);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Per-call drop-in: the hot-path slots of funs_handle_t                 */
+/* ------------------------------------------------------------------------ */
+
+/* Same slots, signatures and index conventions as funs_handle_t:
+ *   widx = log2(width) - 2 (com_tables.c:75-105), CU_SIZE_NUM = 6;
+ *   ipflt[IPFILTER_H_8, H_4, V_8, V_4], ipflt_ext[IPFILTER_EXT_8, EXT_4]
+ *   (com_define.h:251-262); cost_satd[log2w-2][log2h-2] with the six slots
+ *   [0][0] [1][0] [0][1] [1][1] [2][1] [1][2] filled (com_cost.c:1025-1030). */
+typedef struct me_dispatch_t {
+    /* com_util.h:300 */
+    void (*ipcpy[6])(const me_pel *src, int i_src, me_pel *dst, int i_dst, int width, int height);
+    /* com_util.h:301 */
+    void (*ipflt[4][6])(const me_pel *src, int i_src, me_pel *dst, int i_dst, int width, int height,
+                        const int8_t *coeff, int max_val);
+    /* com_util.h:302 */
+    void (*ipflt_ext[2][6])(const me_pel *src, int i_src, me_pel *dst, int i_dst, int width, int height,
+                            const int8_t *coef_x, const int8_t *coef_y, int max_val);
+    /* com_util.h:304 */
+    void (*ip_flt_y_hor)(const me_pel *src, int i_src, me_pel *dst[3], int i_dst, int16_t *dst_tmp[3],
+                         int i_dst_tmp, int width, int height, int8_t (*coeff)[8], int bit_depth);
+    /* com_util.h:305 */
+    void (*ip_flt_y_ver)(const me_pel *src, int i_src, me_pel *dst[3], int i_dst, int width, int height,
+                         int8_t (*coeff)[8], int bit_depth);
+    /* com_util.h:306 */
+    void (*ip_flt_y_ver_ext)(const int16_t *src, int i_src, me_pel *dst[3], int i_dst, int width, int height,
+                             int8_t (*coeff)[8], int bit_depth);
+    /* com_util.h:316 */
+    uint32_t (*cost_sad[6])(me_pel *p_org, int i_org, me_pel *p_pred, int i_pred, int height);
+    /* com_util.h:317 */
+    void (*cost_sad_x3[6])(me_pel *p_org, int i_org, me_pel *pred0, me_pel *pred1, me_pel *pred2, int i_pred,
+                           uint32_t sad[3], int height);
+    /* com_util.h:318 */
+    void (*cost_sad_x4[6])(me_pel *p_org, int i_org, me_pel *pred0, me_pel *pred1, me_pel *pred2, me_pel *pred3,
+                           int i_pred, uint32_t sad[4], int height);
+    /* com_util.h:322 */
+    uint32_t (*cost_satd[3][3])(me_pel *p_org, int i_org, me_pel *p_pred, int i_pred);
+    /* com_util.h:330 */
+    void (*pel_avrg[6])(me_pel *dst, int i_dst, me_pel *src1, me_pel *src2, int height);
+} me_dispatch_t;
+
+/* Overwrites every slot above with its HIP entry, remembering the previous
+ * contents for the optional fallback.  Replaces, for these slots,
+ * uavs3e_funs_init_sse()/uavs3e_funs_init_avx2() (sse.c:213, avx2.c:41). */
+int me_funs_init_hip(me_dispatch_t *table);
+
+/* The entries themselves, for callers that bind them one by one. */
+uint32_t me_hip_sad(int width, me_pel *p_org, int i_org, me_pel *p_pred, int i_pred, int height);        /* com_cost.c:40-58 */
+void me_hip_sad_x3(int width, me_pel *p_org, int i_org, me_pel *p0, me_pel *p1, me_pel *p2, int i_pred,
+                   uint32_t sad[3], int height);                                                          /* com_cost.c:60-85 */
+void me_hip_sad_x4(int width, me_pel *p_org, int i_org, me_pel *p0, me_pel *p1, me_pel *p2, me_pel *p3,
+                   int i_pred, uint32_t sad[4], int height);                                              /* com_cost.c:87-115 */
+uint32_t me_hip_satd(int log2w_m2, int log2h_m2, me_pel *p_org, int i_org, me_pel *p_pred, int i_pred);  /* com_cost.c:176-710 */
+uint32_t me_hip_had(int w, int h, me_pel *org, int s_org, me_pel *cur, int s_cur, int bit_depth);        /* com_had, com_cost.c:712-812 */
+void me_hip_ipcpy(const me_pel *src, int i_src, me_pel *dst, int i_dst, int width, int height);           /* com_mc.c:649-657 */
+void me_hip_ipflt(int type, const me_pel *src, int i_src, me_pel *dst, int i_dst, int width, int height,
+                  const int8_t *coeff, int max_val);                                                      /* com_mc.c:461-491, 555-585 */
+void me_hip_ipflt_ext(int type, const me_pel *src, int i_src, me_pel *dst, int i_dst, int width, int height,
+                      const int8_t *coef_x, const int8_t *coef_y, int max_val);                           /* com_mc.c:493-549, 587-643 */
+void me_hip_pel_avrg(int width, me_pel *dst, int i_dst, me_pel *src1, me_pel *src2, int height);         /* com_util.c:3088-3105 */
+void me_hip_ip_flt_y_hor(const me_pel *src, int i_src, me_pel *dst[3], int i_dst, int16_t *dst_tmp[3], int i_dst_tmp,
+                         int width, int height, int8_t (*coeff)[8], int bit_depth);                       /* com_mc.c:668-702 */
+void me_hip_ip_flt_y_ver(const me_pel *src, int i_src, me_pel *dst[3], int i_dst, int width, int height,
+                         int8_t (*coeff)[8], int bit_depth);                                              /* com_mc.c:704-729 */
+void me_hip_ip_flt_y_ver_ext(const int16_t *src, int i_src, me_pel *dst[3], int i_dst, int width, int height,
+                             int8_t (*coeff)[8], int bit_depth);                                          /* com_mc.c:731-758 */
+
+/* ------------------------------------------------------------------------ */
+/* 2. Batched frame ME (no counterpart in the reference)                    */
+/* ------------------------------------------------------------------------ */
+
+/* One result per (PU, reference). */
+typedef struct me_pu_result_t {
+    int16_t mv_int[2];     /* integer-pel full-search MV (me.c:553 mv_ipel) */
+    int16_t mv_sub[2];     /* quarter-pel refined MV (me_sub_pel_search, me.c:366-500) */
+    uint64_t cost_int;     /* u64 integer-pel cost (me.c:151-187); UINT64_MAX if not searched */
+    uint64_t cost_sub;     /* u64 sub-pel cost (me.c:409-487) */
+    uint32_t cost_final;   /* (u32)((cost_sub + lambda*extra_bits + 2^15) >> 16) (me.c:560-567) */
+    uint32_t valid;        /* 1 if the PU lies inside the picture and its window is not empty */
+} me_pu_result_t;
+
+/* A reference picture resident in device memory: the edge-padded integer
+ * luma plane and its 15 quarter-pel planes, laid out like the reference's
+ * com_pic_t / com_subpel_t (com_util.c:62-91, 251-284): plane index
+ * dy*4+dx, pad ME 144 luma samples (PIC_PAD_SIZE_L), stride pic_w + 288. */
+typedef struct me_ref_s *me_ref_t;
+
+int me_ref_create(me_ref_t *out, int pic_w, int pic_h);
+/* Loads luma (pic_h rows of pic_w samples at `stride`) from host memory, or
+ * from device memory when on_device != 0, pads it (com_img_padding, com_util.c:
+ * 105-165) and builds the 15 planes (com_if_luma_frame, com_mc.c:760-800).
+ * stream: a hipStream_t or NULL. */
+int me_ref_load(me_ref_t ref, const me_pel *luma, int stride, int on_device, void *stream);
+/* Device address of plane `idx` at sample (0,0); *stride and *pad receive the
+ * layout.  Valid until me_ref_destroy. */
+int me_ref_plane(me_ref_t ref, int idx, me_pel **plane, int *stride, int *pad);
+/* Copies the whole padded plane `idx` ((pic_h+288) rows of pic_w+288
+ * samples) to host memory. */
+int me_ref_download_plane(me_ref_t ref, int idx, me_pel *out);
+void me_ref_destroy(me_ref_t ref);
+
+/* Frame-level configuration (the batched workloads use MVP = (0,0) for every
+ * PU and AMVR precision 0). */
+typedef struct me_frame_cfg_t {
+    int pic_w, pic_h;
+    int search_range;      /* R: integer full search over [-R, R]^2 clipped like me.c:502-527 */
+    uint32_t lambda_mv;    /* floor(65536*sqrt(lambda)) (uavs3e.c:851) */
+    uint32_t shape_mask;   /* bit s selects shape s of the 29-shape list below */
+    int num_refs;          /* references searched per PU (tbl_refi_bits row) */
+    int band_rows;         /* tuning: displacement rows per workgroup, 0 = default */
+    int waves;             /* tuning: waves per workgroup (1..4), 0 = default */
+    int dx_cols;           /* tuning: displacement columns per workgroup (multiple of 4), 0 = all */
+} me_frame_cfg_t;
+
+/* The 29 PU shapes, in mask-bit order: 4x8 8x4 4x16 16x4 4x32 32x4 8x8 8x16
+ * 16x8 8x32 32x8 8x64 64x8 16x16 16x32 32x16 16x64 64x16 16x128 128x16 32x32
+ * 32x64 64x32 32x128 128x32 64x64 64x128 128x64 128x128.  PUs of a CTU are
+ * numbered shape by shape in this order, each shape tiled in raster order
+ * over the 128x128 CTU; CTUs are numbered in raster order. */
+int me_shape_w(int s);
+int me_shape_h(int s);
+int me_pus_per_ctu(uint32_t shape_mask);
+
+typedef struct me_plan_s *me_plan_t;
+
+int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg);
+/* Current picture, from host or device memory. */
+int me_plan_set_cur(me_plan_t plan, const me_pel *cur, int stride, int on_device, void *stream);
+/* Enqueues ME of the whole frame against refs[0..num_refs-1] on `stream`
+ * as one graph launch.  Bit r of prepare_mask re-pads and re-interpolates
+ * reference r from its device luma inside the same graph (a reference that
+ * was just reconstructed).  Results stay on the device until me_plan_results. */
+int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, void *stream);
+/* Waits for the last run and copies results to host memory:
+ * res[(ctu*npu + pu)*num_refs + r], best_ref[ctu*npu + pu] (-1 = PU skipped). */
+int me_plan_results(me_plan_t plan, me_pu_result_t *res, int8_t *best_ref);
+int me_plan_num_ctus(me_plan_t plan);
+int me_plan_num_pus(me_plan_t plan);   /* per CTU */
+void me_plan_destroy(me_plan_t plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,172 @@
+"""HIP path vs the oracle (bit-exact), through the C ABI.
+
+Every test here runs the product library on the GPU and compares it with the
+oracle's C restatement on the same seeded inputs."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, MASK_ALL, MASK_16x16, SHAPES
+import uavs3e_amd as ua
+from uavs3e_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+SATD_SLOTS = [(0, 0), (1, 0), (0, 1), (1, 1), (2, 1), (1, 2)]
+
+
+@pytest.fixture(scope="module", params=[8, 10])
+def libs(request):
+    bd = request.param
+    return bd, ua.MeHip(bd), Oracle(bd)
+
+
+def test_block_costs(libs):
+    bd, hip, orc = libs
+    rng = np.random.default_rng(10 + bd)
+    S = 320
+    a = rng.integers(0, 1 << bd, (300, S)).astype(hip.dt)
+    b = rng.integers(0, 1 << bd, (300, S)).astype(hip.dt)
+    for (w, h) in SHAPES + [(4, 4)]:
+        oo = int(rng.integers(0, 100)) * S + int(rng.integers(0, 100))
+        po = int(rng.integers(0, 100)) * S + int(rng.integers(0, 100))
+        assert hip.sad(w, h, a, S, b, S, oo, po) == orc.sad(w, h, a, S, b, S, oo, po), (w, h)
+        assert hip.had(w, h, a, S, b, S, oo, po) == orc.had(w, h, a, S, b, S, oo, po), (w, h)
+        offs = [po + k for k in range(4)]
+        x4 = hip.sad_xn(w, h, a, S, oo, b, S, offs)
+        x3 = hip.sad_xn(w, h, a, S, oo, b, S, offs[:3])
+        ref = [orc.sad(w, h, a, S, b, S, oo, o) for o in offs]
+        assert list(x4) == ref and list(x3) == ref[:3], (w, h)
+    for wl_, hl in SATD_SLOTS:
+        for t in range(20):
+            oo = int(rng.integers(0, 200)) * S + int(rng.integers(0, 200))
+            po = int(rng.integers(0, 200)) * S + int(rng.integers(0, 200))
+            assert hip.satd(wl_, hl, a, S, b, S, oo, po) == orc.satd(wl_, hl, a, S, b, S, oo, po)
+    assert hip.error_count() == 0
+
+
+def test_filters(libs):
+    bd, hip, orc = libs
+    rng = np.random.default_rng(20 + bd)
+    S = 320
+    src = rng.integers(0, 1 << bd, (200, S)).astype(hip.dt)
+    maxv = (1 << bd) - 1
+    off = 10 * S + 10
+    for typ in range(4):
+        n = 8 if typ in (0, 2) else 4
+        for w in (4, 8, 16, 32, 64) + ((128,) if n == 8 else ()):
+            for frac in range(1, 4 if n == 8 else 8):
+                c = orc.luma_taps(frac) if n == 8 else orc.chroma_taps(frac)
+                h = int(rng.choice([4, 8, 16]))
+                assert np.array_equal(hip.ipflt(typ, src, S, off, w, h, c, maxv), orc.ipflt(typ, src, S, off, w, h, c, maxv))
+    for typ in range(2):
+        n = 8 if typ == 0 else 4
+        for w in (4, 8, 16, 32, 64) + ((128,) if n == 8 else ()):
+            for fx in range(1, 4 if n == 8 else 8):
+                fy = int(rng.integers(1, 4 if n == 8 else 8))
+                cx = orc.luma_taps(fx) if n == 8 else orc.chroma_taps(fx)
+                cy = orc.luma_taps(fy) if n == 8 else orc.chroma_taps(fy)
+                h = int(rng.choice([4, 8, 16]))
+                assert np.array_equal(hip.ipflt_ext(typ, src, S, off, w, h, cx, cy, maxv),
+                                      orc.ipflt_ext(typ, src, S, off, w, h, cx, cy, maxv))
+    for w in (4, 8, 16, 32, 64, 128):
+        s1 = rng.integers(0, 1 << bd, (8, w)).astype(hip.dt)
+        s2 = rng.integers(0, 1 << bd, (8, w)).astype(hip.dt)
+        assert np.array_equal(hip.pel_avrg(w, 8, s1, s2), orc.pel_avrg(w, 8, s1, s2))
+        assert np.array_equal(hip.ipcpy(src, S, off, w, 8), src[10:18, 10:10 + w])
+    assert hip.error_count() == 0
+
+
+@pytest.mark.parametrize("size", [(72, 40), (200, 136)])
+def test_planes_device_and_table(libs, size):
+    """Device reference planes and com_if_luma_frame through the per-call
+    frame filters both equal the oracle's 16 padded planes."""
+    bd, hip, orc = libs
+    w, h = size
+    host = ua.MeHost(bd)
+    luma = np.random.default_rng(30 + bd + w).integers(0, 1 << bd, (h, w)).astype(hip.dt)
+    want = orc.make_planes(luma)
+    ref = hip.ref_create(luma)
+    got = hip.ref_planes(ref, w, h)
+    hip.ref_destroy(ref)
+    for i in range(16):
+        assert np.array_equal(got[i], want[i]), "device plane %d" % i
+    t = hip.dispatch_table()
+    via = host.planes_via_table(t, luma)
+    for i in range(16):
+        assert np.array_equal(via[i], want[i]), "table plane %d" % i
+    assert hip.error_count() == 0
+
+
+def _run_plan(hip, cur, refs, R, lam, mask, **kw):
+    h, w = cur.shape
+    dref = [hip.ref_create(r) for r in refs]
+    plan = hip.plan_create(w, h, R, lam, mask, len(refs), **kw)
+    hip.plan_set_cur(plan, cur)
+    hip.plan_run(plan, dref)
+    res, best = hip.plan_results(plan, len(refs))
+    hip.plan_destroy(plan)
+    for r in dref:
+        hip.ref_destroy(r)
+    return res, best
+
+
+def _compare(res, best, ores, obest):
+    valid = ores["valid"] == 1
+    assert np.array_equal(res["valid"], ores["valid"])
+    for f in ("mv_int", "mv_sub", "cost_int", "cost_sub", "cost_final"):
+        a, b = res[f][valid], ores[f][valid]
+        bad = np.nonzero(np.any(a != b, axis=-1) if a.ndim > 1 else a != b)[0]
+        assert len(bad) == 0, "%s differs at %d of %d valid PUs, first %s vs %s" % (
+            f, len(bad), valid.sum(), a[bad[0]], b[bad[0]])
+    assert np.array_equal(best, obest)
+
+
+CASES = [
+    # (bd, w, h, R, mask, nref)
+    (8, 384, 264, 8, MASK_ALL, 2),
+    (8, 384, 264, 40, MASK_16x16, 1),
+    (8, 136, 136, 128, 1 << SHAPES.index((128, 128)), 1),
+    (10, 384, 264, 8, MASK_ALL, 2),
+    (10, 200, 136, 24, MASK_16x16, 2),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "bd%d_%dx%d_R%d_m%x_n%d" % c)
+def test_frame_plan_vs_oracle(case):
+    bd, w, h, R, mask, nref = case
+    base = wl.cfg2() if bd == 8 else wl.cfg3()
+    work = base.scaled(w, h, R)
+    work.ref_shifts_qpel = work.ref_shifts_qpel[:nref]
+    cur, refs = work.frames()
+    hip, orc = ua.MeHip(bd), Oracle(bd)
+    lam = wl.lambda_mv(32)
+    res, best = _run_plan(hip, cur, refs, R, lam, mask)
+    ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], R, lam, mask)
+    _compare(res, best, ores, obest)
+    assert res["valid"].sum() > 0
+
+
+def test_plan_tuning_variants_agree():
+    """Band heights and waves per block change only the schedule."""
+    work = wl.cfg2().scaled(256, 200, 12)
+    cur, refs = work.frames()
+    hip = ua.MeHip(8)
+    lam = wl.lambda_mv(32)
+    base = _run_plan(hip, cur, refs, 12, lam, MASK_ALL)
+    for band, waves, cols in ((1, 1, 0), (5, 2, 8), (25, 4, 12), (3, 4, 4)):
+        res, best = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, band_rows=band, waves=waves, dx_cols=cols)
+        assert np.array_equal(res, base[0]) and np.array_equal(best, base[1]), (band, waves, cols)
+
+
+def test_host_driver_percall_vs_oracle():
+    """The per-call drop-in path: the host driver calling HIP table entries."""
+    work = wl.cfg1().scaled(144, 136, 3)
+    cur, refs = work.frames()
+    hip, orc, host = ua.MeHip(8), Oracle(8), ua.MeHost(8)
+    t = hip.dispatch_table()
+    planes = [host.planes_via_table(t, r) for r in refs]
+    lam = wl.lambda_mv(32)
+    res, best, ms = host.me_frame(t, cur, planes, 3, lam, MASK_16x16, 0, 1, threads=2)
+    ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], 3, lam, MASK_16x16, 0, 1)
+    _compare(res[:1], best[:1], ores[:1], obest[:1])
+    assert hip.error_count() == 0

@@ -1,0 +1,281 @@
+"""uavs3e motion estimation on MI355X (gfx950): Python bindings.
+
+The product is the C-ABI library ``lib/libuavs3e_me_hip{8,10}.so`` declared
+in ``include/uavs3e_me_hip.h``.  This module loads it with ctypes and mirrors
+its two layers for Python callers and tests:
+
+* ``MeHip(bd)`` -- library state, the per-call entries (the hot-path slots of
+  uavs3e's funs_handle_t, com_util.h:300-330), the dispatch table, device
+  reference pictures and frame plans.
+* ``MeHost(bd)`` -- host C helpers: the ME driver that calls costs only
+  through a dispatch table (the per-call drop-in path) and the synthetic
+  frame generator.
+
+Loading fails loudly when a library is missing; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+SHAPES = [(4, 8), (8, 4), (4, 16), (16, 4), (4, 32), (32, 4), (8, 8), (8, 16), (16, 8), (8, 32),
+          (32, 8), (8, 64), (64, 8), (16, 16), (16, 32), (32, 16), (16, 64), (64, 16), (16, 128),
+          (128, 16), (32, 32), (32, 64), (64, 32), (32, 128), (128, 32), (64, 64), (64, 128),
+          (128, 64), (128, 128)]
+MASK_ALL = (1 << 29) - 1
+MASK_16x16 = 1 << SHAPES.index((16, 16))
+PAD = 144
+
+PU_RESULT = np.dtype([("mv_int", "<i2", 2), ("mv_sub", "<i2", 2), ("cost_int", "<u8"),
+                      ("cost_sub", "<u8"), ("cost_final", "<u4"), ("valid", "<u4")])
+
+# me_dispatch_t: 78 function pointers, slot order of the C struct.
+DISPATCH_SLOTS = (
+    [("ipcpy", i) for i in range(6)] +
+    [("ipflt", t, i) for t in range(4) for i in range(6)] +
+    [("ipflt_ext", t, i) for t in range(2) for i in range(6)] +
+    [("ip_flt_y_hor",), ("ip_flt_y_ver",), ("ip_flt_y_ver_ext",)] +
+    [("cost_sad", i) for i in range(6)] +
+    [("cost_sad_x3", i) for i in range(6)] +
+    [("cost_sad_x4", i) for i in range(6)] +
+    [("cost_satd", a, b) for a in range(3) for b in range(3)] +
+    [("pel_avrg", i) for i in range(6)])
+DispatchTable = C.c_void_p * len(DISPATCH_SLOTS)
+
+
+class FrameCfg(C.Structure):
+    _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("search_range", C.c_int), ("lambda_mv", C.c_uint32),
+                ("shape_mask", C.c_uint32), ("num_refs", C.c_int), ("band_rows", C.c_int), ("waves", C.c_int),
+                ("dx_cols", C.c_int)]
+
+
+class MeError(RuntimeError):
+    pass
+
+
+def pel_dtype(bd):
+    return np.uint8 if bd == 8 else np.uint16
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _pat(a, off):
+    return C.c_void_p(a.ctypes.data + int(off) * a.itemsize)
+
+
+def hip_lib_path(bd):
+    return os.path.join(LIB_DIR, "libuavs3e_me_hip%d.so" % bd)
+
+
+def host_lib_path(bd):
+    return os.path.join(LIB_DIR, "libme_host%d.so" % bd)
+
+
+class MeHip:
+    """The HIP library for one bit depth."""
+
+    def __init__(self, bd=8, init=True):
+        path = hip_lib_path(bd)
+        if not os.path.exists(path):
+            raise MeError("HIP library not built: %s (run __graft_entry__.build())" % path)
+        self.bd = bd
+        self.dt = pel_dtype(bd)
+        L = self.lib = C.CDLL(path)
+        L.me_hip_last_error.restype = C.c_char_p
+        L.me_hip_error_count.restype = C.c_uint64
+        for n in ("me_hip_sad", "me_hip_satd", "me_hip_had"):
+            getattr(L, n).restype = C.c_uint32
+        L.me_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+        for n in ("me_plan_num_ctus", "me_plan_num_pus", "me_plan_set_cur", "me_plan_results", "me_ref_load",
+                  "me_ref_plane"):
+            getattr(L, n).argtypes = None
+        L.me_plan_destroy.argtypes = [C.c_void_p]
+        L.me_ref_destroy.argtypes = [C.c_void_p]
+        assert L.me_hip_bit_depth() == bd
+        if init:
+            self.check(L.me_hip_init(0), "me_hip_init")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise MeError("%s failed: %s" % (what, self.lib.me_hip_last_error().decode()))
+
+    def error_count(self):
+        return self.lib.me_hip_error_count()
+
+    # -- dispatch table ----------------------------------------------------
+    def dispatch_table(self):
+        t = DispatchTable()
+        self.check(self.lib.me_funs_init_hip(C.byref(t)), "me_funs_init_hip")
+        return t
+
+    # -- per-call entries (numpy views; offsets are flat element indices) ---
+    def sad(self, w, h, org, i_org, pred, i_pred, org_off=0, pred_off=0):
+        return self.lib.me_hip_sad(w, _pat(org, org_off), i_org, _pat(pred, pred_off), i_pred, h)
+
+    def sad_xn(self, w, h, org, i_org, org_off, pred, i_pred, offs):
+        out = np.zeros(4, np.uint32)
+        ps = [_pat(pred, o) for o in offs]
+        if len(offs) == 3:
+            self.lib.me_hip_sad_x3(w, _pat(org, org_off), i_org, *ps, i_pred, _p(out), h)
+        else:
+            self.lib.me_hip_sad_x4(w, _pat(org, org_off), i_org, *ps, i_pred, _p(out), h)
+        return out[:len(offs)]
+
+    def satd(self, wl, hl, org, i_org, cur, i_cur, org_off=0, cur_off=0):
+        return self.lib.me_hip_satd(wl, hl, _pat(org, org_off), i_org, _pat(cur, cur_off), i_cur)
+
+    def had(self, w, h, org, i_org, cur, i_cur, org_off=0, cur_off=0):
+        return self.lib.me_hip_had(w, h, _pat(org, org_off), i_org, _pat(cur, cur_off), i_cur, self.bd)
+
+    def ipflt(self, typ, src, i_src, src_off, w, h, coef, max_val):
+        dst = np.zeros((h, w), self.dt)
+        coef = np.ascontiguousarray(coef, np.int8)
+        self.lib.me_hip_ipflt(typ, _pat(src, src_off), i_src, _p(dst), w, w, h, _p(coef), max_val)
+        return dst
+
+    def ipflt_ext(self, typ, src, i_src, src_off, w, h, cx, cy, max_val):
+        dst = np.zeros((h, w), self.dt)
+        cx = np.ascontiguousarray(cx, np.int8)
+        cy = np.ascontiguousarray(cy, np.int8)
+        self.lib.me_hip_ipflt_ext(typ, _pat(src, src_off), i_src, _p(dst), w, w, h, _p(cx), _p(cy), max_val)
+        return dst
+
+    def ipcpy(self, src, i_src, src_off, w, h):
+        dst = np.zeros((h, w), self.dt)
+        self.lib.me_hip_ipcpy(_pat(src, src_off), i_src, _p(dst), w, w, h)
+        return dst
+
+    def pel_avrg(self, w, h, s1, s2):
+        dst = np.zeros((h, w), self.dt)
+        s1 = np.ascontiguousarray(s1, self.dt)
+        s2 = np.ascontiguousarray(s2, self.dt)
+        self.lib.me_hip_pel_avrg(w, _p(dst), w, _p(s1), _p(s2), h)
+        return dst
+
+    # -- device references ---------------------------------------------------
+    def ref_create(self, luma):
+        luma = np.ascontiguousarray(luma, self.dt)
+        h, w = luma.shape
+        r = C.c_void_p()
+        self.check(self.lib.me_ref_create(C.byref(r), w, h), "me_ref_create")
+        self.check(self.lib.me_ref_load(r, _p(luma), w, 0, None), "me_ref_load")
+        return r
+
+    def ref_planes(self, ref, w, h, pad=PAD):
+        """Download the 16 padded planes of a device reference: (16, H+2pad, W+2pad)."""
+        planes = np.zeros((16, h + 2 * pad, w + 2 * pad), self.dt)
+        for i in range(16):
+            self.check(self.lib.me_ref_download_plane(ref, i, _p(planes[i])), "me_ref_download_plane")
+        return planes
+
+    def ref_destroy(self, ref):
+        self.lib.me_ref_destroy(ref)
+
+    # -- plans ---------------------------------------------------------------
+    def plan_create(self, w, h, R, lambda_mv, mask, num_refs, band_rows=0, waves=0, dx_cols=0):
+        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols)
+        p = C.c_void_p()
+        self.check(self.lib.me_plan_create(C.byref(p), C.byref(cfg)), "me_plan_create")
+        return p
+
+    def plan_set_cur(self, plan, cur):
+        cur = np.ascontiguousarray(cur, self.dt)
+        self.check(self.lib.me_plan_set_cur(plan, _p(cur), cur.shape[1], 0, None), "me_plan_set_cur")
+
+    def plan_run(self, plan, refs, prepare_mask=0, stream=None):
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        self.check(self.lib.me_plan_run(plan, arr, prepare_mask, stream), "me_plan_run")
+
+    def plan_results(self, plan, num_refs):
+        n_ctu = self.lib.me_plan_num_ctus(plan)
+        npu = self.lib.me_plan_num_pus(plan)
+        res = np.zeros((n_ctu, npu, num_refs), PU_RESULT)
+        best = np.zeros((n_ctu, npu), np.int8)
+        self.check(self.lib.me_plan_results(plan, _p(res), _p(best)), "me_plan_results")
+        return res, best
+
+    def plan_destroy(self, plan):
+        self.lib.me_plan_destroy(plan)
+
+
+class MeHost:
+    """Host C helpers: synthetic frames and the table-driven ME driver."""
+
+    def __init__(self, bd=8):
+        path = host_lib_path(bd)
+        if not os.path.exists(path):
+            raise MeError("host library not built: %s" % path)
+        self.bd = bd
+        self.dt = pel_dtype(bd)
+        L = self.lib = C.CDLL(path)
+        L.me_synth_frame.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_void_p]
+        L.me_synth_key.restype = C.c_uint64
+        L.me_synth_key.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int]
+        L.me_synth_uniform.restype = C.c_double
+        L.me_synth_uniform.argtypes = [C.c_uint64, C.c_uint64]
+        L.me_synth_sin.restype = C.c_double
+        L.me_synth_sin.argtypes = [C.c_double]
+        L.me_synth_region_offset.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_void_p]
+        L.me_host_frame.restype = C.c_double
+        L.me_host_frame.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+        L.me_host_if_luma_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+
+    def synth(self, seed, stream, frame, w, h, sx=0.0, sy=0.0, region_qpel=0, sigma=0.0, noise_purpose=3,
+              offset=0.0):
+        out = np.zeros((h, w), self.dt)
+        self.lib.me_synth_frame(seed, stream, frame, w, h, self.bd, sx, sy, region_qpel, sigma, noise_purpose,
+                                offset, _p(out))
+        return out
+
+    def key(self, seed, stream, frame, purpose):
+        return self.lib.me_synth_key(seed, stream, frame, purpose)
+
+    def uniform(self, key, i):
+        return self.lib.me_synth_uniform(key, i)
+
+    def region_offset(self, seed, stream, frame, w, region_qpel, x, y):
+        out = np.zeros(2, np.int32)
+        self.lib.me_synth_region_offset(seed, stream, frame, w, region_qpel, x, y, _p(out))
+        return tuple(int(v) for v in out)
+
+    def planes_via_table(self, table, luma, pad=PAD):
+        """com_if_luma_frame through a dispatch table: (16, H+2pad, W+2pad)."""
+        h, w = luma.shape
+        stride = w + 2 * pad
+        planes = np.zeros((16, h + 2 * pad, stride), self.dt)
+        planes[0, pad:pad + h, pad:pad + w] = luma
+        ptrs = (C.c_void_p * 16)(*[planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize for i in range(16)])
+        rc = self.lib.me_host_if_luma_frame(C.byref(table), ptrs, stride, w, h, pad, self.bd)
+        if rc:
+            raise MeError("me_host_if_luma_frame failed")
+        return planes
+
+    def me_frame(self, table, cur, ref_planes_list, R, lambda_mv, mask, ctu0=0, ctu1=None, threads=1,
+                 ctu_ms=None, pad=PAD):
+        """Full search + sub-pel through the table for CTUs [ctu0, ctu1)."""
+        h, w = cur.shape
+        n_ctu = ((w + 127) // 128) * ((h + 127) // 128)
+        ctu1 = n_ctu if ctu1 is None else ctu1
+        npu = sum((128 // sw) * (128 // sh) for s, (sw, sh) in enumerate(SHAPES) if mask >> s & 1)
+        nref = len(ref_planes_list)
+        stride = w + 2 * pad
+        ptrs = (C.c_void_p * (16 * nref))()
+        for r, pl in enumerate(ref_planes_list):
+            for i in range(16):
+                ptrs[16 * r + i] = pl[i].ctypes.data + (pad * stride + pad) * pl.itemsize
+        res = np.zeros((n_ctu, npu, nref), PU_RESULT)
+        best = np.full((n_ctu, npu), -1, np.int8)
+        cur = np.ascontiguousarray(cur, self.dt)
+        ms = self.lib.me_host_frame(C.byref(table), w, h, self.bd, _p(cur), w, nref, ptrs, stride, R, lambda_mv, mask,
+                                    ctu0, ctu1, threads, _p(res), _p(best),
+                                    _p(ctu_ms) if ctu_ms is not None else None)
+        return res, best, ms

@@ -1,0 +1,1016 @@
+// Batched frame motion estimation for gfx950.
+//
+// Path per frame (one hipGraph):
+//   [optional] pad + 15 quarter-pel planes of a just-reconstructed reference
+//   k_fill_keys       -> key buffer := all ones
+//   k_int_search      -> integer full search of every PU, every reference
+//   k_subpel          -> me_sub_pel_search + final cost + reference choice
+//
+// Reference semantics (file:line in uavs3e):
+//   integer cost  (sad << (16-(bd-8))) + (bits_x+bits_y)*lambda   me.c:119,151-187
+//   window        get_max_search_range (is_padding = 1)           me.c:502-527
+//   full search   raster y-major, first strict minimum wins        me.c:150-190
+//   sub-pel       me_sub_pel_search, cost_type 0 (SATD)            me.c:366-500
+//   final cost    + lambda*(refi_bits+mvr+(mvr<4)+2), (c+2^15)>>16 me.c:560-567
+#include "common.h"
+#include "satd.h"
+#include "../../include/uavs3e_me_hip.h"
+
+#include <string.h>
+#include <vector>
+
+#define ME_MAX_REFS 4
+
+// ---------------------------------------------------------------------------
+// Reference pictures: padded integer plane + 15 quarter-pel planes
+// ---------------------------------------------------------------------------
+
+struct me_ref_s {
+    int w, h, stride, rows;
+    size_t plane_elems;      // stride * rows
+    pel *planes;             // 16 planes, plane i at planes + i * plane_elems
+    pel *luma;               // unpadded w x h copy (source for re-preparation)
+    s16 *tmp;                // 3 horizontal s16 planes over [-8, w+8) x [-8, h+8)
+};
+
+// Edge padding of the integer plane (com_img_padding(img, .., 0),
+// com_util.c:105-165): every padded sample copies the nearest picture sample.
+__global__ void k_pad_int(const pel *__restrict__ luma, int w, int h, pel *__restrict__ plane0, int stride, int rows)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    int Y = blockIdx.y;
+    if (X >= stride || Y >= rows) return;
+    int x = me_clip3(0, w - 1, X - ME_PAD);
+    int y = me_clip3(0, h - 1, Y - ME_PAD);
+    plane0[(size_t)Y * stride + X] = luma[(size_t)y * w + x];
+}
+
+// Horizontal s16 intermediates of com_if_filter_hor_8 (com_mc.c:668-702):
+// t = sum(taps[dx] * P0), stored as (t + add) >> (bd - 8), over
+// [-8, w+8) x [-8, h+8) for dx = 1..3.
+__global__ void k_htmp(const pel *__restrict__ p0, int stride, int w, int h, s16 *__restrict__ tmp)
+{
+    int tw = w + 16, th = h + 16;
+    int xi = blockIdx.x * blockDim.x + threadIdx.x;
+    int yi = blockIdx.y;
+    int dx = blockIdx.z + 1;
+    if (xi >= tw || yi >= th) return;
+    const pel *s = p0 + (ptrdiff_t)(yi - 8) * stride + (xi - 8);
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) t += (int)s[k - 3] * ME_LUMA_TAPS[dx][k];
+    const int sh = BIT_DEPTH - 8, add = (1 << sh) >> 1;
+    tmp[(size_t)(dx - 1) * tw * th + (size_t)yi * tw + xi] = (s16)((t + add) >> sh);
+}
+
+// The 15 sub-pel planes of com_if_luma_frame (com_mc.c:760-800) including
+// the final com_img_padding(plane, 1, 4): every padded sample equals the
+// filter output at its position clamped into the [-4, w+4) x [-4, h+4) band.
+//   dy == 0: clip((H8(P0) + 32) >> 6)              com_mc.c:668-702
+//   dx == 0: clip((V8(P0) + 32) >> 6)              com_mc.c:704-729
+//   else:    clip((V8(tmp) + 2^(19-bd)) >> (20-bd)) com_mc.c:731-758
+__global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h,
+                         const s16 *__restrict__ tmp)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    int Y = blockIdx.y;
+    int idx = blockIdx.z + 1;
+    if (X >= stride || Y >= rows) return;
+    int dx = idx & 3, dy = idx >> 2;
+    int x = me_clip3(-4, w + 3, X - ME_PAD);
+    int y = me_clip3(-4, h + 3, Y - ME_PAD);
+    const int maxv = (1 << BIT_DEPTH) - 1;
+    const pel *p0 = planes + (size_t)ME_PAD * stride + ME_PAD;
+    int v;
+    if (dy == 0) {
+        const pel *s = p0 + (ptrdiff_t)y * stride + x;
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[k - 3] * ME_LUMA_TAPS[dx][k];
+        v = (t + 32) >> 6;
+    } else if (dx == 0) {
+        const pel *s = p0 + (ptrdiff_t)y * stride + x;
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * stride] * ME_LUMA_TAPS[dy][k];
+        v = (t + 32) >> 6;
+    } else {
+        int tw = w + 16, th = h + 16;
+        const s16 *s = tmp + (size_t)(dx - 1) * tw * th + (size_t)(y + 8) * tw + (x + 8);
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * tw] * ME_LUMA_TAPS[dy][k];
+        const int sh2 = 20 - BIT_DEPTH, add2 = 1 << (sh2 - 1);
+        v = (t + add2) >> sh2;
+    }
+    planes[(size_t)idx * plane_elems + (size_t)Y * stride + X] = (pel)me_clip3(0, maxv, v);
+}
+
+static int ref_prepare_async(me_ref_s *r, hipStream_t st)
+{
+    dim3 b(256);
+    hipLaunchKernelGGL(k_pad_int, dim3((r->stride + 255) / 256, r->rows), b, 0, st, r->luma, r->w, r->h,
+                       r->planes, r->stride, r->rows);
+    hipLaunchKernelGGL(k_htmp, dim3((r->w + 16 + 255) / 256, r->h + 16, 3), b, 0, st,
+                       r->planes + (size_t)ME_PAD * r->stride + ME_PAD, r->stride, r->w, r->h, r->tmp);
+    hipLaunchKernelGGL(k_planes, dim3((r->stride + 255) / 256, r->rows, 15), b, 0, st, r->planes, r->plane_elems,
+                       r->stride, r->rows, r->w, r->h, r->tmp);
+    ME_CHECK(hipGetLastError());
+    return 0;
+}
+
+ME_EXPORT int me_ref_create(me_ref_t *out, int pic_w, int pic_h)
+{
+    if (me_hip_init(-1)) return -1;
+    if (!out || pic_w <= 0 || pic_h <= 0) { me_set_error("me_ref_create: bad size", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    me_ref_s *r = (me_ref_s *)calloc(1, sizeof(me_ref_s));
+    r->w = pic_w; r->h = pic_h;
+    r->stride = pic_w + 2 * ME_PAD;
+    r->rows = pic_h + 2 * ME_PAD;
+    r->plane_elems = (size_t)r->stride * r->rows;
+    hipError_t e = hipMalloc((void **)&r->planes, 16 * r->plane_elems * sizeof(pel));
+    if (e == hipSuccess) e = hipMalloc((void **)&r->luma, (size_t)pic_w * pic_h * sizeof(pel));
+    if (e == hipSuccess) e = hipMalloc((void **)&r->tmp, 3 * (size_t)(pic_w + 16) * (pic_h + 16) * sizeof(s16));
+    if (e != hipSuccess) {
+        me_set_error("me_ref_create: hipMalloc", e, __FILE__, __LINE__);
+        hipFree(r->planes); hipFree(r->luma); hipFree(r->tmp); free(r);
+        return -1;
+    }
+    *out = r;
+    return 0;
+}
+
+ME_EXPORT int me_ref_load(me_ref_t r, const me_pel *luma, int stride, int on_device, void *stream)
+{
+    if (!r || !luma || stride < r->w) { me_set_error("me_ref_load: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    hipStream_t st = (hipStream_t)stream;
+    ME_CHECK(hipMemcpy2DAsync(r->luma, r->w * sizeof(pel), luma, stride * sizeof(pel), r->w * sizeof(pel), r->h,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    if (ref_prepare_async(r, st)) return -1;
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+ME_EXPORT int me_ref_plane(me_ref_t r, int idx, me_pel **plane, int *stride, int *pad)
+{
+    if (!r || idx < 0 || idx > 15) return -1;
+    if (plane) *plane = r->planes + (size_t)idx * r->plane_elems + (size_t)ME_PAD * r->stride + ME_PAD;
+    if (stride) *stride = r->stride;
+    if (pad) *pad = ME_PAD;
+    return 0;
+}
+
+ME_EXPORT int me_ref_download_plane(me_ref_t r, int idx, me_pel *out)
+{
+    if (!r || !out || idx < 0 || idx > 15) { me_set_error("me_ref_download_plane: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    ME_CHECK(hipMemcpy(out, r->planes + (size_t)idx * r->plane_elems, r->plane_elems * sizeof(pel), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+ME_EXPORT void me_ref_destroy(me_ref_t r)
+{
+    if (!r) return;
+    hipFree(r->planes); hipFree(r->luma); hipFree(r->tmp);
+    free(r);
+}
+
+// ---------------------------------------------------------------------------
+// Shapes and PU numbering
+// ---------------------------------------------------------------------------
+
+ME_EXPORT int me_shape_w(int s) { return s >= 0 && s < ME_NSHAPES ? ME_SHAPES_HOST[s][0] : 0; }
+ME_EXPORT int me_shape_h(int s) { return s >= 0 && s < ME_NSHAPES ? ME_SHAPES_HOST[s][1] : 0; }
+ME_EXPORT int me_pus_per_ctu(uint32_t mask)
+{
+    int n = 0;
+    for (int s = 0; s < ME_NSHAPES; s++)
+        if (mask >> s & 1) n += (128 / ME_SHAPES_HOST[s][0]) * (128 / ME_SHAPES_HOST[s][1]);
+    return n;
+}
+
+enum {
+    S4x8 = 0, S8x4, S4x16, S16x4, S4x32, S32x4, S8x8, S8x16, S16x8, S8x32, S32x8, S8x64, S64x8, S16x16,
+    S16x32, S32x16, S16x64, S64x16, S16x128, S128x16, S32x32, S32x64, S64x32, S32x128, S128x32, S64x64,
+    S64x128, S128x64, S128x128
+};
+
+// ---------------------------------------------------------------------------
+// Integer full search
+// ---------------------------------------------------------------------------
+
+// Keys: a candidate's cost c (u64, < 2^41) and its raster index i in the
+// (2R+1)^2 window are packed as key = (c << IB) | i, so the smallest key is
+// the smallest cost with the first raster position on ties -- the reference's
+// strict '<' scan (me.c:169-172, 187).  With cost = (sad << shift) + rate and
+// IB >= 32 - shift, key = sad * 2^(shift+IB) + K(d), K(d) = (rate << IB) | i,
+// so the low word of a key is K's low word and its high word is
+// (sad << (shift+IB-32)) + K_hi(d): one shift-add per candidate.  KEY_BIAS
+// lifts every high word above 2^28 so that keys are normal doubles.
+#define KEY_BIAS_HI 0x10000000u
+#define KEY_INVALID_HI 0x40000000u
+#define KEY_INIT 0x7FF0000000000000ull
+
+struct IntArgs {
+    const pel *cur;          // current picture, stride pic_w
+    const pel *ref0[ME_MAX_REFS];  // integer plane of each reference at (0,0)
+    int ref_stride;
+    int pic_w, pic_h, ctu_cols;
+    int R;                   // window half size
+    int lo, hi;              // uniform displacement clip: [max(-R,-T+4), min(R,T-4)]
+    int IB, sh;              // index bits; sad shift into the high word
+    const u32 *rate;         // [2R+1]: bits(4*(i-R)) * lambda
+    u64 *keys;               // [ctu][npu][nref]
+    int npu, nref;
+    int band_rows, wcols;    // displacement rows per block; LDS window row length
+    int dx_cols, nbx;        // displacement columns per block; blocks along x
+    int base[ME_NSHAPES];    // first PU index of each shape inside a CTU (-1: not selected)
+};
+
+struct KeyCtx {
+    u32 khi, klo;            // uniform key parts of the current displacement
+    int dx, dy;              // current displacement
+    int sh;
+};
+
+template <bool EDGE>
+struct Edge {
+    int px0, py0, wlim, hlim;  // picture position of the lane's region; limits
+};
+
+__device__ __forceinline__ bool node_valid(int px, int py, int dx, int dy, int wlim, int hlim)
+{
+    // is_padding = 1 window of get_max_search_range (me.c:508-512):
+    // -127 - x <= dx <= W-1-x and the same for y (the +-T limit is uniform).
+    return (u32)(px + dx + 127) <= (u32)wlim && (u32)(py + dy + 127) <= (u32)hlim;
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void upd(double &st, u32 sad, const KeyCtx &k, int px, int py, int wlim, int hlim)
+{
+    u32 hi = (sad << k.sh) + k.khi;
+    if (EDGE) {
+        if (!node_valid(px, py, k.dx, k.dy, wlim, hlim)) hi |= KEY_INVALID_HI;
+    }
+    st = me_vmin64(st, __builtin_bit_cast(double, ((u64)hi << 32) | k.klo));
+}
+
+__device__ __forceinline__ u32 xorl(u32 v, int m) { return (u32)__shfl_xor((int)v, m, 64); }
+
+// Node state layout per lane (MODE_ALL):
+//   small[0..7]   4x8   (c, r2)    small[8..15]  8x4  (c2, r)
+//   small[16..19] 4x16  (c)        small[20..23] 16x4 (r)
+//   small[24..27] 8x8   (c2, r2)   small[28..29] 8x16 (c2)
+//   small[30..31] 16x8  (r2)       small[32]     16x16
+//   large[0..3] 32x4 (r)  [4..5] 32x8 (r2) [6] 32x16 [7..10] 4x32 (c) [11..12] 8x32 (c2)
+//   [13] 16x32 [14] 32x32 [15..16] 64x8 (r2) [17] 64x16 [18..19] 8x64 (c2) [20] 16x64
+//   [21] 64x32 [22] 32x64 [23] 64x64 [24] 128x16 [25] 16x128 [26] 128x32 [27] 32x128
+//   [28] 128x64 [29] 64x128 [30] 128x128
+#define N_SMALL 33
+#define N_LARGE 31
+
+template <bool EDGE>
+__device__ __forceinline__ void nodes_all(const u32 L[4][4], double *small, double *large, const KeyCtx &k,
+                                          int X, int Y, int rx, int ry, int cx, int cy, int wlim, int hlim)
+{
+    // picture position of the region
+    const int px = cx + X, py = cy + Y;
+    u32 h8[4][2], v8[2][4], q[2][2], h16[4], v16[4], h16x8[2], v8x16[2];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++) h8[r][c2] = L[r][2 * c2] + L[r][2 * c2 + 1];
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) v8[r2][c] = L[2 * r2][c] + L[2 * r2 + 1][c];
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++)
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++) q[r2][c2] = h8[2 * r2][c2] + h8[2 * r2 + 1][c2];
+#pragma unroll
+    for (int r = 0; r < 4; r++) h16[r] = h8[r][0] + h8[r][1];
+#pragma unroll
+    for (int c = 0; c < 4; c++) v16[c] = v8[0][c] + v8[1][c];
+#pragma unroll
+    for (int i = 0; i < 2; i++) { h16x8[i] = q[i][0] + q[i][1]; v8x16[i] = q[0][i] + q[1][i]; }
+    const u32 s16 = h16x8[0] + h16x8[1];
+
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++) upd<EDGE>(small[c * 2 + r2], v8[r2][c], k, px + 4 * c, py + 8 * r2, wlim, hlim);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) upd<EDGE>(small[8 + c2 * 4 + r], h8[r][c2], k, px + 8 * c2, py + 4 * r, wlim, hlim);
+#pragma unroll
+    for (int c = 0; c < 4; c++) upd<EDGE>(small[16 + c], v16[c], k, px + 4 * c, py, wlim, hlim);
+#pragma unroll
+    for (int r = 0; r < 4; r++) upd<EDGE>(small[20 + r], h16[r], k, px, py + 4 * r, wlim, hlim);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++) upd<EDGE>(small[24 + c2 * 2 + r2], q[r2][c2], k, px + 8 * c2, py + 8 * r2, wlim, hlim);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++) upd<EDGE>(small[28 + c2], v8x16[c2], k, px + 8 * c2, py, wlim, hlim);
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++) upd<EDGE>(small[30 + r2], h16x8[r2], k, px, py + 8 * r2, wlim, hlim);
+    upd<EDGE>(small[32], s16, k, px, py, wlim, hlim);
+
+    // Cross-region sums.  Lane = ry*8 + rx: x neighbours differ in lane bits
+    // 0..2, y neighbours in bits 3..5.  Every lane of a group holds the sum
+    // of the group's node; only one of them writes it out at the end.
+    const int x2 = cx + (X & ~31), x4 = cx + (X & ~63), y2 = cy + (Y & ~31), y4 = cy + (Y & ~63);
+#pragma unroll
+    for (int r = 0; r < 4; r++) upd<EDGE>(large[r], h16[r] + xorl(h16[r], 1), k, x2, py + 4 * r, wlim, hlim);
+    u32 t32x8[2];
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++) {
+        t32x8[r2] = h16x8[r2] + xorl(h16x8[r2], 1);
+        upd<EDGE>(large[4 + r2], t32x8[r2], k, x2, py + 8 * r2, wlim, hlim);
+    }
+    const u32 t32x16 = s16 + xorl(s16, 1);
+    upd<EDGE>(large[6], t32x16, k, x2, py, wlim, hlim);
+#pragma unroll
+    for (int c = 0; c < 4; c++) upd<EDGE>(large[7 + c], v16[c] + xorl(v16[c], 8), k, px + 4 * c, y2, wlim, hlim);
+    u32 t8x32[2];
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++) {
+        t8x32[c2] = v8x16[c2] + xorl(v8x16[c2], 8);
+        upd<EDGE>(large[11 + c2], t8x32[c2], k, px + 8 * c2, y2, wlim, hlim);
+    }
+    const u32 t16x32 = s16 + xorl(s16, 8);
+    upd<EDGE>(large[13], t16x32, k, px, y2, wlim, hlim);
+    const u32 t32x32 = t32x16 + xorl(t32x16, 8);
+    upd<EDGE>(large[14], t32x32, k, x2, y2, wlim, hlim);
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++) upd<EDGE>(large[15 + r2], t32x8[r2] + xorl(t32x8[r2], 2), k, x4, py + 8 * r2, wlim, hlim);
+    const u32 t64x16 = t32x16 + xorl(t32x16, 2);
+    upd<EDGE>(large[17], t64x16, k, x4, py, wlim, hlim);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++) upd<EDGE>(large[18 + c2], t8x32[c2] + xorl(t8x32[c2], 16), k, px + 8 * c2, y4, wlim, hlim);
+    const u32 t16x64 = t16x32 + xorl(t16x32, 16);
+    upd<EDGE>(large[20], t16x64, k, px, y4, wlim, hlim);
+    const u32 t64x32 = t32x32 + xorl(t32x32, 2);
+    upd<EDGE>(large[21], t64x32, k, x4, y2, wlim, hlim);
+    const u32 t32x64 = t32x32 + xorl(t32x32, 16);
+    upd<EDGE>(large[22], t32x64, k, x2, y4, wlim, hlim);
+    const u32 t64x64 = t64x32 + xorl(t64x32, 16);
+    upd<EDGE>(large[23], t64x64, k, x4, y4, wlim, hlim);
+    upd<EDGE>(large[24], t64x16 + xorl(t64x16, 4), k, cx, py, wlim, hlim);
+    upd<EDGE>(large[25], t16x64 + xorl(t16x64, 32), k, px, cy, wlim, hlim);
+    const u32 t128x32 = t64x32 + xorl(t64x32, 4);
+    upd<EDGE>(large[26], t128x32, k, cx, y2, wlim, hlim);
+    upd<EDGE>(large[27], t32x64 + xorl(t32x64, 32), k, x2, cy, wlim, hlim);
+    const u32 t128x64 = t64x64 + xorl(t64x64, 4);
+    upd<EDGE>(large[28], t128x64, k, cx, y4, wlim, hlim);
+    upd<EDGE>(large[29], t64x64 + xorl(t64x64, 32), k, x4, cy, wlim, hlim);
+    upd<EDGE>(large[30], t128x64 + xorl(t128x64, 32), k, cx, cy, wlim, hlim);
+    (void)rx; (void)ry;
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void nodes_16(const u32 L[4][4], double *small, const KeyCtx &k, int px, int py,
+                                         int wlim, int hlim)
+{
+    u32 s = 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) s += L[r][c];
+    upd<EDGE>(small[0], s, k, px, py, wlim, hlim);
+}
+
+__device__ __forceinline__ void atomic_key_min(u64 *p, double st)
+{
+    u64 v = __builtin_bit_cast(u64, st);
+    if (v != KEY_INIT) atomicMin((unsigned long long *)p, (unsigned long long)v);
+}
+
+// PU index inside the CTU of shape s at CTU-local (x, y).
+__device__ __forceinline__ int pu_index(const IntArgs &a, int s, int w, int h, int x, int y)
+{
+    return a.base[s] + (y / h) * (128 / w) + x / w;
+}
+
+// Leaves: the 16 4x4-block SADs of the lane's 16x16 region.
+#if BIT_DEPTH == 8
+// 8-bit: v_qsad_pk_u16_u8 gives the SADs of one 4-sample row at the four
+// displacements dxb..dxb+3 in one instruction (4 x u16, a 4x4 leaf is at most
+// 16*255).  acc[r][c] holds leaf (r, c) for the four displacements.
+struct Leaves {
+    u64 acc[4][4];
+    __device__ __forceinline__ void load(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0,
+                                         int ccol0)
+    {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[r][c] = 0;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + c0);
+            u32 d[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) d[j] = rp[j];
+            const uint4 cw = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0);
+            const u32 cc[4] = {cw.x, cw.y, cw.z, cw.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[r >> 2][j] = __builtin_amdgcn_qsad_pk_u16_u8(((u64)d[j + 1] << 32) | d[j], cc[j], acc[r >> 2][j]);
+        }
+    }
+    __device__ __forceinline__ void get(int k, u32 L[4][4]) const
+    {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) L[r][c] = (u32)(acc[r][c] >> (16 * k)) & 0xFFFFu;
+    }
+};
+#else
+// 10-bit: v_sad_u16 on sample pairs; odd displacements realign the pairs
+// with v_alignbyte.  Leaves are computed per displacement.
+struct Leaves {
+    const pel *win, *curl;
+    int wcols, wrow0, c0, crow0, ccol0;
+    __device__ __forceinline__ void load(const pel *w_, int wc, int wr, int c, const pel *cl, int cr, int cc)
+    {
+        win = w_; wcols = wc; wrow0 = wr; c0 = c; curl = cl; crow0 = cr; ccol0 = cc;
+    }
+    __device__ __forceinline__ void get(int k, u32 L[4][4]) const
+    {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) L[r][c] = 0;
+#pragma unroll 4
+        for (int r = 0; r < 16; r++) {
+            const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + c0 + (k & ~1));
+            u32 d[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) d[j] = rp[j];
+            const uint4 c01 = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0);
+            const uint4 c23 = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0 + 8);
+            const u32 cc[8] = {c01.x, c01.y, c01.z, c01.w, c23.x, c23.y, c23.z, c23.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                u32 p0, p1;
+                if (k & 1) {
+                    p0 = __builtin_amdgcn_alignbyte(d[2 * c + 1], d[2 * c], 2);
+                    p1 = __builtin_amdgcn_alignbyte(d[2 * c + 2], d[2 * c + 1], 2);
+                } else {
+                    p0 = d[2 * c];
+                    p1 = d[2 * c + 1];
+                }
+                L[r >> 2][c] = __builtin_amdgcn_sad_u16(p0, cc[2 * c], __builtin_amdgcn_sad_u16(p1, cc[2 * c + 1], L[r >> 2][c]));
+            }
+        }
+    }
+};
+#endif
+
+// One block = (displacement band, CTU, reference); 64*WAVES threads; each
+// wave covers the whole CTU (lane = 16x16 region) for every WAVES-th
+// displacement row of the band.  The reference window of the band and the
+// current CTU are staged in LDS.
+template <int MODE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_int_search(IntArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int band = blockIdx.x / a.nbx, bandx = blockIdx.x - band * a.nbx, ctu = blockIdx.y, ref = blockIdx.z;
+    const int cx = (ctu % a.ctu_cols) * 128, cy = (ctu / a.ctu_cols) * 128;
+    const int dy0 = -a.R + band * a.band_rows;
+    const int dxs = -a.R + bandx * a.dx_cols;                       // first displacement column of the block
+    const int dxe = dxs + a.dx_cols - 1 < a.hi ? dxs + a.dx_cols - 1 : a.hi;  // last one
+    const int wrows = 128 + a.band_rows - 1;
+    pel *win = (pel *)smem;
+    pel *curl = win + (size_t)wrows * a.wcols;
+    const int tid = threadIdx.x, nthr = 64 * WAVES;
+
+    // Stage the window: reference samples (cx + dxs + c, cy + dy0 + r), clamped
+    // to the padded plane (it is edge-replicated, so clamping is exact), and
+    // the current CTU clamped to the picture.
+    const pel *ref0 = a.ref0[ref];
+    const int xmin = -ME_PAD, xmax = a.pic_w + ME_PAD - 1, ymin = -ME_PAD, ymax = a.pic_h + ME_PAD - 1;
+    for (int i = tid; i < wrows * a.wcols; i += nthr) {
+        int r = i / a.wcols, c = i - r * a.wcols;
+        int x = me_clip3(xmin, xmax, cx + dxs + c), y = me_clip3(ymin, ymax, cy + dy0 + r);
+        win[i] = ref0[(ptrdiff_t)y * a.ref_stride + x];
+    }
+    for (int i = tid; i < 128 * 128; i += nthr) {
+        int r = i >> 7, c = i & 127;
+        int x = me_clip3(0, a.pic_w - 1, cx + c), y = me_clip3(0, a.pic_h - 1, cy + r);
+        curl[i] = a.cur[(size_t)y * a.pic_w + x];
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int rx = lane & 7, ry = lane >> 3;
+    const int X = rx * 16, Y = ry * 16;
+    const bool edge = !(cx + 124 + a.R <= a.pic_w - 1 && cy + 124 + a.R <= a.pic_h - 1 && a.R <= 127);
+    const int wlim = a.pic_w + 126, hlim = a.pic_h + 126;
+    const int span = 2 * a.R + 1;
+
+    double small[MODE == 0 ? N_SMALL : 1], large[MODE == 0 ? N_LARGE : 1];
+#pragma unroll
+    for (int i = 0; i < (MODE == 0 ? N_SMALL : 1); i++) small[i] = __builtin_bit_cast(double, KEY_INIT);
+#pragma unroll
+    for (int i = 0; i < (MODE == 0 ? N_LARGE : 1); i++) large[i] = __builtin_bit_cast(double, KEY_INIT);
+
+    for (int row = wave; row < a.band_rows; row += WAVES) {
+        const int dy = dy0 + row;
+        if (dy < a.lo || dy > a.hi) continue;
+        const int dyi = dy + a.R;
+        const u32 ry_rate = a.rate[dyi];
+        for (int dxb = dxs; dxb <= dxe; dxb += 4) {
+            if (dxb + 3 < a.lo) continue;
+            Leaves lv;
+            lv.load(win, a.wcols, Y + row, X + (dxb - dxs), curl, Y, X);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int dx = dxb + k;
+                if (dx < a.lo || dx > dxe) continue;
+                const int dxi = dx + a.R;
+                const u32 rate = a.rate[dxi] + ry_rate;
+                const u64 K = ((u64)rate << a.IB) | (u64)(dyi * span + dxi);
+                u32 L[4][4];
+                lv.get(k, L);
+                KeyCtx kc;
+                kc.khi = (u32)(K >> 32) + KEY_BIAS_HI;
+                kc.klo = (u32)K;
+                kc.dx = dx; kc.dy = dy; kc.sh = a.sh;
+                if (MODE == 0) {
+                    if (edge) nodes_all<true>(L, small, large, kc, X, Y, rx, ry, cx, cy, wlim, hlim);
+                    else nodes_all<false>(L, small, large, kc, X, Y, rx, ry, cx, cy, wlim, hlim);
+                } else {
+                    if (edge) nodes_16<true>(L, small, kc, cx + X, cy + Y, wlim, hlim);
+                    else nodes_16<false>(L, small, kc, cx + X, cy + Y, wlim, hlim);
+                }
+            }
+        }
+    }
+
+    // Publish: one atomic min per node into keys[ctu][pu][ref].
+    u64 *kb = a.keys + (size_t)ctu * a.npu * a.nref + ref;
+    const int nref = a.nref;
+#define PUB(S, W, H, XX, YY, ST)                                                          \
+    do {                                                                                  \
+        if (a.base[S] >= 0) atomic_key_min(kb + (size_t)pu_index(a, S, W, H, XX, YY) * nref, ST); \
+    } while (0)
+    if (MODE == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int r2 = 0; r2 < 2; r2++) PUB(S4x8, 4, 8, X + 4 * c, Y + 8 * r2, small[c * 2 + r2]);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) PUB(S8x4, 8, 4, X + 8 * c2, Y + 4 * r, small[8 + c2 * 4 + r]);
+#pragma unroll
+        for (int c = 0; c < 4; c++) PUB(S4x16, 4, 16, X + 4 * c, Y, small[16 + c]);
+#pragma unroll
+        for (int r = 0; r < 4; r++) PUB(S16x4, 16, 4, X, Y + 4 * r, small[20 + r]);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+            for (int r2 = 0; r2 < 2; r2++) PUB(S8x8, 8, 8, X + 8 * c2, Y + 8 * r2, small[24 + c2 * 2 + r2]);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++) PUB(S8x16, 8, 16, X + 8 * c2, Y, small[28 + c2]);
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++) PUB(S16x8, 16, 8, X, Y + 8 * r2, small[30 + r2]);
+        PUB(S16x16, 16, 16, X, Y, small[32]);
+
+        const bool ex1 = !(rx & 1), ex2 = !(rx & 3), ex3 = rx == 0;
+        const bool ey1 = !(ry & 1), ey2 = !(ry & 3), ey3 = ry == 0;
+        if (ex1) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) PUB(S32x4, 32, 4, X, Y + 4 * r, large[r]);
+            PUB(S32x8, 32, 8, X, Y, large[4]);
+            PUB(S32x8, 32, 8, X, Y + 8, large[5]);
+            PUB(S32x16, 32, 16, X, Y, large[6]);
+        }
+        if (ey1) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) PUB(S4x32, 4, 32, X + 4 * c, Y, large[7 + c]);
+            PUB(S8x32, 8, 32, X, Y, large[11]);
+            PUB(S8x32, 8, 32, X + 8, Y, large[12]);
+            PUB(S16x32, 16, 32, X, Y, large[13]);
+        }
+        if (ex1 && ey1) PUB(S32x32, 32, 32, X, Y, large[14]);
+        if (ex2) {
+            PUB(S64x8, 64, 8, X, Y, large[15]);
+            PUB(S64x8, 64, 8, X, Y + 8, large[16]);
+            PUB(S64x16, 64, 16, X, Y, large[17]);
+        }
+        if (ey2) {
+            PUB(S8x64, 8, 64, X, Y, large[18]);
+            PUB(S8x64, 8, 64, X + 8, Y, large[19]);
+            PUB(S16x64, 16, 64, X, Y, large[20]);
+        }
+        if (ex2 && ey1) PUB(S64x32, 64, 32, X, Y, large[21]);
+        if (ex1 && ey2) PUB(S32x64, 32, 64, X, Y, large[22]);
+        if (ex2 && ey2) PUB(S64x64, 64, 64, X, Y, large[23]);
+        if (ex3) PUB(S128x16, 128, 16, X, Y, large[24]);
+        if (ey3) PUB(S16x128, 16, 128, X, Y, large[25]);
+        if (ex3 && ey1) PUB(S128x32, 128, 32, X, Y, large[26]);
+        if (ex1 && ey3) PUB(S32x128, 32, 128, X, Y, large[27]);
+        if (ex3 && ey2) PUB(S128x64, 128, 64, X, Y, large[28]);
+        if (ex2 && ey3) PUB(S64x128, 64, 128, X, Y, large[29]);
+        if (ex3 && ey3) PUB(S128x128, 128, 128, X, Y, large[30]);
+    } else {
+        PUB(S16x16, 16, 16, X, Y, small[0]);
+    }
+#undef PUB
+}
+
+__global__ void k_fill_keys(u64 *keys, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = ~0ull;
+}
+
+// ---------------------------------------------------------------------------
+// Sub-pel refinement (me_sub_pel_search, me.c:366-500, cost_type 0, bi 0)
+// ---------------------------------------------------------------------------
+
+struct SubArgs {
+    const pel *cur;
+    const pel *planes[ME_MAX_REFS];  // plane 0 base (padded buffer start) of each reference
+    size_t plane_elems;
+    int stride;
+    int pic_w, pic_h, ctu_cols;
+    int R, IB;
+    u32 lambda;
+    const u8 *tab;              // MV-bits table, centred (tab[0] is mv 0)
+    const int *pu_tab;          // per CTU-local PU: shape, x, y
+    const u64 *keys;
+    me_pu_result_t *res;
+    s8 *best;
+    int npu, nref;
+};
+
+__device__ __forceinline__ const pel *plane_ptr(const SubArgs &a, int ref, int idx, int x, int y)
+{
+    return a.planes[ref] + (size_t)idx * a.plane_elems + (size_t)(y + ME_PAD) * a.stride + (x + ME_PAD);
+}
+
+// com_mc_blk_luma_pointer (com_mc.c:89-101).
+__device__ __forceinline__ const pel *qpel_ptr(const SubArgs &a, int ref, int qx, int qy)
+{
+    int ix = me_clip3(-ME_MAX_CU - 4, a.pic_w + 4, qx >> 2);
+    int iy = me_clip3(-ME_MAX_CU - 4, a.pic_h + 4, qy >> 2);
+    return plane_ptr(a, ref, (qy & 3) * 4 + (qx & 3), ix, iy);
+}
+
+// One block (64 threads) per PU: every reference in turn, then the reference
+// choice (smallest final cost, lower index on ties: inter.c:901-904).
+__global__ __launch_bounds__(64) void k_subpel(SubArgs a)
+{
+    __shared__ int sums[9];
+    __shared__ int sel[2];
+    const int pu = blockIdx.x, ctu = blockIdx.y;
+    const int lane = threadIdx.x;
+    const int s = a.pu_tab[pu * 3], ox = a.pu_tab[pu * 3 + 1], oy = a.pu_tab[pu * 3 + 2];
+    int pw, ph;
+    {
+        // shape sizes from a tiny local table
+        const int SW[ME_NSHAPES] = {4, 8, 4, 16, 4, 32, 8, 8, 16, 8, 32, 8, 64, 16, 16, 32, 16, 64, 16, 128, 32, 32, 64, 32, 128, 64, 64, 128, 128};
+        const int SH[ME_NSHAPES] = {8, 4, 16, 4, 32, 4, 8, 16, 8, 32, 8, 64, 8, 16, 32, 16, 64, 16, 128, 16, 32, 64, 32, 128, 32, 64, 128, 64, 128};
+        pw = SW[s]; ph = SH[s];
+    }
+    const int x = (ctu % a.ctu_cols) * 128 + ox, y = (ctu / a.ctu_cols) * 128 + oy;
+    const size_t base = (size_t)ctu * a.npu + pu;
+    const bool inside = x + pw <= a.pic_w && y + ph <= a.pic_h;
+    int tt, tw, th;
+    had_tiling(pw, ph, &tt, &tw, &th);
+    const int tpr = pw / tw, ntiles = tpr * (ph / th);
+    const pel *org = a.cur + (size_t)y * a.pic_w + x;
+    const int span = 2 * a.R + 1;
+    u32 best_final = 0xFFFFFFFFu;
+    int best_ref = -1;
+
+    for (int r = 0; r < a.nref; r++) {
+        me_pu_result_t out;
+        out.mv_int[0] = out.mv_int[1] = out.mv_sub[0] = out.mv_sub[1] = 0;
+        out.cost_int = out.cost_sub = ~0ull;
+        out.cost_final = 0xFFFFFFFFu;
+        out.valid = 0;
+        const u64 key = a.keys[base * a.nref + r];
+        if (!inside || key == ~0ull) {
+            if (lane == 0) a.res[base * a.nref + r] = out;
+            continue;
+        }
+        const u64 k = key - ((u64)KEY_BIAS_HI << 32);
+        const int idx = (int)(k & ((1ull << a.IB) - 1));
+        const int dx = idx % span - a.R, dy = idx / span - a.R;
+        out.mv_int[0] = (s16)dx; out.mv_int[1] = (s16)dy;
+        out.cost_int = k >> a.IB;
+
+        // Phase 1: centre (integer plane, no clipping: me.c:400) and the 8
+        // half-pel points of me.c:415-417.
+        const int HP[8][2] = {{-2, 0}, {-2, 2}, {0, 2}, {2, 2}, {2, 0}, {2, -2}, {0, -2}, {-2, -2}};
+        const int QP[8][2] = {{-1, 0}, {0, 1}, {1, 0}, {0, -1}, {-1, 1}, {1, 1}, {-1, -1}, {1, -1}};
+        if (lane < 9) sums[lane] = 0;
+        __syncthreads();
+        const int cmx = dx * 4, cmy = dy * 4;
+        for (int t = lane; t < 9 * ntiles; t += 64) {
+            const int p = t / ntiles, ti = t - p * ntiles;
+            const int tx = (ti % tpr) * tw, ty = (ti / tpr) * th;
+            const pel *pred;
+            if (p == 0) pred = plane_ptr(a, r, 0, x + dx, y + dy);
+            else pred = qpel_ptr(a, r, cmx + HP[p - 1][0] + (x << 2), cmy + HP[p - 1][1] + (y << 2));
+            int v = satd_any(tt, org + ty * a.pic_w + tx, a.pic_w, pred + ty * a.stride + tx, a.stride);
+            atomicAdd(&sums[p], v);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const u8 *tb = a.tab;
+            int bx = cmx, by = cmy;
+            u64 best = (u64)(a.lambda * (u32)(tb[cmx] + tb[cmy])) + ((u64)(sums[0] >> (BIT_DEPTH - 8)) << 16);
+            for (int i = 0; i < 8; i++) {
+                int mx = cmx + HP[i][0], my = cmy + HP[i][1];
+                u64 c = (u64)(a.lambda * (u32)(tb[mx] + tb[my])) + ((u64)(sums[i + 1] >> (BIT_DEPTH - 8)) << 16);
+                if (c < best) { best = c; bx = mx; by = my; }
+            }
+            sel[0] = bx; sel[1] = by;
+            out.cost_sub = best;
+        }
+        __syncthreads();
+        const int hx = sel[0], hy = sel[1];
+        if (lane < 8) sums[lane] = 0;
+        __syncthreads();
+        // Phase 2: the 8 quarter-pel points of me.c:460-462 around the best
+        // half-pel point.
+        for (int t = lane; t < 8 * ntiles; t += 64) {
+            const int p = t / ntiles, ti = t - p * ntiles;
+            const int tx = (ti % tpr) * tw, ty = (ti / tpr) * th;
+            const pel *pred = qpel_ptr(a, r, hx + QP[p][0] + (x << 2), hy + QP[p][1] + (y << 2));
+            int v = satd_any(tt, org + ty * a.pic_w + tx, a.pic_w, pred + ty * a.stride + tx, a.stride);
+            atomicAdd(&sums[p], v);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const u8 *tb = a.tab;
+            u64 best = out.cost_sub;
+            int bx = hx, by = hy;
+            for (int i = 0; i < 8; i++) {
+                int mx = hx + QP[i][0], my = hy + QP[i][1];
+                u64 c = (u64)(a.lambda * (u32)(tb[mx] + tb[my])) + ((u64)(sums[i] >> (BIT_DEPTH - 8)) << 16);
+                if (c < best) { best = c; bx = mx; by = my; }
+            }
+            out.mv_sub[0] = (s16)bx; out.mv_sub[1] = (s16)by;
+            out.cost_sub = best;
+            // refi bits: tbl_refi_bits[n][r] (tables.c:52-70); mvr 0 adds 0 + 1 + 2
+            int rb = a.nref <= 1 ? 0 : (r < a.nref - 1 ? r + 1 : a.nref - 1);
+            u64 cf = best + (u64)(a.lambda * (u32)(rb + 3));
+            out.cost_final = (u32)((cf + (1u << 15)) >> 16);
+            out.valid = 1;
+            a.res[base * a.nref + r] = out;
+            if (out.cost_final < best_final) { best_final = out.cost_final; best_ref = r; }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) a.best[base] = (s8)best_ref;
+}
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+
+struct me_plan_s {
+    me_frame_cfg_t cfg;
+    int ctu_cols, ctu_rows, n_ctu, npu, nref;
+    int IB, sh, lo, hi, span;
+    int mode, waves, band_rows, nbands, wcols, dx_cols, nbx;
+    size_t lds;
+    int base[ME_NSHAPES];
+    u32 *d_rate;
+    u8 *d_tab_base;
+    int tab_off;
+    int *d_pu_tab;
+    pel *d_cur;
+    u64 *d_keys;
+    me_pu_result_t *d_res;
+    s8 *d_best;
+    hipStream_t own;
+    hipGraphExec_t exec;
+    me_ref_t graph_refs[ME_MAX_REFS];
+    u32 graph_mask;
+    hipStream_t graph_stream;
+};
+
+// Closed form of the MV-bits table of inter_search_create (inter.c:2029-2058).
+static void mvbits_fill(u8 *base, int off)
+{
+    base[off] = 1;
+    base[0] = base[2 * off] = 0;
+    for (int v = 1; v < off; v++) {
+        int l = 0;
+        while ((2 << l) <= v + 1) l++;
+        base[off + v] = base[off - v] = (u8)(2 * (l + 1));
+    }
+}
+
+ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
+{
+    if (me_hip_init(-1)) return -1;
+    if (!out || !cfg || cfg->pic_w <= 0 || cfg->pic_h <= 0 || cfg->num_refs < 1 || cfg->num_refs > ME_MAX_REFS ||
+        cfg->search_range < 0 || cfg->search_range > 255 || !(cfg->shape_mask & ((1u << ME_NSHAPES) - 1))) {
+        me_set_error("me_plan_create: bad configuration", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    me_plan_s *p = (me_plan_s *)calloc(1, sizeof(me_plan_s));
+    p->cfg = *cfg;
+    p->ctu_cols = (cfg->pic_w + 127) / 128;
+    p->ctu_rows = (cfg->pic_h + 127) / 128;
+    p->n_ctu = p->ctu_cols * p->ctu_rows;
+    p->nref = cfg->num_refs;
+    const int R = cfg->search_range;
+    p->span = 2 * R + 1;
+    int need = 0;
+    while ((1 << need) < p->span * p->span) need++;
+    const int shift = 16 - (BIT_DEPTH - 8);
+    p->IB = need > 32 - shift ? need : 32 - shift;
+    p->sh = shift + p->IB - 32;
+    // Uniform part of the window: |mv| <= T - 4 with T = max(W,H) (me.c:504-512).
+    const int T = cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h;
+    p->lo = -R > -T + 4 ? -R : -T + 4;
+    p->hi = R < T - 4 ? R : T - 4;
+    const uint32_t m = cfg->shape_mask & ((1u << ME_NSHAPES) - 1);
+    p->mode = m == (1u << S16x16) ? 1 : 0;
+    int n = 0;
+    for (int s = 0; s < ME_NSHAPES; s++) {
+        if (m >> s & 1) { p->base[s] = n; n += (128 / ME_SHAPES_HOST[s][0]) * (128 / ME_SHAPES_HOST[s][1]); }
+        else p->base[s] = -1;
+    }
+    p->npu = n;
+    p->waves = cfg->waves > 0 ? (cfg->waves > 4 ? 4 : cfg->waves) : 4;
+    p->band_rows = cfg->band_rows > 0 ? cfg->band_rows : 16;
+    if (p->band_rows > p->span) p->band_rows = p->span;
+    p->nbands = (p->span + p->band_rows - 1) / p->band_rows;
+    // Columns: multiples of 4 displacements (one v_qsad_pk_u16_u8 step).
+    const int full_cols = (p->span + 3) & ~3;
+    p->dx_cols = cfg->dx_cols > 0 ? ((cfg->dx_cols + 3) & ~3) : full_cols;
+    if (p->dx_cols > full_cols) p->dx_cols = full_cols;
+    p->nbx = (p->span + p->dx_cols - 1) / p->dx_cols;
+    // A lane reads up to 20 samples from column X + (dxb - dxs) <= 112 + dx_cols - 4.
+    p->wcols = ((128 + p->dx_cols + 16 + 15) / 16) * 16;
+    p->lds = (size_t)(128 + p->band_rows - 1) * p->wcols * sizeof(pel) + 128 * 128 * sizeof(pel);
+
+    // rate[i] = bits(4*(i-R)) * lambda (GET_MVBITS_IPEL, util.h:43-44)
+    p->tab_off = ((cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h) << 2) + 3;
+    std::vector<u8> tab(2 * p->tab_off + 1);
+    mvbits_fill(tab.data(), p->tab_off);
+    std::vector<u32> rate(p->span);
+    for (int i = 0; i < p->span; i++) rate[i] = (u32)tab[p->tab_off + 4 * (i - R)] * cfg->lambda_mv;
+    std::vector<int> putab(3 * n);
+    int k = 0;
+    for (int s = 0; s < ME_NSHAPES; s++) {
+        if (!(m >> s & 1)) continue;
+        int w = ME_SHAPES_HOST[s][0], h = ME_SHAPES_HOST[s][1];
+        for (int y = 0; y < 128; y += h)
+            for (int x = 0; x < 128; x += w) { putab[3 * k] = s; putab[3 * k + 1] = x; putab[3 * k + 2] = y; k++; }
+    }
+    const size_t nkeys = (size_t)p->n_ctu * p->npu * p->nref;
+    hipError_t e = hipSuccess;
+#define ALLOC(ptr, bytes) if (e == hipSuccess) e = hipMalloc((void **)&(ptr), (bytes))
+    ALLOC(p->d_rate, p->span * sizeof(u32));
+    ALLOC(p->d_tab_base, tab.size());
+    ALLOC(p->d_pu_tab, putab.size() * sizeof(int));
+    ALLOC(p->d_cur, (size_t)cfg->pic_w * cfg->pic_h * sizeof(pel));
+    ALLOC(p->d_keys, nkeys * sizeof(u64));
+    ALLOC(p->d_res, nkeys * sizeof(me_pu_result_t));
+    ALLOC(p->d_best, (size_t)p->n_ctu * p->npu);
+#undef ALLOC
+    if (e == hipSuccess) e = hipStreamCreate(&p->own);
+    if (e == hipSuccess) e = hipMemcpy(p->d_rate, rate.data(), p->span * sizeof(u32), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_tab_base, tab.data(), tab.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_pu_tab, putab.data(), putab.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        me_set_error("me_plan_create", e, __FILE__, __LINE__);
+        me_plan_destroy(p);
+        return -1;
+    }
+    *out = p;
+    return 0;
+}
+
+ME_EXPORT int me_plan_set_cur(me_plan_t p, const me_pel *cur, int stride, int on_device, void *stream)
+{
+    if (!p || !cur || stride < p->cfg.pic_w) { me_set_error("me_plan_set_cur: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    hipStream_t st = stream ? (hipStream_t)stream : p->own;
+    ME_CHECK(hipMemcpy2DAsync(p->d_cur, p->cfg.pic_w * sizeof(pel), cur, stride * sizeof(pel), p->cfg.pic_w * sizeof(pel),
+                              p->cfg.pic_h, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+static int plan_enqueue(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
+{
+    for (int r = 0; r < p->nref; r++)
+        if (prepare_mask >> r & 1)
+            if (ref_prepare_async(refs[r], st)) return -1;
+    const size_t nkeys = (size_t)p->n_ctu * p->npu * p->nref;
+    hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, st, p->d_keys, nkeys);
+
+    IntArgs a;
+    memset(&a, 0, sizeof(a));
+    a.cur = p->d_cur;
+    for (int r = 0; r < p->nref; r++) a.ref0[r] = refs[r]->planes + (size_t)ME_PAD * refs[r]->stride + ME_PAD;
+    a.ref_stride = refs[0]->stride;
+    a.pic_w = p->cfg.pic_w; a.pic_h = p->cfg.pic_h; a.ctu_cols = p->ctu_cols;
+    a.R = p->cfg.search_range; a.lo = p->lo; a.hi = p->hi; a.IB = p->IB; a.sh = p->sh;
+    a.rate = p->d_rate; a.keys = p->d_keys; a.npu = p->npu; a.nref = p->nref;
+    a.band_rows = p->band_rows; a.wcols = p->wcols; a.dx_cols = p->dx_cols; a.nbx = p->nbx;
+    memcpy(a.base, p->base, sizeof(a.base));
+    dim3 grid(p->nbands * p->nbx, p->n_ctu, p->nref);
+    if (p->lds > 65536) {
+        const void *fn[6] = {(const void *)k_int_search<0, 1>, (const void *)k_int_search<0, 2>, (const void *)k_int_search<0, 4>,
+                             (const void *)k_int_search<1, 1>, (const void *)k_int_search<1, 2>, (const void *)k_int_search<1, 4>};
+        for (int i = 0; i < 6; i++) ME_CHECK(hipFuncSetAttribute(fn[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds));
+    }
+    if (p->mode == 0) {
+        switch (p->waves) {
+        case 1: hipLaunchKernelGGL((k_int_search<0, 1>), grid, dim3(64), p->lds, st, a); break;
+        case 2: hipLaunchKernelGGL((k_int_search<0, 2>), grid, dim3(128), p->lds, st, a); break;
+        default: hipLaunchKernelGGL((k_int_search<0, 4>), grid, dim3(256), p->lds, st, a); break;
+        }
+    } else {
+        switch (p->waves) {
+        case 1: hipLaunchKernelGGL((k_int_search<1, 1>), grid, dim3(64), p->lds, st, a); break;
+        case 2: hipLaunchKernelGGL((k_int_search<1, 2>), grid, dim3(128), p->lds, st, a); break;
+        default: hipLaunchKernelGGL((k_int_search<1, 4>), grid, dim3(256), p->lds, st, a); break;
+        }
+    }
+
+    SubArgs b;
+    memset(&b, 0, sizeof(b));
+    b.cur = p->d_cur;
+    for (int r = 0; r < p->nref; r++) b.planes[r] = refs[r]->planes;
+    b.plane_elems = refs[0]->plane_elems;
+    b.stride = refs[0]->stride;
+    b.pic_w = p->cfg.pic_w; b.pic_h = p->cfg.pic_h; b.ctu_cols = p->ctu_cols;
+    b.R = p->cfg.search_range; b.IB = p->IB; b.lambda = p->cfg.lambda_mv;
+    b.tab = p->d_tab_base + p->tab_off;
+    b.pu_tab = p->d_pu_tab; b.keys = p->d_keys; b.res = p->d_res; b.best = p->d_best;
+    b.npu = p->npu; b.nref = p->nref;
+    hipLaunchKernelGGL(k_subpel, dim3(p->npu, p->n_ctu), dim3(64), 0, st, b);
+    ME_CHECK(hipGetLastError());
+    return 0;
+}
+
+ME_EXPORT int me_plan_run(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, void *stream)
+{
+    if (!p || !refs) { me_set_error("me_plan_run: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    for (int r = 0; r < p->nref; r++)
+        if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h) {
+            me_set_error("me_plan_run: reference size differs from the plan", hipErrorInvalidValue, __FILE__, __LINE__);
+            return -1;
+        }
+    if (p->lds > 160 * 1024) { me_set_error("me_plan_run: LDS window too large", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    hipStream_t st = stream ? (hipStream_t)stream : p->own;
+    bool same = p->exec && p->graph_mask == prepare_mask && p->graph_stream == st;
+    for (int r = 0; same && r < p->nref; r++) same = p->graph_refs[r] == refs[r];
+    if (!same) {
+        if (p->exec) { hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+        hipGraph_t g;
+        ME_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = plan_enqueue(p, refs, prepare_mask, st);
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) return -1;
+        if (e != hipSuccess) { me_set_error("hipStreamEndCapture", e, __FILE__, __LINE__); return -1; }
+        e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) { me_set_error("hipGraphInstantiate", e, __FILE__, __LINE__); p->exec = nullptr; return -1; }
+        for (int r = 0; r < p->nref; r++) p->graph_refs[r] = refs[r];
+        p->graph_mask = prepare_mask;
+        p->graph_stream = st;
+    }
+    ME_CHECK(hipGraphLaunch(p->exec, st));
+    return 0;
+}
+
+ME_EXPORT int me_plan_results(me_plan_t p, me_pu_result_t *res, int8_t *best_ref)
+{
+    if (!p) return -1;
+    hipStream_t st = p->graph_stream ? p->graph_stream : p->own;
+    ME_CHECK(hipStreamSynchronize(st));
+    const size_t n = (size_t)p->n_ctu * p->npu;
+    if (res) ME_CHECK(hipMemcpy(res, p->d_res, n * p->nref * sizeof(me_pu_result_t), hipMemcpyDeviceToHost));
+    if (best_ref) ME_CHECK(hipMemcpy(best_ref, p->d_best, n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+ME_EXPORT int me_plan_num_ctus(me_plan_t p) { return p ? p->n_ctu : 0; }
+ME_EXPORT int me_plan_num_pus(me_plan_t p) { return p ? p->npu : 0; }
+
+ME_EXPORT void me_plan_destroy(me_plan_t p)
+{
+    if (!p) return;
+    if (p->exec) hipGraphExecDestroy(p->exec);
+    if (p->own) hipStreamDestroy(p->own);
+    hipFree(p->d_rate); hipFree(p->d_tab_base); hipFree(p->d_pu_tab); hipFree(p->d_cur);
+    hipFree(p->d_keys); hipFree(p->d_res); hipFree(p->d_best);
+    free(p);
+}
