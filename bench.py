@@ -1,0 +1,228 @@
+"""Benchmark: batched frame motion estimation, 1080p 8-bit, 2 references,
+all 29 PU shapes, full search +-64 then quarter-pel refinement (the
+1080p-all-PU workload).  Prints one JSON line.
+
+A step is one frame: re-pad + re-interpolate the newest reference (each frame
+of an encoder contributes one new reference picture) and run integer full
+search, sub-pel refinement and reference choice for every PU of the frame
+against both references, as one hipGraph launch.  Inputs are resident in HBM
+before timing starts.
+
+Beside the batched number the line carries:
+  per_call     the per-call drop-in path (host driver, every SAD/SATD/filter a
+               synchronous HIP call) on a bounded sample, as CTUs/s
+  cpu_baseline the reference uavs3e kernels (C+SSE+AVX2 table, compiled from
+               the reference sources into oracle/_ref) on the host's cores, on
+               a bounded sample of CTUs, else the oracle restatement
+  roofline     the integer-search kernel against HBM bandwidth, plus its VALU
+               fraction
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK_OPS = 256 * 64 * 2.4e9     # lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a, dtype=np.float64), q))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-percall", action="store_true", help="skip the per-call leg")
+    ap.add_argument("--band-rows", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--dx-cols", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the ME path on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import uavs3e_amd as ua
+    from uavs3e_amd import workloads as wl
+
+    work = wl.ALL[args.workload]()
+    bd = work.bd
+    t0 = time.time()
+    cur, refs = work.frames()
+    gen_s = time.time() - t0
+    print("[bench] %s frames generated in %.1fs" % (work.name, gen_s), file=sys.stderr, flush=True)
+
+    hip = ua.MeHip(bd)
+    dref = [hip.ref_create(r) for r in refs]
+    plan = hip.plan_create(work.w, work.h, work.R, work.lam, work.mask, work.num_refs,
+                           band_rows=args.band_rows, waves=args.waves, dx_cols=args.dx_cols)
+    hip.plan_set_cur(plan, cur)
+    n_ctu = hip.lib.me_plan_num_ctus(plan)
+    npu = hip.lib.me_plan_num_pus(plan)
+
+    # ---- batched: one graph launch per frame --------------------------------
+    prep = 1   # the newest reference is rebuilt inside every step
+    for _ in range(args.warmup):
+        hip.plan_run(plan, dref, prep)
+    hip.plan_results(plan, work.num_refs)
+    torch.cuda.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        hip.plan_run(plan, dref, prep)
+        hip.lib.me_plan_results(plan, None, None)   # waits for this frame
+        lat.append((time.perf_counter() - s0) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_step = elapsed * 1e3 / args.steps
+    ctus_per_s = n_ctu * args.steps / elapsed
+    print("[bench] batched: %.3f ms/frame, %.0f CTUs/s" % (ms_step, ctus_per_s), file=sys.stderr, flush=True)
+
+    # candidate evaluations per frame (integer full search, all valid PUs and refs)
+    res, best = hip.plan_results(plan, work.num_refs)
+    valid = res["valid"] == 1
+    geo = []
+    for s in range(29):
+        if work.mask >> s & 1:
+            sw, sh = ua.SHAPES[s]
+            geo += [(ox, oy) for oy in range(0, 128, sh) for ox in range(0, 128, sw)]
+    geo = np.array(geo)
+    cols = (work.w + 127) // 128
+    T = max(work.w, work.h)
+    ctu_xy = np.array([((c % cols) * 128, (c // cols) * 128) for c in range(n_ctu)])
+    px = ctu_xy[:, None, 0] + geo[None, :, 0]
+    py = ctu_xy[:, None, 1] + geo[None, :, 1]
+    nx = np.minimum(np.minimum(work.w - 1 - px, T - 4), work.R) - np.maximum(np.maximum(-127 - px, -T + 4), -work.R) + 1
+    ny = np.minimum(np.minimum(work.h - 1 - py, T - 4), work.R) - np.maximum(np.maximum(-127 - py, -T + 4), -work.R) + 1
+    cands = int((np.clip(nx, 0, None) * np.clip(ny, 0, None) * valid[:, :, 0]).sum()) * work.num_refs
+    shapes = [s for s in range(29) if work.mask >> s & 1]
+    cand_per_s = cands * args.steps / elapsed
+
+    # ---- stage timing with HIP events on the plan's stream -------------------
+    stage = hip.plan_profile(plan, dref, prep, iters=5)
+    k_ms = stage[2]
+    pel = 1 if bd == 8 else 2
+    alg_bytes = work.w * work.h * pel * (1 + work.num_refs) + n_ctu * npu * work.num_refs * 8
+    achieved_gbs = alg_bytes / (k_ms * 1e-3) / 1e9
+    # algorithmic VALU work of the integer search: every candidate SAD built
+    # from 4x4-block SADs (16 abs-diffs per 4x4 block and displacement, 4 per
+    # v_sad-class lane op) plus one compare per PU candidate
+    blocks4 = (work.w // 4) * (work.h // 4)
+    span = 2 * work.R + 1
+    valu_ops = blocks4 * span * span * work.num_refs * 16 / 4 + cands
+    valu_frac = valu_ops / (k_ms * 1e-3) / VALU_PEAK_OPS
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % work.name)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    # ---- per-call drop-in leg -----------------------------------------------
+    per_call = None
+    if not args.no_percall:
+        host = ua.MeHost(bd)
+        table = hip.dispatch_table()
+        sample_w = work.scaled(256, 256, 8)   # picture of 4 CTUs; window +-8
+        scur, srefs = sample_w.frames(host)
+        splanes = [host.planes_via_table(table, r) for r in srefs]
+        pmask = ua.MASK_16x16
+        t0 = time.perf_counter()
+        pres, pbest, pms = host.me_frame(table, scur, splanes, 8, work.lam, pmask, 0, 1, threads=1)
+        p_el = time.perf_counter() - t0
+        pvalid = pres["valid"][0] == 1
+        pc = int(pvalid.sum()) * 17 * 17   # candidates of the sample (one CTU, 16x16 PUs, +-8, 2 refs)
+        p_cand_s = pc / p_el
+        cands_per_ctu = cands / max(1, int(valid[:, :, 0].any(axis=1).sum()))
+        per_call = {"value": p_cand_s / cands_per_ctu, "unit": "CTUs/s", "candidates_per_s": p_cand_s,
+                    "hip_errors": int(hip.error_count()),
+                    "sample": "1 CTU of %s with 16x16 PUs, +-8, %d refs through the per-call table (%.1f s); "
+                              "CTUs/s = candidates/s / %.0f candidates per %s CTU"
+                              % (sample_w.name, work.num_refs, p_el, cands_per_ctu, work.name)}
+        print("[bench] per-call: %.0f candidates/s" % p_cand_s, file=sys.stderr, flush=True)
+
+    # ---- CPU baseline leg -----------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(os.cpu_count() or 1, 16))
+        rows = (work.h + 127) // 128
+        c0 = (rows // 2 - 1) * cols
+        c1 = min(n_ctu, c0 + 2 * cols)
+        from oracle.oracle import RefLib, Oracle
+        if RefLib.available(bd):
+            rl = RefLib(bd, level=2)
+            pics = [rl.pic_create(r) for r in refs]
+            cres, cbest, cms = rl.me_frame(cur, pics, work.R, work.lam, work.mask, c0, c1, threads=threads)
+            for pc_ in pics:
+                rl.pic_free(pc_)
+            kind = "reference"
+            what = "uavs3e kernels (C+SSE4.2+AVX2 dispatch table) compiled from the reference sources"
+        else:
+            orc = Oracle(bd)
+            t0 = time.perf_counter()
+            cres, cbest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], work.R, work.lam, work.mask,
+                                       c0, c0 + 1)
+            cms = (time.perf_counter() - t0) * 1e3
+            c1 = c0 + 1
+            threads = 1
+            kind = "port"
+            what = "the scalar C oracle"
+        same = bool(np.array_equal(cres[c0:c1], res[c0:c1]) and np.array_equal(cbest[c0:c1], best[c0:c1]))
+        cpu = {"value": (c1 - c0) / (cms * 1e-3), "unit": "CTUs/s", "cores": threads, "kind": kind,
+               "sample": "CTUs %d..%d (%d interior CTU rows) of %s, %s, %d threads, %.1f s; results bit-exact "
+                         "with the GPU: %s" % (c0, c1 - 1, (c1 - c0) // cols, work.name, what, threads,
+                                               cms * 1e-3, same)}
+        print("[bench] cpu baseline: %.2f CTUs/s (%s)" % (cpu["value"], kind), file=sys.stderr, flush=True)
+
+    hip.plan_destroy(plan)
+    for r in dref:
+        hip.ref_destroy(r)
+
+    line = {
+        "metric": "1080p ME CTUs/s batched vs per-call vs AVX2, bit-exact MVs; p50/p99 CTU latency",
+        "value": ctus_per_s,
+        "unit": "CTUs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8" if bd == 8 else "u16",
+        "data": "synthetic (seeded sinusoid field + noise, global + per-region motion)",
+        "config": {"workload": work.name, "pic": "%dx%d" % (work.w, work.h), "bit_depth": bd,
+                   "refs": work.num_refs, "search_range": work.R, "pu_shapes": len(shapes),
+                   "pus_per_frame": int(valid[:, :, 0].sum()), "ctus_per_frame": n_ctu,
+                   "step": "rebuild newest reference planes + full search + sub-pel + ref choice, one hipGraph"},
+        "candidate_evals_per_s": cand_per_s,
+        "ctu_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99),
+                           "note": "batched: every CTU of a frame completes with the frame"},
+        "stage_ms": {"prepare_ref": stage[0], "fill": stage[1], "int_search": stage[2], "subpel": stage[3]},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_int_search", "algorithmic_bytes": alg_bytes, "kernel_ms": k_ms,
+                     "valu": {"ops": valu_ops, "frac": valu_frac, "peak_lane_ops_per_s": VALU_PEAK_OPS}},
+        "per_call": per_call,
+        "speedup_vs_per_call": (ctus_per_s / per_call["value"]) if per_call else None,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -199,6 +199,10 @@ int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, voi
 /* Waits for the last run and copies results to host memory:
  * res[(ctu*npu + pu)*num_refs + r], best_ref[ctu*npu + pu] (-1 = PU skipped). */
 int me_plan_results(me_plan_t plan, me_pu_result_t *res, int8_t *best_ref);
+/* Runs `iters` frames stage by stage (no graph) on the plan's own stream and
+ * writes the average milliseconds of each stage to ms[0..3]: reference
+ * preparation, key initialisation, integer search, sub-pel + reference choice. */
+int me_plan_profile(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, int iters, double ms[4]);
 int me_plan_num_ctus(me_plan_t plan);
 int me_plan_num_pus(me_plan_t plan);   /* per CTU */
 void me_plan_destroy(me_plan_t plan);

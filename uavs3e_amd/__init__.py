@@ -191,6 +191,13 @@ class MeHip:
         arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
         self.check(self.lib.me_plan_run(plan, arr, prepare_mask, stream), "me_plan_run")
 
+    def plan_profile(self, plan, refs, prepare_mask=0, iters=3):
+        """Average ms per stage: [prepare, fill, integer search, sub-pel]."""
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        ms = (C.c_double * 4)()
+        self.check(self.lib.me_plan_profile(plan, arr, prepare_mask, iters, ms), "me_plan_profile")
+        return list(ms)
+
     def plan_results(self, plan, num_refs):
         n_ctu = self.lib.me_plan_num_ctus(plan)
         npu = self.lib.me_plan_num_pus(plan)

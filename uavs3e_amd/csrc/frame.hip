@@ -906,13 +906,17 @@ ME_EXPORT int me_plan_set_cur(me_plan_t p, const me_pel *cur, int stride, int on
     return 0;
 }
 
-static int plan_enqueue(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
+// Enqueues the frame; ev (optional, 5 events) brackets the four stages.
+static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st, hipEvent_t *ev)
 {
+    if (ev) ME_CHECK(hipEventRecord(ev[0], st));
     for (int r = 0; r < p->nref; r++)
         if (prepare_mask >> r & 1)
             if (ref_prepare_async(refs[r], st)) return -1;
+    if (ev) ME_CHECK(hipEventRecord(ev[1], st));
     const size_t nkeys = (size_t)p->n_ctu * p->npu * p->nref;
     hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, st, p->d_keys, nkeys);
+    if (ev) ME_CHECK(hipEventRecord(ev[2], st));
 
     IntArgs a;
     memset(&a, 0, sizeof(a));
@@ -955,9 +959,16 @@ static int plan_enqueue(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mas
     b.tab = p->d_tab_base + p->tab_off;
     b.pu_tab = p->d_pu_tab; b.keys = p->d_keys; b.res = p->d_res; b.best = p->d_best;
     b.npu = p->npu; b.nref = p->nref;
+    if (ev) ME_CHECK(hipEventRecord(ev[3], st));
     hipLaunchKernelGGL(k_subpel, dim3(p->npu, p->n_ctu), dim3(64), 0, st, b);
     ME_CHECK(hipGetLastError());
+    if (ev) ME_CHECK(hipEventRecord(ev[4], st));
     return 0;
+}
+
+static int plan_enqueue(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
+{
+    return plan_enqueue_stages(p, refs, prepare_mask, st, nullptr);
 }
 
 ME_EXPORT int me_plan_run(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, void *stream)
@@ -989,6 +1000,34 @@ ME_EXPORT int me_plan_run(me_plan_t p, me_ref_t const *refs, uint32_t prepare_ma
     }
     ME_CHECK(hipGraphLaunch(p->exec, st));
     return 0;
+}
+
+// Stage timing: runs `iters` frames stage by stage (no graph) with HIP
+// events on the plan's stream between the stages and returns the average
+// milliseconds of: [0] reference preparation (prepare_mask), [1] key fill,
+// [2] integer search, [3] sub-pel + reference choice.
+ME_EXPORT int me_plan_profile(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, int iters, double *ms)
+{
+    if (!p || !refs || !ms || iters < 1) return -1;
+    hipStream_t st = p->own;
+    hipEvent_t ev[5];
+    for (int i = 0; i < 5; i++) ME_CHECK(hipEventCreate(&ev[i]));
+    double acc[4] = {0, 0, 0, 0};
+    int rc = 0;
+    for (int it = 0; it < iters && !rc; it++) {
+        rc = plan_enqueue_stages(p, refs, prepare_mask, st, ev);
+        if (!rc) {
+            ME_CHECK(hipEventSynchronize(ev[4]));
+            for (int k = 0; k < 4; k++) {
+                float t = 0;
+                ME_CHECK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                acc[k] += t;
+            }
+        }
+    }
+    for (int i = 0; i < 5; i++) hipEventDestroy(ev[i]);
+    for (int k = 0; k < 4; k++) ms[k] = acc[k] / iters;
+    return rc;
 }
 
 ME_EXPORT int me_plan_results(me_plan_t p, me_pu_result_t *res, int8_t *best_ref)
