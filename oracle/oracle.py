@@ -59,6 +59,19 @@ class _Lib:
         self.dt = pel_dtype(bd)
 
 
+class OrcCtx(C.Structure):
+    """orc_ctx_t of me_oracle.c."""
+    _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("bd", C.c_int), ("org", C.c_void_p), ("i_org", C.c_int),
+                ("planes", C.c_void_p), ("i_ref", C.c_int), ("tab_mvbits", C.c_void_p), ("tab_off", C.c_int),
+                ("lambda_mv", C.c_uint32), ("is_padding", C.c_int), ("mvr", C.c_int), ("cost_type", C.c_int)]
+
+
+class OrcTz(C.Structure):
+    """orc_tz_t of me_oracle.c."""
+    _fields_ = [("gop_size", C.c_int), ("max_search_range", C.c_int), ("adaptive_raster_range", C.c_int),
+                ("ptr", C.c_longlong), ("ref_ptr", C.c_longlong), ("num_refp", C.c_int), ("mv_ipel", C.c_int16 * 2)]
+
+
 class Oracle(_Lib):
     def __init__(self, bd=8):
         super().__init__(os.path.join(HERE, "lib", "liboracle%d.so" % bd), bd)
@@ -78,6 +91,37 @@ class Oracle(_Lib):
         L.orc_subpel_ptr.restype = C.c_void_p
         assert L.orc_bit_depth() == bd
         assert L.orc_pu_res_size() == PU_RES_DTYPE.itemsize
+        assert L.orc_ctx_size() == C.sizeof(OrcCtx) and L.orc_tz_size() == C.sizeof(OrcTz)
+
+    def search_ctx(self, cur, x, y, planes, lambda_mv, mvr=0, cost_type=0, is_padding=1, pad=PAD):
+        """An orc_ctx_t for the PU at (x, y) of `cur` against padded `planes`.
+        Returns (ctx, keepalive)."""
+        h, w = cur.shape
+        stride = w + 2 * pad
+        ptrs = (C.c_void_p * 16)(*[planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize for i in range(16)])
+        off, tab = self.mvbits(w, h)
+        c = OrcCtx(w, h, self.bd, cur.ctypes.data + (y * w + x) * cur.itemsize, w, C.cast(ptrs, C.c_void_p).value,
+                   stride, tab.ctypes.data + off, off, lambda_mv, is_padding, mvr, cost_type)
+        return c, (ptrs, tab, cur, planes)
+
+    def me_search_tz(self, cur, planes, x, y, w, h, refi, num_refp, cur_poc, ref_poc, gop, msr, mvr, cost_type,
+                     adaptive, lambda_mv, mvp):
+        c, keep = self.search_ctx(cur, x, y, planes, lambda_mv, mvr, cost_type)
+        t = OrcTz(gop, msr, adaptive, cur_poc, ref_poc, num_refp)
+        mvp_a = np.array(mvp, np.int16)
+        mv = np.zeros(2, np.int16)
+        mvi = np.zeros(2, np.int16)
+        c64 = C.c_uint64()
+        r = self.lib.orc_me_search_tz(C.byref(c), C.byref(t), x, y, w, h, refi, ptr(mvp_a), ptr(mv), ptr(mvi),
+                                      C.byref(c64))
+        return (int(mv[0]), int(mv[1]), int(mvi[0]), int(mvi[1])), r
+
+    def subpel(self, cur, planes, x, y, w, h, mvr, cost_type, lambda_mv, imv, mvp):
+        c, keep = self.search_ctx(cur, x, y, planes, lambda_mv, mvr, cost_type)
+        mvp_a = np.array(mvp, np.int16)
+        mv = np.array([imv[0] * 4, imv[1] * 4], np.int16)
+        r = self.lib.orc_subpel(C.byref(c), ptr(mvp_a), x, y, w, h, ptr(mv))
+        return (int(mv[0]), int(mv[1])), r
 
     # -- block costs --------------------------------------------------------
     def sad(self, w, h, org, i_org, pred, i_pred, org_off=0, pred_off=0):
@@ -200,15 +244,21 @@ class RefLib(_Lib):
     def sad(self, w, h, org, i_org, pred, i_pred, org_off=0, pred_off=0):
         return self.lib.refh_sad(w, ptr_at(org, org_off), i_org, ptr_at(pred, pred_off), i_pred, h)
 
-    def sad_x4(self, w, h, org, i_org, pred, i_pred, offs):
+    def sad_x4(self, w, h, org, i_org, pred, i_pred, offs, org_off=0):
         out = np.zeros(4, np.uint32)
-        self.lib.refh_sad_x4(w, ptr(org), i_org, *[ptr_at(pred, o) for o in offs], i_pred, ptr(out), h)
+        self.lib.refh_sad_x4(w, ptr_at(org, org_off), i_org, *[ptr_at(pred, o) for o in offs], i_pred, ptr(out), h)
         return out
 
-    def sad_x3(self, w, h, org, i_org, pred, i_pred, offs):
+    def sad_x3(self, w, h, org, i_org, pred, i_pred, offs, org_off=0):
         out = np.zeros(3, np.uint32)
-        self.lib.refh_sad_x3(w, ptr(org), i_org, *[ptr_at(pred, o) for o in offs], i_pred, ptr(out), h)
+        self.lib.refh_sad_x3(w, ptr_at(org, org_off), i_org, *[ptr_at(pred, o) for o in offs], i_pred, ptr(out), h)
         return out
+
+    def luma_coef(self, frac):
+        return np.ctypeslib.as_array(C.cast(self.lib.refh_luma_coef(frac), C.POINTER(C.c_int8)), (8,)).copy()
+
+    def chroma_coef(self, frac):
+        return np.ctypeslib.as_array(C.cast(self.lib.refh_chroma_coef(frac), C.POINTER(C.c_int8)), (4,)).copy()
 
     def satd(self, wl, hl, org, s_org, cur, s_cur, org_off=0, cur_off=0):
         return self.lib.refh_satd(wl, hl, ptr_at(org, org_off), s_org, ptr_at(cur, cur_off), s_cur)
