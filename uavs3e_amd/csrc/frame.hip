@@ -737,11 +737,12 @@ struct SubArgs {
     int R, IB;
     u32 lambda;
     const u8 *tab;              // MV-bits table, centred (tab[0] is mv 0)
-    const int *pu_tab;          // per CTU-local PU: shape, x, y
     const u64 *keys;
     me_pu_result_t *res;
     s8 *best;
-    int npu, nref;
+    int npu, nref, n_ctu;
+    // the shape handled by one launch
+    int pw, ph, tw, th, tpr, base, per_ctu;
 };
 
 __device__ __forceinline__ const pel *plane_ptr(const SubArgs &a, int ref, int idx, int x, int y)
@@ -757,115 +758,246 @@ __device__ __forceinline__ const pel *qpel_ptr(const SubArgs &a, int ref, int qx
     return plane_ptr(a, ref, (qy & 3) * 4 + (qx & 3), ix, iy);
 }
 
-// One block (64 threads) per PU: every reference in turn, then the reference
-// choice (smallest final cost, lower index on ties: inter.c:901-904).
-__global__ __launch_bounds__(64) void k_subpel(SubArgs a)
+// N consecutive samples from an arbitrary address as int, read with aligned
+// dword loads and v_alignbyte (may read up to 3 bytes past the last sample,
+// which stays inside the padded planes and the slack of the current frame).
+template <int N>
+__device__ __forceinline__ void load_row(const pel *p, int *out)
 {
-    __shared__ int sums[9];
-    __shared__ int sel[2];
-    const int pu = blockIdx.x, ctu = blockIdx.y;
-    const int lane = threadIdx.x;
-    const int s = a.pu_tab[pu * 3], ox = a.pu_tab[pu * 3 + 1], oy = a.pu_tab[pu * 3 + 2];
-    int pw, ph;
-    {
-        // shape sizes from a tiny local table
-        const int SW[ME_NSHAPES] = {4, 8, 4, 16, 4, 32, 8, 8, 16, 8, 32, 8, 64, 16, 16, 32, 16, 64, 16, 128, 32, 32, 64, 32, 128, 64, 64, 128, 128};
-        const int SH[ME_NSHAPES] = {8, 4, 16, 4, 32, 4, 8, 16, 8, 32, 8, 64, 8, 16, 32, 16, 64, 16, 128, 16, 32, 64, 32, 128, 32, 64, 128, 64, 128};
-        pw = SW[s]; ph = SH[s];
+    const uintptr_t ad = (uintptr_t)p;
+    const u32 *q = (const u32 *)(ad & ~(uintptr_t)3);
+    const u32 sh = (u32)(ad & 3);
+    constexpr int NW = (N * (int)sizeof(pel) + 3) / 4;
+    u32 d[NW + 1];
+#pragma unroll
+    for (int i = 0; i <= NW; i++) d[i] = q[i];
+    u32 w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#if BIT_DEPTH == 8
+        out[i] = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFF);
+#else
+        out[i] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xFFFF);
+#endif
     }
-    const int x = (ctu % a.ctu_cols) * 128 + ox, y = (ctu / a.ctu_cols) * 128 + oy;
-    const size_t base = (size_t)ctu * a.npu + pu;
-    const bool inside = x + pw <= a.pic_w && y + ph <= a.pic_h;
-    int tt, tw, th;
-    had_tiling(pw, ph, &tt, &tw, &th);
-    const int tpr = pw / tw, ntiles = tpr * (ph / th);
-    const pel *org = a.cur + (size_t)y * a.pic_w + x;
+}
+
+template <int N>
+__device__ __forceinline__ void fwht(int *v, int step)
+{
+#pragma unroll
+    for (int hs = 1; hs < N; hs <<= 1)
+#pragma unroll
+        for (int i = 0; i < N; i += hs << 1)
+#pragma unroll
+            for (int j = i; j < i + hs; j++) {
+                int p = v[j * step], q = v[(j + hs) * step];
+                v[j * step] = p + q;
+                v[(j + hs) * step] = p - q;
+            }
+}
+
+// Sum of |coef| of the 2-D Walsh-Hadamard transform of a TW x TH residual.
+template <int TW, int TH>
+__device__ __forceinline__ int had_abs(const pel *org, int so, const pel *cur, int sc)
+{
+    int m[TW * TH];
+#pragma unroll
+    for (int y = 0; y < TH; y++) {
+        int o[TW], c[TW];
+        load_row<TW>(org + y * so, o);
+        load_row<TW>(cur + y * sc, c);
+#pragma unroll
+        for (int x = 0; x < TW; x++) m[y * TW + x] = o[x] - c[x];
+    }
+#pragma unroll
+    for (int y = 0; y < TH; y++) fwht<TW>(m + y * TW, 1);
+#pragma unroll
+    for (int x = 0; x < TW; x++) fwht<TH>(m + x, TW);
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < TW * TH; i++) s += abs(m[i]);
+    return s;
+}
+
+// One normalised cost_satd tile (com_cost.c:259, 340, 467, 586, 649, 708).
+// A 16-point WHT is the 8-point WHT of the half sums and of the half
+// differences, so a 16x8 (8x16) tile is two 8x8 transforms.
+template <int TT>
+__device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, int sc)
+{
+    const double SQRT32 = 0x1.6a09e667f3bcdp+2, SQRT128 = 0x1.6a09e667f3bcdp+3;
+    if (TT == 0) return (had_abs<4, 4>(org, so, cur, sc) + 1) >> 1;
+    if (TT == 1) return (had_abs<8, 8>(org, so, cur, sc) + 2) >> 2;
+    if (TT == 2) return (int)((double)had_abs<8, 4>(org, so, cur, sc) / SQRT32 * 2.0);
+    if (TT == 4) return (int)((double)had_abs<4, 8>(org, so, cur, sc) / SQRT32 * 2.0);
+    // 16x8 (TT 3) and 8x16 (TT 5): rows of the two 8x8 halves are combined
+    // into sums and differences as they are read
+    int mS[64], mD[64];
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        int o1[8], c1[8], o2[8], c2[8];
+        if (TT == 3) {
+            load_row<8>(org + y * so, o1);
+            load_row<8>(cur + y * sc, c1);
+            load_row<8>(org + y * so + 8, o2);
+            load_row<8>(cur + y * sc + 8, c2);
+        } else {
+            load_row<8>(org + y * so, o1);
+            load_row<8>(cur + y * sc, c1);
+            load_row<8>(org + (y + 8) * so, o2);
+            load_row<8>(cur + (y + 8) * sc, c2);
+        }
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            const int l = o1[x] - c1[x], r = o2[x] - c2[x];
+            mS[y * 8 + x] = l + r;
+            mD[y * 8 + x] = l - r;
+        }
+    }
+    int s = 0;
+#pragma unroll
+    for (int y = 0; y < 8; y++) { fwht<8>(mS + y * 8, 1); fwht<8>(mD + y * 8, 1); }
+#pragma unroll
+    for (int x = 0; x < 8; x++) { fwht<8>(mS + x, 8); fwht<8>(mD + x, 8); }
+#pragma unroll
+    for (int i = 0; i < 64; i++) s += abs(mS[i]) + abs(mD[i]);
+    return (int)((double)s / SQRT128 * 2.0);
+}
+
+// Sub-pel search of one shape.  A unit is one (CTU, PU of this shape,
+// reference); LPU lanes share a unit and each takes T/LPU of its com_had
+// tiles (com_cost.c:719-811).  Per search position the lanes' tile SATDs are
+// summed across the unit's lanes, so every lane of a unit holds the same
+// costs and makes the same (strict '<', reference order) decisions.
+template <int TT, int T>
+__device__ __forceinline__ void subpel_body(const SubArgs &a)
+{
+    constexpr int LPU = T < 64 ? T : 64, TPL = T / LPU, UPB = 256 / LPU;
+    const int li = threadIdx.x % LPU;
+    const long unit = (long)blockIdx.x * UPB + threadIdx.x / LPU;
+    const long nunits = (long)a.n_ctu * a.per_ctu * a.nref;
+    if (unit >= nunits) return;
+    const int r = (int)(unit % a.nref);
+    const long cp = unit / a.nref;
+    const int i = (int)(cp % a.per_ctu), ctu = (int)(cp / a.per_ctu);
+    const int cols = 128 / a.pw;
+    const int x = (ctu % a.ctu_cols) * 128 + (i % cols) * a.pw, y = (ctu / a.ctu_cols) * 128 + (i / cols) * a.ph;
+    const size_t slot = ((size_t)ctu * a.npu + a.base + i) * a.nref + r;
+    const bool inside = x + a.pw <= a.pic_w && y + a.ph <= a.pic_h;
+    const u64 key = a.keys[slot];
+    me_pu_result_t out;
+    out.mv_int[0] = out.mv_int[1] = out.mv_sub[0] = out.mv_sub[1] = 0;
+    out.cost_int = out.cost_sub = ~0ull;
+    out.cost_final = 0xFFFFFFFFu;
+    out.valid = 0;
+    if (!inside || key == ~0ull) {
+        if (li == 0) a.res[slot] = out;
+        return;
+    }
+    const u64 k = key - ((u64)KEY_BIAS_HI << 32);
     const int span = 2 * a.R + 1;
-    u32 best_final = 0xFFFFFFFFu;
-    int best_ref = -1;
+    const int idx = (int)(k & ((1ull << a.IB) - 1));
+    const int dx = idx % span - a.R, dy = idx / span - a.R;
+    out.mv_int[0] = (s16)dx;
+    out.mv_int[1] = (s16)dy;
+    out.cost_int = k >> a.IB;
 
-    for (int r = 0; r < a.nref; r++) {
-        me_pu_result_t out;
-        out.mv_int[0] = out.mv_int[1] = out.mv_sub[0] = out.mv_sub[1] = 0;
-        out.cost_int = out.cost_sub = ~0ull;
-        out.cost_final = 0xFFFFFFFFu;
-        out.valid = 0;
-        const u64 key = a.keys[base * a.nref + r];
-        if (!inside || key == ~0ull) {
-            if (lane == 0) a.res[base * a.nref + r] = out;
-            continue;
-        }
-        const u64 k = key - ((u64)KEY_BIAS_HI << 32);
-        const int idx = (int)(k & ((1ull << a.IB) - 1));
-        const int dx = idx % span - a.R, dy = idx / span - a.R;
-        out.mv_int[0] = (s16)dx; out.mv_int[1] = (s16)dy;
-        out.cost_int = k >> a.IB;
-
-        // Phase 1: centre (integer plane, no clipping: me.c:400) and the 8
-        // half-pel points of me.c:415-417.
-        const int HP[8][2] = {{-2, 0}, {-2, 2}, {0, 2}, {2, 2}, {2, 0}, {2, -2}, {0, -2}, {-2, -2}};
-        const int QP[8][2] = {{-1, 0}, {0, 1}, {1, 0}, {0, -1}, {-1, 1}, {1, 1}, {-1, -1}, {1, -1}};
-        if (lane < 9) sums[lane] = 0;
-        __syncthreads();
-        const int cmx = dx * 4, cmy = dy * 4;
-        for (int t = lane; t < 9 * ntiles; t += 64) {
-            const int p = t / ntiles, ti = t - p * ntiles;
-            const int tx = (ti % tpr) * tw, ty = (ti / tpr) * th;
-            const pel *pred;
-            if (p == 0) pred = plane_ptr(a, r, 0, x + dx, y + dy);
-            else pred = qpel_ptr(a, r, cmx + HP[p - 1][0] + (x << 2), cmy + HP[p - 1][1] + (y << 2));
-            int v = satd_any(tt, org + ty * a.pic_w + tx, a.pic_w, pred + ty * a.stride + tx, a.stride);
-            atomicAdd(&sums[p], v);
-        }
-        __syncthreads();
-        if (lane == 0) {
-            const u8 *tb = a.tab;
-            int bx = cmx, by = cmy;
-            u64 best = (u64)(a.lambda * (u32)(tb[cmx] + tb[cmy])) + ((u64)(sums[0] >> (BIT_DEPTH - 8)) << 16);
-            for (int i = 0; i < 8; i++) {
-                int mx = cmx + HP[i][0], my = cmy + HP[i][1];
-                u64 c = (u64)(a.lambda * (u32)(tb[mx] + tb[my])) + ((u64)(sums[i + 1] >> (BIT_DEPTH - 8)) << 16);
-                if (c < best) { best = c; bx = mx; by = my; }
-            }
-            sel[0] = bx; sel[1] = by;
-            out.cost_sub = best;
-        }
-        __syncthreads();
-        const int hx = sel[0], hy = sel[1];
-        if (lane < 8) sums[lane] = 0;
-        __syncthreads();
-        // Phase 2: the 8 quarter-pel points of me.c:460-462 around the best
-        // half-pel point.
-        for (int t = lane; t < 8 * ntiles; t += 64) {
-            const int p = t / ntiles, ti = t - p * ntiles;
-            const int tx = (ti % tpr) * tw, ty = (ti / tpr) * th;
-            const pel *pred = qpel_ptr(a, r, hx + QP[p][0] + (x << 2), hy + QP[p][1] + (y << 2));
-            int v = satd_any(tt, org + ty * a.pic_w + tx, a.pic_w, pred + ty * a.stride + tx, a.stride);
-            atomicAdd(&sums[p], v);
-        }
-        __syncthreads();
-        if (lane == 0) {
-            const u8 *tb = a.tab;
-            u64 best = out.cost_sub;
-            int bx = hx, by = hy;
-            for (int i = 0; i < 8; i++) {
-                int mx = hx + QP[i][0], my = hy + QP[i][1];
-                u64 c = (u64)(a.lambda * (u32)(tb[mx] + tb[my])) + ((u64)(sums[i] >> (BIT_DEPTH - 8)) << 16);
-                if (c < best) { best = c; bx = mx; by = my; }
-            }
-            out.mv_sub[0] = (s16)bx; out.mv_sub[1] = (s16)by;
-            out.cost_sub = best;
-            // refi bits: tbl_refi_bits[n][r] (tables.c:52-70); mvr 0 adds 0 + 1 + 2
-            int rb = a.nref <= 1 ? 0 : (r < a.nref - 1 ? r + 1 : a.nref - 1);
-            u64 cf = best + (u64)(a.lambda * (u32)(rb + 3));
-            out.cost_final = (u32)((cf + (1u << 15)) >> 16);
-            out.valid = 1;
-            a.res[base * a.nref + r] = out;
-            if (out.cost_final < best_final) { best_final = out.cost_final; best_ref = r; }
-        }
-        __syncthreads();
+    const pel *org = a.cur + (size_t)y * a.pic_w + x;
+    int toff_o[TPL], toff_p[TPL];
+#pragma unroll
+    for (int j = 0; j < TPL; j++) {
+        const int t = li + LPU * j;
+        const int tx = (t % a.tpr) * a.tw, ty = (t / a.tpr) * a.th;
+        toff_o[j] = ty * a.pic_w + tx;
+        toff_p[j] = ty * a.stride + tx;
     }
-    if (lane == 0) a.best[base] = (s8)best_ref;
+    auto had = [&](const pel *pred) -> u32 {
+        int v = 0;
+#pragma unroll 1
+        for (int j = 0; j < TPL; j++) v += satd_t<TT>(org + toff_o[j], a.pic_w, pred + toff_p[j], a.stride);
+#pragma unroll
+        for (int m = 1; m < LPU; m <<= 1) v += __shfl_xor(v, m, 64);
+        return (u32)(v >> (BIT_DEPTH - 8));   // com_had: sum >> (bd - 8)
+    };
+    const u8 *tb = a.tab;
+    const int cmx = dx * 4, cmy = dy * 4;
+    // centre: the integer plane, not clipped (me.c:400)
+    u64 best = (u64)(a.lambda * (u32)(tb[cmx] + tb[cmy])) + ((u64)had(plane_ptr(a, r, 0, x + dx, y + dy)) << 16);
+    int bx = cmx, by = cmy;
+    const int HP[8][2] = {{-2, 0}, {-2, 2}, {0, 2}, {2, 2}, {2, 0}, {2, -2}, {0, -2}, {-2, -2}};
+    const int QP[8][2] = {{-1, 0}, {0, 1}, {1, 0}, {0, -1}, {-1, 1}, {1, 1}, {-1, -1}, {1, -1}};
+#pragma unroll 1
+    for (int p = 0; p < 8; p++) {
+        const int mx = cmx + HP[p][0], my = cmy + HP[p][1];
+        const u64 c = (u64)(a.lambda * (u32)(tb[mx] + tb[my])) + ((u64)had(qpel_ptr(a, r, mx + (x << 2), my + (y << 2))) << 16);
+        if (c < best) { best = c; bx = mx; by = my; }
+    }
+    const int hx = bx, hy = by;
+#pragma unroll 1
+    for (int p = 0; p < 8; p++) {
+        const int mx = hx + QP[p][0], my = hy + QP[p][1];
+        const u64 c = (u64)(a.lambda * (u32)(tb[mx] + tb[my])) + ((u64)had(qpel_ptr(a, r, mx + (x << 2), my + (y << 2))) << 16);
+        if (c < best) { best = c; bx = mx; by = my; }
+    }
+    if (li == 0) {
+        out.mv_sub[0] = (s16)bx;
+        out.mv_sub[1] = (s16)by;
+        out.cost_sub = best;
+        // tbl_refi_bits[n][r] (tables.c:52-70); AMVR 0 adds 0 + 1 + 2 (me.c:560-561)
+        const int rb = a.nref <= 1 ? 0 : (r < a.nref - 1 ? r + 1 : a.nref - 1);
+        const u64 cf = best + (u64)(a.lambda * (u32)(rb + 3));
+        out.cost_final = (u32)((cf + (1u << 15)) >> 16);
+        out.valid = 1;
+        a.res[slot] = out;
+    }
+}
+
+// 8x8-class tiles fit 4 waves per SIMD; the 16x8 / 8x16 tiles (two 8x8
+// halves) get the register budget of 2.
+template <int TT, int T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_subpel(SubArgs a)
+{
+    subpel_body<TT, T>(a);
+}
+template <int TT, int T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_subpel_wide(SubArgs a)
+{
+    subpel_body<TT, T>(a);
+}
+
+// Reference choice per PU: smallest final cost, lower index on ties
+// (inter.c:901-904); -1 for a PU that was not searched.
+__global__ void k_select(const me_pu_result_t *res, s8 *best, int n, int nref)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u32 bc = 0xFFFFFFFFu;
+    int br = -1;
+    for (int r = 0; r < nref; r++) {
+        const me_pu_result_t &o = res[(size_t)i * nref + r];
+        if (o.valid && o.cost_final < bc) { bc = o.cost_final; br = r; }
+    }
+    best[i] = (s8)br;
+}
+
+typedef void (*subpel_fn)(SubArgs);
+
+// Kernel instance for a shape's (tile type, tile count).
+static subpel_fn subpel_kernel(int tt, int T)
+{
+#define K(TT_, T_) if (tt == TT_ && T == T_) return k_subpel<TT_, T_>
+    K(4, 1); K(4, 2); K(4, 4);
+    K(2, 1); K(2, 2); K(2, 4);
+    K(1, 1); K(1, 4); K(1, 16); K(1, 64); K(1, 256);
+#undef K
+#define K(TT_, T_) if (tt == TT_ && T == T_) return k_subpel_wide<TT_, T_>
+    K(5, 1); K(5, 2); K(5, 4); K(5, 8); K(5, 16); K(5, 32); K(5, 64);
+    K(3, 1); K(3, 2); K(3, 4); K(3, 8); K(3, 16); K(3, 32); K(3, 64);
+#undef K
+    return nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -972,7 +1104,7 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     ALLOC(p->d_rate, p->span * sizeof(u32));
     ALLOC(p->d_tab_base, tab.size());
     ALLOC(p->d_pu_tab, putab.size() * sizeof(int));
-    ALLOC(p->d_cur, (size_t)cfg->pic_w * cfg->pic_h * sizeof(pel));
+    ALLOC(p->d_cur, (size_t)cfg->pic_w * cfg->pic_h * sizeof(pel) + 64);   // slack for dword row reads
     ALLOC(p->d_keys, nkeys * sizeof(u64));
     ALLOC(p->d_res, nkeys * sizeof(me_pu_result_t));
     ALLOC(p->d_best, (size_t)p->n_ctu * p->npu);
@@ -1051,11 +1183,25 @@ static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prep
     b.pic_w = p->cfg.pic_w; b.pic_h = p->cfg.pic_h; b.ctu_cols = p->ctu_cols;
     b.R = p->cfg.search_range; b.IB = p->IB; b.lambda = p->cfg.lambda_mv;
     b.tab = p->d_tab_base + p->tab_off;
-    b.pu_tab = p->d_pu_tab; b.keys = p->d_keys; b.res = p->d_res; b.best = p->d_best;
-    b.npu = p->npu; b.nref = p->nref;
+    b.keys = p->d_keys; b.res = p->d_res; b.best = p->d_best;
+    b.npu = p->npu; b.nref = p->nref; b.n_ctu = p->n_ctu;
     if (ev) ME_CHECK(hipEventRecord(ev[3], st));
-    hipLaunchKernelGGL(k_subpel, dim3(p->npu, p->n_ctu), dim3(64), 0, st, b);
-    ME_CHECK(hipGetLastError());
+    for (int s = 0; s < ME_NSHAPES; s++) {
+        if (p->base[s] < 0) continue;
+        const int w = ME_SHAPES_HOST[s][0], h = ME_SHAPES_HOST[s][1];
+        int tt, tw, th;
+        had_tiling(w, h, &tt, &tw, &th);
+        const int T = (w / tw) * (h / th);
+        subpel_fn fn = subpel_kernel(tt, T);
+        if (!fn) { me_set_error("no sub-pel kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+        b.pw = w; b.ph = h; b.tw = tw; b.th = th; b.tpr = w / tw; b.base = p->base[s];
+        b.per_ctu = (128 / w) * (128 / h);
+        const int lpu = T < 64 ? T : 64, upb = 256 / lpu;
+        const long units = (long)p->n_ctu * b.per_ctu * p->nref;
+        hipLaunchKernelGGL(fn, dim3((unsigned)((units + upb - 1) / upb)), dim3(256), 0, st, b);
+    }
+    const int npus = p->n_ctu * p->npu;
+    hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, p->d_res, p->d_best, npus, p->nref);
     if (ev) ME_CHECK(hipEventRecord(ev[4], st));
     return 0;
 }
