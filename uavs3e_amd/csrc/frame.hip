@@ -223,6 +223,7 @@ struct IntArgs {
     int npu, nref;
     int band_rows, wcols;    // displacement rows per block; LDS window row length
     int dx_cols, nbx;        // displacement columns per block; blocks along x
+    int wpitch;              // LDS row pitch of the window (wcols + skew room)
     int base[ME_NSHAPES];    // first PU index of each shape inside a CTU (-1: not selected)
 };
 
@@ -461,6 +462,12 @@ __device__ __forceinline__ int pu_index(const IntArgs &a, int s, int w, int h, i
     return a.base[s] + (y / h) * (128 / w) + x / w;
 }
 
+// The LDS window is stored with every block of 16 rows shifted right by
+// ((row >> 4) & 3) dwords.  Lanes read rows 16 apart at columns 16 samples
+// apart, so without the shift a half-wave's 32 dword reads fall into only 8
+// of the 32 banks; with it they fall into 32 distinct banks (8-bit).
+__device__ __forceinline__ int wskew(int row) { return ((row >> 4) & 3) * (4 / (int)sizeof(pel)); }
+
 // Leaves: the 16 4x4-block SADs of the lane's 16x16 region, NK displacements
 // at a time, as L[r][c].
 #if BIT_DEPTH == 8
@@ -477,7 +484,7 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
         for (int c = 0; c < 4; c++) acc[r][c] = 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + c0);
+        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0);
         u32 d[5];
 #pragma unroll
         for (int j = 0; j < 5; j++) d[j] = rp[j];
@@ -504,7 +511,7 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
         for (int c = 0; c < 4; c++) L[r][c].v = 0;
 #pragma unroll 4
     for (int r = 0; r < 16; r++) {
-        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + c0 + (k0 & ~1));
+        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0 + (k0 & ~1));
         u32 d[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) d[j] = rp[j];
@@ -570,7 +577,7 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const Block &b, const
                     }
                 }
                 Val L[4][4];
-                leaves(win, a.wcols, Y + row, X + (dxb - b.dxs), curl, Y, X, k0, L);
+                leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, k0, L);
                 Region g;
                 g.build(L);
                 if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
@@ -669,7 +676,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_int_search(IntArgs a)
     b.dxe = b.dxs + a.dx_cols - 1 < a.hi ? b.dxs + a.dx_cols - 1 : a.hi;
     const int wrows = 128 + a.band_rows - 1;
     pel *win = (pel *)smem;
-    pel *curl = win + (size_t)wrows * a.wcols;
+    pel *curl = win + (size_t)wrows * a.wpitch;
     const int tid = threadIdx.x, nthr = 64 * WAVES;
 
     // Stage the window: reference samples (cx + dxs + c, cy + dy0 + r), clamped
@@ -680,7 +687,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_int_search(IntArgs a)
     for (int i = tid; i < wrows * a.wcols; i += nthr) {
         int r = i / a.wcols, c = i - r * a.wcols;
         int x = me_clip3(xmin, xmax, b.cx + b.dxs + c), y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
-        win[i] = ref0[(ptrdiff_t)y * a.ref_stride + x];
+        win[r * a.wpitch + wskew(r) + c] = ref0[(ptrdiff_t)y * a.ref_stride + x];
     }
     for (int i = tid; i < 128 * 128; i += nthr) {
         int r = i >> 7, c = i & 127;
@@ -875,7 +882,9 @@ __device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, in
 template <int TT, int T>
 __device__ __forceinline__ void subpel_body(const SubArgs &a)
 {
-    constexpr int LPU = T < 64 ? T : 64, TPL = T / LPU, UPB = 256 / LPU;
+    // lanes per unit: one per tile up to a wave; the 256-tile 128x128 PU
+    // takes a whole 256-thread block
+    constexpr int LPU = T < 64 ? T : (T < 256 ? 64 : 256), TPL = T / LPU, UPB = 256 / LPU;
     const int li = threadIdx.x % LPU;
     const long unit = (long)blockIdx.x * UPB + threadIdx.x / LPU;
     const long nunits = (long)a.n_ctu * a.per_ctu * a.nref;
@@ -919,7 +928,14 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a)
 #pragma unroll 1
         for (int j = 0; j < TPL; j++) v += satd_t<TT>(org + toff_o[j], a.pic_w, pred + toff_p[j], a.stride);
 #pragma unroll
-        for (int m = 1; m < LPU; m <<= 1) v += __shfl_xor(v, m, 64);
+        for (int m = 1; m < (LPU < 64 ? LPU : 64); m <<= 1) v += __shfl_xor(v, m, 64);
+        if (LPU > 64) {
+            __shared__ int red[4];
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+            __syncthreads();
+            v = red[0] + red[1] + red[2] + red[3];
+            __syncthreads();
+        }
         return (u32)(v >> (BIT_DEPTH - 8));   // com_had: sum >> (bd - 8)
     };
     const u8 *tb = a.tab;
@@ -1008,7 +1024,7 @@ struct me_plan_s {
     me_frame_cfg_t cfg;
     int ctu_cols, ctu_rows, n_ctu, npu, nref;
     int IB, sh, lo, hi, span;
-    int mode, waves, band_rows, nbands, wcols, dx_cols, nbx;
+    int mode, waves, band_rows, nbands, wcols, wpitch, dx_cols, nbx;
     size_t lds;
     int base[ME_NSHAPES];
     u32 *d_rate;
@@ -1082,7 +1098,8 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     p->nbx = (p->span + p->dx_cols - 1) / p->dx_cols;
     // A lane reads up to 20 samples from column X + (dxb - dxs) <= 112 + dx_cols - 4.
     p->wcols = ((128 + p->dx_cols + 16 + 15) / 16) * 16;
-    p->lds = (size_t)(128 + p->band_rows - 1) * p->wcols * sizeof(pel) + 128 * 128 * sizeof(pel);
+    p->wpitch = p->wcols + 16;
+    p->lds = (size_t)(128 + p->band_rows - 1) * p->wpitch * sizeof(pel) + 128 * 128 * sizeof(pel);
 
     // rate[i] = bits(4*(i-R)) * lambda (GET_MVBITS_IPEL, util.h:43-44)
     p->tab_off = ((cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h) << 2) + 3;
@@ -1152,7 +1169,7 @@ static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prep
     a.pic_w = p->cfg.pic_w; a.pic_h = p->cfg.pic_h; a.ctu_cols = p->ctu_cols;
     a.R = p->cfg.search_range; a.lo = p->lo; a.hi = p->hi; a.IB = p->IB; a.sh = p->sh;
     a.rate = p->d_rate; a.keys = p->d_keys; a.npu = p->npu; a.nref = p->nref;
-    a.band_rows = p->band_rows; a.wcols = p->wcols; a.dx_cols = p->dx_cols; a.nbx = p->nbx;
+    a.band_rows = p->band_rows; a.wcols = p->wcols; a.dx_cols = p->dx_cols; a.nbx = p->nbx; a.wpitch = p->wpitch;
     memcpy(a.base, p->base, sizeof(a.base));
     dim3 grid(p->nbands * p->nbx, p->n_ctu, p->nref);
     if (p->lds > 65536) {
@@ -1196,7 +1213,7 @@ static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prep
         if (!fn) { me_set_error("no sub-pel kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
         b.pw = w; b.ph = h; b.tw = tw; b.th = th; b.tpr = w / tw; b.base = p->base[s];
         b.per_ctu = (128 / w) * (128 / h);
-        const int lpu = T < 64 ? T : 64, upb = 256 / lpu;
+        const int lpu = T < 64 ? T : (T < 256 ? 64 : 256), upb = 256 / lpu;
         const long units = (long)p->n_ctu * b.per_ctu * p->nref;
         hipLaunchKernelGGL(fn, dim3((unsigned)((units + upb - 1) / upb)), dim3(256), 0, st, b);
     }
