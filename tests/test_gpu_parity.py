@@ -111,14 +111,8 @@ def _run_plan(hip, cur, refs, R, lam, mask, **kw):
 
 
 def _compare(res, best, ores, obest):
-    valid = ores["valid"] == 1
-    assert np.array_equal(res["valid"], ores["valid"])
-    for f in ("mv_int", "mv_sub", "cost_int", "cost_sub", "cost_final"):
-        a, b = res[f][valid], ores[f][valid]
-        bad = np.nonzero(np.any(a != b, axis=-1) if a.ndim > 1 else a != b)[0]
-        assert len(bad) == 0, "%s differs at %d of %d valid PUs, first %s vs %s" % (
-            f, len(bad), valid.sum(), a[bad[0]], b[bad[0]])
-    assert np.array_equal(best, obest)
+    msg = ua.compare_results(res, best, ores, obest)
+    assert msg is None, msg
 
 
 CASES = [
@@ -144,6 +138,25 @@ def test_frame_plan_vs_oracle(case):
     ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], R, lam, mask)
     _compare(res, best, ores, obest)
     assert res["valid"].sum() > 0
+
+
+@pytest.mark.parametrize("bd,nref", [(8, 2), (10, 1)])
+def test_frame_plan_high_contrast(bd, nref):
+    """Independent uniform-noise frames: mean |difference| of about a third of
+    the sample range, so a 128x128 SAD exceeds 2^20 (8-bit) and 2^22
+    (10-bit) at every displacement -- the largest costs the keys must order."""
+    rng = np.random.default_rng(7 + bd)
+    w, h, R = 256, 256, 16
+    dt = np.uint8 if bd == 8 else np.uint16
+    cur = rng.integers(0, 1 << bd, (h, w)).astype(dt)
+    refs = [rng.integers(0, 1 << bd, (h, w)).astype(dt) for _ in range(nref)]
+    hip, orc = ua.MeHip(bd), Oracle(bd)
+    lam = wl.lambda_mv(32)
+    res, best = _run_plan(hip, cur, refs, R, lam, MASK_ALL)
+    ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], R, lam, MASK_ALL)
+    _compare(res, best, ores, obest)
+    big = ores["cost_int"][:, -1, 0] >> (16 - (bd - 8))   # 128x128 SADs of the four CTUs
+    assert (big > (1 << (20 if bd == 8 else 22))).all()
 
 
 def test_plan_tuning_variants_agree():

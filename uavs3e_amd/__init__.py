@@ -46,6 +46,28 @@ DISPATCH_SLOTS = (
 DispatchTable = C.c_void_p * len(DISPATCH_SLOTS)
 
 
+RESULT_FIELDS = ("mv_int", "mv_sub", "cost_int", "cost_sub", "cost_final")
+
+
+def compare_results(res, best, ores, obest):
+    """Compares two frame results ([ctu][pu][ref] PU_RESULT arrays and
+    [ctu][pu] best-reference arrays).  Entries of PUs outside the picture
+    (valid == 0) carry no result and are not compared.  Returns None when
+    equal, else a message naming the first difference."""
+    if not np.array_equal(res["valid"], ores["valid"]):
+        return "valid flags differ at %d entries" % int((res["valid"] != ores["valid"]).sum())
+    valid = ores["valid"] == 1
+    for f in RESULT_FIELDS:
+        a, b = res[f][valid], ores[f][valid]
+        bad = np.nonzero(np.any(a != b, axis=-1) if a.ndim > 1 else a != b)[0]
+        if len(bad):
+            return "%s differs at %d of %d valid PUs, first %s vs %s" % (
+                f, len(bad), int(valid.sum()), a[bad[0]], b[bad[0]])
+    if not np.array_equal(best, obest):
+        return "best reference differs at %d PUs" % int((best != obest).sum())
+    return None
+
+
 class FrameCfg(C.Structure):
     _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("search_range", C.c_int), ("lambda_mv", C.c_uint32),
                 ("shape_mask", C.c_uint32), ("num_refs", C.c_int), ("band_rows", C.c_int), ("waves", C.c_int),

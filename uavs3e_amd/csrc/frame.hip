@@ -206,8 +206,14 @@ enum {
 // so the low word of a key is K's low word and its high word is
 // (sad << (shift+IB-32)) + K_hi(d): one shift-add per candidate.  KEY_BIAS
 // lifts every high word above 2^28 so that keys are normal doubles.
+// Valid high words stay below 2^29 (sad < 2^24 for a 128x128 PU at 10 bits,
+// K_hi < 2^15).  A node outside the search window sets bit 30; a lane whose
+// displacement lies past the block's range (the tail of the last group of
+// four) gets KEY_OUT_HI plus its SAD.  Both stay below 0x7FF00000, so every
+// key is a finite positive double that v_min_f64 orders like the u64 key.
 #define KEY_BIAS_HI 0x10000000u
 #define KEY_INVALID_HI 0x40000000u
+#define KEY_OUT_HI 0x60000000u
 #define KEY_INIT 0x7FF0000000000000ull
 
 struct IntArgs {
@@ -567,7 +573,7 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const Block &b, const
                     const int dx = dxb + k0 + k;
                     kc.dx[k] = dx;
                     if (dx < a.lo || dx > b.dxe) {
-                        kc.khi[k] = 0x7FE00000u;   // can never be the minimum
+                        kc.khi[k] = KEY_OUT_HI;   // above every valid key, never NaN
                         kc.klo[k] = 0;
                     } else {
                         const int dxi = dx + a.R;
