@@ -39,11 +39,33 @@ def host_lib(bd):
     return os.path.join(LIB, "libme_host%d.so" % bd)
 
 
-def _build_hip(bd, extra=()):
-    srcs = [os.path.join(CSRC, f) for f in ("percall.hip", "frame.hip")]
+# Translation units of the HIP library: (source, extra defines).  The
+# integer-search and sub-pel kernels are split by instance so that the
+# expensive device compiles run in parallel.
+HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []),
+             ("intsearch.hip", ["-DME_IS_WAVES=1"]), ("intsearch.hip", ["-DME_IS_WAVES=2"]),
+             ("intsearch.hip", ["-DME_IS_WAVES=4"]),
+             ("subpel.hip", ["-DME_SP_PART=0"]), ("subpel.hip", ["-DME_SP_PART=1"]),
+             ("subpel.hip", ["-DME_SP_PART=2"])]
+BUILD_DIR = os.path.join(ROOT, "_build")
+
+
+def _obj_name(bd, src, defs):
+    tag = "".join("_" + d.split("=")[-1] for d in defs)
+    return os.path.join(BUILD_DIR, "bd%d_%s%s.o" % (bd, os.path.splitext(src)[0], tag))
+
+
+def _compile_unit(bd, src, defs, extra=()):
+    obj = _obj_name(bd, src, defs)
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+    _run([HIPCC] + flags + list(extra) + defs + ["-DBIT_DEPTH=%d" % bd, "-c", os.path.join(CSRC, src), "-o", obj])
+    return obj
+
+
+def _link_hip(bd, objs):
     out = hip_lib(bd)
     tmp = out + ".tmp%d" % os.getpid()
-    _run([HIPCC] + HIP_FLAGS + list(extra) + ["-DBIT_DEPTH=%d" % bd] + srcs + ["-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -59,12 +81,15 @@ def _build_host(bd):
     return out
 
 
-def build(jobs=4):
+def build(jobs=None, bit_depths=(8, 10)):
     os.makedirs(LIB, exist_ok=True)
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    jobs = jobs or max(1, min(16, os.cpu_count() or 1))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(_build_hip, 8), ex.submit(_build_hip, 10), ex.submit(_build_host, 8),
-                ex.submit(_build_host, 10)]
-        for f in futs:
+        units = {bd: [ex.submit(_compile_unit, bd, src, defs) for src, defs in HIP_UNITS] for bd in bit_depths}
+        hosts = [ex.submit(_build_host, bd) for bd in bit_depths]
+        links = [ex.submit(_link_hip, bd, [f.result() for f in units[bd]]) for bd in bit_depths]
+        for f in hosts + links:
             f.result()
 
 

@@ -1,0 +1,161 @@
+// Internal declarations shared by the batched ME translation units
+// (refs.hip, intsearch.hip, subpel.hip, frame.hip, server.hip).
+#pragma once
+#include "common.h"
+#include "satd.h"
+#include "../../include/uavs3e_me_hip.h"
+
+#define ME_MAX_REFS 4
+
+// A reference picture resident in device memory: the edge-padded integer
+// plane + 15 quarter-pel planes (com_pic_t / com_subpel_t, com_util.c:62-91,
+// 251-284), plus the sources from which they are rebuilt.
+struct me_ref_s {
+    int w, h, stride, rows;
+    size_t plane_elems;      // stride * rows
+    pel *planes;             // 16 planes, plane i at planes + i * plane_elems
+    pel *luma;               // unpadded w x h copy (source for re-preparation)
+    s16 *tmp;                // 3 horizontal s16 planes over [-8, w+8) x [-8, h+8)
+};
+
+// Pads the integer plane of r from r->luma and rebuilds its 15 sub-pel
+// planes on stream st (com_img_padding + com_if_luma_frame).
+int me_ref_prepare_async(me_ref_s *r, hipStream_t st);
+
+enum {
+    S4x8 = 0, S8x4, S4x16, S16x4, S4x32, S32x4, S8x8, S8x16, S16x8, S8x32, S32x8, S8x64, S64x8, S16x16,
+    S16x32, S32x16, S16x64, S64x16, S16x128, S128x16, S32x32, S32x64, S64x32, S32x128, S128x32, S64x64,
+    S64x128, S128x64, S128x128
+};
+
+// One current picture and the references its PUs are searched against.
+struct PicDesc {
+    const pel *cur;                  // current picture, stride pic_w (+64 B of read slack)
+    const pel *planes[ME_MAX_REFS];  // plane-0 base (padded buffer start) of each reference
+    size_t plane_elems;              // samples per plane (stride * rows)
+    int ref_stride;                  // pic_w + 2 * ME_PAD
+    int pic_w, pic_h, ctu_cols;
+};
+
+// The CTUs a launch works on, always indexed by uniform (workgroup-level)
+// values so that descriptor reads are scalar loads.  jobs == nullptr: job j
+// is CTU j of pics[0] (a whole frame).  Otherwise jobs[j] = (picture index
+// << 16) | CTU index.  pics is device memory.  Keys and results are indexed
+// by job.
+struct JobSet {
+    const PicDesc *pics;
+    const u32 *jobs;
+    int n_jobs;
+};
+
+__device__ __forceinline__ const PicDesc *job_decode(const JobSet &js, int job, int &ctu)
+{
+    if (js.jobs) {
+        const u32 j = js.jobs[job];
+        ctu = (int)(j & 0xFFFFu);
+        return js.pics + (j >> 16);
+    }
+    ctu = job;
+    return js.pics;
+}
+
+// ---------------------------------------------------------------------------
+// Integer full search
+// ---------------------------------------------------------------------------
+
+// Keys: a candidate's cost c (u64, < 2^41) and its raster index i in the
+// (2R+1)^2 window are packed as key = (c << IB) | i, so the smallest key is
+// the smallest cost with the first raster position on ties -- the reference's
+// strict '<' scan (me.c:169-172, 187).  With cost = (sad << shift) + rate and
+// IB >= 32 - shift, key = sad * 2^(shift+IB) + K(d), K(d) = (rate << IB) | i,
+// so the low word of a key is K's low word and its high word is
+// (sad << (shift+IB-32)) + K_hi(d): one shift-add per candidate.  KEY_BIAS
+// lifts every high word above 2^28 so that keys are normal doubles.
+// Valid high words stay below 2^29 (sad < 2^24 for a 128x128 PU at 10 bits,
+// K_hi < 2^15).  A node outside the search window sets bit 30; a lane whose
+// displacement lies past the block's range (the tail of the last group of
+// four) gets KEY_OUT_HI plus its SAD.  Both stay below 0x7FF00000, so every
+// key is a finite positive double that v_min_f64 orders like the u64 key.
+#define KEY_BIAS_HI 0x10000000u
+#define KEY_INVALID_HI 0x40000000u
+#define KEY_OUT_HI 0x60000000u
+#define KEY_INIT 0x7FF0000000000000ull
+
+struct IntArgs {
+    JobSet js;
+    int R;                   // window half size
+    int lo, hi;              // uniform displacement clip: [max(-R,-T+4), min(R,T-4)]
+    int IB, sh;              // index bits; sad shift into the high word
+    const u32 *rate;         // [2R+1]: bits(4*(i-R)) * lambda
+    u64 *keys;               // [job][npu][nref]
+    int npu, nref;
+    int band_rows, wcols;    // displacement rows per block; LDS window row length
+    int dx_cols, nbx;        // displacement columns per block; blocks along x
+    int wpitch;              // LDS row pitch of the window (wcols + skew room)
+    int base[ME_NSHAPES];    // first PU index of each shape inside a CTU (-1: not selected)
+};
+
+// Launches k_int_search<mode, waves> (waves 1, 2 or 4; mode 1 = 16x16 PUs
+// only) on grid (bands * nbx, jobs, refs) with `lds` bytes of LDS.
+int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
+int me_int_search_launch_w1(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
+int me_int_search_launch_w2(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
+int me_int_search_launch_w4(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Sub-pel refinement (me_sub_pel_search, me.c:366-500, cost_type 0, bi 0)
+// ---------------------------------------------------------------------------
+
+struct SubArgs {
+    JobSet js;
+    int R, IB;
+    u32 lambda;
+    const u8 *tab;              // MV-bits table, centred (tab[0] is mv 0)
+    const u64 *keys;
+    me_pu_result_t *res;
+    int npu, nref;
+    // the shape handled by one launch
+    int pw, ph, tw, th, tpr, base, per_ctu;
+    int lpu;                    // lanes per PU: tiles, 2 x tiles for 16x8 / 8x16 tiles
+};
+
+typedef void (*subpel_fn)(SubArgs);
+// Kernel for a com_had tile type (had_tiling); nullptr for an unknown type.
+subpel_fn me_subpel_kernel(int tt);
+subpel_fn me_subpel_kernel_small(int tt);   // tile types 0, 1, 2, 4
+subpel_fn me_subpel_kernel_16x8(int tt);    // tile type 3
+subpel_fn me_subpel_kernel_8x16(int tt);    // tile type 5
+
+// ---------------------------------------------------------------------------
+// One search configuration: window geometry, key layout, schedule and the
+// device rate tables, shared by frame plans and the request server.
+// ---------------------------------------------------------------------------
+
+struct MeSearch {
+    int R, span, IB, sh, lo, hi;
+    int npu, nref, mode;
+    int waves, band_rows, nbands, wcols, wpitch, dx_cols, nbx;
+    size_t lds;
+    int base[ME_NSHAPES];
+    u32 lambda;
+    u32 *d_rate;             // [2R+1] integer-pel rate per displacement component
+    u8 *d_tab_base;          // MV-bits table, tab_off samples each side of 0
+    int tab_off;
+};
+
+// max_dim = max(pic_w, pic_h) of the pictures searched (the |mv| <= T - 4
+// limit and the MV-bits table size of inter_search_create).  band_rows,
+// waves and dx_cols are schedule choices (0 = default); results do not
+// depend on them.
+int me_search_init(MeSearch *s, int R, u32 lambda, u32 shape_mask, int nref, int max_dim, int band_rows, int waves,
+                   int dx_cols);
+void me_search_free(MeSearch *s);
+// Enqueues key initialisation, integer search, sub-pel refinement and the
+// reference choice for the jobs of js.  keys/res are [job][npu][nref],
+// best is [job][npu].  ev, if given, receives 4 events recorded before the
+// key fill, before the integer search, before sub-pel and at the end.
+int me_search_enqueue(const MeSearch *s, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
+                      hipEvent_t *ev);
+// The MV-bits table of inter_search_create (inter.c:2029-2058) in closed
+// form: tab[off + v] for |v| < off, 0 at +-off.
+void me_mvbits_fill(u8 *base, int off);

@@ -1,0 +1,526 @@
+// Integer full search of every PU of a set of CTUs (gfx950).
+//
+//   integer cost  (sad << (16-(bd-8))) + (bits_x+bits_y)*lambda   me.c:119,151-187
+//   window        get_max_search_range (is_padding = 1)           me.c:502-527
+//   full search   raster y-major, first strict minimum wins        me.c:150-190
+//
+// This file is compiled once per workgroup size (ME_IS_WAVES = 1, 2, 4) so
+// that the instances build in parallel.
+#include "frame.h"
+
+#ifndef ME_IS_WAVES
+#define ME_IS_WAVES 4
+#endif
+
+// Values of NK displacements of one node: at 8 bits four displacements are
+// packed as u16 lanes (lo = d0,d1; hi = d2,d3) -- a sum of at most 256
+// samples stays below 2^16, so plain 32-bit adds never carry between lanes;
+// at 10 bits one displacement per u32.
+#if BIT_DEPTH == 8
+#define NK 4
+struct Val {
+    u32 lo, hi;
+    __device__ __forceinline__ u32 get(int k) const
+    {
+        const u32 w = k < 2 ? lo : hi;
+        return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
+    }
+};
+__device__ __forceinline__ Val operator+(Val a, Val b) { return {a.lo + b.lo, a.hi + b.hi}; }
+__device__ __forceinline__ Val vsel(bool c, Val a, Val b) { return c ? a : b; }
+#else
+#define NK 1
+struct Val {
+    u32 v;
+    __device__ __forceinline__ u32 get(int) const { return v; }
+};
+__device__ __forceinline__ Val operator+(Val a, Val b) { return {a.v + b.v}; }
+__device__ __forceinline__ Val vsel(bool c, Val a, Val b) { return c ? a : b; }
+#endif
+
+// Uniform key parts of the NK displacements being processed.  A displacement
+// outside the block's range gets a high word that can never win.
+struct KeyCtx {
+    u32 khi[NK], klo[NK];
+    int dx[NK];
+    int dy, sh;
+};
+
+typedef u32 u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ bool node_valid(int px, int py, int dx, int dy, int wlim, int hlim)
+{
+    // is_padding = 1 window of get_max_search_range (me.c:508-512):
+    // -127 - x <= dx <= W-1-x and the same for y (the +-T limit is uniform).
+    return (u32)(px + dx + 127) <= (u32)wlim && (u32)(py + dy + 127) <= (u32)hlim;
+}
+
+// One candidate: key = ((sad << sh) + khi) : klo, kept as the bits of a
+// double; st = min(st, key).  SH0: sh is known to be 0 (every configuration
+// except 8-bit with R > 127), so the add can take a 16-bit lane directly.
+template <bool EDGE, bool SH0>
+__device__ __forceinline__ void upd1(double &st, u32 sad, const KeyCtx &c, int k, int px, int py, int wlim, int hlim)
+{
+    u32 hi = (SH0 ? sad : (sad << c.sh)) + c.khi[k];
+    if (EDGE) {
+        if (!node_valid(px, py, c.dx[k], c.dy, wlim, hlim)) hi |= KEY_INVALID_HI;
+    }
+    u32x2 key = {c.klo[k], hi};
+    st = me_vmin64(st, __builtin_bit_cast(double, key));
+}
+
+// Cross-lane partner values.  Lane = ry*8 + rx (16x16 regions of the CTU).
+//   xor 1, xor 2 : DPP quad permutes        (x pairs, x quads)
+//   x octet      : DPP row_half_mirror       (the other x quad of a row of 8)
+//   xor 8        : DPP row_ror:8             (y pairs)
+//   xor 16       : ds_swizzle xor mode       (y quads)
+//   xor 32       : ds_bpermute               (the other half of the CTU)
+__device__ __forceinline__ u32 px1(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false); }
+__device__ __forceinline__ u32 px2(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false); }
+__device__ __forceinline__ u32 px4(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false); }
+__device__ __forceinline__ u32 py1(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false); }
+__device__ __forceinline__ u32 py2(u32 v) { return (u32)__builtin_amdgcn_ds_swizzle((int)v, 0x401F); }
+__device__ __forceinline__ u32 py4(u32 v) { return (u32)__shfl_xor((int)v, 32, 64); }
+#if BIT_DEPTH == 8
+__device__ __forceinline__ Val vpx1(Val a) { return {px1(a.lo), px1(a.hi)}; }
+__device__ __forceinline__ Val vpy1(Val a) { return {py1(a.lo), py1(a.hi)}; }
+#else
+__device__ __forceinline__ Val vpx1(Val a) { return {px1(a.v)}; }
+__device__ __forceinline__ Val vpy1(Val a) { return {py1(a.v)}; }
+#endif
+
+// Region-level composition from the 16 leaves L[r][c] (4x4 blocks, row r,
+// column c of the lane's 16x16 region).
+struct Region {
+    Val h8[4][2], v8[2][4], q[2][2], h16[4], v16[4], h16x8[2], v8x16[2], s16;
+    __device__ __forceinline__ void build(const Val L[4][4])
+    {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c2 = 0; c2 < 2; c2++) h8[r][c2] = L[r][2 * c2] + L[r][2 * c2 + 1];
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) v8[r2][c] = L[2 * r2][c] + L[2 * r2 + 1][c];
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++)
+#pragma unroll
+            for (int c2 = 0; c2 < 2; c2++) q[r2][c2] = h8[2 * r2][c2] + h8[2 * r2 + 1][c2];
+#pragma unroll
+        for (int r = 0; r < 4; r++) h16[r] = h8[r][0] + h8[r][1];
+#pragma unroll
+        for (int c = 0; c < 4; c++) v16[c] = v8[0][c] + v8[1][c];
+#pragma unroll
+        for (int i = 0; i < 2; i++) { h16x8[i] = q[i][0] + q[i][1]; v8x16[i] = q[0][i] + q[1][i]; }
+        s16 = h16x8[0] + h16x8[1];
+    }
+};
+
+// Pass S: the 33 PUs inside the lane's 16x16 region.
+//   st[0..7] 4x8 (c, r2)   st[8..15] 8x4 (c2, r)  st[16..19] 4x16 (c)
+//   st[20..23] 16x4 (r)    st[24..27] 8x8 (c2,r2) st[28..29] 8x16 (c2)
+//   st[30..31] 16x8 (r2)   st[32] 16x16
+#define N_SMALL 33
+template <bool EDGE, bool SH0>
+__device__ __forceinline__ void pass_small(const Region &g, double *st, const KeyCtx &c, int px, int py, int wlim, int hlim)
+{
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+#define U(I, V, X_, Y_) upd1<EDGE, SH0>(st[I], (V).get(k), c, k, X_, Y_, wlim, hlim)
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+            for (int r2 = 0; r2 < 2; r2++) U(cc * 2 + r2, g.v8[r2][cc], px + 4 * cc, py + 8 * r2);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) U(8 + c2 * 4 + r, g.h8[r][c2], px + 8 * c2, py + 4 * r);
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) U(16 + cc, g.v16[cc], px + 4 * cc, py);
+#pragma unroll
+        for (int r = 0; r < 4; r++) U(20 + r, g.h16[r], px, py + 4 * r);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+            for (int r2 = 0; r2 < 2; r2++) U(24 + c2 * 2 + r2, g.q[r2][c2], px + 8 * c2, py + 8 * r2);
+#pragma unroll
+        for (int c2 = 0; c2 < 2; c2++) U(28 + c2, g.v8x16[c2], px + 8 * c2, py);
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++) U(30 + r2, g.h16x8[r2], px, py + 8 * r2);
+        U(32, g.s16, px, py);
+#undef U
+    }
+}
+
+// Pass L: the 609 PUs larger than a 16x16 region.
+// Pairs of regions (x pairs, y pairs) hold 14 nodes between two lanes; each
+// lane keeps half of them (3 "dense" slots per direction, chosen by the lane's
+// rank in the pair).  The 19 sparser classes are summed on every lane of their
+// group and kept by all of them; one lane per group publishes.
+//   st[0..2] x-pair slots: 32x4 rows {0,2}, {1,3} by rank; 32x8 row = rank
+//   st[3..5] y-pair slots: 4x32 cols {0,2}, {1,3}; 8x32 col = rank
+//   st[6] 32x16  st[7] 16x32  st[8] 32x32  st[9..10] 64x8 (r2)  st[11] 64x16
+//   st[12..13] 8x64 (c2)  st[14] 16x64  st[15] 64x32  st[16] 32x64  st[17] 64x64
+//   st[18] 128x16  st[19] 16x128  st[20] 128x32  st[21] 32x128  st[22] 128x64
+//   st[23] 64x128  st[24] 128x128
+#define N_LARGE 25
+template <bool EDGE, bool SH0>
+__device__ __forceinline__ void pass_large(const Region &g, double *st, const KeyCtx &c, int rx, int ry, int cx, int cy,
+                                           int wlim, int hlim)
+{
+    const int X = rx * 16, Y = ry * 16, px = cx + X, py = cy + Y;
+    const bool xr = rx & 1, yr = ry & 1;
+    const int x2 = cx + (X & ~31), x4 = cx + (X & ~63), y2 = cy + (Y & ~31), y4 = cy + (Y & ~63);
+    // dense x pairs
+    Val t32x4[4], t32x8[2], t4x32[4], t8x32[2];
+#pragma unroll
+    for (int r = 0; r < 4; r++) t32x4[r] = g.h16[r] + vpx1(g.h16[r]);
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++) t32x8[r2] = g.h16x8[r2] + vpx1(g.h16x8[r2]);
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) t4x32[cc] = g.v16[cc] + vpy1(g.v16[cc]);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++) t8x32[c2] = g.v8x16[c2] + vpy1(g.v8x16[c2]);
+    const Val d0 = vsel(xr, t32x4[2], t32x4[0]), d1 = vsel(xr, t32x4[3], t32x4[1]), d2 = vsel(xr, t32x8[1], t32x8[0]);
+    const Val d3 = vsel(yr, t4x32[2], t4x32[0]), d4 = vsel(yr, t4x32[3], t4x32[1]), d5 = vsel(yr, t8x32[1], t8x32[0]);
+    // sparse classes are summed one displacement at a time (sums exceed 16 bits)
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        upd1<EDGE, SH0>(st[0], d0.get(k), c, k, x2, py + (xr ? 8 : 0), wlim, hlim);
+        upd1<EDGE, SH0>(st[1], d1.get(k), c, k, x2, py + (xr ? 12 : 4), wlim, hlim);
+        upd1<EDGE, SH0>(st[2], d2.get(k), c, k, x2, py + (xr ? 8 : 0), wlim, hlim);
+        upd1<EDGE, SH0>(st[3], d3.get(k), c, k, px + (yr ? 8 : 0), y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[4], d4.get(k), c, k, px + (yr ? 12 : 4), y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[5], d5.get(k), c, k, px + (yr ? 8 : 0), y2, wlim, hlim);
+        const u32 s = g.s16.get(k);
+        const u32 t32x16 = s + px1(s), t16x32 = s + py1(s);
+        const u32 t32x32 = t32x16 + py1(t32x16);
+        const u32 a8 = t32x8[0].get(k), b8 = t32x8[1].get(k), c8 = t8x32[0].get(k), d8 = t8x32[1].get(k);
+        const u32 t64x8a = a8 + px2(a8), t64x8b = b8 + px2(b8);
+        const u32 t8x64a = c8 + py2(c8), t8x64b = d8 + py2(d8);
+        const u32 t64x16 = t32x16 + px2(t32x16), t16x64 = t16x32 + py2(t16x32);
+        const u32 t64x32 = t32x32 + px2(t32x32), t32x64 = t32x32 + py2(t32x32);
+        const u32 t64x64 = t64x32 + py2(t64x32);
+        const u32 t128x16 = t64x16 + px4(t64x16), t16x128 = t16x64 + py4(t16x64);
+        const u32 t128x32 = t64x32 + px4(t64x32), t32x128 = t32x64 + py4(t32x64);
+        const u32 t128x64 = t64x64 + px4(t64x64), t64x128 = t64x64 + py4(t64x64);
+        const u32 t128x128 = t128x64 + py4(t128x64);
+        upd1<EDGE, SH0>(st[6], t32x16, c, k, x2, py, wlim, hlim);
+        upd1<EDGE, SH0>(st[7], t16x32, c, k, px, y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[8], t32x32, c, k, x2, y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[9], t64x8a, c, k, x4, py, wlim, hlim);
+        upd1<EDGE, SH0>(st[10], t64x8b, c, k, x4, py + 8, wlim, hlim);
+        upd1<EDGE, SH0>(st[11], t64x16, c, k, x4, py, wlim, hlim);
+        upd1<EDGE, SH0>(st[12], t8x64a, c, k, px, y4, wlim, hlim);
+        upd1<EDGE, SH0>(st[13], t8x64b, c, k, px + 8, y4, wlim, hlim);
+        upd1<EDGE, SH0>(st[14], t16x64, c, k, px, y4, wlim, hlim);
+        upd1<EDGE, SH0>(st[15], t64x32, c, k, x4, y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[16], t32x64, c, k, x2, y4, wlim, hlim);
+        upd1<EDGE, SH0>(st[17], t64x64, c, k, x4, y4, wlim, hlim);
+        upd1<EDGE, SH0>(st[18], t128x16, c, k, cx, py, wlim, hlim);
+        upd1<EDGE, SH0>(st[19], t16x128, c, k, px, cy, wlim, hlim);
+        upd1<EDGE, SH0>(st[20], t128x32, c, k, cx, y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[21], t32x128, c, k, x2, cy, wlim, hlim);
+        upd1<EDGE, SH0>(st[22], t128x64, c, k, cx, y4, wlim, hlim);
+        upd1<EDGE, SH0>(st[23], t64x128, c, k, x4, cy, wlim, hlim);
+        upd1<EDGE, SH0>(st[24], t128x128, c, k, cx, cy, wlim, hlim);
+    }
+}
+
+template <bool EDGE, bool SH0>
+__device__ __forceinline__ void pass_16(const Region &g, double *st, const KeyCtx &c, int px, int py, int wlim, int hlim)
+{
+#pragma unroll
+    for (int k = 0; k < NK; k++) upd1<EDGE, SH0>(st[0], g.s16.get(k), c, k, px, py, wlim, hlim);
+}
+
+__device__ __forceinline__ void atomic_key_min(u64 *p, double st)
+{
+    u64 v = __builtin_bit_cast(u64, st);
+    if (v != KEY_INIT) atomicMin((unsigned long long *)p, (unsigned long long)v);
+}
+
+// PU index inside the CTU of shape s at CTU-local (x, y).
+__device__ __forceinline__ int pu_index(const IntArgs &a, int s, int w, int h, int x, int y)
+{
+    return a.base[s] + (y / h) * (128 / w) + x / w;
+}
+
+// The LDS window is stored with every block of 16 rows shifted right by
+// ((row >> 4) & 3) dwords.  Lanes read rows 16 apart at columns 16 samples
+// apart, so without the shift a half-wave's 32 dword reads fall into only 8
+// of the 32 banks; with it they fall into 32 distinct banks (8-bit).
+__device__ __forceinline__ int wskew(int row) { return ((row >> 4) & 3) * (4 / (int)sizeof(pel)); }
+
+// Leaves: the 16 4x4-block SADs of the lane's 16x16 region, NK displacements
+// at a time, as L[r][c].
+#if BIT_DEPTH == 8
+// v_qsad_pk_u16_u8 gives the SADs of one 4-sample row at the four
+// displacements dxb..dxb+3 in one instruction (4 x u16).
+__device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0, int ccol0,
+                                       int k0, Val L[4][4])
+{
+    (void)k0;
+    u64 acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[r][c] = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0);
+        u32 d[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) d[j] = rp[j];
+        const uint4 cw = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0);
+        const u32 cc[4] = {cw.x, cw.y, cw.z, cw.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            acc[r >> 2][j] = __builtin_amdgcn_qsad_pk_u16_u8(((u64)d[j + 1] << 32) | d[j], cc[j], acc[r >> 2][j]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) L[r][c] = {(u32)acc[r][c], (u32)(acc[r][c] >> 32)};
+}
+#else
+// v_sad_u16 on sample pairs; odd displacements realign the pairs with
+// v_alignbyte.  One displacement (dxb + k0) per call.
+__device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0, int ccol0,
+                                       int k0, Val L[4][4])
+{
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) L[r][c].v = 0;
+#pragma unroll 4
+    for (int r = 0; r < 16; r++) {
+        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0 + (k0 & ~1));
+        u32 d[9];
+#pragma unroll
+        for (int j = 0; j < 9; j++) d[j] = rp[j];
+        const uint4 c01 = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0);
+        const uint4 c23 = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0 + 8);
+        const u32 cc[8] = {c01.x, c01.y, c01.z, c01.w, c23.x, c23.y, c23.z, c23.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            u32 p0, p1;
+            if (k0 & 1) {
+                p0 = __builtin_amdgcn_alignbyte(d[2 * c + 1], d[2 * c], 2);
+                p1 = __builtin_amdgcn_alignbyte(d[2 * c + 2], d[2 * c + 1], 2);
+            } else {
+                p0 = d[2 * c];
+                p1 = d[2 * c + 1];
+            }
+            L[r >> 2][c].v = __builtin_amdgcn_sad_u16(p0, cc[2 * c], __builtin_amdgcn_sad_u16(p1, cc[2 * c + 1], L[r >> 2][c].v));
+        }
+    }
+}
+#endif
+
+struct Block {
+    int cx, cy, dy0, dxs, dxe, ref;
+};
+
+// One pass over the block's displacements: PASS 0 = small PUs, 1 = large
+// PUs, 2 = 16x16 only.  Returns the node states in st.
+template <int PASS, int WAVES, bool EDGE, bool SH0, int NS>
+__device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, const Block &b, const pel *win, const pel *curl,
+                                         double *st)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rx = lane & 7, ry = lane >> 3, X = rx * 16, Y = ry * 16;
+    const int wlim = P.pic_w + 126, hlim = P.pic_h + 126;
+    const int span = 2 * a.R + 1;
+#pragma unroll
+    for (int i = 0; i < NS; i++) st[i] = __builtin_bit_cast(double, KEY_INIT);
+    for (int row = wave; row < a.band_rows; row += WAVES) {
+        const int dy = b.dy0 + row;
+        if (dy < a.lo || dy > a.hi) continue;
+        const int dyi = dy + a.R;
+        const u32 ry_rate = a.rate[dyi];
+        for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
+            if (dxb + 3 < a.lo) continue;
+#pragma unroll
+            for (int k0 = 0; k0 < 4; k0 += NK) {
+                if (NK == 1 && (dxb + k0 < a.lo || dxb + k0 > b.dxe)) continue;
+                KeyCtx kc;
+                kc.dy = dy;
+                kc.sh = a.sh;
+#pragma unroll
+                for (int k = 0; k < NK; k++) {
+                    const int dx = dxb + k0 + k;
+                    kc.dx[k] = dx;
+                    if (dx < a.lo || dx > b.dxe) {
+                        kc.khi[k] = KEY_OUT_HI;   // above every valid key, never NaN
+                        kc.klo[k] = 0;
+                    } else {
+                        const int dxi = dx + a.R;
+                        const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
+                        kc.khi[k] = (u32)(K >> 32) + KEY_BIAS_HI;
+                        kc.klo[k] = (u32)K;
+                    }
+                }
+                Val L[4][4];
+                leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, k0, L);
+                Region g;
+                g.build(L);
+                if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+                else if (PASS == 1) pass_large<EDGE, SH0>(g, st, kc, rx, ry, b.cx, b.cy, wlim, hlim);
+                else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+            }
+        }
+    }
+}
+
+#define PUB(S, W, H, XX, YY, ST)                                                                    \
+    do {                                                                                            \
+        if (a.base[S] >= 0) atomic_key_min(kb + (size_t)pu_index(a, S, W, H, XX, YY) * a.nref, ST); \
+    } while (0)
+
+__device__ __forceinline__ void publish_small(const IntArgs &a, u64 *kb, const double *st, int X, int Y)
+{
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++) PUB(S4x8, 4, 8, X + 4 * c, Y + 8 * r2, st[c * 2 + r2]);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) PUB(S8x4, 8, 4, X + 8 * c2, Y + 4 * r, st[8 + c2 * 4 + r]);
+#pragma unroll
+    for (int c = 0; c < 4; c++) PUB(S4x16, 4, 16, X + 4 * c, Y, st[16 + c]);
+#pragma unroll
+    for (int r = 0; r < 4; r++) PUB(S16x4, 16, 4, X, Y + 4 * r, st[20 + r]);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+        for (int r2 = 0; r2 < 2; r2++) PUB(S8x8, 8, 8, X + 8 * c2, Y + 8 * r2, st[24 + c2 * 2 + r2]);
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++) PUB(S8x16, 8, 16, X + 8 * c2, Y, st[28 + c2]);
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++) PUB(S16x8, 16, 8, X, Y + 8 * r2, st[30 + r2]);
+    PUB(S16x16, 16, 16, X, Y, st[32]);
+}
+
+__device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, const double *st, int rx, int ry)
+{
+    const int X = rx * 16, Y = ry * 16, X2 = X & ~31, Y2 = Y & ~31, X4 = X & ~63, Y4 = Y & ~63;
+    const bool xr = rx & 1, yr = ry & 1;
+    // dense slots: every lane owns distinct nodes
+    PUB(S32x4, 32, 4, X2, Y + (xr ? 8 : 0), st[0]);
+    PUB(S32x4, 32, 4, X2, Y + (xr ? 12 : 4), st[1]);
+    PUB(S32x8, 32, 8, X2, Y + (xr ? 8 : 0), st[2]);
+    PUB(S4x32, 4, 32, X + (yr ? 8 : 0), Y2, st[3]);
+    PUB(S4x32, 4, 32, X + (yr ? 12 : 4), Y2, st[4]);
+    PUB(S8x32, 8, 32, X + (yr ? 8 : 0), Y2, st[5]);
+    // sparse classes: the group's top-left lane publishes
+    const bool ex1 = !(rx & 1), ex2 = !(rx & 3), ex3 = rx == 0;
+    const bool ey1 = !(ry & 1), ey2 = !(ry & 3), ey3 = ry == 0;
+    if (ex1) PUB(S32x16, 32, 16, X, Y, st[6]);
+    if (ey1) PUB(S16x32, 16, 32, X, Y, st[7]);
+    if (ex1 && ey1) PUB(S32x32, 32, 32, X, Y, st[8]);
+    if (ex2) {
+        PUB(S64x8, 64, 8, X, Y, st[9]);
+        PUB(S64x8, 64, 8, X, Y + 8, st[10]);
+        PUB(S64x16, 64, 16, X, Y, st[11]);
+    }
+    if (ey2) {
+        PUB(S8x64, 8, 64, X, Y, st[12]);
+        PUB(S8x64, 8, 64, X + 8, Y, st[13]);
+        PUB(S16x64, 16, 64, X, Y, st[14]);
+    }
+    if (ex2 && ey1) PUB(S64x32, 64, 32, X, Y, st[15]);
+    if (ex1 && ey2) PUB(S32x64, 32, 64, X, Y, st[16]);
+    if (ex2 && ey2) PUB(S64x64, 64, 64, X, Y, st[17]);
+    if (ex3) PUB(S128x16, 128, 16, X, Y, st[18]);
+    if (ey3) PUB(S16x128, 16, 128, X, Y, st[19]);
+    if (ex3 && ey1) PUB(S128x32, 128, 32, X, Y, st[20]);
+    if (ex1 && ey3) PUB(S32x128, 32, 128, X, Y, st[21]);
+    if (ex3 && ey2) PUB(S128x64, 128, 64, X, Y, st[22]);
+    if (ex2 && ey3) PUB(S64x128, 64, 128, X, Y, st[23]);
+    if (ex3 && ey3) PUB(S128x128, 128, 128, X, Y, st[24]);
+}
+
+// One block = (displacement tile, CTU, reference); 64*WAVES threads.  The
+// reference window of the tile and the current CTU are staged in LDS once;
+// then each wave covers the whole CTU (lane = 16x16 region) for every
+// WAVES-th displacement row of the tile, in two passes (small PUs, large
+// PUs) so that each pass's node states fit in registers.
+template <int MODE, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_int_search(IntArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    Block b;
+    const int band = blockIdx.x / a.nbx, bandx = blockIdx.x - band * a.nbx, job = blockIdx.y;
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    b.ref = blockIdx.z;
+    b.cx = (ctu % P.ctu_cols) * 128;
+    b.cy = (ctu / P.ctu_cols) * 128;
+    b.dy0 = -a.R + band * a.band_rows;
+    b.dxs = -a.R + bandx * a.dx_cols;
+    b.dxe = b.dxs + a.dx_cols - 1 < a.hi ? b.dxs + a.dx_cols - 1 : a.hi;
+    const int wrows = 128 + a.band_rows - 1;
+    pel *win = (pel *)smem;
+    pel *curl = win + (size_t)wrows * a.wpitch;
+    const int tid = threadIdx.x, nthr = 64 * WAVES;
+
+    // Stage the window: reference samples (cx + dxs + c, cy + dy0 + r), clamped
+    // to the padded plane (it is edge-replicated, so clamping is exact), and
+    // the current CTU clamped to the picture.
+    const pel *ref0 = P.planes[b.ref] + (size_t)ME_PAD * P.ref_stride + ME_PAD;
+    const int xmin = -ME_PAD, xmax = P.pic_w + ME_PAD - 1, ymin = -ME_PAD, ymax = P.pic_h + ME_PAD - 1;
+    for (int i = tid; i < wrows * a.wcols; i += nthr) {
+        int r = i / a.wcols, c = i - r * a.wcols;
+        int x = me_clip3(xmin, xmax, b.cx + b.dxs + c), y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
+        win[r * a.wpitch + wskew(r) + c] = ref0[(ptrdiff_t)y * P.ref_stride + x];
+    }
+    for (int i = tid; i < 128 * 128; i += nthr) {
+        int r = i >> 7, c = i & 127;
+        int x = me_clip3(0, P.pic_w - 1, b.cx + c), y = me_clip3(0, P.pic_h - 1, b.cy + r);
+        curl[i] = P.cur[(size_t)y * P.pic_w + x];
+    }
+    __syncthreads();
+
+    const int lane = tid & 63;
+    const int rx = lane & 7, ry = lane >> 3;
+    const bool edge = !(b.cx + 124 + a.R <= P.pic_w - 1 && b.cy + 124 + a.R <= P.pic_h - 1 && a.R <= 127);
+    u64 *kb = a.keys + (size_t)job * a.npu * a.nref + b.ref;
+    // Interior blocks with sh == 0 (the common case) take the fast path;
+    // the others test each node's window and shift the SAD.
+    const bool fast = !edge && a.sh == 0;
+    if (MODE == 1) {
+        double st[1];
+        if (fast) run_pass<2, WAVES, false, true, 1>(a, P, b, win, curl, st);
+        else run_pass<2, WAVES, true, false, 1>(a, P, b, win, curl, st);
+        PUB(S16x16, 16, 16, rx * 16, ry * 16, st[0]);
+        return;
+    }
+    {
+        double st[N_SMALL];
+        if (fast) run_pass<0, WAVES, false, true, N_SMALL>(a, P, b, win, curl, st);
+        else run_pass<0, WAVES, true, false, N_SMALL>(a, P, b, win, curl, st);
+        publish_small(a, kb, st, rx * 16, ry * 16);
+    }
+    {
+        double st[N_LARGE];
+        if (fast) run_pass<1, WAVES, false, true, N_LARGE>(a, P, b, win, curl, st);
+        else run_pass<1, WAVES, true, false, N_LARGE>(a, P, b, win, curl, st);
+        publish_large(a, kb, st, rx, ry);
+    }
+}
+#undef PUB
+
+#define ME_IS_CAT2(a, b) a##b
+#define ME_IS_CAT(a, b) ME_IS_CAT2(a, b)
+int ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st)
+{
+    const void *fn = mode == 1 ? (const void *)k_int_search<1, ME_IS_WAVES> : (const void *)k_int_search<0, ME_IS_WAVES>;
+    if (lds > 65536) ME_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (mode == 1) hipLaunchKernelGGL((k_int_search<1, ME_IS_WAVES>), grid, dim3(64 * ME_IS_WAVES), lds, st, a);
+    else hipLaunchKernelGGL((k_int_search<0, ME_IS_WAVES>), grid, dim3(64 * ME_IS_WAVES), lds, st, a);
+    ME_CHECK(hipGetLastError());
+    return 0;
+}

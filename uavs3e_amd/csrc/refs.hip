@@ -1,0 +1,165 @@
+// Device reference pictures for batched ME (gfx950): edge padding and the
+// 15 quarter-pel planes of com_if_luma_frame, plus the PU shape API.
+#include "frame.h"
+
+#include <string.h>
+
+// ---------------------------------------------------------------------------
+// Reference pictures: padded integer plane + 15 quarter-pel planes
+// ---------------------------------------------------------------------------
+
+
+// Edge padding of the integer plane (com_img_padding(img, .., 0),
+// com_util.c:105-165): every padded sample copies the nearest picture sample.
+__global__ void k_pad_int(const pel *__restrict__ luma, int w, int h, pel *__restrict__ plane0, int stride, int rows)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    int Y = blockIdx.y;
+    if (X >= stride || Y >= rows) return;
+    int x = me_clip3(0, w - 1, X - ME_PAD);
+    int y = me_clip3(0, h - 1, Y - ME_PAD);
+    plane0[(size_t)Y * stride + X] = luma[(size_t)y * w + x];
+}
+
+// Horizontal s16 intermediates of com_if_filter_hor_8 (com_mc.c:668-702):
+// t = sum(taps[dx] * P0), stored as (t + add) >> (bd - 8), over
+// [-8, w+8) x [-8, h+8) for dx = 1..3.
+__global__ void k_htmp(const pel *__restrict__ p0, int stride, int w, int h, s16 *__restrict__ tmp)
+{
+    int tw = w + 16, th = h + 16;
+    int xi = blockIdx.x * blockDim.x + threadIdx.x;
+    int yi = blockIdx.y;
+    int dx = blockIdx.z + 1;
+    if (xi >= tw || yi >= th) return;
+    const pel *s = p0 + (ptrdiff_t)(yi - 8) * stride + (xi - 8);
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) t += (int)s[k - 3] * ME_LUMA_TAPS[dx][k];
+    const int sh = BIT_DEPTH - 8, add = (1 << sh) >> 1;
+    tmp[(size_t)(dx - 1) * tw * th + (size_t)yi * tw + xi] = (s16)((t + add) >> sh);
+}
+
+// The 15 sub-pel planes of com_if_luma_frame (com_mc.c:760-800) including
+// the final com_img_padding(plane, 1, 4): every padded sample equals the
+// filter output at its position clamped into the [-4, w+4) x [-4, h+4) band.
+//   dy == 0: clip((H8(P0) + 32) >> 6)              com_mc.c:668-702
+//   dx == 0: clip((V8(P0) + 32) >> 6)              com_mc.c:704-729
+//   else:    clip((V8(tmp) + 2^(19-bd)) >> (20-bd)) com_mc.c:731-758
+__global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h,
+                         const s16 *__restrict__ tmp)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    int Y = blockIdx.y;
+    int idx = blockIdx.z + 1;
+    if (X >= stride || Y >= rows) return;
+    int dx = idx & 3, dy = idx >> 2;
+    int x = me_clip3(-4, w + 3, X - ME_PAD);
+    int y = me_clip3(-4, h + 3, Y - ME_PAD);
+    const int maxv = (1 << BIT_DEPTH) - 1;
+    const pel *p0 = planes + (size_t)ME_PAD * stride + ME_PAD;
+    int v;
+    if (dy == 0) {
+        const pel *s = p0 + (ptrdiff_t)y * stride + x;
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[k - 3] * ME_LUMA_TAPS[dx][k];
+        v = (t + 32) >> 6;
+    } else if (dx == 0) {
+        const pel *s = p0 + (ptrdiff_t)y * stride + x;
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * stride] * ME_LUMA_TAPS[dy][k];
+        v = (t + 32) >> 6;
+    } else {
+        int tw = w + 16, th = h + 16;
+        const s16 *s = tmp + (size_t)(dx - 1) * tw * th + (size_t)(y + 8) * tw + (x + 8);
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * tw] * ME_LUMA_TAPS[dy][k];
+        const int sh2 = 20 - BIT_DEPTH, add2 = 1 << (sh2 - 1);
+        v = (t + add2) >> sh2;
+    }
+    planes[(size_t)idx * plane_elems + (size_t)Y * stride + X] = (pel)me_clip3(0, maxv, v);
+}
+
+int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
+{
+    dim3 b(256);
+    hipLaunchKernelGGL(k_pad_int, dim3((r->stride + 255) / 256, r->rows), b, 0, st, r->luma, r->w, r->h,
+                       r->planes, r->stride, r->rows);
+    hipLaunchKernelGGL(k_htmp, dim3((r->w + 16 + 255) / 256, r->h + 16, 3), b, 0, st,
+                       r->planes + (size_t)ME_PAD * r->stride + ME_PAD, r->stride, r->w, r->h, r->tmp);
+    hipLaunchKernelGGL(k_planes, dim3((r->stride + 255) / 256, r->rows, 15), b, 0, st, r->planes, r->plane_elems,
+                       r->stride, r->rows, r->w, r->h, r->tmp);
+    ME_CHECK(hipGetLastError());
+    return 0;
+}
+
+ME_EXPORT int me_ref_create(me_ref_t *out, int pic_w, int pic_h)
+{
+    if (me_hip_init(-1)) return -1;
+    if (!out || pic_w <= 0 || pic_h <= 0) { me_set_error("me_ref_create: bad size", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    me_ref_s *r = (me_ref_s *)calloc(1, sizeof(me_ref_s));
+    r->w = pic_w; r->h = pic_h;
+    r->stride = pic_w + 2 * ME_PAD;
+    r->rows = pic_h + 2 * ME_PAD;
+    r->plane_elems = (size_t)r->stride * r->rows;
+    hipError_t e = hipMalloc((void **)&r->planes, 16 * r->plane_elems * sizeof(pel));
+    if (e == hipSuccess) e = hipMalloc((void **)&r->luma, (size_t)pic_w * pic_h * sizeof(pel));
+    if (e == hipSuccess) e = hipMalloc((void **)&r->tmp, 3 * (size_t)(pic_w + 16) * (pic_h + 16) * sizeof(s16));
+    if (e != hipSuccess) {
+        me_set_error("me_ref_create: hipMalloc", e, __FILE__, __LINE__);
+        hipFree(r->planes); hipFree(r->luma); hipFree(r->tmp); free(r);
+        return -1;
+    }
+    *out = r;
+    return 0;
+}
+
+ME_EXPORT int me_ref_load(me_ref_t r, const me_pel *luma, int stride, int on_device, void *stream)
+{
+    if (!r || !luma || stride < r->w) { me_set_error("me_ref_load: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    hipStream_t st = (hipStream_t)stream;
+    ME_CHECK(hipMemcpy2DAsync(r->luma, r->w * sizeof(pel), luma, stride * sizeof(pel), r->w * sizeof(pel), r->h,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    if (me_ref_prepare_async(r, st)) return -1;
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+ME_EXPORT int me_ref_plane(me_ref_t r, int idx, me_pel **plane, int *stride, int *pad)
+{
+    if (!r || idx < 0 || idx > 15) return -1;
+    if (plane) *plane = r->planes + (size_t)idx * r->plane_elems + (size_t)ME_PAD * r->stride + ME_PAD;
+    if (stride) *stride = r->stride;
+    if (pad) *pad = ME_PAD;
+    return 0;
+}
+
+ME_EXPORT int me_ref_download_plane(me_ref_t r, int idx, me_pel *out)
+{
+    if (!r || !out || idx < 0 || idx > 15) { me_set_error("me_ref_download_plane: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    ME_CHECK(hipMemcpy(out, r->planes + (size_t)idx * r->plane_elems, r->plane_elems * sizeof(pel), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+ME_EXPORT void me_ref_destroy(me_ref_t r)
+{
+    if (!r) return;
+    hipFree(r->planes); hipFree(r->luma); hipFree(r->tmp);
+    free(r);
+}
+
+// ---------------------------------------------------------------------------
+// Shapes and PU numbering
+// ---------------------------------------------------------------------------
+
+ME_EXPORT int me_shape_w(int s) { return s >= 0 && s < ME_NSHAPES ? ME_SHAPES_HOST[s][0] : 0; }
+ME_EXPORT int me_shape_h(int s) { return s >= 0 && s < ME_NSHAPES ? ME_SHAPES_HOST[s][1] : 0; }
+ME_EXPORT int me_pus_per_ctu(uint32_t mask)
+{
+    int n = 0;
+    for (int s = 0; s < ME_NSHAPES; s++)
+        if (mask >> s & 1) n += (128 / ME_SHAPES_HOST[s][0]) * (128 / ME_SHAPES_HOST[s][1]);
+    return n;
+}

@@ -36,7 +36,8 @@ def kernel_stats(d):
             c = sqlite3.connect(f)
             for n, calls, tot, avg, pct in c.execute(
                     "select name, total_calls, total_duration, average, percentage from top_kernels"):
-                rows.append((n, int(calls), float(tot) / 1e3, float(avg) / 1e3, float(pct)))
+                # the rocpd view already reports microseconds
+                rows.append((n, int(calls), float(tot), float(avg), float(pct)))
     rows.sort(key=lambda r: -r[2])
     return rows
 
