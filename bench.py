@@ -48,6 +48,9 @@ def main():
                     help="after the timed region, run a plain byte copy of this many bytes each way "
                          "(three launches, torch bitwise_not) to scale HBM counters against")
     ap.add_argument("--no-percall", action="store_true", help="skip the per-call leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-frames", type=int, default=32, help="frames per stream in the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=40000.0, help="offered load of the serving leg, CTUs/s")
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--dx-cols", type=int, default=0)
@@ -198,6 +201,31 @@ def main():
                             "yes" if diff is None else "NO (%s)" % diff)}
         print("[bench] cpu baseline: %.2f CTUs/s (%s)" % (cpu["value"], kind), file=sys.stderr, flush=True)
 
+    # ---- serving leg: 6 concurrent streams, Poisson per-CTU requests ---------
+    serving = None
+    if not args.no_serve:
+        sizes = [(1280, 720)] * 3 + [(1920, 1080)] * 2 + [(3840, 2160)]
+        seeds = [0x40 + i for i in range(len(sizes))]
+        # 4 displacement rows per workgroup: short per-batch latency at light load
+        cfg = ua.ServerCfg(32, wl.lambda_mv(32), ua.MASK_16x16, 1, 64 << 30, 128, 4, 100, 4, 4, 0)
+        host8 = ua.MeHost(8)
+        lat, info, st, _ = host8.serve_load(cfg, sizes, seeds, args.serve_frames, args.serve_rate, arrival_seed=7)
+        lat_ms = lat / 1e3
+        serving = {"streams": "3x1280x720 + 2x1920x1080 + 1x3840x2160, 8-bit, %d frames each" % args.serve_frames,
+                   "job": "one CTU: 16x16 PUs, full search +-32, 1 reference (the previous frame), QP 32, sub-pel",
+                   "offered_ctus_per_s": args.serve_rate, "requests": info["requests"],
+                   "throughput_ctus_per_s": info["requests"] / (info["wall_us"] * 1e-6),
+                   "latency_ms": {"p50": pct(lat_ms, 50), "p99": pct(lat_ms, 99), "max": float(lat_ms.max())},
+                   "server": {"hbm_budget_gb": 64, "max_batch_ctus": 128, "max_inflight": 4, "max_wait_us": 100,
+                              "band_rows": 4,
+                              "batches": st["batches"], "mean_batch": info["requests"] / max(1, st["batches"]),
+                              "mixed_batches": st["mixed_batches"], "max_inflight_seen": st["max_inflight_seen"],
+                              "peak_picture_bytes_end": st["picture_bytes"]},
+                   "frame_synthesis_s": info["synth_s"]}
+        print("[bench] serving: p50 %.3f ms p99 %.3f ms, %.0f CTUs/s" % (
+            serving["latency_ms"]["p50"], serving["latency_ms"]["p99"], serving["throughput_ctus_per_s"]),
+            file=sys.stderr, flush=True)
+
     if args.calib_copy:
         src = torch.zeros(args.calib_copy, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
@@ -228,8 +256,11 @@ def main():
                    "pus_per_frame": int(valid[:, :, 0].sum()), "ctus_per_frame": n_ctu,
                    "step": "rebuild newest reference planes + full search + sub-pel + ref choice, one hipGraph"},
         "candidate_evals_per_s": cand_per_s,
-        "ctu_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99),
-                           "note": "batched: every CTU of a frame completes with the frame"},
+        "ctu_latency_ms": ({"p50": serving["latency_ms"]["p50"], "p99": serving["latency_ms"]["p99"],
+                            "note": "serving leg: per-CTU request arrival to result on the host"}
+                           if serving else None),
+        "frame_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99),
+                             "note": "batched: every CTU of a frame completes with the frame"},
         "stage_ms": {"prepare_ref": stage[0], "fill": stage[1], "int_search": stage[2], "subpel": stage[3]},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
@@ -237,6 +268,7 @@ def main():
                      "valu": {"ops": valu_ops, "frac": valu_frac, "peak_lane_ops_per_s": VALU_PEAK_OPS}},
         "per_call": per_call,
         "speedup_vs_per_call": (ctus_per_s / per_call["value"]) if per_call else None,
+        "serving": serving,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
     }

@@ -207,6 +207,71 @@ int me_plan_num_ctus(me_plan_t plan);
 int me_plan_num_pus(me_plan_t plan);   /* per CTU */
 void me_plan_destroy(me_plan_t plan);
 
+/* ------------------------------------------------------------------------ */
+/* 3. Request server (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------------ */
+
+/* Continuous batching of per-CTU ME requests from several concurrent
+ * streams of different resolutions.  Pictures stay resident on the device
+ * (integer plane + 15 quarter-pel planes each) and are admitted against an
+ * HBM budget; requests are gathered into batches of up to max_batch_ctus
+ * CTUs from any stream, up to max_inflight batches run on the device at
+ * once on their own streams, and a request that arrives while batches run
+ * joins the next batch as soon as a batch slot frees up.  Every request's
+ * result is the same as for the frame API (and the same whatever else is in
+ * its batch).  All entry points are thread-safe. */
+typedef struct me_server_cfg_t {
+    int search_range;       /* R of the integer full search (the same for every stream) */
+    uint32_t lambda_mv;     /* floor(65536*sqrt(lambda)) (uavs3e.c:851) */
+    uint32_t shape_mask;    /* PU shapes searched in each CTU (29-shape list) */
+    int num_refs;           /* references per request, 1..4 */
+    uint64_t hbm_budget;    /* device bytes pictures may hold in total */
+    int max_batch_ctus;     /* CTUs per batch, 1..65535 */
+    int max_inflight;       /* batches on the device at once, 1..8 */
+    int max_wait_us;        /* while other batches run, a partial batch waits at most this long to fill */
+    int band_rows, waves, dx_cols;  /* search schedule, as in me_frame_cfg_t (0 = default) */
+} me_server_cfg_t;
+
+typedef struct me_server_stats_t {
+    uint64_t requests_done;
+    uint64_t batches;
+    uint64_t max_batch_seen;      /* largest batch launched */
+    uint64_t max_inflight_seen;   /* most batches on the device at once */
+    uint64_t mixed_batches;       /* batches holding CTUs of more than one stream */
+    uint64_t picture_bytes;       /* device bytes held by pictures now (live + pooled) */
+    uint64_t admission_waits;     /* me_server_put_picture calls that had to wait for budget */
+} me_server_stats_t;
+
+typedef struct me_server_s *me_server_t;
+
+int me_server_create(me_server_t *out, const me_server_cfg_t *cfg);
+/* Registers a stream of pic_w x pic_h pictures; *stream_id receives its id. */
+int me_server_open_stream(me_server_t srv, int pic_w, int pic_h, int *stream_id);
+/* Makes picture `poc` of a stream resident: uploads its luma (host memory,
+ * `stride` samples per row) and builds its padded planes.  Waits while the
+ * HBM budget is exhausted, up to timeout_ms (-1: no limit).  Returns 0 when
+ * resident, 1 when the budget did not free up in time, negative on error. */
+int me_server_put_picture(me_server_t srv, int stream_id, int64_t poc, const me_pel *luma, int stride, int timeout_ms);
+/* No further requests will name this picture; its memory returns to the
+ * budget once the requests already submitted with it have completed. */
+int me_server_release_picture(me_server_t srv, int stream_id, int64_t poc);
+/* Submits ME of CTU `ctu` (raster index) of picture cur_poc against the
+ * pictures ref_pocs[0..num_refs-1] of the same stream.  On completion
+ * res[pu * num_refs + r] and best_ref[pu] (host memory that must stay valid
+ * until then; PU numbering as in the frame API) are written.  Returns a
+ * ticket (> 0) or a negative value on error. */
+int64_t me_server_submit(me_server_t srv, int stream_id, int64_t cur_poc, int ctu, const int64_t *ref_pocs,
+                         me_pu_result_t *res, int8_t *best_ref);
+/* Waits for a ticket up to timeout_ms (-1: no limit).  Returns 0 when done
+ * (the ticket is then retired), 1 on timeout.  done_us (optional) receives
+ * the completion time on the me_server_now_us clock. */
+int me_server_wait(me_server_t srv, int64_t ticket, int timeout_ms, double *done_us);
+/* Monotonic clock of the server in microseconds (CLOCK_MONOTONIC). */
+double me_server_now_us(void);
+int me_server_stats(me_server_t srv, me_server_stats_t *stats);
+/* Waits for every submitted request, then frees the server and its pictures. */
+void me_server_destroy(me_server_t srv);
+
 #ifdef __cplusplus
 }
 #endif

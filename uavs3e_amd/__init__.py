@@ -74,6 +74,19 @@ class FrameCfg(C.Structure):
                 ("dx_cols", C.c_int)]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("search_range", C.c_int), ("lambda_mv", C.c_uint32), ("shape_mask", C.c_uint32),
+                ("num_refs", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch_ctus", C.c_int),
+                ("max_inflight", C.c_int), ("max_wait_us", C.c_int), ("band_rows", C.c_int), ("waves", C.c_int),
+                ("dx_cols", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("batches", C.c_uint64), ("max_batch_seen", C.c_uint64),
+                ("max_inflight_seen", C.c_uint64), ("mixed_batches", C.c_uint64), ("picture_bytes", C.c_uint64),
+                ("admission_waits", C.c_uint64)]
+
+
 class MeError(RuntimeError):
     pass
 
@@ -118,6 +131,16 @@ class MeHip:
             getattr(L, n).argtypes = None
         L.me_plan_destroy.argtypes = [C.c_void_p]
         L.me_ref_destroy.argtypes = [C.c_void_p]
+        L.me_server_create.argtypes = [C.c_void_p, C.c_void_p]
+        L.me_server_open_stream.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.me_server_put_picture.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_int, C.c_int]
+        L.me_server_release_picture.argtypes = [C.c_void_p, C.c_int, C.c_int64]
+        L.me_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.me_server_submit.restype = C.c_int64
+        L.me_server_wait.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p]
+        L.me_server_now_us.restype = C.c_double
+        L.me_server_stats.argtypes = [C.c_void_p, C.c_void_p]
+        L.me_server_destroy.argtypes = [C.c_void_p]
         assert L.me_hip_bit_depth() == bd
         if init:
             self.check(L.me_hip_init(0), "me_hip_init")
@@ -232,6 +255,82 @@ class MeHip:
         self.lib.me_plan_destroy(plan)
 
 
+def server_picture_bytes(w, h, bd):
+    """Device bytes one resident picture takes in the server's HBM budget:
+    16 padded planes, the unpadded luma (+64 B read slack) and three s16
+    filter intermediates over (w+16) x (h+16)."""
+    pel = 1 if bd == 8 else 2
+    return 16 * (w + 2 * PAD) * (h + 2 * PAD) * pel + w * h * pel + 64 + 3 * (w + 16) * (h + 16) * 2
+
+
+class MeServer:
+    """The request server of the HIP library (continuous batching of per-CTU
+    ME requests from several streams).  Result buffers handed to submit() are
+    kept alive here until their ticket is waited for."""
+
+    def __init__(self, hip, R, lambda_mv, mask, num_refs, hbm_budget=64 << 30, max_batch_ctus=64,
+                 max_inflight=3, max_wait_us=200, band_rows=0, waves=0, dx_cols=0):
+        self.hip = hip
+        self.lib = hip.lib
+        self.num_refs = num_refs
+        self.npu = sum((128 // w) * (128 // h) for s, (w, h) in enumerate(SHAPES) if mask >> s & 1)
+        cfg = ServerCfg(R, lambda_mv, mask, num_refs, hbm_budget, max_batch_ctus, max_inflight, max_wait_us,
+                        band_rows, waves, dx_cols)
+        self.srv = C.c_void_p()
+        hip.check(self.lib.me_server_create(C.byref(self.srv), C.byref(cfg)), "me_server_create")
+        self._bufs = {}
+
+    def open_stream(self, w, h):
+        sid = C.c_int()
+        self.hip.check(self.lib.me_server_open_stream(self.srv, w, h, C.byref(sid)), "me_server_open_stream")
+        return sid.value
+
+    def put_picture(self, sid, poc, luma, timeout_ms=-1):
+        """True when resident, False when the HBM budget did not free up in time."""
+        luma = np.ascontiguousarray(luma, self.hip.dt)
+        rc = self.lib.me_server_put_picture(self.srv, sid, poc, _p(luma), luma.shape[1], timeout_ms)
+        if rc < 0:
+            self.hip.check(rc, "me_server_put_picture")
+        return rc == 0
+
+    def release_picture(self, sid, poc):
+        self.hip.check(self.lib.me_server_release_picture(self.srv, sid, poc), "me_server_release_picture")
+
+    def submit(self, sid, cur_poc, ctu, ref_pocs):
+        res = np.zeros((self.npu, self.num_refs), PU_RESULT)
+        best = np.zeros(self.npu, np.int8)
+        refs = (C.c_int64 * self.num_refs)(*ref_pocs)
+        t = self.lib.me_server_submit(self.srv, sid, cur_poc, ctu, refs, _p(res), _p(best))
+        if t <= 0:
+            raise MeError("me_server_submit failed: %s" % self.lib.me_hip_last_error().decode())
+        self._bufs[t] = (res, best)
+        return t
+
+    def wait(self, ticket, timeout_ms=-1):
+        """(res, best, done_us) of a ticket, or None on timeout."""
+        done = C.c_double()
+        rc = self.lib.me_server_wait(self.srv, ticket, timeout_ms, C.byref(done))
+        if rc == 1:
+            return None
+        res, best = self._bufs.pop(ticket)
+        if rc != 0:
+            raise MeError("request %d failed" % ticket)
+        return res, best, done.value
+
+    def now_us(self):
+        return self.lib.me_server_now_us()
+
+    def stats(self):
+        st = ServerStats()
+        self.hip.check(self.lib.me_server_stats(self.srv, C.byref(st)), "me_server_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.srv:
+            self.lib.me_server_destroy(self.srv)
+            self.srv = None
+
+
 class MeHost:
     """Host C helpers: synthetic frames and the table-driven ME driver."""
 
@@ -257,6 +356,8 @@ class MeHost:
                                     C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
         L.me_host_if_luma_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.me_host_serve_load.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                                         C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
 
     def synth(self, seed, stream, frame, w, h, sx=0.0, sy=0.0, region_qpel=0, sigma=0.0, noise_purpose=3,
               offset=0.0):
@@ -275,6 +376,29 @@ class MeHost:
         out = np.zeros(2, np.int32)
         self.lib.me_synth_region_offset(seed, stream, frame, w, region_qpel, x, y, _p(out))
         return tuple(int(v) for v in out)
+
+    def serve_load(self, cfg, sizes, seeds, frames, rate, arrival_seed=1, keep_results=False):
+        """Drives a request server (created from the ServerCfg `cfg`) with
+        one uploader/submitter/collector thread set per stream (host C,
+        csrc/serve_load.c).  Returns (latency_us per request, info dict,
+        server stats dict, (res, best) or None)."""
+        wh = np.array([v for s in sizes for v in s], np.int32)
+        sd = np.array(seeds, np.uint64)
+        n = sum(((w + 127) // 128) * ((h + 127) // 128) for w, h in sizes) * (frames - 1)
+        npu = sum((128 // sw) * (128 // sh) for s, (sw, sh) in enumerate(SHAPES) if cfg.shape_mask >> s & 1)
+        lat = np.zeros(n, np.float64)
+        info = np.zeros(3, np.float64)
+        st = ServerStats()
+        res = np.zeros((n, npu, cfg.num_refs), PU_RESULT) if keep_results else None
+        best = np.zeros((n, npu), np.int8) if keep_results else None
+        rc = self.lib.me_host_serve_load(C.byref(cfg), len(sizes), _p(wh), _p(sd), frames, rate, arrival_seed,
+                                         _p(lat), _p(res) if keep_results else None,
+                                         _p(best) if keep_results else None, _p(info), C.byref(st))
+        if rc:
+            raise MeError("me_host_serve_load failed")
+        stats = {f: getattr(st, f) for f, _ in ServerStats._fields_}
+        return lat, {"wall_us": info[0], "requests": int(info[1]), "synth_s": info[2]}, stats, \
+            ((res, best) if keep_results else None)
 
     def planes_via_table(self, table, luma, pad=PAD):
         """com_if_luma_frame through a dispatch table: (16, H+2pad, W+2pad)."""

@@ -42,7 +42,7 @@ def host_lib(bd):
 # Translation units of the HIP library: (source, extra defines).  The
 # integer-search and sub-pel kernels are split by instance so that the
 # expensive device compiles run in parallel.
-HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []),
+HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []), ("server.hip", []),
              ("intsearch.hip", ["-DME_IS_WAVES=1"]), ("intsearch.hip", ["-DME_IS_WAVES=2"]),
              ("intsearch.hip", ["-DME_IS_WAVES=4"]),
              ("subpel.hip", ["-DME_SP_PART=0"]), ("subpel.hip", ["-DME_SP_PART=1"]),
@@ -71,12 +71,14 @@ def _link_hip(bd, objs):
 
 
 def _build_host(bd):
-    srcs = [os.path.join(CSRC, f) for f in ("me_host.c", "synth.c")]
-    srcs = [s for s in srcs if os.path.exists(s)]
+    """Host C: the per-call ME driver, the synthetic generator and the
+    serving load (which calls the HIP library's C ABI, so it links it)."""
+    srcs = [os.path.join(CSRC, f) for f in ("me_host.c", "synth.c", "serve_load.c")]
     out = host_lib(bd)
     tmp = out + ".tmp%d" % os.getpid()
     _run(["gcc", "-O2", "-std=gnu99", "-fPIC", "-shared", "-ffp-contract=off", "-DBIT_DEPTH=%d" % bd,
-          "-I" + os.path.join(ROOT, "include")] + srcs + ["-o", tmp, "-lm", "-lpthread"])
+          "-I" + os.path.join(ROOT, "include")] + srcs +
+         ["-o", tmp, "-L" + LIB, "-luavs3e_me_hip%d" % bd, "-Wl,-rpath,$ORIGIN", "-lm", "-lpthread"])
     os.replace(tmp, out)
     return out
 
@@ -87,9 +89,11 @@ def build(jobs=None, bit_depths=(8, 10)):
     jobs = jobs or max(1, min(16, os.cpu_count() or 1))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         units = {bd: [ex.submit(_compile_unit, bd, src, defs) for src, defs in HIP_UNITS] for bd in bit_depths}
-        hosts = [ex.submit(_build_host, bd) for bd in bit_depths]
         links = [ex.submit(_link_hip, bd, [f.result() for f in units[bd]]) for bd in bit_depths]
-        for f in hosts + links:
+        for f in links:
+            f.result()
+        hosts = [ex.submit(_build_host, bd) for bd in bit_depths]   # links the HIP library
+        for f in hosts:
             f.result()
 
 

@@ -33,6 +33,8 @@ typedef int64_t s64;
 // Error reporting: every HIP call is checked; the library records the last
 // error string and returns a negative code from its C entry points.
 void me_set_error(const char *what, hipError_t e, const char *file, int line);
+// The device selected by me_hip_init (threads the library starts select it).
+int me_hip_device(void);
 #define ME_CHECK(call)                                                        \
     do {                                                                      \
         hipError_t e_ = (call);                                               \

@@ -265,8 +265,12 @@ ME_EXPORT int me_plan_set_cur(me_plan_t p, const me_pel *cur, int stride, int on
 {
     if (!p || !cur || stride < p->cfg.pic_w) { me_set_error("me_plan_set_cur: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
     hipStream_t st = stream ? (hipStream_t)stream : p->own;
-    ME_CHECK(hipMemcpy2DAsync(p->d_cur, p->cfg.pic_w * sizeof(pel), cur, stride * sizeof(pel), p->cfg.pic_w * sizeof(pel),
-                              p->cfg.pic_h, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (stride == p->cfg.pic_w)   // one copy: a 2-D copy from pageable memory goes row by row
+        ME_CHECK(hipMemcpyAsync(p->d_cur, cur, (size_t)p->cfg.pic_w * p->cfg.pic_h * sizeof(pel), kind, st));
+    else
+        ME_CHECK(hipMemcpy2DAsync(p->d_cur, p->cfg.pic_w * sizeof(pel), cur, stride * sizeof(pel),
+                                  p->cfg.pic_w * sizeof(pel), p->cfg.pic_h, kind, st));
     ME_CHECK(hipStreamSynchronize(st));
     return 0;
 }

@@ -34,6 +34,7 @@ void me_set_error(const char *what, hipError_t e, const char *file, int line)
 }
 
 ME_EXPORT const char *me_hip_last_error(void) { return t_err; }
+int me_hip_device(void) { return g_device; }
 ME_EXPORT int me_hip_bit_depth(void) { return BIT_DEPTH; }
 ME_EXPORT uint64_t me_hip_error_count(void) { return g_errors.load(); }
 ME_EXPORT void me_hip_set_fallback(int enable) { g_fallback.store(enable ? 1 : 0); }

@@ -105,7 +105,8 @@ ME_EXPORT int me_ref_create(me_ref_t *out, int pic_w, int pic_h)
     r->rows = pic_h + 2 * ME_PAD;
     r->plane_elems = (size_t)r->stride * r->rows;
     hipError_t e = hipMalloc((void **)&r->planes, 16 * r->plane_elems * sizeof(pel));
-    if (e == hipSuccess) e = hipMalloc((void **)&r->luma, (size_t)pic_w * pic_h * sizeof(pel));
+    // +64 B: the unpadded luma doubles as a current picture, read with dword row loads
+    if (e == hipSuccess) e = hipMalloc((void **)&r->luma, (size_t)pic_w * pic_h * sizeof(pel) + 64);
     if (e == hipSuccess) e = hipMalloc((void **)&r->tmp, 3 * (size_t)(pic_w + 16) * (pic_h + 16) * sizeof(s16));
     if (e != hipSuccess) {
         me_set_error("me_ref_create: hipMalloc", e, __FILE__, __LINE__);
@@ -120,8 +121,12 @@ ME_EXPORT int me_ref_load(me_ref_t r, const me_pel *luma, int stride, int on_dev
 {
     if (!r || !luma || stride < r->w) { me_set_error("me_ref_load: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
     hipStream_t st = (hipStream_t)stream;
-    ME_CHECK(hipMemcpy2DAsync(r->luma, r->w * sizeof(pel), luma, stride * sizeof(pel), r->w * sizeof(pel), r->h,
-                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (stride == r->w)   // one copy: a 2-D copy from pageable memory goes row by row
+        ME_CHECK(hipMemcpyAsync(r->luma, luma, (size_t)r->w * r->h * sizeof(pel), kind, st));
+    else
+        ME_CHECK(hipMemcpy2DAsync(r->luma, r->w * sizeof(pel), luma, stride * sizeof(pel), r->w * sizeof(pel), r->h,
+                                  kind, st));
     if (me_ref_prepare_async(r, st)) return -1;
     ME_CHECK(hipStreamSynchronize(st));
     return 0;
