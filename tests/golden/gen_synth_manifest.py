@@ -16,7 +16,7 @@ sys.path.insert(0, ROOT)
 
 from uavs3e_amd import workloads as wl  # noqa: E402
 
-SIZES = {"cfg1": None, "cfg2": None, "cfg3": (640, 360), "cfg5": (640, 360)}
+SIZES = {"hd_16x16": None, "hd_allpu": None, "uhd10_allpu": (640, 360), "fuhd10_bi": (640, 360)}
 
 
 def digests():

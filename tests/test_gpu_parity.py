@@ -128,7 +128,7 @@ CASES = [
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "bd%d_%dx%d_R%d_m%x_n%d" % c)
 def test_frame_plan_vs_oracle(case):
     bd, w, h, R, mask, nref = case
-    base = wl.cfg2() if bd == 8 else wl.cfg3()
+    base = wl.hd_allpu() if bd == 8 else wl.uhd10_allpu()
     work = base.scaled(w, h, R)
     work.ref_shifts_qpel = work.ref_shifts_qpel[:nref]
     cur, refs = work.frames()
@@ -161,7 +161,7 @@ def test_frame_plan_high_contrast(bd, nref):
 
 def test_plan_tuning_variants_agree():
     """Band heights and waves per block change only the schedule."""
-    work = wl.cfg2().scaled(256, 200, 12)
+    work = wl.hd_allpu().scaled(256, 200, 12)
     cur, refs = work.frames()
     hip = ua.MeHip(8)
     lam = wl.lambda_mv(32)
@@ -173,7 +173,7 @@ def test_plan_tuning_variants_agree():
 
 def test_host_driver_percall_vs_oracle():
     """The per-call drop-in path: the host driver calling HIP table entries."""
-    work = wl.cfg1().scaled(144, 136, 3)
+    work = wl.hd_16x16().scaled(144, 136, 3)
     cur, refs = work.frames()
     hip, orc, host = ua.MeHip(8), Oracle(8), ua.MeHost(8)
     t = hip.dispatch_table()

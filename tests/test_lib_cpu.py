@@ -68,7 +68,7 @@ def test_synth_sin_accuracy():
 def test_synth_manifest():
     with open(os.path.join(ROOT, "tests", "golden", "synth_manifest.json")) as f:
         man = json.load(f)
-    sizes = {"cfg1": None, "cfg2": None, "cfg3": (640, 360), "cfg5": (640, 360)}
+    sizes = {"hd_16x16": None, "hd_allpu": None, "uhd10_allpu": (640, 360), "fuhd10_bi": (640, 360)}
     for name, size in sizes.items():
         w = wl.ALL[name]()
         if size:
@@ -84,7 +84,7 @@ def test_workload_motion_is_recovered():
     (global (+13, -6) quarter-pel plus the per-64x64 offset): the errors are
     centred on zero and within one pel for most PUs (the field is
     smooth and noisy, so sub-pel precision is limited)."""
-    w = wl.cfg1().scaled(256, 256, 12)
+    w = wl.hd_16x16().scaled(256, 256, 12)
     cur, refs = w.frames()
     o = Oracle(8)
     res, best = o.me_frame(cur, [o.make_planes(refs[0])], w.R, w.lam, ua.MASK_16x16)

@@ -3,10 +3,10 @@
 Each workload is generated from its seed by the host generator
 (csrc/synth.c); the same seed gives the same samples on any machine.
 
-  cfg1  1920x1080 8-bit, 1 reference, 16x16 PUs, full search +-32, QP 32
-  cfg2  1920x1080 8-bit, 2 references, all 29 PU shapes, +-64      (bench line)
-  cfg3  3840x2160 10-bit, 4 references (POC distance 1,2,4,8), 29 shapes, +-128
-  cfg5  7680x4320 10-bit, 2 L0 + 2 L1 references, 29 shapes, +-64
+  hd_16x16    1920x1080 8-bit, 1 reference, 16x16 PUs, full search +-32, QP 32
+  hd_allpu    1920x1080 8-bit, 2 references, all 29 PU shapes, +-64      (bench line)
+  uhd10_allpu 3840x2160 10-bit, 4 references (POC distance 1,2,4,8), 29 shapes, +-128
+  fuhd10_bi   7680x4320 10-bit, 2 L0 + 2 L1 references, 29 shapes, +-64
 
 Motion: the current frame samples the field at (x, y) + global + region
 offset; the global motion is (+13, -6) quarter-pel per frame and every
@@ -83,29 +83,29 @@ def _second_ref_shift(seed):
     return (int(host.uniform(k, 0) * 17) - 8, int(host.uniform(k, 1) * 17) - 8)
 
 
-def cfg1():
-    return Workload("cfg1_1080p_16x16_r32", 1920, 1080, 8, 32, MASK_16x16, ref_shifts_qpel=[(0, 0)],
+def hd_16x16():
+    return Workload("1080p8_16x16_1ref_r32", 1920, 1080, 8, 32, MASK_16x16, ref_shifts_qpel=[(0, 0)],
                     cur_shift_qpel=GLOBAL_QPEL, seed=1)
 
 
-def cfg2():
+def hd_allpu():
     s = _second_ref_shift(2)
-    return Workload("cfg2_1080p_allpu_2ref_r64", 1920, 1080, 8, 64, MASK_ALL,
+    return Workload("1080p8_allpu_2ref_r64", 1920, 1080, 8, 64, MASK_ALL,
                     ref_shifts_qpel=[(0, 0), (-s[0], -s[1])], cur_shift_qpel=GLOBAL_QPEL, seed=2)
 
 
-def cfg3():
+def uhd10_allpu():
     gx, gy = GLOBAL_QPEL
-    return Workload("cfg3_2160p10_allpu_4ref_r128", 3840, 2160, 10, 128, MASK_ALL,
+    return Workload("2160p10_allpu_4ref_r128", 3840, 2160, 10, 128, MASK_ALL,
                     ref_shifts_qpel=[((8 - d) * gx, (8 - d) * gy) for d in (1, 2, 4, 8)],
                     cur_shift_qpel=(8 * gx, 8 * gy), sigma_ref=12.0, sigma_cur=12.0, seed=3)
 
 
-def cfg5():
+def fuhd10_bi():
     gx, gy = GLOBAL_QPEL
-    return Workload("cfg5_4320p10_allpu_l0l1_r64", 7680, 4320, 10, 64, MASK_ALL,
+    return Workload("4320p10_allpu_l0l1_r64", 7680, 4320, 10, 64, MASK_ALL,
                     ref_shifts_qpel=[(o * gx, o * gy) for o in (-1, -2, 1, 2)],
                     cur_shift_qpel=(0, 0), sigma_ref=12.0, sigma_cur=12.0, seed=5, drift=True)
 
 
-ALL = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg5": cfg5}
+ALL = {"hd_16x16": hd_16x16, "hd_allpu": hd_allpu, "uhd10_allpu": uhd10_allpu, "fuhd10_bi": fuhd10_bi}
