@@ -712,20 +712,25 @@ EXPORT int orc_pu_res_size(void) { return (int)sizeof(orc_pu_res_t); }
 
 /* Full search + sub-pel refinement for every PU of CTUs [ctu0, ctu1) and every
  * reference.  MVP = (0,0) for every PU; PUs not fully inside the picture are
- * skipped (valid = 0).  Results are indexed [ctu][pu][ref]; best_ref[ctu][pu]
- * is the reference with the smallest final u32 cost, ties to the lower index
- * (inter.c:901-904), or -1 for a skipped PU.
+ * skipped (valid = 0).  Results are indexed [ctu][pu][ref].
+ * References [0, num_refs - num_refs_l1) form list 0, the rest list 1; the
+ * final cost of reference r uses tbl_refi_bits of its list's size and its
+ * index in the list (me.c:560-561 with num_refp of that list).
+ * best_ref[(ctu*npu + pu)*nl + l] (nl = 1 without list 1, else 2) is the
+ * reference of list l with the smallest final u32 cost, ties to the lower
+ * index (inter.c:901-904), or -1 for a skipped PU.
  *   ref_planes: num_refs x 16 plane pointers, each at (0,0) of a padded plane. */
-EXPORT void orc_me_frame(int pic_w, int pic_h, int bd, const pel *cur, int i_cur,
-                         int num_refs, const pel *const *ref_planes, int i_ref,
-                         int R, u32 lambda_mv, int cost_type, u32 shape_mask,
-                         int ctu0, int ctu1, orc_pu_res_t *res, s8 *best_ref)
+EXPORT void orc_me_frame_lists(int pic_w, int pic_h, int bd, const pel *cur, int i_cur,
+                               int num_refs, int num_refs_l1, const pel *const *ref_planes, int i_ref,
+                               int R, u32 lambda_mv, int cost_type, u32 shape_mask,
+                               int ctu0, int ctu1, orc_pu_res_t *res, s8 *best_ref)
 {
     int off = orc_mvbits_offset(pic_w, pic_h);
     u8 *tab = (u8 *)malloc((size_t)(2 * off + 1));
     orc_mvbits_fill(tab, off);
     int ctu_cols = (pic_w + 127) / 128;
     int npu = orc_pus_per_ctu(shape_mask);
+    const int n0 = num_refs - num_refs_l1, nl = num_refs_l1 > 0 ? 2 : 1;
     s16 mvp[2] = {0, 0};
     for (int ctu = ctu0; ctu < ctu1; ctu++) {
         int cx = (ctu % ctu_cols) * 128, cy = (ctu / ctu_cols) * 128;
@@ -738,8 +743,8 @@ EXPORT void orc_me_frame(int pic_w, int pic_h, int bd, const pel *cur, int i_cur
                     int x = cx + ox, y = cy + oy;
                     size_t base = ((size_t)ctu * npu + pu);
                     int inside = x + w <= pic_w && y + h <= pic_h;
-                    u32 best_cost = 0xFFFFFFFFu;
-                    s8 bref = -1;
+                    u32 best_cost[2] = {0xFFFFFFFFu, 0xFFFFFFFFu};
+                    s8 bref[2] = {-1, -1};
                     for (int r = 0; r < num_refs; r++) {
                         orc_pu_res_t *o = &res[base * num_refs + r];
                         memset(o, 0, sizeof(*o));
@@ -761,13 +766,155 @@ EXPORT void orc_me_frame(int pic_w, int pic_h, int bd, const pel *cur, int i_cur
                         u64 cs = orc_subpel(&c, mvp, x, y, w, h, mv);
                         o->mv_sub[0] = mv[0]; o->mv_sub[1] = mv[1];
                         o->cost_sub = cs;
-                        o->cost_final = orc_final_cost(cs, lambda_mv, num_refs, r, 0);
+                        const int l = r < n0 ? 0 : 1;
+                        o->cost_final = l == 0 ? orc_final_cost(cs, lambda_mv, n0, r, 0)
+                                               : orc_final_cost(cs, lambda_mv, num_refs_l1, r - n0, 0);
                         o->valid = 1;
-                        if (o->cost_final < best_cost) { best_cost = o->cost_final; bref = (s8)r; }
+                        if (o->cost_final < best_cost[l]) { best_cost[l] = o->cost_final; bref[l] = (s8)r; }
                     }
-                    best_ref[base] = bref;
+                    for (int l = 0; l < nl; l++) best_ref[base * nl + l] = bref[l];
                 }
         }
     }
     free(tab);
+}
+
+/* The single-list form: every reference in list 0. */
+EXPORT void orc_me_frame(int pic_w, int pic_h, int bd, const pel *cur, int i_cur,
+                         int num_refs, const pel *const *ref_planes, int i_ref,
+                         int R, u32 lambda_mv, int cost_type, u32 shape_mask,
+                         int ctu0, int ctu1, orc_pu_res_t *res, s8 *best_ref)
+{
+    orc_me_frame_lists(pic_w, pic_h, bd, cur, i_cur, num_refs, 0, ref_planes, i_ref, R, lambda_mv, cost_type,
+                       shape_mask, ctu0, ctu1, res, best_ref);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Motion compensation                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* com_mc_blk_luma with hp_flag 0 (com_mc.c:41-87): quarter-pel position ->
+ * copy of w x h samples from plane [qy&3][qx&3] at the integer position
+ * clipped to [-MAX_CU_SIZE-4, pic+4] (max_posx = pic_w + 4, com_mc.c:144). */
+static void mc_luma(const pel *const *planes, int i_ref, int pic_w, int pic_h, int qx, int qy, int w, int h,
+                    pel *dst, int i_dst)
+{
+    int ix = CLIP3(-128 - 4, pic_w + 4, qx >> 2);
+    int iy = CLIP3(-128 - 4, pic_h + 4, qy >> 2);
+    orc_ipcpy(planes[(qy & 3) * 4 + (qx & 3)] + iy * i_ref + ix, i_ref, dst, i_dst, w, h);
+}
+
+/* com_mc_blk_chroma with hp_flag 0 (com_mc.c:103-138): the luma quarter-pel
+ * position is a 1/8-pel chroma position; the integer part is clipped to
+ * [-(MAX_CU_SIZE/2)-2, (pic+4)/2] and the 4-tap filters of com_tbl_mc_c_coeff
+ * run on the padded chroma plane (copy, H_4, V_4 or EXT_4). */
+static void mc_chroma(const pel *plane, int i_c, int pic_w, int pic_h, int qx, int qy, int w, int h,
+                      pel *dst, int i_dst, int maxv)
+{
+    int dx = qx & 7, dy = qy & 7;
+    int ix = CLIP3(-64 - 2, (pic_w + 4) >> 1, qx >> 3);
+    int iy = CLIP3(-64 - 2, (pic_h + 4) >> 1, qy >> 3);
+    const pel *src = plane + iy * i_c + ix;
+    if (dx == 0 && dy == 0) orc_ipcpy(src, i_c, dst, i_dst, w, h);
+    else if (dy == 0) orc_ipflt(1, src, i_c, dst, i_dst, w, h, CHROMA_TAPS[dx], maxv);
+    else if (dx == 0) orc_ipflt(3, src, i_c, dst, i_dst, w, h, CHROMA_TAPS[dy], maxv);
+    else orc_ipflt_ext(1, src, i_c, dst, i_dst, w, h, CHROMA_TAPS[dx], CHROMA_TAPS[dy], maxv);
+}
+
+/* com_mc_cu (com_mc.c:140-200) for one PU at (x, y), w x h, from nl lists:
+ * list l predicts from luma planes planes_l[l] (16 plane pointers at (0,0),
+ * stride i_ref) and chroma planes u_l[l], v_l[l] (padded, at (0,0), stride
+ * i_c) at quarter-pel MV mv[l]; two lists are averaged with pel_avrg,
+ * (a + b + 1) >> 1 (com_util.c:3088-3105).  When both lists name the same
+ * picture with the same MV the reference keeps the L0 prediction, which the
+ * average reproduces ((a + a + 1) >> 1 = a).  Chroma (4:2:0, (w/2) x (h/2))
+ * only when do_chroma.  Outputs are packed (stride w, resp. w/2). */
+EXPORT void orc_mc_pu(int pic_w, int pic_h, int bd, int nl, const pel *const *const *planes_l, int i_ref,
+                      const pel *const *u_l, const pel *const *v_l, int i_c, int x, int y, int w, int h,
+                      const s16 *mv, int do_chroma, pel *pred_y, pel *pred_u, pel *pred_v)
+{
+    const int maxv = (1 << bd) - 1;
+    pel *t0 = (pel *)malloc(sizeof(pel) * 128 * 128), *t1 = (pel *)malloc(sizeof(pel) * 128 * 128);
+    for (int comp = 0; comp < (do_chroma ? 3 : 1); comp++) {
+        int cw = comp ? w / 2 : w, ch = comp ? h / 2 : h;
+        pel *out = comp == 0 ? pred_y : (comp == 1 ? pred_u : pred_v);
+        for (int l = 0; l < nl; l++) {
+            int qx = (x << 2) + mv[2 * l], qy = (y << 2) + mv[2 * l + 1];
+            pel *dst = l == 0 ? t0 : t1;
+            if (comp == 0) mc_luma(planes_l[l], i_ref, pic_w, pic_h, qx, qy, cw, ch, dst, cw);
+            else mc_chroma(comp == 1 ? u_l[l] : v_l[l], i_c, pic_w, pic_h, qx, qy, cw, ch, dst, cw, maxv);
+        }
+        if (nl == 2) orc_pel_avrg(cw, out, cw, t0, t1, ch);
+        else memcpy(out, t0, sizeof(pel) * (size_t)cw * ch);
+    }
+    free(t0);
+    free(t1);
+}
+
+/* Prediction of every PU of CTUs [ctu0, ctu1) from the ME results: orc_mc_pu
+ * with the best reference of each list (best_ref, as orc_me_frame_lists
+ * writes it) and its quarter-pel MV.  Chroma is predicted for PUs of width
+ * and height >= 8 (the chroma slots widx - 1 of com_mc_cu exist only there).
+ * Output layout, per CTU: the selected shapes in table order, each shape's
+ * PUs in raster order, each PU's w x h samples row by row; luma takes
+ * 16384 samples per shape and chroma (u and v each) 4096 per eligible
+ * shape.  Skipped PUs (a best_ref < 0) are written as zeros.
+ *   ref_planes: num_refs x 16 luma planes at (0,0), stride i_ref
+ *   ref_u, ref_v: num_refs chroma planes at (0,0), padded, stride i_c */
+EXPORT void orc_mc_frame(int pic_w, int pic_h, int bd, int num_refs, int num_refs_l1, const pel *const *ref_planes,
+                         int i_ref, const pel *const *ref_u, const pel *const *ref_v, int i_c, u32 shape_mask,
+                         int ctu0, int ctu1, const orc_pu_res_t *res, const s8 *best_ref,
+                         pel *pred_y, pel *pred_u, pel *pred_v)
+{
+    const int nl = num_refs_l1 > 0 ? 2 : 1;
+    int ctu_cols = (pic_w + 127) / 128;
+    int npu = orc_pus_per_ctu(shape_mask);
+    int nsel = 0, nsel_c = 0;
+    for (int s = 0; s < 29; s++)
+        if (shape_mask >> s & 1) { nsel++; if (SHAPES[s][0] >= 8 && SHAPES[s][1] >= 8) nsel_c++; }
+    for (int ctu = ctu0; ctu < ctu1; ctu++) {
+        int cx = (ctu % ctu_cols) * 128, cy = (ctu / ctu_cols) * 128;
+        int pu = 0, rank = 0, rank_c = 0;
+        for (int s = 0; s < 29; s++) {
+            if (!(shape_mask >> s & 1)) continue;
+            int w = SHAPES[s][0], h = SHAPES[s][1];
+            int has_c = w >= 8 && h >= 8;
+            pel *py = pred_y + (size_t)ctu * nsel * 16384 + (size_t)rank * 16384;
+            pel *pu_ = pred_u + (size_t)ctu * nsel_c * 4096 + (size_t)rank_c * 4096;
+            pel *pv_ = pred_v + (size_t)ctu * nsel_c * 4096 + (size_t)rank_c * 4096;
+            int i = 0;
+            for (int oy = 0; oy < 128; oy += h)
+                for (int ox = 0; ox < 128; ox += w, pu++, i++) {
+                    size_t base = ((size_t)ctu * npu + pu);
+                    pel *oy_ = py + (size_t)i * w * h;
+                    pel *ou = has_c ? pu_ + (size_t)i * (w / 2) * (h / 2) : NULL;
+                    pel *ov = has_c ? pv_ + (size_t)i * (w / 2) * (h / 2) : NULL;
+                    int ok = 1;
+                    for (int l = 0; l < nl; l++) ok &= best_ref[base * nl + l] >= 0;
+                    if (!ok) {
+                        memset(oy_, 0, sizeof(pel) * (size_t)w * h);
+                        if (has_c) {
+                            memset(ou, 0, sizeof(pel) * (size_t)(w / 2) * (h / 2));
+                            memset(ov, 0, sizeof(pel) * (size_t)(w / 2) * (h / 2));
+                        }
+                        continue;
+                    }
+                    const pel *const *pl[2];
+                    const pel *ul[2], *vl[2];
+                    s16 mv[4];
+                    for (int l = 0; l < nl; l++) {
+                        int r = best_ref[base * nl + l];
+                        pl[l] = ref_planes + 16 * r;
+                        ul[l] = ref_u ? ref_u[r] : NULL;
+                        vl[l] = ref_v ? ref_v[r] : NULL;
+                        mv[2 * l] = res[base * num_refs + r].mv_sub[0];
+                        mv[2 * l + 1] = res[base * num_refs + r].mv_sub[1];
+                    }
+                    orc_mc_pu(pic_w, pic_h, bd, nl, pl, i_ref, ul, vl, i_c, cx + ox, cy + oy, w, h, mv,
+                              has_c && ref_u && ref_v, oy_, ou, ov);
+                }
+            rank++;
+            rank_c += has_c;
+        }
+    }
 }

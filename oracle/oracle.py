@@ -184,25 +184,91 @@ class Oracle(_Lib):
         return self.lib.orc_lambda_mv(qp)
 
     # -- frame -------------------------------------------------------------
-    def me_frame(self, cur, ref_planes_list, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, pad=PAD):
+    @staticmethod
+    def _plane_ptrs(ref_planes_list, w, pad):
+        stride = w + 2 * pad
+        ptrs = (C.c_void_p * (16 * len(ref_planes_list)))()
+        for r, planes in enumerate(ref_planes_list):
+            for i in range(16):
+                ptrs[16 * r + i] = planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize
+        return ptrs, stride
+
+    def me_frame(self, cur, ref_planes_list, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, pad=PAD,
+                 num_refs_l1=0):
         """Full search + sub-pel for CTUs [ctu0, ctu1); returns (res, best_ref)
-        with res shaped (n_ctu, npu, nref) of PU_RES_DTYPE."""
+        with res shaped (n_ctu, npu, nref) of PU_RES_DTYPE and best_ref
+        (n_ctu, npu), or (n_ctu, npu, 2) when the last num_refs_l1 references
+        form list 1."""
         h, w = cur.shape
         n_ctu = ((w + 127) // 128) * ((h + 127) // 128)
         ctu1 = n_ctu if ctu1 is None else ctu1
         npu = self.lib.orc_pus_per_ctu(C.c_uint32(mask))
         nref = len(ref_planes_list)
-        stride = w + 2 * pad
-        ptrs = (C.c_void_p * (16 * nref))()
-        for r, planes in enumerate(ref_planes_list):
-            for i in range(16):
-                ptrs[16 * r + i] = planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize
+        ptrs, stride = self._plane_ptrs(ref_planes_list, w, pad)
         res = np.zeros((n_ctu, npu, nref), PU_RES_DTYPE)
-        best = np.full((n_ctu, npu), -1, np.int8)
+        best = np.full((n_ctu, npu, 2) if num_refs_l1 else (n_ctu, npu), -1, np.int8)
         cur = np.ascontiguousarray(cur)
-        self.lib.orc_me_frame(w, h, self.bd, ptr(cur), w, nref, ptrs, stride, R, C.c_uint32(lambda_mv),
-                              cost_type, C.c_uint32(mask), ctu0, ctu1, ptr(res), ptr(best))
+        self.lib.orc_me_frame_lists(w, h, self.bd, ptr(cur), w, nref, num_refs_l1, ptrs, stride, R,
+                                    C.c_uint32(lambda_mv), cost_type, C.c_uint32(mask), ctu0, ctu1, ptr(res),
+                                    ptr(best))
         return res, best
+
+    def make_chroma(self, plane, pad=PAD // 2):
+        """A chroma plane edge-padded by `pad` (PIC_PAD_SIZE_C, com_img_padding
+        with ext 0): (H/2 + 2pad, W/2 + 2pad)."""
+        ch, cw = plane.shape
+        out = np.zeros((ch + 2 * pad, cw + 2 * pad), self.dt)
+        out[pad:pad + ch, pad:pad + cw] = plane
+        self.lib.orc_pad_plane(ptr_at(out, pad * out.shape[1] + pad), out.shape[1], cw, ch, pad, 0)
+        return out
+
+    def mc_pu(self, pic_w, pic_h, planes_l, chroma_l, x, y, w, h, mvs, pad=PAD, cpad=PAD // 2):
+        """com_mc_cu of one PU from len(planes_l) lists (luma planes per list,
+        (u, v) padded chroma per list or None, quarter-pel MV per list);
+        returns (pred_y, pred_u, pred_v) (chroma None without chroma)."""
+        nl = len(planes_l)
+        stride = pic_w + 2 * pad
+        keep = []
+        pl = (C.c_void_p * nl)()
+        for l, planes in enumerate(planes_l):
+            a = (C.c_void_p * 16)(*[planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize for i in range(16)])
+            keep.append(a)
+            pl[l] = C.cast(a, C.c_void_p).value
+        do_c = chroma_l is not None
+        cs = pic_w // 2 + 2 * cpad
+        ul = (C.c_void_p * nl)(*[chroma_l[l][0].ctypes.data + (cpad * cs + cpad) * self.dt().itemsize
+                                 for l in range(nl)]) if do_c else None
+        vl = (C.c_void_p * nl)(*[chroma_l[l][1].ctypes.data + (cpad * cs + cpad) * self.dt().itemsize
+                                 for l in range(nl)]) if do_c else None
+        mv = np.array(mvs, np.int16).reshape(-1)
+        py = np.zeros((h, w), self.dt)
+        pu = np.zeros((h // 2, w // 2), self.dt)
+        pv = np.zeros((h // 2, w // 2), self.dt)
+        self.lib.orc_mc_pu(pic_w, pic_h, self.bd, nl, pl, stride, ul, vl, cs, x, y, w, h, ptr(mv), int(do_c),
+                           ptr(py), ptr(pu), ptr(pv))
+        return py, (pu if do_c else None), (pv if do_c else None)
+
+    def mc_frame(self, pic_w, pic_h, ref_planes_list, chroma_list, num_refs_l1, mask, res, best, pad=PAD,
+                 cpad=PAD // 2):
+        """Predictions of every PU from ME results (orc_mc_frame layout):
+        returns flat (pred_y, pred_u, pred_v).  chroma_list: padded (u, v)
+        per reference."""
+        n_ctu = res.shape[0]
+        nsel = sum(1 for s in range(29) if mask >> s & 1)
+        nsel_c = sum(1 for s in range(29) if mask >> s & 1 and SHAPES[s][0] >= 8 and SHAPES[s][1] >= 8)
+        ptrs, stride = self._plane_ptrs(ref_planes_list, pic_w, pad)
+        cs = pic_w // 2 + 2 * cpad
+        isz = np.dtype(self.dt).itemsize
+        ul = (C.c_void_p * len(chroma_list))(*[u.ctypes.data + (cpad * cs + cpad) * isz for u, v in chroma_list])
+        vl = (C.c_void_p * len(chroma_list))(*[v.ctypes.data + (cpad * cs + cpad) * isz for u, v in chroma_list])
+        py = np.zeros(n_ctu * nsel * 16384, self.dt)
+        pu = np.zeros(max(1, n_ctu * nsel_c * 4096), self.dt)
+        pv = np.zeros(max(1, n_ctu * nsel_c * 4096), self.dt)
+        res = np.ascontiguousarray(res)
+        best = np.ascontiguousarray(best)
+        self.lib.orc_mc_frame(pic_w, pic_h, self.bd, res.shape[2], num_refs_l1, ptrs, stride, ul, vl, cs,
+                              C.c_uint32(mask), 0, n_ctu, ptr(res), ptr(best), ptr(py), ptr(pu), ptr(pv))
+        return py, pu, pv
 
 
 class RefLib(_Lib):
@@ -307,6 +373,22 @@ class RefLib(_Lib):
 
     def pic_free(self, pic):
         self.lib.refh_pic_free(pic)
+
+    def pic_set_chroma(self, pic, u, v):
+        u = np.ascontiguousarray(u, self.dt)
+        v = np.ascontiguousarray(v, self.dt)
+        self.lib.refh_pic_set_chroma(C.c_void_p(pic), ptr(u), ptr(v))
+
+    def mc_cu(self, x, y, pic_w, pic_h, w, h, p0, p1, mv0, mv1, chroma=True):
+        """com_mc_cu of one PU (p0 / p1: L0 / L1 pictures or None)."""
+        py = np.zeros((h, w), self.dt)
+        pu = np.zeros((h // 2, w // 2), self.dt)
+        pv = np.zeros((h // 2, w // 2), self.dt)
+        m0 = np.array(mv0 if mv0 is not None else (0, 0), np.int16)
+        m1 = np.array(mv1 if mv1 is not None else (0, 0), np.int16)
+        self.lib.refh_mc_cu(x, y, pic_w, pic_h, w, h, C.c_void_p(p0) if p0 else None, C.c_void_p(p1) if p1 else None,
+                            ptr(m0), ptr(m1), 0 if chroma else 1, ptr(py), ptr(pu), ptr(pv))
+        return py, (pu if chroma else None), (pv if chroma else None)
 
     def me_frame(self, cur, pics, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, threads=1, ctu_ms=None):
         h, w = cur.shape

@@ -130,6 +130,45 @@ RH_EXPORT void refh_pic_plane(void *p, int idx, pel *out)
     memcpy(out, base, sizeof(pel) * (size_t)s * (r->h + 2 * pad));
 }
 
+/* Sets the picture's 4:2:0 chroma planes ((w/2) x (h/2) samples each, rows
+ * packed) and edge-pads them as the encoder pads a reconstructed picture
+ * (com_img_padding(img, 3, 0), uavs3e.c:1087; the luma plane is padded
+ * again, which leaves it unchanged). */
+RH_EXPORT void refh_pic_set_chroma(void *p, const pel *u, const pel *v)
+{
+    refh_pic_t *r = (refh_pic_t *)p;
+    int cw = r->w / 2, chh = r->h / 2;
+    for (int y = 0; y < chh; y++) {
+        memcpy(r->pic->u + y * r->pic->stride_chroma, u + y * cw, sizeof(pel) * cw);
+        memcpy(r->pic->v + y * r->pic->stride_chroma, v + y * cw, sizeof(pel) * cw);
+    }
+    com_img_padding(r->pic->img, 3, 0);
+}
+
+/* com_mc_cu (com_mc.c:140-200) for one PU at (x, y), w x h: p0 / p1 are the
+ * L0 / L1 reference pictures (either may be NULL for uni-prediction), mv0 /
+ * mv1 their quarter-pel MVs.  pred_y receives w x h luma samples (stride w),
+ * pred_u / pred_v (w/2) x (h/2) chroma samples (stride w/2); channel 0 =
+ * luma and chroma, 1 = luma only. */
+RH_EXPORT void refh_mc_cu(int x, int y, int pic_w, int pic_h, int w, int h, void *p0, void *p1, const s16 *mv0,
+                          const s16 *mv1, int channel, pel *pred_y, pel *pred_u, pel *pred_v)
+{
+    com_ref_pic_t refp[1][REFP_NUM];
+    memset(refp, 0, sizeof(refp));
+    s8 refi[REFP_NUM] = {-1, -1};
+    s16 mv[REFP_NUM][MV_D] = {{0, 0}, {0, 0}};
+    if (p0) { refp[0][REFP_0].pic = ((refh_pic_t *)p0)->pic; refp[0][REFP_0].ptr = refp[0][REFP_0].pic->ptr; refi[REFP_0] = 0; mv[REFP_0][0] = mv0[0]; mv[REFP_0][1] = mv0[1]; }
+    if (p1) { refp[0][REFP_1].pic = ((refh_pic_t *)p1)->pic; refp[0][REFP_1].ptr = refp[0][REFP_1].pic->ptr; refi[REFP_1] = 0; mv[REFP_1][0] = mv1[0]; mv[REFP_1][1] = mv1[1]; }
+    static __thread pel buf[N_C][MAX_CU_DIM] __attribute__((aligned(32)));
+    com_mc_cu(x, y, pic_w, pic_h, w, h, refi, mv, refp, buf, w, channel ? CHANNEL_L : CHANNEL_LC, BIT_DEPTH);
+    for (int i = 0; i < h; i++) memcpy(pred_y + i * w, buf[Y_C] + i * w, sizeof(pel) * w);
+    if (!channel)
+        for (int i = 0; i < h / 2; i++) {
+            memcpy(pred_u + i * (w / 2), buf[U_C] + i * (w / 2), sizeof(pel) * (w / 2));
+            memcpy(pred_v + i * (w / 2), buf[V_C] + i * (w / 2), sizeof(pel) * (w / 2));
+        }
+}
+
 RH_EXPORT void refh_pic_free(void *p)
 {
     refh_pic_t *r = (refh_pic_t *)p;

@@ -331,3 +331,127 @@ def test_final_cost_and_refi_bits():
             assert o.lib.orc_refi_bits(n, i) == v
     # (c + lambda*(refi_bits + mvr + (mvr<4) + 2) + 2^15) >> 16
     assert o.lib.orc_final_cost(10 << 16, 314169, 2, 1, 0) == (((10 << 16) + 314169 * 4) + 32768) >> 16
+
+
+# --------------------------------------------------------------------------
+# Motion compensation and reference lists
+# --------------------------------------------------------------------------
+
+MC_W, MC_H = 160, 96
+
+
+def mc_golden(bd):
+    return np.load(os.path.join(GOLD, "mc_bd%d.npz" % bd), allow_pickle=False)
+
+
+def test_golden_mc_pu(bd):
+    """orc_mc_pu against com_mc_cu of the compiled reference (C table):
+    luma copies from the sub-pel planes, chroma 1/8-pel 4-tap filters,
+    clipped border positions, bi-averaging, and both lists on one picture
+    with one MV."""
+    d = mc_golden(bd)
+    o = Oracle(bd)
+    planes = [o.make_planes(l) for l in d["luma"]]
+    chroma = [(o.make_chroma(u), o.make_chroma(v)) for u, v in zip(d["u"], d["v"])]
+    preds, k = d["preds"], 0
+    kinds = set()
+    for x, y, w, h, nl, r0, m0x, m0y, r1, m1x, m1y in d["cases"].tolist():
+        ch = w >= 8 and h >= 8
+        rs, mvs = [r0, r1][:nl], [(m0x, m0y), (m1x, m1y)][:nl]
+        py, pu, pv = o.mc_pu(MC_W, MC_H, [planes[r] for r in rs], [chroma[r] for r in rs] if ch else None,
+                             x, y, w, h, mvs)
+        for a in ([py, pu, pv] if ch else [py]):
+            n = a.size
+            assert np.array_equal(a.ravel(), preds[k:k + n]), (x, y, w, h, nl, mvs)
+            k += n
+        kinds.add((nl, (m0x & 7) != 0 and (m0y & 7) != 0))
+    assert k == preds.size and kinds == {(1, False), (1, True), (2, False), (2, True)}
+    if bd == 10:
+        assert len(d["simd_mismatch"]) > 0   # the reference's AVX2 chroma filter differs from its C path
+
+
+def test_mc_frame_layout_and_lists(bd):
+    """orc_mc_frame writes orc_mc_pu of each PU's best L0 / L1 reference at
+    its offset in the frame layout, and orc_me_frame_lists picks the best
+    reference per list with the list's own tbl_refi_bits row."""
+    rng = np.random.default_rng(40 + bd)
+    dt = np.uint8 if bd == 8 else np.uint16
+    o = Oracle(bd)
+    w, h = 136, 136
+    lumas = [rng.integers(0, 1 << bd, (h, w)).astype(dt) for _ in range(4)]
+    cur = np.clip(lumas[0].astype(int) + rng.integers(-3, 4, (h, w)), 0, (1 << bd) - 1).astype(dt)
+    planes = [o.make_planes(l) for l in lumas]
+    chroma = [(o.make_chroma(rng.integers(0, 1 << bd, (h // 2, w // 2)).astype(dt)),
+               o.make_chroma(rng.integers(0, 1 << bd, (h // 2, w // 2)).astype(dt))) for _ in range(4)]
+    mask = sum(1 << SHAPES.index(s) for s in [(4, 8), (8, 8), (16, 8), (32, 32)])
+    lam = o.lambda_mv(32)
+    res, best = o.me_frame(cur, planes, 3, lam, mask, num_refs_l1=2)
+    res1, best1 = o.me_frame(cur, planes, 3, lam, mask)
+    assert np.array_equal(res[["mv_sub", "cost_sub"]], res1[["mv_sub", "cost_sub"]])
+    valid = res["valid"][..., 0] == 1
+    for l, (r0, n) in enumerate([(0, 2), (2, 2)]):
+        fc = np.stack([res["cost_final"][..., r0 + i] for i in range(n)], -1).astype(np.int64)
+        for i in range(n):
+            want = [o.lib.orc_final_cost(int(c), lam, n, i, 0) for c in res["cost_sub"][..., r0 + i][valid]]
+            assert (fc[..., i][valid] == np.array(want)).all()
+        assert (best[..., l][valid] == r0 + np.argmin(fc, -1)[valid]).all()   # argmin takes the first on ties
+    py, pu, pv = o.mc_frame(w, h, planes, chroma, 2, mask, res, best)
+    nsel, nsel_c = 4, 3
+    n_ctu = res.shape[0]
+    pu_i = 0
+    for s in [SHAPES.index(t) for t in [(4, 8), (8, 8), (16, 8), (32, 32)]]:
+        sw, sh = SHAPES[s]
+        rank = sorted([SHAPES.index(t) for t in [(4, 8), (8, 8), (16, 8), (32, 32)]]).index(s)
+        rank_c = rank - 1 if sw >= 8 else None
+        for i, (oy, ox) in enumerate([(a, b) for a in range(0, 128, sh) for b in range(0, 128, sw)]):
+            for ctu in range(n_ctu):
+                cx, cy = (ctu % 2) * 128, (ctu // 2) * 128
+                b = best[ctu, pu_i + i]
+                off = ctu * nsel * 16384 + rank * 16384 + i * sw * sh
+                got = py[off:off + sw * sh].reshape(sh, sw)
+                if (b < 0).any():
+                    assert not got.any()
+                    continue
+                mvs = [tuple(int(v) for v in res[ctu, pu_i + i, r]["mv_sub"]) for r in b]
+                ey, eu, ev = o.mc_pu(w, h, [planes[r] for r in b], [chroma[r] for r in b] if sw >= 8 else None,
+                                     cx + ox, cy + oy, sw, sh, mvs)
+                assert np.array_equal(got, ey), (s, i, ctu)
+                if sw >= 8:
+                    co = ctu * nsel_c * 4096 + rank_c * 4096 + i * (sw // 2) * (sh // 2)
+                    assert np.array_equal(pu[co:co + eu.size].reshape(eu.shape), eu)
+                    assert np.array_equal(pv[co:co + ev.size].reshape(ev.shape), ev)
+        pu_i += (128 // sw) * (128 // sh)
+
+
+@pytest.mark.skipif(not RefLib.available(8), reason="oracle/_ref not built (reference tree absent)")
+def test_live_reference_mc_sweep(bd):
+    """orc_mc_pu against com_mc_cu of the compiled reference's C table on
+    fresh random pictures, every shape, uni and bi."""
+    rng = np.random.default_rng(77 + bd)
+    dt = np.uint8 if bd == 8 else np.uint16
+    o, ref = Oracle(bd), RefLib(bd, level=0)
+    w, h = 200, 136
+    lumas = [rng.integers(0, 1 << bd, (h, w)).astype(dt) for _ in range(2)]
+    cs = [(rng.integers(0, 1 << bd, (h // 2, w // 2)).astype(dt), rng.integers(0, 1 << bd, (h // 2, w // 2)).astype(dt))
+          for _ in range(2)]
+    pics = []
+    for i in range(2):
+        p = ref.pic_create(lumas[i], poc=i)
+        ref.pic_set_chroma(p, *cs[i])
+        pics.append(p)
+    planes = [o.make_planes(l) for l in lumas]
+    chroma = [(o.make_chroma(u), o.make_chroma(v)) for u, v in cs]
+    try:
+        for pw, ph in SHAPES:
+            for nl in (1, 2):
+                x = int(rng.integers(0, max(1, (w - pw) // 4 + 1))) * 4
+                y = int(rng.integers(0, max(1, (h - ph) // 4 + 1))) * 4
+                mvs = [tuple(int(v) for v in rng.integers(-300, 300, 2)) for _ in range(2)]
+                chf = pw >= 8 and ph >= 8
+                got = o.mc_pu(w, h, planes[:nl], chroma[:nl] if chf else None, x, y, pw, ph, mvs[:nl])
+                want = ref.mc_cu(x, y, w, h, pw, ph, pics[0], pics[1] if nl == 2 else None, mvs[0], mvs[1], chroma=chf)
+                for a, b in zip(got, want):
+                    assert (a is None and b is None) or np.array_equal(a, b), (pw, ph, nl)
+    finally:
+        for p in pics:
+            ref.pic_free(p)
