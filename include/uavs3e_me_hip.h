@@ -156,6 +156,11 @@ int me_ref_create(me_ref_t *out, int pic_w, int pic_h);
  * 105-165) and builds the 15 planes (com_if_luma_frame, com_mc.c:760-800).
  * stream: a hipStream_t or NULL. */
 int me_ref_load(me_ref_t ref, const me_pel *luma, int stride, int on_device, void *stream);
+/* Loads the 4:2:0 chroma planes (pic_h/2 rows of pic_w/2 samples each, at
+ * `stride`) used by me_plan_mc; reads outside them see the nearest edge
+ * sample, as in the reference's padded chroma planes (pad 72, com_util.c:
+ * 148-165). */
+int me_ref_load_chroma(me_ref_t ref, const me_pel *u, const me_pel *v, int stride, int on_device, void *stream);
 /* Device address of plane `idx` at sample (0,0); *stride and *pad receive the
  * layout.  Valid until me_ref_destroy. */
 int me_ref_plane(me_ref_t ref, int idx, me_pel **plane, int *stride, int *pad);
@@ -175,6 +180,10 @@ typedef struct me_frame_cfg_t {
     int band_rows;         /* tuning: displacement rows per workgroup, 0 = default */
     int waves;             /* tuning: waves per workgroup (1..4), 0 = default */
     int dx_cols;           /* tuning: displacement columns per workgroup (multiple of 4), 0 = all */
+    int num_refs_l1;       /* 0: every reference is in list 0.  n > 0: refs[num_refs-n ..] form list 1
+                              (bi-directional frames); the final cost of a reference then uses
+                              tbl_refi_bits of its own list (tables.c:52-70) and the best reference is
+                              chosen per list */
 } me_frame_cfg_t;
 
 /* The 29 PU shapes, in mask-bit order: 4x8 8x4 4x16 16x4 4x32 32x4 8x8 8x16
@@ -197,8 +206,28 @@ int me_plan_set_cur(me_plan_t plan, const me_pel *cur, int stride, int on_device
  * was just reconstructed).  Results stay on the device until me_plan_results. */
 int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, void *stream);
 /* Waits for the last run and copies results to host memory:
- * res[(ctu*npu + pu)*num_refs + r], best_ref[ctu*npu + pu] (-1 = PU skipped). */
+ * res[(ctu*npu + pu)*num_refs + r], best_ref[(ctu*npu + pu)*L + l] with L = 1
+ * (single list) or 2 (num_refs_l1 > 0): the reference index (into refs) of
+ * list l's best reference, -1 = PU skipped. */
 int me_plan_results(me_plan_t plan, me_pu_result_t *res, int8_t *best_ref);
+/* Motion compensation of every PU from the last run's results: com_mc_cu
+ * (com_mc.c:140-200) with the best reference of each list and its
+ * quarter-pel MV -- luma copied from the quarter-pel planes, 4:2:0 chroma
+ * by the 1/8-pel 4-tap filters of com_mc_blk_chroma (com_mc.c:103-138), and
+ * with two lists the rounding average (a + b + 1) >> 1 of pel_avrg
+ * (com_util.c:3088-3105).  Chroma is predicted for shapes of width and
+ * height >= 8 (the chroma slots widx - 1 of com_mc_cu exist only there) and
+ * needs me_ref_load_chroma on every reference.
+ * Layout per CTU: the selected shapes in table order, each shape's PUs in
+ * raster order, each PU's w x h samples row by row; luma takes 16384
+ * samples per shape, u and v 4096 per chroma shape; skipped PUs are zero.
+ * pred_* are device buffers of me_plan_mc_size elements, or all NULL to use
+ * buffers owned by the plan (read back with me_plan_mc_download).
+ * Enqueued on `stream` (NULL: the plan's stream) after the ME of the frame. */
+int me_plan_mc(me_plan_t plan, me_ref_t const *refs, me_pel *pred_y, me_pel *pred_u, me_pel *pred_v, void *stream);
+int me_plan_mc_size(me_plan_t plan, uint64_t *luma_elems, uint64_t *chroma_elems);
+/* Waits for the plan's stream and copies the plan-owned predictions to host. */
+int me_plan_mc_download(me_plan_t plan, me_pel *pred_y, me_pel *pred_u, me_pel *pred_v);
 /* Runs `iters` frames stage by stage (no graph) on the plan's own stream and
  * writes the average milliseconds of each stage to ms[0..3]: reference
  * preparation, key initialisation, integer search, sub-pel + reference choice. */

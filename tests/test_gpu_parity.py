@@ -183,3 +183,40 @@ def test_host_driver_percall_vs_oracle():
     ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], 3, lam, MASK_16x16, 0, 1)
     _compare(res[:1], best[:1], ores[:1], obest[:1])
     assert hip.error_count() == 0
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_bi_plan_and_mc_vs_oracle(bd):
+    """Two reference lists (2 past, 2 future pictures): per-list final costs
+    and best references, then motion compensation of every PU -- luma from
+    the quarter-pel planes, 1/8-pel chroma, rounding bi-average -- bit-exact
+    against the oracle."""
+    work = wl.fuhd10_bi().scaled(264, 200, 6)
+    if bd == 8:
+        import dataclasses
+        work = dataclasses.replace(work, bd=8, sigma_ref=3.0, sigma_cur=3.0)
+    cur, refs = work.frames()
+    chroma = work.ref_chroma()
+    hip, orc = ua.MeHip(bd), Oracle(bd)
+    lam = wl.lambda_mv(32)
+    dref = [hip.ref_create(r) for r in refs]
+    for d, (u, v) in zip(dref, chroma):
+        hip.ref_load_chroma(d, u, v)
+    plan = hip.plan_create(work.w, work.h, work.R, lam, MASK_ALL, 4, num_refs_l1=2)
+    hip.plan_set_cur(plan, cur)
+    hip.plan_run(plan, dref)
+    hip.plan_mc(plan, dref)
+    res, best = hip.plan_results(plan, 4)
+    py, pu, pv = hip.plan_mc_download(plan)
+    hip.plan_destroy(plan)
+    for r in dref:
+        hip.ref_destroy(r)
+    planes = [orc.make_planes(r) for r in refs]
+    ores, obest = orc.me_frame(cur, planes, work.R, lam, MASK_ALL, num_refs_l1=2)
+    _compare(res, best, ores, obest)
+    assert best.shape[-1] == 2 and (best[..., 0] < 2).all() and ((best[..., 1] >= 2) | (best[..., 1] < 0)).all()
+    oy, ou, ov = orc.mc_frame(work.w, work.h, planes, [(orc.make_chroma(u), orc.make_chroma(v)) for u, v in chroma],
+                              2, MASK_ALL, ores, obest)
+    assert np.array_equal(py, oy), "luma predictions differ at %d samples" % int((py != oy).sum())
+    assert np.array_equal(pu, ou) and np.array_equal(pv, ov), "chroma predictions differ"
+    assert py.any() and pu.any()

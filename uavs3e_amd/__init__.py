@@ -71,7 +71,23 @@ def compare_results(res, best, ores, obest):
 class FrameCfg(C.Structure):
     _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("search_range", C.c_int), ("lambda_mv", C.c_uint32),
                 ("shape_mask", C.c_uint32), ("num_refs", C.c_int), ("band_rows", C.c_int), ("waves", C.c_int),
-                ("dx_cols", C.c_int)]
+                ("dx_cols", C.c_int), ("num_refs_l1", C.c_int)]
+
+
+def mc_layout(mask):
+    """Elements per CTU of the luma and of each chroma prediction buffer of
+    me_plan_mc, and the per-shape offsets {shape: (luma_off, chroma_off or
+    None)} inside a CTU's block."""
+    ly = lc = 0
+    offs = {}
+    for s, (w, h) in enumerate(SHAPES):
+        if not mask >> s & 1:
+            continue
+        has_c = w >= 8 and h >= 8
+        offs[s] = (ly, lc if has_c else None)
+        ly += 16384
+        lc += 4096 if has_c else 0
+    return ly, lc, offs
 
 
 class ServerCfg(C.Structure):
@@ -129,6 +145,10 @@ class MeHip:
         for n in ("me_plan_num_ctus", "me_plan_num_pus", "me_plan_set_cur", "me_plan_results", "me_ref_load",
                   "me_ref_plane"):
             getattr(L, n).argtypes = None
+        L.me_ref_load_chroma.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.me_plan_mc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.me_plan_mc_size.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.me_plan_mc_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
         L.me_ref_destroy.argtypes = [C.c_void_p]
         L.me_server_create.argtypes = [C.c_void_p, C.c_void_p]
@@ -211,6 +231,11 @@ class MeHip:
         self.check(self.lib.me_ref_load(r, _p(luma), w, 0, None), "me_ref_load")
         return r
 
+    def ref_load_chroma(self, ref, u, v):
+        u = np.ascontiguousarray(u, self.dt)
+        v = np.ascontiguousarray(v, self.dt)
+        self.check(self.lib.me_ref_load_chroma(ref, _p(u), _p(v), u.shape[1], 0, None), "me_ref_load_chroma")
+
     def ref_planes(self, ref, w, h, pad=PAD):
         """Download the 16 padded planes of a device reference: (16, H+2pad, W+2pad)."""
         planes = np.zeros((16, h + 2 * pad, w + 2 * pad), self.dt)
@@ -222,10 +247,12 @@ class MeHip:
         self.lib.me_ref_destroy(ref)
 
     # -- plans ---------------------------------------------------------------
-    def plan_create(self, w, h, R, lambda_mv, mask, num_refs, band_rows=0, waves=0, dx_cols=0):
-        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols)
+    def plan_create(self, w, h, R, lambda_mv, mask, num_refs, band_rows=0, waves=0, dx_cols=0, num_refs_l1=0):
+        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols, num_refs_l1)
         p = C.c_void_p()
         self.check(self.lib.me_plan_create(C.byref(p), C.byref(cfg)), "me_plan_create")
+        self._nl = getattr(self, "_nl", {})
+        self._nl[p.value] = 2 if num_refs_l1 else 1
         return p
 
     def plan_set_cur(self, plan, cur):
@@ -244,14 +271,32 @@ class MeHip:
         return list(ms)
 
     def plan_results(self, plan, num_refs):
+        """(res [ctu][pu][ref], best [ctu][pu] or [ctu][pu][list] for two lists)."""
         n_ctu = self.lib.me_plan_num_ctus(plan)
         npu = self.lib.me_plan_num_pus(plan)
+        nl = getattr(self, "_nl", {}).get(plan.value, 1)
         res = np.zeros((n_ctu, npu, num_refs), PU_RESULT)
-        best = np.zeros((n_ctu, npu), np.int8)
+        best = np.zeros((n_ctu, npu, 2) if nl == 2 else (n_ctu, npu), np.int8)
         self.check(self.lib.me_plan_results(plan, _p(res), _p(best)), "me_plan_results")
         return res, best
 
+    def plan_mc(self, plan, refs, stream=None):
+        """me_plan_mc into the plan's own device buffers."""
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        self.check(self.lib.me_plan_mc(plan, arr, None, None, None, stream), "me_plan_mc")
+
+    def plan_mc_download(self, plan):
+        """Flat (pred_y, pred_u, pred_v) of the last me_plan_mc."""
+        ly, lc = C.c_uint64(), C.c_uint64()
+        self.check(self.lib.me_plan_mc_size(plan, C.byref(ly), C.byref(lc)), "me_plan_mc_size")
+        py = np.zeros(ly.value, self.dt)
+        pu = np.zeros(max(1, lc.value), self.dt)
+        pv = np.zeros(max(1, lc.value), self.dt)
+        self.check(self.lib.me_plan_mc_download(plan, _p(py), _p(pu), _p(pv)), "me_plan_mc_download")
+        return py, pu, pv
+
     def plan_destroy(self, plan):
+        getattr(self, "_nl", {}).pop(plan.value, None)
         self.lib.me_plan_destroy(plan)
 
 

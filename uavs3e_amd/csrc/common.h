@@ -59,6 +59,10 @@ __constant__ static const s8 ME_LUMA_TAPS[4][8] = {
     {-1, 4, -11, 40, 40, -11, 4, -1},
     {-1, 3, -7, 19, 57, -10, 4, -1}};
 
+__constant__ static const s8 ME_CHROMA_TAPS[8][4] = {
+    {0, 64, 0, 0}, {-4, 62, 6, 0}, {-6, 56, 15, -1}, {-5, 47, 25, -3},
+    {-4, 36, 36, -4}, {-3, 25, 47, -5}, {-1, 15, 56, -6}, {0, 6, 62, -4}};
+
 __device__ __forceinline__ int me_clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // One wave64 min of two u64 keys held as the bit patterns of positive normal

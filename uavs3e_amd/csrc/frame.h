@@ -16,6 +16,7 @@ struct me_ref_s {
     pel *planes;             // 16 planes, plane i at planes + i * plane_elems
     pel *luma;               // unpadded w x h copy (source for re-preparation)
     s16 *tmp;                // 3 horizontal s16 planes over [-8, w+8) x [-8, h+8)
+    pel *chroma;             // optional 4:2:0 u then v, (w/2) x (h/2) each, unpadded
 };
 
 // Pads the integer plane of r from r->luma and rebuilds its 15 sub-pel
@@ -35,6 +36,7 @@ struct PicDesc {
     size_t plane_elems;              // samples per plane (stride * rows)
     int ref_stride;                  // pic_w + 2 * ME_PAD
     int pic_w, pic_h, ctu_cols;
+    const pel *chroma[ME_MAX_REFS];  // u plane of each reference (v follows), or null
 };
 
 // The CTUs a launch works on, always indexed by uniform (workgroup-level)
@@ -108,6 +110,7 @@ int me_int_search_launch_w4(int mode, const IntArgs &a, dim3 grid, size_t lds, h
 
 struct SubArgs {
     JobSet js;
+    int n0;                     // list-0 size (references [n0, nref) form list 1)
     int R, IB;
     u32 lambda;
     const u8 *tab;              // MV-bits table, centred (tab[0] is mv 0)
@@ -133,7 +136,7 @@ subpel_fn me_subpel_kernel_8x16(int tt);    // tile type 5
 
 struct MeSearch {
     int R, span, IB, sh, lo, hi;
-    int npu, nref, mode;
+    int npu, nref, nref_l1, nl, mode;   // nl: lists (1, or 2 when nref_l1 > 0)
     int waves, band_rows, nbands, wcols, wpitch, dx_cols, nbx;
     size_t lds;
     int base[ME_NSHAPES];
@@ -147,15 +150,27 @@ struct MeSearch {
 // limit and the MV-bits table size of inter_search_create).  band_rows,
 // waves and dx_cols are schedule choices (0 = default); results do not
 // depend on them.
-int me_search_init(MeSearch *s, int R, u32 lambda, u32 shape_mask, int nref, int max_dim, int band_rows, int waves,
-                   int dx_cols);
+int me_search_init(MeSearch *s, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
+                   int waves, int dx_cols);
 void me_search_free(MeSearch *s);
 // Enqueues key initialisation, integer search, sub-pel refinement and the
 // reference choice for the jobs of js.  keys/res are [job][npu][nref],
-// best is [job][npu].  ev, if given, receives 4 events recorded before the
+// best is [job][npu][nl].  ev, if given, receives 4 events recorded before the
 // key fill, before the integer search, before sub-pel and at the end.
 int me_search_enqueue(const MeSearch *s, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
                       hipEvent_t *ev);
 // The MV-bits table of inter_search_create (inter.c:2029-2058) in closed
 // form: tab[off + v] for |v| < off, 0 at +-off.
 void me_mvbits_fill(u8 *base, int off);
+
+// ---------------------------------------------------------------------------
+// Motion compensation of every PU from a search's results
+// ---------------------------------------------------------------------------
+
+// Enqueues the predictions (layout of me_plan_mc) for the jobs of js from
+// res / best of me_search_enqueue; the pictures' chroma pointers must be set
+// when pred_u / pred_v are given.
+int me_mc_enqueue(const MeSearch *s, const JobSet &js, const me_pu_result_t *res, const s8 *best, pel *pred_y,
+                  pel *pred_u, pel *pred_v, hipStream_t st);
+// Elements per CTU of the luma and of each chroma prediction buffer.
+void me_mc_sizes(const MeSearch *s, size_t *luma_per_ctu, size_t *chroma_per_ctu);

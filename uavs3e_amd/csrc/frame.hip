@@ -26,19 +26,22 @@ __global__ void k_fill_keys(u64 *keys, size_t n)
     if (i < n) keys[i] = ~0ull;
 }
 
-// Reference choice per PU: smallest final cost, lower index on ties
-// (inter.c:901-904); -1 for a PU that was not searched.
-__global__ void k_select(const me_pu_result_t *res, s8 *best, int n, int nref)
+// Reference choice per PU and list: smallest final cost, lower index on
+// ties (inter.c:901-904); -1 for a PU that was not searched.  References
+// [0, n0) form list 0, [n0, nref) list 1; best[i * nl + l].
+__global__ void k_select(const me_pu_result_t *res, s8 *best, int n, int nref, int n0, int nl)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    u32 bc = 0xFFFFFFFFu;
-    int br = -1;
-    for (int r = 0; r < nref; r++) {
-        const me_pu_result_t &o = res[(size_t)i * nref + r];
-        if (o.valid && o.cost_final < bc) { bc = o.cost_final; br = r; }
+    for (int l = 0; l < nl; l++) {
+        u32 bc = 0xFFFFFFFFu;
+        int br = -1;
+        for (int r = l ? n0 : 0; r < (l ? nref : n0); r++) {
+            const me_pu_result_t &o = res[(size_t)i * nref + r];
+            if (o.valid && o.cost_final < bc) { bc = o.cost_final; br = r; }
+        }
+        best[(size_t)i * nl + l] = (s8)br;
     }
-    best[i] = (s8)br;
 }
 
 int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st)
@@ -75,18 +78,20 @@ void me_mvbits_fill(u8 *base, int off)
     }
 }
 
-int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int max_dim, int band_rows, int waves,
-                   int dx_cols)
+int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
+                   int waves, int dx_cols)
 {
     memset(p, 0, sizeof(*p));
     const uint32_t m = shape_mask & ((1u << ME_NSHAPES) - 1);
-    if (R < 0 || R > 255 || nref < 1 || nref > ME_MAX_REFS || !m || max_dim < 1) {
+    if (R < 0 || R > 255 || nref < 1 || nref > ME_MAX_REFS || nref_l1 < 0 || nref_l1 >= nref || !m || max_dim < 1) {
         me_set_error("me_search_init: bad configuration", hipErrorInvalidValue, __FILE__, __LINE__);
         return -1;
     }
     p->R = R;
     p->lambda = lambda;
     p->nref = nref;
+    p->nref_l1 = nref_l1;
+    p->nl = nref_l1 > 0 ? 2 : 1;
     p->span = 2 * R + 1;
     int need = 0;
     while ((1 << need) < p->span * p->span) need++;
@@ -172,6 +177,7 @@ int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_resu
     SubArgs b;
     memset(&b, 0, sizeof(b));
     b.js = js;
+    b.n0 = p->nref - p->nref_l1;
     b.R = p->R; b.IB = p->IB; b.lambda = p->lambda;
     b.tab = p->d_tab_base + p->tab_off;
     b.keys = keys; b.res = res;
@@ -197,7 +203,8 @@ int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_resu
         ME_CHECK(hipGetLastError());
     }
     const int npus = js.n_jobs * p->npu;
-    hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, res, best, npus, p->nref);
+    hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, res, best, npus, p->nref,
+                       p->nref - p->nref_l1, p->nl);
     ME_CHECK(hipGetLastError());
     if (ev) ME_CHECK(hipEventRecord(ev[3], st));
     return 0;
@@ -214,6 +221,7 @@ struct me_plan_s {
     PicDesc pic;          // host copy of *d_pic
     PicDesc *d_pic;       // the frame's picture descriptor (current + references)
     pel *d_cur;
+    pel *d_pred;          // plan-owned predictions of me_plan_mc (y, then u, then v), or null
     u64 *d_keys;
     me_pu_result_t *d_res;
     s8 *d_best;
@@ -237,8 +245,8 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     p->ctu_rows = (cfg->pic_h + 127) / 128;
     p->n_ctu = p->ctu_cols * p->ctu_rows;
     const int T = cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h;
-    if (me_search_init(&p->s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, T, cfg->band_rows,
-                       cfg->waves, cfg->dx_cols)) {
+    if (me_search_init(&p->s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, cfg->num_refs_l1, T,
+                       cfg->band_rows, cfg->waves, cfg->dx_cols)) {
         free(p);
         return -1;
     }
@@ -249,7 +257,7 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     ALLOC(p->d_pic, sizeof(PicDesc));
     ALLOC(p->d_keys, nkeys * sizeof(u64));
     ALLOC(p->d_res, nkeys * sizeof(me_pu_result_t));
-    ALLOC(p->d_best, (size_t)p->n_ctu * p->s.npu);
+    ALLOC(p->d_best, (size_t)p->n_ctu * p->s.npu * p->s.nl);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&p->own);
     if (e != hipSuccess) {
@@ -290,6 +298,7 @@ static int plan_set_pic(me_plan_s *p, me_ref_t const *refs, hipStream_t st)
     d.pic_w = p->cfg.pic_w;
     d.pic_h = p->cfg.pic_h;
     d.ctu_cols = p->ctu_cols;
+    for (int r = 0; r < p->s.nref; r++) d.chroma[r] = refs[r]->chroma;
     if (memcmp(&d, &p->pic, sizeof(d)) == 0) return 0;
     ME_CHECK(hipStreamSynchronize(st));
     if (p->graph_stream && p->graph_stream != st) ME_CHECK(hipStreamSynchronize(p->graph_stream));
@@ -386,7 +395,64 @@ ME_EXPORT int me_plan_results(me_plan_t p, me_pu_result_t *res, int8_t *best_ref
     ME_CHECK(hipStreamSynchronize(st));
     const size_t n = (size_t)p->n_ctu * p->s.npu;
     if (res) ME_CHECK(hipMemcpy(res, p->d_res, n * p->s.nref * sizeof(me_pu_result_t), hipMemcpyDeviceToHost));
-    if (best_ref) ME_CHECK(hipMemcpy(best_ref, p->d_best, n, hipMemcpyDeviceToHost));
+    if (best_ref) ME_CHECK(hipMemcpy(best_ref, p->d_best, n * p->s.nl, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+ME_EXPORT int me_plan_mc_size(me_plan_t p, uint64_t *luma_elems, uint64_t *chroma_elems)
+{
+    if (!p) return -1;
+    size_t ly, lc;
+    me_mc_sizes(&p->s, &ly, &lc);
+    if (luma_elems) *luma_elems = (uint64_t)ly * p->n_ctu;
+    if (chroma_elems) *chroma_elems = (uint64_t)lc * p->n_ctu;
+    return 0;
+}
+
+ME_EXPORT int me_plan_mc(me_plan_t p, me_ref_t const *refs, me_pel *pred_y, me_pel *pred_u, me_pel *pred_v,
+                         void *stream)
+{
+    if (!p || !refs || (!pred_y && (pred_u || pred_v))) {
+        me_set_error("me_plan_mc: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    size_t ly, lc;
+    me_mc_sizes(&p->s, &ly, &lc);
+    ly *= p->n_ctu;
+    lc *= p->n_ctu;
+    for (int r = 0; r < p->s.nref; r++)
+        if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h || (lc && !refs[r]->chroma)) {
+            me_set_error("me_plan_mc: reference missing, of another size, or without chroma", hipErrorInvalidValue,
+                         __FILE__, __LINE__);
+            return -1;
+        }
+    hipStream_t st = stream ? (hipStream_t)stream : (p->graph_stream ? p->graph_stream : p->own);
+    if (!pred_y) {
+        if (!p->d_pred) ME_CHECK(hipMalloc((void **)&p->d_pred, (ly + 2 * lc) * sizeof(pel)));
+        pred_y = p->d_pred;
+        pred_u = lc ? p->d_pred + ly : nullptr;
+        pred_v = lc ? p->d_pred + ly + lc : nullptr;
+    }
+    if (plan_set_pic(p, refs, st)) return -1;
+    JobSet js;
+    js.pics = p->d_pic;
+    js.jobs = nullptr;
+    js.n_jobs = p->n_ctu;
+    return me_mc_enqueue(&p->s, js, p->d_res, p->d_best, pred_y, pred_u, pred_v, st);
+}
+
+ME_EXPORT int me_plan_mc_download(me_plan_t p, me_pel *pred_y, me_pel *pred_u, me_pel *pred_v)
+{
+    if (!p || !p->d_pred) { me_set_error("me_plan_mc_download: no plan-owned predictions", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    size_t ly, lc;
+    me_mc_sizes(&p->s, &ly, &lc);
+    ly *= p->n_ctu;
+    lc *= p->n_ctu;
+    ME_CHECK(hipStreamSynchronize(p->graph_stream ? p->graph_stream : p->own));
+    ME_CHECK(hipStreamSynchronize(p->own));
+    if (pred_y) ME_CHECK(hipMemcpy(pred_y, p->d_pred, ly * sizeof(pel), hipMemcpyDeviceToHost));
+    if (pred_u && lc) ME_CHECK(hipMemcpy(pred_u, p->d_pred + ly, lc * sizeof(pel), hipMemcpyDeviceToHost));
+    if (pred_v && lc) ME_CHECK(hipMemcpy(pred_v, p->d_pred + ly + lc, lc * sizeof(pel), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -401,6 +467,7 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     me_search_free(&p->s);
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);
+    (void)hipFree(p->d_pred);
     (void)hipFree(p->d_keys);
     (void)hipFree(p->d_res);
     (void)hipFree(p->d_best);

@@ -1,0 +1,216 @@
+// Motion compensation of every PU of a set of CTUs (gfx950): com_mc_cu
+// (com_mc.c:140-200) with the best reference of each list and its
+// quarter-pel MV, for all selected PU shapes at once.
+//
+//   luma     com_mc_blk_luma, hp_flag 0 (com_mc.c:41-87): a copy from plane
+//            [qy&3][qx&3] at the integer position clipped to
+//            [-MAX_CU_SIZE-4, pic+4]
+//   chroma   com_mc_blk_chroma, hp_flag 0 (com_mc.c:103-138): the luma
+//            quarter-pel position read as a 1/8-pel chroma position, integer
+//            part clipped to [-(MAX_CU_SIZE/2)-2, (pic+4)/2], then copy /
+//            H_4 / V_4 / EXT_4 with com_tbl_mc_c_coeff (com_mc.c:555-657)
+//   bi       pel_avrg, (a + b + 1) >> 1 (com_util.c:3088-3105)
+//
+// The device keeps chroma unpadded; every read clamps its coordinates into
+// the plane, which is what the reference's edge-replicated pad of 72
+// samples holds at every position these reads can reach (the clipped
+// origin is >= -66, filter taps reach 1 sample before and 2 after, and a
+// 64-sample block starting at (pic+4)/2 ends 68 samples past the edge).
+#include "frame.h"
+
+#include <string.h>
+
+struct McArgs {
+    JobSet js;
+    const me_pu_result_t *res;
+    const s8 *best;
+    int npu, nref, nl;
+    int pw, ph, base, per_ctu;
+    size_t luma_off, chroma_off;   // this shape's offset inside a CTU's block
+    size_t luma_ctu, chroma_ctu;   // elements per CTU
+    int chroma;                    // 1 if this shape has chroma
+    pel *py, *pu, *pv;
+};
+
+// Four luma samples from an arbitrary address (aligned dword loads and
+// v_alignbyte; may read 3 bytes past the last, inside the padded plane).
+__device__ __forceinline__ void load4(const pel *p, int o[4])
+{
+    const uintptr_t ad = (uintptr_t)p;
+    const u32 *q = (const u32 *)(ad & ~(uintptr_t)3);
+    const u32 sh = (u32)(ad & 3);
+#if BIT_DEPTH == 8
+    const u32 w = __builtin_amdgcn_alignbyte(q[1], q[0], sh);
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = (int)((w >> (8 * i)) & 0xFF);
+#else
+    const u32 w0 = __builtin_amdgcn_alignbyte(q[1], q[0], sh), w1 = __builtin_amdgcn_alignbyte(q[2], q[1], sh);
+    o[0] = (int)(w0 & 0xFFFF);
+    o[1] = (int)(w0 >> 16);
+    o[2] = (int)(w1 & 0xFFFF);
+    o[3] = (int)(w1 >> 16);
+#endif
+}
+
+// One chroma sample (cx, cy) of a PU's (w/2) x (h/2) block predicted from
+// plane `pl` (cw x chh, unpadded) at luma quarter-pel position (qx, qy).
+__device__ __forceinline__ int mc_chroma_sample(const pel *pl, int cw, int chh, int pic_w, int pic_h, int qx, int qy,
+                                                int cx, int cy)
+{
+    const int fx = qx & 7, fy = qy & 7;
+    const int ix = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_w + 4) >> 1, qx >> 3) + cx;
+    const int iy = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_h + 4) >> 1, qy >> 3) + cy;
+    const int maxv = (1 << BIT_DEPTH) - 1;
+    auto S = [&](int xx, int yy) -> int {
+        return (int)pl[(size_t)me_clip3(0, chh - 1, yy) * cw + me_clip3(0, cw - 1, xx)];
+    };
+    if (fx == 0 && fy == 0) return S(ix, iy);
+    if (fy == 0) {
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fx][k] * S(ix - 1 + k, iy);
+        return me_clip3(0, maxv, (t + 32) >> 6);
+    }
+    if (fx == 0) {
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fy][k] * S(ix, iy - 1 + k);
+        return me_clip3(0, maxv, (t + 32) >> 6);
+    }
+    // 2-D: horizontal pass stored as s16, then vertical (com_mc.c:587-643;
+    // 8-bit: shift1 0, shift2 12; 10-bit: shift1 2, shift2 10)
+    const int shift1 = BIT_DEPTH == 8 ? 0 : 2, shift2 = BIT_DEPTH == 8 ? 12 : 10;
+    const int add1 = (1 << shift1) >> 1, add2 = 1 << (shift2 - 1);
+    int v = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int t = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fx][k] * S(ix - 1 + k, iy - 1 + j);
+        v += ME_CHROMA_TAPS[fy][j] * (int)(s16)((t + add1) >> shift1);
+    }
+    return me_clip3(0, maxv, (v + add2) >> shift2);
+}
+
+// grid (16, jobs): 4096 threads per (CTU, shape); thread t writes luma
+// samples 4t..4t+3 of the shape's 16384 and, for chroma shapes, chroma
+// sample t of u and of v.
+__global__ __launch_bounds__(256) void k_mc(McArgs a)
+{
+    const int job = blockIdx.y;
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int cx0 = (ctu % P.ctu_cols) * 128, cy0 = (ctu / P.ctu_cols) * 128;
+    const int cols = 128 / a.pw;
+    const size_t pu0 = (size_t)job * a.npu + a.base;
+
+    // luma
+    {
+        const int area4 = a.pw * a.ph / 4;
+        const int i = t / area4, q = t - i * area4;
+        const int row = q / (a.pw / 4), c4 = (q - row * (a.pw / 4)) * 4;
+        const int x = cx0 + (i % cols) * a.pw, y = cy0 + (i / cols) * a.ph;
+        const s8 *bp = a.best + (pu0 + i) * a.nl;
+        int acc[4] = {0, 0, 0, 0};
+        bool ok = true;
+        for (int l = 0; l < a.nl; l++) ok &= bp[l] >= 0;
+        if (ok) {
+            for (int l = 0; l < a.nl; l++) {
+                const int r = bp[l];
+                const me_pu_result_t &o = a.res[(pu0 + i) * a.nref + r];
+                const int qx = (x << 2) + o.mv_sub[0], qy = (y << 2) + o.mv_sub[1];
+                const int ix = me_clip3(-ME_MAX_CU - 4, P.pic_w + 4, qx >> 2);
+                const int iy = me_clip3(-ME_MAX_CU - 4, P.pic_h + 4, qy >> 2);
+                const pel *src = P.planes[r] + (size_t)((qy & 3) * 4 + (qx & 3)) * P.plane_elems +
+                                 (size_t)(iy + row + ME_PAD) * P.ref_stride + (ix + c4 + ME_PAD);
+                int v[4];
+                load4(src, v);
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc[k] = l ? (acc[k] + v[k] + 1) >> 1 : v[k];
+            }
+        }
+        pel *dst = a.py + (size_t)job * a.luma_ctu + a.luma_off + (size_t)i * a.pw * a.ph +
+                   (size_t)row * a.pw + c4;
+#if BIT_DEPTH == 8
+        *(u32 *)dst = (u32)acc[0] | ((u32)acc[1] << 8) | ((u32)acc[2] << 16) | ((u32)acc[3] << 24);
+#else
+        *(uint2 *)dst = make_uint2((u32)acc[0] | ((u32)acc[1] << 16), (u32)acc[2] | ((u32)acc[3] << 16));
+#endif
+    }
+
+    // chroma
+    if (a.chroma) {
+        const int cw = a.pw / 2, chh = a.ph / 2, area = cw * chh;
+        const int i = t / area, q = t - i * area;
+        const int row = q / cw, col = q - row * cw;
+        const int x = cx0 + (i % cols) * a.pw, y = cy0 + (i / cols) * a.ph;
+        const s8 *bp = a.best + (pu0 + i) * a.nl;
+        const int pcw = P.pic_w / 2, pch = P.pic_h / 2;
+        int u = 0, v = 0;
+        bool ok = true;
+        for (int l = 0; l < a.nl; l++) ok &= bp[l] >= 0;
+        if (ok) {
+            for (int l = 0; l < a.nl; l++) {
+                const int r = bp[l];
+                const me_pu_result_t &o = a.res[(pu0 + i) * a.nref + r];
+                const int qx = (x << 2) + o.mv_sub[0], qy = (y << 2) + o.mv_sub[1];
+                const pel *pu = P.chroma[r], *pv = pu + (size_t)pcw * pch;
+                const int su = mc_chroma_sample(pu, pcw, pch, P.pic_w, P.pic_h, qx, qy, col, row);
+                const int sv = mc_chroma_sample(pv, pcw, pch, P.pic_w, P.pic_h, qx, qy, col, row);
+                u = l ? (u + su + 1) >> 1 : su;
+                v = l ? (v + sv + 1) >> 1 : sv;
+            }
+        }
+        const size_t o = (size_t)job * a.chroma_ctu + a.chroma_off + (size_t)i * area + q;
+        a.pu[o] = (pel)u;
+        a.pv[o] = (pel)v;
+    }
+}
+
+void me_mc_sizes(const MeSearch *s, size_t *luma_per_ctu, size_t *chroma_per_ctu)
+{
+    size_t ly = 0, lc = 0;
+    for (int k = 0; k < ME_NSHAPES; k++) {
+        if (s->base[k] < 0) continue;
+        ly += 16384;
+        if (ME_SHAPES_HOST[k][0] >= 8 && ME_SHAPES_HOST[k][1] >= 8) lc += 4096;
+    }
+    *luma_per_ctu = ly;
+    *chroma_per_ctu = lc;
+}
+
+int me_mc_enqueue(const MeSearch *s, const JobSet &js, const me_pu_result_t *res, const s8 *best, pel *pred_y,
+                  pel *pred_u, pel *pred_v, hipStream_t st)
+{
+    McArgs a;
+    memset(&a, 0, sizeof(a));
+    a.js = js;
+    a.res = res;
+    a.best = best;
+    a.npu = s->npu;
+    a.nref = s->nref;
+    a.nl = s->nl;
+    me_mc_sizes(s, &a.luma_ctu, &a.chroma_ctu);
+    a.py = pred_y;
+    a.pu = pred_u;
+    a.pv = pred_v;
+    const bool with_chroma = pred_u && pred_v && a.chroma_ctu > 0;
+    size_t lo = 0, co = 0;
+    for (int k = 0; k < ME_NSHAPES; k++) {
+        if (s->base[k] < 0) continue;
+        a.pw = ME_SHAPES_HOST[k][0];
+        a.ph = ME_SHAPES_HOST[k][1];
+        a.base = s->base[k];
+        a.per_ctu = (128 / a.pw) * (128 / a.ph);
+        a.luma_off = lo;
+        a.chroma_off = co;
+        const bool has_c = a.pw >= 8 && a.ph >= 8;
+        a.chroma = with_chroma && has_c;
+        hipLaunchKernelGGL(k_mc, dim3(16, js.n_jobs), dim3(256), 0, st, a);
+        ME_CHECK(hipGetLastError());
+        lo += 16384;
+        co += has_c ? 4096 : 0;
+    }
+    return 0;
+}

@@ -132,6 +132,26 @@ ME_EXPORT int me_ref_load(me_ref_t r, const me_pel *luma, int stride, int on_dev
     return 0;
 }
 
+ME_EXPORT int me_ref_load_chroma(me_ref_t r, const me_pel *u, const me_pel *v, int stride, int on_device, void *stream)
+{
+    const int cw = r ? r->w / 2 : 0, chh = r ? r->h / 2 : 0;
+    if (!r || !u || !v || stride < cw || cw < 1 || chh < 1) {
+        me_set_error("me_ref_load_chroma: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (!r->chroma) ME_CHECK(hipMalloc((void **)&r->chroma, 2 * (size_t)cw * chh * sizeof(pel)));
+    hipStream_t st = (hipStream_t)stream;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const me_pel *src[2] = {u, v};
+    for (int c = 0; c < 2; c++) {
+        pel *dst = r->chroma + (size_t)c * cw * chh;
+        if (stride == cw) ME_CHECK(hipMemcpyAsync(dst, src[c], (size_t)cw * chh * sizeof(pel), kind, st));
+        else ME_CHECK(hipMemcpy2DAsync(dst, cw * sizeof(pel), src[c], stride * sizeof(pel), cw * sizeof(pel), chh, kind, st));
+    }
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
 ME_EXPORT int me_ref_plane(me_ref_t r, int idx, me_pel **plane, int *stride, int *pad)
 {
     if (!r || idx < 0 || idx > 15) return -1;
@@ -151,7 +171,10 @@ ME_EXPORT int me_ref_download_plane(me_ref_t r, int idx, me_pel *out)
 ME_EXPORT void me_ref_destroy(me_ref_t r)
 {
     if (!r) return;
-    hipFree(r->planes); hipFree(r->luma); hipFree(r->tmp);
+    (void)hipFree(r->planes);
+    (void)hipFree(r->luma);
+    (void)hipFree(r->tmp);
+    (void)hipFree(r->chroma);
     free(r);
 }
 

@@ -254,8 +254,10 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a)
         out.mv_sub[0] = (s16)bx;
         out.mv_sub[1] = (s16)by;
         out.cost_sub = best;
-        // tbl_refi_bits[n][r] (tables.c:52-70); AMVR 0 adds 0 + 1 + 2 (me.c:560-561)
-        const int rb = a.nref <= 1 ? 0 : (r < a.nref - 1 ? r + 1 : a.nref - 1);
+        // tbl_refi_bits[n][i] of the reference's list: n its size, i its index
+        // in it (tables.c:52-70); AMVR 0 adds 0 + 1 + 2 (me.c:560-561)
+        const int n = r < a.n0 ? a.n0 : a.nref - a.n0, ri = r < a.n0 ? r : r - a.n0;
+        const int rb = n <= 1 ? 0 : (ri < n - 1 ? ri + 1 : n - 1);
         const u64 cf = best + (u64)(a.lambda * (u32)(rb + 3));
         out.cost_final = (u32)((cf + (1u << 15)) >> 16);
         out.valid = 1;

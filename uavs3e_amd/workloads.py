@@ -41,10 +41,17 @@ class Workload:
     sigma_cur: float = 3.0
     seed: int = 1
     drift: bool = False
+    l1: int = 0
 
     @property
     def num_refs(self):
         return len(self.ref_shifts_qpel)
+
+    @property
+    def num_refs_l1(self):
+        """References of list 1 (the last ones): the bi-directional workload
+        has 2 L0 (past) and 2 L1 (future) references."""
+        return self.l1
 
     @property
     def lam(self):
@@ -76,6 +83,20 @@ class Workload:
         cur = host.synth(self.seed, 0, 1, self.w, self.h, cx / 4.0, cy / 4.0, 16, self.sigma_cur, 4, off)
         return cur, refs
 
+    def ref_chroma(self, host=None):
+        """4:2:0 chroma (u, v) of each reference: two more fields of the same
+        kind (streams 1 and 2 of the seed) at half resolution, moved by half
+        the luma shift."""
+        host = host or MeHost(self.bd)
+        out = []
+        for i, (qx, qy) in enumerate(self.ref_shifts_qpel):
+            uv = []
+            for c in (1, 2):
+                uv.append(host.synth(self.seed, c, 10 + i, self.w // 2, self.h // 2, qx / 8.0, qy / 8.0, 0,
+                                     self.sigma_ref, 3, 0.0))
+            out.append(tuple(uv))
+        return out
+
 
 def _second_ref_shift(seed):
     host = MeHost(8)
@@ -105,7 +126,7 @@ def fuhd10_bi():
     gx, gy = GLOBAL_QPEL
     return Workload("4320p10_allpu_l0l1_r64", 7680, 4320, 10, 64, MASK_ALL,
                     ref_shifts_qpel=[(o * gx, o * gy) for o in (-1, -2, 1, 2)],
-                    cur_shift_qpel=(0, 0), sigma_ref=12.0, sigma_cur=12.0, seed=5, drift=True)
+                    cur_shift_qpel=(0, 0), sigma_ref=12.0, sigma_cur=12.0, seed=5, drift=True, l1=2)
 
 
 ALL = {"hd_16x16": hd_16x16, "hd_allpu": hd_allpu, "uhd10_allpu": uhd10_allpu, "fuhd10_bi": fuhd10_bi}
