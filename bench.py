@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-frames", type=int, default=32, help="frames per stream in the serving leg")
     ap.add_argument("--serve-rate", type=float, default=40000.0, help="offered load of the serving leg, CTUs/s")
+    ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of the autotuner's")
+    ap.add_argument("--tune-iters", type=int, default=3)
+    ap.add_argument("--tune-log", default="", help="write the autotuner's timing table to this JSON file")
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--dx-cols", type=int, default=0)
@@ -78,6 +81,22 @@ def main():
     if bi:   # bi-directional frames: ME of both lists, then MC (luma + 4:2:0 chroma, bi-average)
         for d, (u, v) in zip(dref, work.ref_chroma()):
             hip.ref_load_chroma(d, u, v)
+    # integer-search schedule: the autotuner's pick unless given on the command line
+    tune = None
+    if not (args.band_rows or args.waves or args.dx_cols) and not args.no_tune:
+        t0 = time.perf_counter()
+        pick, table = hip.autotune(work.w, work.h, work.R, work.lam, work.mask, dref, cur, work.num_refs_l1,
+                                   iters=args.tune_iters)
+        tune = {"pick": pick, "candidates": len(table), "seconds": time.perf_counter() - t0,
+                "fastest_ms": min(t["int_search_ms"] for t in table),
+                "default_ms": next((t["int_search_ms"] for t in table
+                                    if t["band_rows"] == 16 and t["waves"] == 4 and t["dx_cols"] == 0), None)}
+        args.band_rows, args.waves, args.dx_cols = pick["band_rows"], pick["waves"], pick["dx_cols"]
+        print("[bench] autotune: %s (%.3f ms; default %s ms) over %d schedules in %.1fs" % (
+            pick, tune["fastest_ms"], tune["default_ms"], len(table), tune["seconds"]), file=sys.stderr, flush=True)
+        if args.tune_log:
+            with open(args.tune_log, "w") as f:
+                json.dump({"workload": work.name, "pick": pick, "table": table}, f, indent=1)
     plan = hip.plan_create(work.w, work.h, work.R, work.lam, work.mask, work.num_refs,
                            band_rows=args.band_rows, waves=args.waves, dx_cols=args.dx_cols,
                            num_refs_l1=work.num_refs_l1)
@@ -285,6 +304,8 @@ def main():
                    "step": "rebuild newest reference planes + full search + sub-pel + ref choice, one hipGraph" +
                            (" + MC of every PU (luma, 4:2:0 chroma, bi-average)" if bi else ""),
                    "lists": "2 L0 + 2 L1" if bi else "L0"},
+        "autotune": tune,
+        "schedule": {"band_rows": args.band_rows, "waves": args.waves, "dx_cols": args.dx_cols},
         "candidate_evals_per_s": cand_per_s,
         "ctu_latency_ms": ({"p50": serving["latency_ms"]["p50"], "p99": serving["latency_ms"]["p99"],
                             "note": "serving leg: per-CTU request arrival to result on the host"}

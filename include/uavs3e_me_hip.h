@@ -205,6 +205,21 @@ int me_plan_set_cur(me_plan_t plan, const me_pel *cur, int stride, int on_device
  * reference r from its device luma inside the same graph (a reference that
  * was just reconstructed).  Results stay on the device until me_plan_results. */
 int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, void *stream);
+/* Autotuner of the integer-search schedule (no counterpart in the
+ * reference).  Times the integer search of cfg's frame against refs, with
+ * the current picture `cur` (host memory, `stride` samples per row), for
+ * every candidate workgroup size (1, 2, 4 waves) x displacement rows per
+ * workgroup (4..32) x displacement columns per workgroup (all, 64, 32) whose
+ * LDS window fits, `iters` runs each (HIP events on the plan's stream), and
+ * writes the fastest into cfg->band_rows / waves / dx_cols.  The schedule
+ * never changes results.  log (optional) receives up to log_cap timings. */
+typedef struct me_tune_result_t {
+    int band_rows, waves, dx_cols;
+    double int_search_ms;
+} me_tune_result_t;
+int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const me_pel *cur, int stride, int iters,
+                     me_tune_result_t *log, int log_cap, int *n_log);
+
 /* Waits for the last run and copies results to host memory:
  * res[(ctu*npu + pu)*num_refs + r], best_ref[(ctu*npu + pu)*L + l] with L = 1
  * (single list) or 2 (num_refs_l1 > 0): the reference index (into refs) of

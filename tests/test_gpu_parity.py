@@ -220,3 +220,22 @@ def test_bi_plan_and_mc_vs_oracle(bd):
     assert np.array_equal(py, oy), "luma predictions differ at %d samples" % int((py != oy).sum())
     assert np.array_equal(pu, ou) and np.array_equal(pv, ov), "chroma predictions differ"
     assert py.any() and pu.any()
+
+
+def test_autotune_picks_fastest_and_keeps_results():
+    """The autotuner times every schedule that fits and returns the fastest;
+    the tuned plan's results equal the default plan's (and the oracle's)."""
+    work = wl.hd_allpu().scaled(392, 264, 10)
+    cur, refs = work.frames()
+    hip, orc = ua.MeHip(8), Oracle(8)
+    lam = wl.lambda_mv(32)
+    dref = [hip.ref_create(r) for r in refs]
+    pick, table = hip.autotune(work.w, work.h, work.R, lam, MASK_ALL, dref, cur, iters=2)
+    assert len(table) >= 6 and all(t["int_search_ms"] > 0 for t in table)
+    fastest = min(table, key=lambda t: t["int_search_ms"])
+    assert pick == {k: fastest[k] for k in ("band_rows", "waves", "dx_cols")}
+    for r in dref:
+        hip.ref_destroy(r)
+    res, best = _run_plan(hip, cur, refs, work.R, lam, MASK_ALL, **pick)
+    ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], work.R, lam, MASK_ALL)
+    _compare(res, best, ores, obest)

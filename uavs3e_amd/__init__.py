@@ -90,6 +90,10 @@ def mc_layout(mask):
     return ly, lc, offs
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("band_rows", C.c_int), ("waves", C.c_int), ("dx_cols", C.c_int), ("int_search_ms", C.c_double)]
+
+
 class ServerCfg(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda_mv", C.c_uint32), ("shape_mask", C.c_uint32),
                 ("num_refs", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch_ctus", C.c_int),
@@ -149,6 +153,8 @@ class MeHip:
         L.me_plan_mc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_mc_size.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_mc_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                       C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
         L.me_ref_destroy.argtypes = [C.c_void_p]
         L.me_server_create.argtypes = [C.c_void_p, C.c_void_p]
@@ -254,6 +260,20 @@ class MeHip:
         self._nl = getattr(self, "_nl", {})
         self._nl[p.value] = 2 if num_refs_l1 else 1
         return p
+
+    def autotune(self, w, h, R, lambda_mv, mask, refs, cur, num_refs_l1=0, iters=3):
+        """me_plan_autotune: ({band_rows, waves, dx_cols} of the fastest
+        integer-search schedule, [timings of every candidate])."""
+        cfg = FrameCfg(w, h, R, lambda_mv, mask, len(refs), 0, 0, 0, num_refs_l1)
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        cur = np.ascontiguousarray(cur, self.dt)
+        log = (TuneResult * 64)()
+        n = C.c_int()
+        self.check(self.lib.me_plan_autotune(C.byref(cfg), arr, _p(cur), cur.shape[1], iters, log, 64, C.byref(n)),
+                   "me_plan_autotune")
+        table = [{"band_rows": t.band_rows, "waves": t.waves, "dx_cols": t.dx_cols, "int_search_ms": t.int_search_ms}
+                 for t in log[:n.value]]
+        return {"band_rows": cfg.band_rows, "waves": cfg.waves, "dx_cols": cfg.dx_cols}, table
 
     def plan_set_cur(self, plan, cur):
         cur = np.ascontiguousarray(cur, self.dt)

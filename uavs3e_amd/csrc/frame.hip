@@ -399,6 +399,50 @@ ME_EXPORT int me_plan_results(me_plan_t p, me_pu_result_t *res, int8_t *best_ref
     return 0;
 }
 
+ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const me_pel *cur, int stride, int iters,
+                               me_tune_result_t *log, int log_cap, int *n_log)
+{
+    if (!cfg || !refs || !cur || iters < 1) {
+        me_set_error("me_plan_autotune: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    const int span = 2 * cfg->search_range + 1;
+    const int full = (span + 3) & ~3;
+    const int bands[] = {4, 8, 16, 32}, waves[] = {1, 2, 4}, cols[] = {0, 64, 32};
+    double best = 1e30;
+    me_frame_cfg_t c = *cfg, pick = *cfg;
+    int n = 0;
+    for (int w : waves)
+        for (int b : bands)
+            for (int dc : cols) {
+                if (dc && dc >= full) continue;   // same as "all columns"
+                if (b > span) continue;
+                c.band_rows = b;
+                c.waves = w;
+                c.dx_cols = dc;
+                me_plan_t p;
+                if (me_plan_create(&p, &c)) continue;   // LDS window too large
+                double ms[4] = {0, 0, 0, 0};
+                int rc = me_plan_set_cur(p, cur, stride, 0, nullptr);
+                if (!rc) rc = me_plan_profile(p, refs, 0, 1, ms);          // warm-up
+                if (!rc) rc = me_plan_profile(p, refs, 0, iters, ms);
+                me_plan_destroy(p);
+                if (rc) return -1;
+                if (log && n < log_cap) log[n] = {b, w, dc, ms[2]};
+                n++;
+                if (ms[2] < best) { best = ms[2]; pick = c; }
+            }
+    if (n_log) *n_log = n < log_cap ? n : log_cap;
+    if (best >= 1e30) {
+        me_set_error("me_plan_autotune: no schedule fits", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    cfg->band_rows = pick.band_rows;
+    cfg->waves = pick.waves;
+    cfg->dx_cols = pick.dx_cols;
+    return 0;
+}
+
 ME_EXPORT int me_plan_mc_size(me_plan_t p, uint64_t *luma_elems, uint64_t *chroma_elems)
 {
     if (!p) return -1;
