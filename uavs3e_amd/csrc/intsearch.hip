@@ -46,6 +46,7 @@ struct KeyCtx {
     int dy, sh;
 };
 
+
 typedef u32 u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ bool node_valid(int px, int py, int dx, int dy, int wlim, int hlim)
@@ -65,8 +66,14 @@ __device__ __forceinline__ void upd1(double &st, u32 sad, const KeyCtx &c, int k
     if (EDGE) {
         if (!node_valid(px, py, c.dx[k], c.dy, wlim, hlim)) hi |= KEY_INVALID_HI;
     }
+#if BIT_DEPTH == 8
+    // a two-word vector: measured fastest at 8 bits
     u32x2 key = {c.klo[k], hi};
     st = me_vmin64(st, __builtin_bit_cast(double, key));
+#else
+    // a 64-bit or: keeps the 10-bit kernel free of private-memory traffic
+    st = me_vmin64(st, __builtin_bit_cast(double, ((u64)hi << 32) | c.klo[k]));
+#endif
 }
 
 // Cross-lane partner values.  Lane = ry*8 + rx (16x16 regions of the CTU).
@@ -294,7 +301,7 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
     for (int r = 0; r < 4; r++)
 #pragma unroll
         for (int c = 0; c < 4; c++) L[r][c].v = 0;
-#pragma unroll 4
+#pragma unroll
     for (int r = 0; r < 16; r++) {
         const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0 + (k0 & ~1));
         u32 d[9];
@@ -329,7 +336,9 @@ template <int PASS, int WAVES, bool EDGE, bool SH0, int NS>
 __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, const Block &b, const pel *win, const pel *curl,
                                          double *st)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform across the wave: keep it (and everything the
+    // displacement loop derives from it) in scalar registers
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rx = lane & 7, ry = lane >> 3, X = rx * 16, Y = ry * 16;
     const int wlim = P.pic_w + 126, hlim = P.pic_h + 126;
     const int span = 2 * a.R + 1;
@@ -358,8 +367,8 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                     } else {
                         const int dxi = dx + a.R;
                         const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
-                        kc.khi[k] = (u32)(K >> 32) + KEY_BIAS_HI;
-                        kc.klo[k] = (u32)K;
+                        kc.khi[k] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
+                        kc.klo[k] = __builtin_amdgcn_readfirstlane((u32)K);
                     }
                 }
                 Val L[4][4];
@@ -449,7 +458,9 @@ __device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, const d
 // WAVES-th displacement row of the tile, in two passes (small PUs, large
 // PUs) so that each pass's node states fit in registers.
 template <int MODE, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_int_search(IntArgs a)
+// The LDS window allows at most 2 workgroups per CU (8 waves: 2 per SIMD),
+// so the register budget of 2 waves per SIMD costs no occupancy.
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_int_search(IntArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     Block b;
