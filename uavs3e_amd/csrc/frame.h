@@ -6,6 +6,7 @@
 #include "../../include/uavs3e_me_hip.h"
 
 #define ME_MAX_REFS 4
+#define ME_SIDE_STREAMS 3
 
 // A reference picture resident in device memory: the edge-padded integer
 // plane + 15 quarter-pel planes (com_pic_t / com_subpel_t, com_util.c:62-91,
@@ -144,6 +145,10 @@ struct MeSearch {
     u32 *d_rate;             // [2R+1] integer-pel rate per displacement component
     u8 *d_tab_base;          // MV-bits table, tab_off samples each side of 0
     int tab_off;
+    // side streams on which the sub-pel launches of different shapes run
+    // concurrently (fork/join by events, so graph capture records branches)
+    hipStream_t side[ME_SIDE_STREAMS];
+    hipEvent_t fork, join[ME_SIDE_STREAMS];
 };
 
 // max_dim = max(pic_w, pic_h) of the pictures searched (the |mv| <= T - 4

@@ -136,6 +136,11 @@ int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int
     for (int i = 0; i < p->span; i++) rate[i] = (u32)tab[p->tab_off + 4 * (i - R)] * lambda;
     hipError_t e = hipMalloc((void **)&p->d_rate, p->span * sizeof(u32));
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tab_base, tab.size());
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+    for (int i = 0; i < ME_SIDE_STREAMS && e == hipSuccess; i++) {
+        e = hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipMemcpy(p->d_rate, rate.data(), p->span * sizeof(u32), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_tab_base, tab.data(), tab.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -152,6 +157,14 @@ void me_search_free(MeSearch *p)
     if (p->d_tab_base) (void)hipFree(p->d_tab_base);
     p->d_rate = nullptr;
     p->d_tab_base = nullptr;
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    p->fork = nullptr;
+    for (int i = 0; i < ME_SIDE_STREAMS; i++) {
+        if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
+        if (p->join[i]) (void)hipEventDestroy(p->join[i]);
+        p->side[i] = nullptr;
+        p->join[i] = nullptr;
+    }
 }
 
 int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
@@ -182,6 +195,17 @@ int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_resu
     b.tab = p->d_tab_base + p->tab_off;
     b.keys = keys; b.res = res;
     b.npu = p->npu; b.nref = p->nref;
+    // Each shape's launch fills only a few hundred workgroups, so the shapes
+    // are spread over the caller's stream and the side streams (forked from
+    // and joined back into st by events).
+    int nshapes = 0;
+    for (int s = 0; s < ME_NSHAPES; s++) nshapes += p->base[s] >= 0;
+    const int nst = nshapes > 1 ? 1 + ME_SIDE_STREAMS : 1;
+    if (nst > 1) {
+        ME_CHECK(hipEventRecord(p->fork, st));
+        for (int i = 0; i < ME_SIDE_STREAMS; i++) ME_CHECK(hipStreamWaitEvent(p->side[i], p->fork, 0));
+    }
+    int li = 0;
     for (int s = 0; s < ME_NSHAPES; s++) {
         if (p->base[s] < 0) continue;
         const int w = ME_SHAPES_HOST[s][0], h = ME_SHAPES_HOST[s][1];
@@ -199,9 +223,16 @@ int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_resu
         // lanes per (CTU, reference): 256..1024; a unit of more than 64
         // lanes is a workgroup of its own
         const int lanes = b.per_ctu * b.lpu, bs = b.lpu > 64 ? b.lpu : (lanes < 256 ? lanes : 256);
-        hipLaunchKernelGGL(fn, dim3(lanes / bs, js.n_jobs, p->nref), dim3(bs), 0, st, b);
+        hipStream_t ls = (li % nst) == 0 ? st : p->side[li % nst - 1];
+        li++;
+        hipLaunchKernelGGL(fn, dim3(lanes / bs, js.n_jobs, p->nref), dim3(bs), 0, ls, b);
         ME_CHECK(hipGetLastError());
     }
+    if (nst > 1)
+        for (int i = 0; i < ME_SIDE_STREAMS; i++) {
+            ME_CHECK(hipEventRecord(p->join[i], p->side[i]));
+            ME_CHECK(hipStreamWaitEvent(st, p->join[i], 0));
+        }
     const int npus = js.n_jobs * p->npu;
     hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, res, best, npus, p->nref,
                        p->nref - p->nref_l1, p->nl);
