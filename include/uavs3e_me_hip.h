@@ -220,6 +220,29 @@ typedef struct me_tune_result_t {
 int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const me_pel *cur, int stride, int iters,
                      me_tune_result_t *log, int log_cap, int *n_log);
 
+/* The reference encoder's own search instead of the full search:
+ * me_search_tz (me.c:529-568) -- diamond, conditional raster and refinement
+ * around each PU's MVP, then me_sub_pel_search -- for every PU of the frame
+ * and every reference, uni-prediction.  mvp: quarter-pel MVPs
+ * [(ctu*npu + pu)*num_refs + r][2] (host memory); mv_ipel_prev: the previous
+ * integer MV of each (PU, reference) for the adaptive raster range
+ * (inter_search_t.mv_ipel, me.c:553), or NULL for zeros.  Results and the
+ * per-list best reference are read with me_plan_results (cost_int: the
+ * integer-pel cost; cost_sub: the cost after sub-pel, before the extra
+ * bits; cost_final: the u32 me_search_tz returns).  Runs on `stream` (NULL:
+ * the plan's stream), outside the plan's graph. */
+typedef struct me_tz_cfg_t {
+    int gop_size;               /* get_raster_range (me.c:65-100) */
+    int max_search_range;       /* inter_search_t.max_search_range */
+    int adaptive_raster_range;
+    int mvr;                    /* AMVR index 0..4 */
+    int cost_type;              /* sub-pel: 0 = SATD, 1 = SAD, then SATD at the winner */
+    int64_t cur_poc;
+    int64_t ref_poc[4];         /* POC of each reference */
+} me_tz_cfg_t;
+int me_plan_run_tz(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_ipel_prev,
+                   const me_tz_cfg_t *tz, void *stream);
+
 /* Waits for the last run and copies results to host memory:
  * res[(ctu*npu + pu)*num_refs + r], best_ref[(ctu*npu + pu)*L + l] with L = 1
  * (single list) or 2 (num_refs_l1 > 0): the reference index (into refs) of

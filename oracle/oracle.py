@@ -114,6 +114,7 @@ class Oracle(_Lib):
         c64 = C.c_uint64()
         r = self.lib.orc_me_search_tz(C.byref(c), C.byref(t), x, y, w, h, refi, ptr(mvp_a), ptr(mv), ptr(mvi),
                                       C.byref(c64))
+        self.last_cost64 = c64.value
         return (int(mv[0]), int(mv[1]), int(mvi[0]), int(mvi[1])), r
 
     def subpel(self, cur, planes, x, y, w, h, mvr, cost_type, lambda_mv, imv, mvp):

@@ -2,8 +2,12 @@
 
 Every test here runs the product library on the GPU and compares it with the
 oracle's C restatement on the same seeded inputs."""
+import ctypes
+
 import numpy as np
 import pytest
+
+C_int64 = ctypes.c_int64
 
 from oracle.oracle import Oracle, MASK_ALL, MASK_16x16, SHAPES
 import uavs3e_amd as ua
@@ -239,3 +243,61 @@ def test_autotune_picks_fastest_and_keeps_results():
     res, best = _run_plan(hip, cur, refs, work.R, lam, MASK_ALL, **pick)
     ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], work.R, lam, MASK_ALL)
     _compare(res, best, ores, obest)
+
+
+TZ_MASK = sum(1 << SHAPES.index(s) for s in [(4, 8), (8, 16), (16, 8), (32, 4), (32, 32), (64, 128), (128, 128)])
+
+
+@pytest.mark.parametrize("bd,mvr,cost_type", [(8, 0, 0), (8, 1, 1), (8, 2, 0), (8, 4, 1), (10, 0, 0), (10, 3, 1)])
+def test_search_tz_vs_oracle(bd, mvr, cost_type):
+    """me_search_tz (diamond, raster, refinement, sub-pel, final cost) for
+    every PU from random MVPs, against the oracle's restatement (itself
+    pinned by the compiled reference's me_search_tz golden vectors)."""
+    base = wl.hd_allpu() if bd == 8 else wl.uhd10_allpu()
+    work = base.scaled(200, 136, 8)
+    work.ref_shifts_qpel = work.ref_shifts_qpel[:2]
+    cur, refs = work.frames()
+    hip, orc = ua.MeHip(bd), Oracle(bd)
+    lam = wl.lambda_mv(32)
+    n_ctu, nref = 4, 2
+    npu = sum((128 // w) * (128 // h) for i, (w, h) in enumerate(SHAPES) if TZ_MASK >> i & 1)
+    rng = np.random.default_rng(100 * bd + 10 * mvr + cost_type)
+    mvp = rng.integers(-60, 61, (n_ctu, npu, nref, 2)).astype(np.int16)
+    tz = ua.TzCfg(8, 64, 1, mvr, cost_type, 8, (C_int64 * 4)(7, 4, 0, 0))
+    dref = [hip.ref_create(r) for r in refs]
+    plan = hip.plan_create(work.w, work.h, work.R, lam, TZ_MASK, nref)
+    hip.plan_set_cur(plan, cur)
+    hip.plan_run_tz(plan, dref, mvp, tz)
+    res, best = hip.plan_results(plan, nref)
+    hip.plan_destroy(plan)
+    for r in dref:
+        hip.ref_destroy(r)
+    planes = [orc.make_planes(r) for r in refs]
+    pocs = [7, 4]
+    checked = 0
+    for ctu in range(n_ctu):
+        cx, cy = (ctu % 2) * 128, (ctu // 2) * 128
+        pu = 0
+        for si, (w, h) in enumerate(SHAPES):
+            if not TZ_MASK >> si & 1:
+                continue
+            for oy in range(0, 128, h):
+                for ox in range(0, 128, w):
+                    x, y = cx + ox, cy + oy
+                    for r in range(nref):
+                        g = res[ctu, pu, r]
+                        if x + w > work.w or y + h > work.h:
+                            assert g["valid"] == 0
+                            continue
+                        (mx, my, ix, iy), rc = orc.me_search_tz(cur, planes[r], x, y, w, h, r, nref, 8, pocs[r], 8, 64,
+                                                                mvr, cost_type, 1, lam, tuple(mvp[ctu, pu, r]))
+                        if rc == (1 << 64) - 1:
+                            assert g["valid"] == 0
+                            continue
+                        extra = orc.lib.orc_refi_bits(nref, r) + mvr + (mvr < 4) + 2
+                        got = (int(g["mv_sub"][0]), int(g["mv_sub"][1]), int(g["mv_int"][0]), int(g["mv_int"][1]),
+                               int(g["cost_final"]), int(g["cost_sub"]) + lam * extra)
+                        assert got == (mx, my, ix, iy, rc, orc.last_cost64), (x, y, w, h, r)
+                        checked += 1
+                    pu += 1
+    assert checked > 1000

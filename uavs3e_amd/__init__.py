@@ -94,6 +94,11 @@ class TuneResult(C.Structure):
     _fields_ = [("band_rows", C.c_int), ("waves", C.c_int), ("dx_cols", C.c_int), ("int_search_ms", C.c_double)]
 
 
+class TzCfg(C.Structure):
+    _fields_ = [("gop_size", C.c_int), ("max_search_range", C.c_int), ("adaptive_raster_range", C.c_int),
+                ("mvr", C.c_int), ("cost_type", C.c_int), ("cur_poc", C.c_int64), ("ref_poc", C.c_int64 * 4)]
+
+
 class ServerCfg(C.Structure):
     _fields_ = [("search_range", C.c_int), ("lambda_mv", C.c_uint32), ("shape_mask", C.c_uint32),
                 ("num_refs", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch_ctus", C.c_int),
@@ -153,6 +158,7 @@ class MeHip:
         L.me_plan_mc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_mc_size.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_mc_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.me_plan_run_tz.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
@@ -299,6 +305,15 @@ class MeHip:
         best = np.zeros((n_ctu, npu, 2) if nl == 2 else (n_ctu, npu), np.int8)
         self.check(self.lib.me_plan_results(plan, _p(res), _p(best)), "me_plan_results")
         return res, best
+
+    def plan_run_tz(self, plan, refs, mvp, tz, mvi_prev=None, stream=None):
+        """me_plan_run_tz: mvp int16 [ctu][pu][ref][2] quarter-pel MVPs; tz a
+        TzCfg.  Results via plan_results."""
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        mvp = np.ascontiguousarray(mvp, np.int16)
+        mvi = np.ascontiguousarray(mvi_prev, np.int16) if mvi_prev is not None else None
+        self.check(self.lib.me_plan_run_tz(plan, arr, _p(mvp), _p(mvi) if mvi is not None else None, C.byref(tz),
+                                           stream), "me_plan_run_tz")
 
     def plan_mc(self, plan, refs, stream=None):
         """me_plan_mc into the plan's own device buffers."""

@@ -179,3 +179,13 @@ int me_mc_enqueue(const MeSearch *s, const JobSet &js, const me_pu_result_t *res
                   pel *pred_u, pel *pred_v, hipStream_t st);
 // Elements per CTU of the luma and of each chroma prediction buffer.
 void me_mc_sizes(const MeSearch *s, size_t *luma_per_ctu, size_t *chroma_per_ctu);
+
+// ---------------------------------------------------------------------------
+// me_search_tz of every PU from per-PU MVPs
+// ---------------------------------------------------------------------------
+
+// Enqueues the reference's search (me.c:529-568) for the jobs of js: mvp and
+// mvi_prev (optional) are [job][npu][nref][2] device arrays; results go to
+// res [job][npu][nref].
+int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mvi_prev,
+                  const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st);

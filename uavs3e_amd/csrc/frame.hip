@@ -253,6 +253,7 @@ struct me_plan_s {
     PicDesc *d_pic;       // the frame's picture descriptor (current + references)
     pel *d_cur;
     pel *d_pred;          // plan-owned predictions of me_plan_mc (y, then u, then v), or null
+    s16 *d_mvp;           // me_plan_run_tz inputs: MVPs, then previous integer MVs
     u64 *d_keys;
     me_pu_result_t *d_res;
     s8 *d_best;
@@ -474,6 +475,39 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
     return 0;
 }
 
+ME_EXPORT int me_plan_run_tz(me_plan_t p, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_ipel_prev,
+                             const me_tz_cfg_t *tz, void *stream)
+{
+    if (!p || !refs || !mvp || !tz || tz->gop_size < 1 || tz->mvr < 0 || tz->mvr > 4 || tz->cost_type < 0 ||
+        tz->cost_type > 1) {
+        me_set_error("me_plan_run_tz: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    for (int r = 0; r < p->s.nref; r++)
+        if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h) {
+            me_set_error("me_plan_run_tz: reference size differs from the plan", hipErrorInvalidValue, __FILE__, __LINE__);
+            return -1;
+        }
+    hipStream_t st = stream ? (hipStream_t)stream : (p->graph_stream ? p->graph_stream : p->own);
+    const size_t n = (size_t)p->n_ctu * p->s.npu * p->s.nref * 2;
+    if (!p->d_mvp) ME_CHECK(hipMalloc((void **)&p->d_mvp, 2 * n * sizeof(s16)));
+    if (plan_set_pic(p, refs, st)) return -1;
+    ME_CHECK(hipMemcpyAsync(p->d_mvp, mvp, n * sizeof(s16), hipMemcpyHostToDevice, st));
+    if (mv_ipel_prev) ME_CHECK(hipMemcpyAsync(p->d_mvp + n, mv_ipel_prev, n * sizeof(s16), hipMemcpyHostToDevice, st));
+    JobSet js;
+    js.pics = p->d_pic;
+    js.jobs = nullptr;
+    js.n_jobs = p->n_ctu;
+    if (me_tz_enqueue(&p->s, js, p->d_mvp, mv_ipel_prev ? p->d_mvp + n : nullptr, tz, p->d_res, st)) return -1;
+    const int npus = p->n_ctu * p->s.npu;
+    hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, p->d_res, p->d_best, npus, p->s.nref,
+                       p->s.nref - p->s.nref_l1, p->s.nl);
+    ME_CHECK(hipGetLastError());
+    // the host arrays may be reused once the call returns
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
 ME_EXPORT int me_plan_mc_size(me_plan_t p, uint64_t *luma_elems, uint64_t *chroma_elems)
 {
     if (!p) return -1;
@@ -543,6 +577,7 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);
     (void)hipFree(p->d_pred);
+    (void)hipFree(p->d_mvp);
     (void)hipFree(p->d_keys);
     (void)hipFree(p->d_res);
     (void)hipFree(p->d_best);

@@ -11,6 +11,11 @@
 #ifndef ME_IS_WAVES
 #define ME_IS_WAVES 4
 #endif
+// 1: one pass over the displacements updates all 58 node classes of a lane;
+// 0: two passes (33 small, then 25 large classes), recomputing the leaves.
+#ifndef ME_IS_FUSED
+#define ME_IS_FUSED 0
+#endif
 
 // Values of NK displacements of one node: at 8 bits four displacements are
 // packed as u16 lanes (lo = d0,d1; hi = d2,d3) -- a sum of at most 256
@@ -377,7 +382,10 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                 g.build(L);
                 if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
                 else if (PASS == 1) pass_large<EDGE, SH0>(g, st, kc, rx, ry, b.cx, b.cy, wlim, hlim);
-                else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+                else if (PASS == 3) {
+                    pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+                    pass_large<EDGE, SH0>(g, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
+                } else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
             }
         }
     }
@@ -497,7 +505,12 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
 
     const int lane = tid & 63;
     const int rx = lane & 7, ry = lane >> 3;
-    const bool edge = !(b.cx + 124 + a.R <= P.pic_w - 1 && b.cy + 124 + a.R <= P.pic_h - 1 && a.R <= 127);
+    // A node's window (me.c:508-512) binds only on the right and bottom for
+    // R <= 127: dx <= W-1-px, dy <= H-1-py.  Blocks whose displacement tile
+    // keeps every node of the CTU (px <= cx+124, py <= cy+124) inside skip
+    // the per-node test.
+    const int dy_last = b.dy0 + a.band_rows - 1 < a.hi ? b.dy0 + a.band_rows - 1 : a.hi;
+    const bool edge = !(b.cx + 124 + b.dxe <= P.pic_w - 1 && b.cy + 124 + dy_last <= P.pic_h - 1 && a.R <= 127);
     u64 *kb = a.keys + (size_t)job * a.npu * a.nref + b.ref;
     // Interior blocks with sh == 0 (the common case) take the fast path;
     // the others test each node's window and shift the SAD.
@@ -509,6 +522,15 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         PUB(S16x16, 16, 16, rx * 16, ry * 16, st[0]);
         return;
     }
+#if ME_IS_FUSED
+    {
+        double st[N_SMALL + N_LARGE];
+        if (fast) run_pass<3, WAVES, false, true, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
+        else run_pass<3, WAVES, true, false, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
+        publish_small(a, kb, st, rx * 16, ry * 16);
+        publish_large(a, kb, st + N_SMALL, rx, ry);
+    }
+#else
     {
         double st[N_SMALL];
         if (fast) run_pass<0, WAVES, false, true, N_SMALL>(a, P, b, win, curl, st);
@@ -521,6 +543,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         else run_pass<1, WAVES, true, false, N_LARGE>(a, P, b, win, curl, st);
         publish_large(a, kb, st, rx, ry);
     }
+#endif
 }
 #undef PUB
 

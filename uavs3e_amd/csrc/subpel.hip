@@ -5,152 +5,11 @@
 // This file is compiled once per group of com_had tile types (ME_SP_PART =
 // 0: 4x4, 8x8, 8x4, 4x8 tiles; 1: 16x8; 2: 8x16) so that the instances build
 // in parallel.
-#include "frame.h"
+#include "had.h"
 
 #ifndef ME_SP_PART
 #define ME_SP_PART 0
 #endif
-
-// Search patterns of me_sub_pel_search in evaluation order, packed 2 bits
-// per point (field q = offset + 1, in half-pel resp. quarter-pel steps):
-//   half-pel    {-2,0} {-2,2} {0,2} {2,2} {2,0} {2,-2} {0,-2} {-2,-2}  me.c:415-417
-//   quarter-pel {-1,0} {0,1} {1,0} {0,-1} {-1,1} {1,1} {-1,-1} {1,-1}  me.c:460-462
-#define ME_SUBPEL_HALF_X 0x1A90u
-#define ME_SUBPEL_HALF_Y 0x01A9u
-#define ME_SUBPEL_QUARTER_X 0x8864u
-#define ME_SUBPEL_QUARTER_Y 0x0A19u
-__device__ __forceinline__ int pat(u32 packed, int q) { return (int)((packed >> (2 * q)) & 3u) - 1; }
-
-// Sum over aligned groups of w lanes (w a power of two, w <= 64), result in
-// every lane of the group; w is uniform.  Quad permutes and row mirrors run
-// as DPP moves, the 32-lane step as ds_swizzle.
-__device__ __forceinline__ int group_sum(int v, int w)
-{
-    if (w >= 2) v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
-    if (w >= 4) v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
-    if (w >= 8) v += __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false);   // row_half_mirror
-    if (w >= 16) v += __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false);  // row_mirror
-    if (w >= 32) v += __builtin_amdgcn_ds_swizzle(v, 0x401F);               // lane ^ 16
-    if (w >= 64) v += __shfl_xor(v, 32, 64);
-    return v;
-}
-
-__device__ __forceinline__ const pel *plane_ptr(const PicDesc &P, int ref, int idx, int x, int y)
-{
-    return P.planes[ref] + (size_t)idx * P.plane_elems + (size_t)(y + ME_PAD) * P.ref_stride + (x + ME_PAD);
-}
-
-// com_mc_blk_luma_pointer (com_mc.c:89-101).
-__device__ __forceinline__ const pel *qpel_ptr(const PicDesc &P, int ref, int qx, int qy)
-{
-    int ix = me_clip3(-ME_MAX_CU - 4, P.pic_w + 4, qx >> 2);
-    int iy = me_clip3(-ME_MAX_CU - 4, P.pic_h + 4, qy >> 2);
-    return plane_ptr(P, ref, (qy & 3) * 4 + (qx & 3), ix, iy);
-}
-
-// N consecutive samples from an arbitrary address as int, read with aligned
-// dword loads and v_alignbyte (may read up to 3 bytes past the last sample,
-// which stays inside the padded planes and the slack of the current frame).
-template <int N>
-__device__ __forceinline__ void load_row(const pel *p, int *out)
-{
-    const uintptr_t ad = (uintptr_t)p;
-    const u32 *q = (const u32 *)(ad & ~(uintptr_t)3);
-    const u32 sh = (u32)(ad & 3);
-    constexpr int NW = (N * (int)sizeof(pel) + 3) / 4;
-    u32 d[NW + 1];
-#pragma unroll
-    for (int i = 0; i <= NW; i++) d[i] = q[i];
-    u32 w[NW];
-#pragma unroll
-    for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-#if BIT_DEPTH == 8
-        out[i] = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFF);
-#else
-        out[i] = (int)((w[i >> 1] >> (16 * (i & 1))) & 0xFFFF);
-#endif
-    }
-}
-
-template <int N>
-__device__ __forceinline__ void fwht(int *v, int step)
-{
-#pragma unroll
-    for (int hs = 1; hs < N; hs <<= 1)
-#pragma unroll
-        for (int i = 0; i < N; i += hs << 1)
-#pragma unroll
-            for (int j = i; j < i + hs; j++) {
-                int p = v[j * step], q = v[(j + hs) * step];
-                v[j * step] = p + q;
-                v[(j + hs) * step] = p - q;
-            }
-}
-
-// Sum of |coef| of the 2-D Walsh-Hadamard transform of a TW x TH residual.
-template <int TW, int TH>
-__device__ __forceinline__ int had_abs(const pel *org, int so, const pel *cur, int sc)
-{
-    int m[TW * TH];
-#pragma unroll
-    for (int y = 0; y < TH; y++) {
-        int o[TW], c[TW];
-        load_row<TW>(org + y * so, o);
-        load_row<TW>(cur + y * sc, c);
-#pragma unroll
-        for (int x = 0; x < TW; x++) m[y * TW + x] = o[x] - c[x];
-        fwht<TW>(m + y * TW, 1);
-    }
-#pragma unroll
-    for (int x = 0; x < TW; x++) fwht<TH>(m + x, TW);
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < TW * TH; i++) s += abs(m[i]);
-    return s;
-}
-
-// A 16-point WHT is the 8-point WHT of the half sums and of the half
-// differences, so a 16x8 (TT 3) or 8x16 (TT 5) tile is two 8x8 transforms:
-// half 0 of the sums, half 1 of the differences of its two 8x8 halves.
-// Returns that half's sum of |coef|.
-template <int TT>
-__device__ __forceinline__ int had_half(const pel *org, int so, const pel *cur, int sc, int half)
-{
-    const int sg = half ? -1 : 1;
-    const int o2 = TT == 3 ? 8 : 8 * so, c2 = TT == 3 ? 8 : 8 * sc;
-    int m[64];
-#pragma unroll
-    for (int y = 0; y < 8; y++) {
-        int oa[8], ca[8], ob[8], cb[8];
-        load_row<8>(org + y * so, oa);
-        load_row<8>(cur + y * sc, ca);
-        load_row<8>(org + y * so + o2, ob);
-        load_row<8>(cur + y * sc + c2, cb);
-#pragma unroll
-        for (int x = 0; x < 8; x++) m[y * 8 + x] = (oa[x] - ca[x]) + sg * (ob[x] - cb[x]);
-        fwht<8>(m + y * 8, 1);
-    }
-#pragma unroll
-    for (int x = 0; x < 8; x++) fwht<8>(m + x, 8);
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < 64; i++) s += abs(m[i]);
-    return s;
-}
-
-// One normalised cost_satd tile (com_cost.c:259, 340, 467, 586, 649, 708)
-// for the tile types done by one lane (0, 1, 2, 4).
-template <int TT>
-__device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, int sc)
-{
-    const double SQRT32 = 0x1.6a09e667f3bcdp+2;
-    if (TT == 0) return (had_abs<4, 4>(org, so, cur, sc) + 1) >> 1;
-    if (TT == 1) return (had_abs<8, 8>(org, so, cur, sc) + 2) >> 2;
-    if (TT == 2) return (int)((double)had_abs<8, 4>(org, so, cur, sc) / SQRT32 * 2.0);
-    return (int)((double)had_abs<4, 8>(org, so, cur, sc) / SQRT32 * 2.0);
-}
 
 // Sub-pel search of one shape.  A unit is one (CTU, PU of this shape,
 // reference); a.lpu lanes share a unit: one lane per com_had tile

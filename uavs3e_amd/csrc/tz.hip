@@ -1,0 +1,381 @@
+// The reference encoder's own integer + sub-pel search, me_search_tz
+// (me.c:529-568), for every PU of a set of CTUs (gfx950): uni-prediction
+// (bi = 0) from a per-PU MVP, any AMVR precision and both sub-pel cost types.
+//
+//   window        get_max_search_range + com_ipel_range_rounding   me.c:48-55, 502-527
+//   diamond       centre, core square, multilayer diamond rounds   me.c:199-355
+//   raster        get_raster_range, rounds of halving steps        me.c:65-197
+//   refine        diamond with max step 8                          me.c:546-551
+//   sub-pel       me_sub_pel_search (AMVR 0: half + quarter,       me.c:366-500
+//                 AMVR 1: half; cost type 1: SAD, then SATD)
+//   final cost    + lambda*(refi_bits+mvr+(mvr<4)+2), (c+2^15)>>16 me.c:560-567
+//
+// One wave per (PU, reference).  The search decisions are made on values
+// every lane holds (uniform control flow); each candidate's SAD or com_had
+// is computed by all lanes of the wave together and summed across them.
+// The running best is replaced only by a strictly cheaper candidate in the
+// reference's evaluation order, exactly as the sequential loops do.
+#include "had.h"
+
+#include <string.h>
+
+// compiled once per group of com_had tile types, like subpel.hip (ME_TZ_PART
+// 0: 4x4, 8x8, 8x4, 4x8 tiles and the launcher; 1: 16x8; 2: 8x16)
+#ifndef ME_TZ_PART
+#define ME_TZ_PART 0
+#endif
+
+struct TzArgs {
+    JobSet js;
+    const s16 *mvp;          // [job][npu][nref][2], quarter pel
+    const s16 *mvi_prev;     // previous integer MV per (job, pu, ref) for the adaptive raster range, or null
+    me_pu_result_t *res;
+    const u8 *tab;           // centred MV-bits table
+    int tab_off;
+    u32 lambda;
+    int npu, nref, n0;       // n0: list-0 size
+    int pw, ph, base, per_ctu, tt, tw, th;
+    int gop, msr, adaptive, mvr, cost_type;
+    long long cur_poc, ref_poc[ME_MAX_REFS];
+};
+
+// Diamond patterns of search_diamond (me.c:330-333): 8 points of radius 2,
+// and the 16-point ring of radius 4 in the same rotation.
+__constant__ static const s8 TZ_DIA8[8][2] = {{-2, 0}, {-1, 1}, {0, 2}, {1, 1}, {2, 0}, {1, -1}, {0, -2}, {-1, -1}};
+__constant__ static const s8 TZ_DIA16[16][2] = {{-4, 0}, {-3, 1}, {-2, 2}, {-1, 3}, {0, 4}, {1, 3}, {2, 2}, {3, 1},
+                                                {4, 0}, {3, -1}, {2, -2}, {1, -3}, {0, -4}, {-1, -3}, {-2, -2}, {-3, -1}};
+
+__device__ __forceinline__ int wave_sum(int v) { return group_sum(v, 64); }
+
+// SAD of a w x h block, all lanes of the wave; every lane gets the sum.
+__device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred, int sp, int w, int h)
+{
+    const int lane = threadIdx.x & 63, lw = __builtin_ctz((unsigned)w);
+    int s = 0;
+    for (int i = lane; i < w * h; i += 64) {
+        const int xx = i & (w - 1), yy = i >> lw;
+        s += abs((int)org[yy * so + xx] - (int)pred[yy * sp + xx]);
+    }
+    return (u32)wave_sum(s);
+}
+
+// One normalised com_had tile of type TT (com_cost.c:259, 340, 467, 586,
+// 649, 708), done by one lane.
+template <int TT>
+__device__ __forceinline__ int satd_tile_fn(const pel *org, int so, const pel *cur, int sc)
+{
+    if (TT == 3 || TT == 5) {
+        const double SQRT128 = 0x1.6a09e667f3bcdp+3;
+        return (int)((double)(had_half<TT>(org, so, cur, sc, 0) + had_half<TT>(org, so, cur, sc, 1)) / SQRT128 * 2.0);
+    }
+    return satd_t<TT>(org, so, cur, sc);
+}
+
+// com_had (com_cost.c:712-812): the PU's tiles spread over the lanes; the
+// tile sum is shifted by bd - 8.  One out-of-line copy (it is called from
+// four places of the sub-pel search).
+template <int TT>
+__device__ __noinline__ u32 wave_had(const TzArgs &a, const pel *org, int so, const pel *pred, int sp)
+{
+    const int lane = threadIdx.x & 63, tpr = a.pw / a.tw, T = tpr * (a.ph / a.th);
+    int s = 0;
+    for (int t = lane; t < T; t += 64) {
+        const int tx = (t % tpr) * a.tw, ty = (t / tpr) * a.th;
+        s += satd_tile_fn<TT>(org + ty * so + tx, so, pred + ty * sp + tx, sp);
+    }
+    return (u32)(wave_sum(s) >> (BIT_DEPTH - 8));
+}
+
+struct TzUnit {
+    const TzArgs *a;
+    const PicDesc *P;
+    int r, x, y, w, h;
+    const pel *org;
+    int mvpx, mvpy;
+    int lo[2], hi[2];
+    u64 best;
+    int mv[2];
+    int shift;
+
+    __device__ __forceinline__ u32 ipel_bits(int mx, int my) const
+    {
+        const int m = a->mvr;
+        return (u32)a->tab[((mx << 2) >> m) - (mvpx >> m)] + (u32)a->tab[((my << 2) >> m) - (mvpy >> m)];
+    }
+    __device__ __forceinline__ u32 qpel_bits(int mx, int my) const
+    {
+        const int m = a->mvr;
+        return (u32)a->tab[(mx >> m) - (mvpx >> m)] + (u32)a->tab[(my >> m) - (mvpy >> m)];
+    }
+    // try_ipel: one integer candidate of a running search (me.c:158-189)
+    __device__ __forceinline__ bool try_ipel(int mx, int my)
+    {
+        const pel *pred = plane_ptr(*P, r, 0, x + mx, y + my);
+        const u32 sad = wave_sad(org, P->pic_w, pred, P->ref_stride, w, h);
+        const u64 cost = ((u64)sad << shift) + (u64)(ipel_bits(mx, my) * a->lambda);
+        if (cost < best) {
+            best = cost;
+            mv[0] = mx;
+            mv[1] = my;
+            return true;
+        }
+        return false;
+    }
+};
+
+// search_diamond (me.c:199-355) for bi = 0; returns the step of the last
+// improvement (0: centre, 2: core square, else the diamond step).
+__device__ __noinline__ int tz_diamond(TzUnit &u, int max_step)
+{
+    const int mvr = u.a->mvr;
+    int best_step = 0;
+    int core = 2, cstep = 1;
+    const int cx = u.mv[0], cy = u.mv[1];
+    if (mvr > 2) { core <<= mvr - 2; cstep <<= mvr - 2; }
+    int x0 = cx - core, y0 = cy - core, x1 = cx + core, y1 = cy + core;
+    while (x0 <= u.lo[0]) x0 += cstep;
+    while (y0 <= u.lo[1]) y0 += cstep;
+    while (x1 >= u.hi[0]) x1 -= cstep;
+    while (y1 >= u.hi[1]) y1 -= cstep;
+    if (u.try_ipel(cx, cy)) best_step = 0;
+    if (max_step)
+        for (int my = y0; my <= y1; my += cstep)
+            for (int mx = x0; mx <= x1; mx += cstep) u.try_ipel(mx, my);
+    if (u.mv[0] != cx || u.mv[1] != cy) best_step = 2;
+    // multilayer diamonds: 4 points at step 4, 8 at step 8, 16 beyond
+    // (me.c:330-333), scaled by the AMVR precision
+    const int scale = mvr - 2 > 0 ? mvr - 2 : 0;
+    for (int step = 4, round = 0; step < max_step; step <<= 1, round++) {
+        const int ss = (step << scale) >> 1;
+        const int num = round < 2 ? 8 : 16, add = round < 1 ? 2 : 1, big = round < 2 ? 0 : 1;
+        for (int i = 0; i < num; i += add) {
+            const int dx = big ? TZ_DIA16[i][0] : TZ_DIA8[i][0], dy = big ? TZ_DIA16[i][1] : TZ_DIA8[i][1];
+            const int mx = cx + (s16)(ss >> big) * dx, my = cy + (s16)(ss >> big) * dy;
+            if (mx <= u.hi[0] && mx >= u.lo[0] && my <= u.hi[1] && my >= u.lo[1])
+                if (u.try_ipel(mx, my)) best_step = step;
+        }
+    }
+    return best_step;
+}
+
+// search_raster (me.c:103-197) with get_raster_range (me.c:65-100).
+__device__ __noinline__ void tz_raster(TzUnit &u, int refi, int mvi_prev_x, int mvi_prev_y)
+{
+    const TzArgs &a = *u.a;
+    const int mvr = a.mvr;
+    const int step_base = mvr == 3 ? 6 : (mvr == 4 ? 8 : 5), step_min = mvr == 3 ? 2 : (mvr == 4 ? 4 : 1);
+    const int mn = u.w < u.h ? u.w : u.h;
+    int step = (step_base > (mn >> 1) ? step_base : (mn >> 1)) * (refi + 1);
+    // get_raster_range
+    long long dl = a.cur_poc - a.ref_poc[u.r];
+    const int dist = (int)(dl < 0 ? -dl : dl);
+    const int msr = a.msr >> (3 - mvr > 0 ? 3 - mvr : 0);
+    int rxy = (msr * dist + (a.gop >> 1)) / a.gop;
+    rxy = me_clip3(msr >> 2, msr, rxy);
+    int rx = rxy, ry = rxy;
+    if (mvr && a.adaptive) {
+        rx = abs(mvi_prev_x - u.mv[0]) * 4;
+        ry = abs(mvi_prev_y - u.mv[1]) * 2;
+        if (rx < (msr >> 2)) rx = msr >> 2;
+        if (ry < (msr >> 2)) ry = msr >> 2;
+        if (rx < step * 8) rx = step * 8;
+        if (ry < step * 4) ry = step * 4;
+        if (rx > rxy) rx = rxy;
+        if (ry > rxy) ry = rxy;
+    }
+    if (mvr > 2) { const int sh = mvr - 2; rx = (rx >> sh) << sh; ry = (ry >> sh) << sh; }
+    const int r0x = me_clip3(u.lo[0], u.hi[0], u.mv[0] - (s16)rx), r1x = me_clip3(u.lo[0], u.hi[0], u.mv[0] + (s16)rx);
+    const int r0y = me_clip3(u.lo[1], u.hi[1], u.mv[1] - (s16)ry), r1y = me_clip3(u.lo[1], u.hi[1], u.mv[1] + (s16)ry);
+    for (int round = 0; step >= step_min; round++) {
+        int x0, y0, x1, y1;
+        if (round == 0) { x0 = r0x; y0 = r0y; x1 = r1x; y1 = r1y; }
+        else {
+            x0 = u.mv[0] - step > u.lo[0] ? u.mv[0] - step : u.lo[0];
+            y0 = u.mv[1] - step > u.lo[1] ? u.mv[1] - step : u.lo[1];
+            x1 = u.mv[0] + step < u.hi[0] ? u.mv[0] + step : u.hi[0];
+            y1 = u.mv[1] + step < u.hi[1] ? u.mv[1] + step : u.hi[1];
+        }
+        for (int my = y0; my <= y1; my += step) {
+            if (u.best == ~0ull) continue;   // the reference's s64 tmp_best (me.c:158) never improves then
+            for (int mx = x0; mx <= x1; mx += step) u.try_ipel(mx, my);
+        }
+        step >>= 1;
+        if (mvr > 2) { const int sh = mvr - 2; step = (step >> sh) << sh; }
+    }
+}
+
+// me_sub_pel_search (me.c:366-500) for bi = 0; u.mv holds the quarter-pel
+// integer result on entry and the refined MV on return.
+template <int TT>
+__device__ __noinline__ u64 tz_subpel(TzUnit &u)
+{
+    const TzArgs &a = *u.a;
+    const int shift = 16 - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
+    auto dist = [&](const pel *p) -> u32 {
+        return a.cost_type == 0 ? wave_had<TT>(a, u.org, u.P->pic_w, p, u.P->ref_stride)
+                                : wave_sad(u.org, u.P->pic_w, p, u.P->ref_stride, u.w, u.h);
+    };
+    const int cx = u.mv[0], cy = u.mv[1];
+    u32 d = dist(plane_ptr(*u.P, u.r, 0, u.x + (cx >> 2), u.y + (cy >> 2)));
+    u64 best = (u64)(a.lambda * u.qpel_bits(cx, cy)) + ((u64)d << shift);
+    for (int p = 0; p < 8; p++) {
+        const int mx = cx + 2 * pat(ME_SUBPEL_HALF_X, p), my = cy + 2 * pat(ME_SUBPEL_HALF_Y, p);
+        d = dist(qpel_ptr(*u.P, u.r, mx + (u.x << 2), my + (u.y << 2)));
+        const u64 c = (u64)(a.lambda * u.qpel_bits(mx, my)) + ((u64)d << shift);
+        if (c < best) { best = c; u.mv[0] = mx; u.mv[1] = my; }
+    }
+    if (a.mvr == 0) {
+        const int hx = u.mv[0], hy = u.mv[1];
+        for (int p = 0; p < 8; p++) {
+            const int mx = hx + pat(ME_SUBPEL_QUARTER_X, p), my = hy + pat(ME_SUBPEL_QUARTER_Y, p);
+            d = dist(qpel_ptr(*u.P, u.r, mx + (u.x << 2), my + (u.y << 2)));
+            const u64 c = (u64)(a.lambda * u.qpel_bits(mx, my)) + ((u64)d << shift);
+            if (c < best) { best = c; u.mv[0] = mx; u.mv[1] = my; }
+        }
+    }
+    if (a.cost_type) {
+        d = wave_had<TT>(a, u.org, u.P->pic_w, qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)),
+                     u.P->ref_stride);
+        best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << 16);
+    }
+    return best;
+}
+
+// grid (ceil(per_ctu / 4), jobs, refs), 4 waves: one (PU, reference) each.
+// TT: the shape's com_had tile type.
+template <int TT>
+__global__ __launch_bounds__(256) void k_tz(TzArgs a)
+{
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = blockIdx.x * 4 + wv;
+    if (i >= a.per_ctu) return;
+    const int job = blockIdx.y, r = blockIdx.z;
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    const int cols = 128 / a.pw;
+    TzUnit u;
+    u.a = &a;
+    u.P = &P;
+    u.r = r;
+    u.w = a.pw;
+    u.h = a.ph;
+    u.x = (ctu % P.ctu_cols) * 128 + (i % cols) * a.pw;
+    u.y = (ctu / P.ctu_cols) * 128 + (i / cols) * a.ph;
+    const size_t slot = ((size_t)job * a.npu + a.base + i) * a.nref + r;
+    me_pu_result_t out;
+    out.mv_int[0] = out.mv_int[1] = out.mv_sub[0] = out.mv_sub[1] = 0;
+    out.cost_int = out.cost_sub = ~0ull;
+    out.cost_final = 0xFFFFFFFFu;
+    out.valid = 0;
+    u.mvpx = a.mvp[slot * 2];
+    u.mvpy = a.mvp[slot * 2 + 1];
+    // get_max_search_range (me.c:502-527), is_padding = 1
+    const int mvr = a.mvr;
+    const int T = (a.tab_off << mvr) >> 2;
+    const int px = u.mvpx >> 2, py = u.mvpy >> 2;
+    int lox = -128 + 1 - u.x, loy = -128 + 1 - u.y, hix = P.pic_w - 1 - u.x, hiy = P.pic_h - 1 - u.y;
+    if (px - T + 4 > lox) lox = px - T + 4;
+    if (py - T + 4 > loy) loy = py - T + 4;
+    if (px + T - 4 < hix) hix = px + T - 4;
+    if (py + T - 4 < hiy) hiy = py + T - 4;
+    int v[4] = {(s16)lox, (s16)loy, (s16)hix, (s16)hiy};
+    if (mvr > 2) {   // com_ipel_range_rounding (me.c:48-55)
+        const int sh = mvr - 2;
+        for (int k = 0; k < 4; k++) v[k] = (s16)(v[k] >= 0 ? ((v[k] >> sh) << sh) : -(((-v[k]) >> sh) << sh));
+    }
+    u.lo[0] = v[0]; u.lo[1] = v[1]; u.hi[0] = v[2]; u.hi[1] = v[3];
+    const bool inside = u.x + u.w <= P.pic_w && u.y + u.h <= P.pic_h;
+    if (!inside || u.lo[0] > u.hi[0] || u.lo[1] > u.hi[1]) {
+        if (lane == 0) a.res[slot] = out;
+        return;
+    }
+    u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
+    u.shift = 16 - (BIT_DEPTH - 8);
+    u.best = ~0ull;
+    u.mv[0] = me_clip3(u.lo[0], u.hi[0], (u.mvpx + 2) >> 2);
+    u.mv[1] = me_clip3(u.lo[1], u.hi[1], (u.mvpy + 2) >> 2);
+    const int refi = r < a.n0 ? r : r - a.n0, nlist = r < a.n0 ? a.n0 : a.nref - a.n0;
+    const int bs = tz_diamond(u, 32);
+    if (abs(u.mvpx - (u.mv[0] << 2)) > 2 || abs(u.mvpy - (u.mv[1] << 2)) > 2) {
+        if (bs > 5) {
+            const int pvx = a.mvi_prev ? a.mvi_prev[slot * 2] : 0, pvy = a.mvi_prev ? a.mvi_prev[slot * 2 + 1] : 0;
+            tz_raster(u, refi, pvx, pvy);
+        }
+        tz_diamond(u, 8);
+    }
+    out.mv_int[0] = (s16)u.mv[0];
+    out.mv_int[1] = (s16)u.mv[1];
+    out.cost_int = u.best;
+    u.mv[0] *= 4;
+    u.mv[1] *= 4;
+    u64 best = u.best;
+    if (mvr < 2) best = tz_subpel<TT>(u);
+    out.mv_sub[0] = (s16)u.mv[0];
+    out.mv_sub[1] = (s16)u.mv[1];
+    out.cost_sub = best;
+    const int rb = nlist <= 1 ? 0 : (refi < nlist - 1 ? refi + 1 : nlist - 1);   // tbl_refi_bits
+    const u64 cf = best + (u64)(a.lambda * (u32)(rb + mvr + (mvr < 4) + 2));
+    out.cost_final = (u32)((cf + (1u << 15)) >> 16);
+    out.valid = 1;
+    if (lane == 0) a.res[slot] = out;
+}
+
+typedef void (*tz_fn)(TzArgs);
+#if ME_TZ_PART == 0
+tz_fn me_tz_kernel_small(int tt)
+{
+    switch (tt) {
+    case 0: return k_tz<0>;
+    case 1: return k_tz<1>;
+    case 2: return k_tz<2>;
+    case 4: return k_tz<4>;
+    default: return nullptr;
+    }
+}
+#elif ME_TZ_PART == 1
+tz_fn me_tz_kernel_16x8(int tt) { return tt == 3 ? k_tz<3> : nullptr; }
+#else
+tz_fn me_tz_kernel_8x16(int tt) { return tt == 5 ? k_tz<5> : nullptr; }
+#endif
+
+#if ME_TZ_PART == 0
+tz_fn me_tz_kernel_16x8(int tt);
+tz_fn me_tz_kernel_8x16(int tt);
+
+int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mvi_prev,
+                  const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st)
+{
+    TzArgs a;
+    memset(&a, 0, sizeof(a));
+    a.js = js;
+    a.mvp = mvp;
+    a.mvi_prev = mvi_prev;
+    a.res = res;
+    a.tab = s->d_tab_base + s->tab_off;
+    a.tab_off = s->tab_off;
+    a.lambda = s->lambda;
+    a.npu = s->npu;
+    a.nref = s->nref;
+    a.n0 = s->nref - s->nref_l1;
+    a.gop = tz->gop_size;
+    a.msr = tz->max_search_range;
+    a.adaptive = tz->adaptive_raster_range;
+    a.mvr = tz->mvr;
+    a.cost_type = tz->cost_type;
+    a.cur_poc = tz->cur_poc;
+    for (int r = 0; r < ME_MAX_REFS; r++) a.ref_poc[r] = tz->ref_poc[r];
+    for (int k = 0; k < ME_NSHAPES; k++) {
+        if (s->base[k] < 0) continue;
+        a.pw = ME_SHAPES_HOST[k][0];
+        a.ph = ME_SHAPES_HOST[k][1];
+        a.base = s->base[k];
+        a.per_ctu = (128 / a.pw) * (128 / a.ph);
+        had_tiling(a.pw, a.ph, &a.tt, &a.tw, &a.th);
+        tz_fn fn = a.tt == 3 ? me_tz_kernel_16x8(a.tt) : (a.tt == 5 ? me_tz_kernel_8x16(a.tt) : me_tz_kernel_small(a.tt));
+        if (!fn) { me_set_error("no search kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+        hipLaunchKernelGGL(fn, dim3((a.per_ctu + 3) / 4, js.n_jobs, s->nref), dim3(256), 0, st, a);
+        ME_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+#endif
