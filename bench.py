@@ -49,6 +49,7 @@ def main():
                          "(three launches, torch bitwise_not) to scale HBM counters against")
     ap.add_argument("--no-percall", action="store_true", help="skip the per-call leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-tz", action="store_true", help="skip the me_search_tz leg")
     ap.add_argument("--serve-frames", type=int, default=32, help="frames per stream in the serving leg")
     ap.add_argument("--serve-rate", type=float, default=40000.0, help="offered load of the serving leg, CTUs/s")
     ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of the autotuner's")
@@ -173,6 +174,26 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
+
+    # ---- the reference's own search (me_search_tz) on the GPU ----------------
+    search_tz = None
+    if not args.no_tz and not bi:
+        import ctypes
+        mvp = np.zeros((n_ctu, npu, work.num_refs, 2), np.int16)   # MVP (0,0), as the full search
+        tz = ua.TzCfg(8, work.R, 0, 0, 0, 8, (ctypes.c_int64 * 4)(*([7, 6, 4, 0][:work.num_refs] +
+                                                                       [0] * (4 - work.num_refs))))
+        hip.plan_run_tz(plan, dref, mvp, tz)     # warm-up
+        t0 = time.perf_counter()
+        for _ in range(3):
+            hip.plan_run_tz(plan, dref, mvp, tz)
+        tz_ms = (time.perf_counter() - t0) * 1e3 / 3
+        tres, _ = hip.plan_results(plan, work.num_refs)
+        search_tz = {"ms_per_frame": tz_ms, "ctus_per_s": n_ctu / (tz_ms * 1e-3),
+                     "note": "me_search_tz (diamond, raster, refinement, sub-pel) for every PU and reference, "
+                             "MVP (0,0), GOP 8, max search range %d, AMVR 0, SATD sub-pel; one wave per "
+                             "(PU, reference); includes the MVP upload" % work.R,
+                     "pus_searched": int((tres["valid"] == 1).sum())}
+        print("[bench] me_search_tz: %.2f ms/frame" % tz_ms, file=sys.stderr, flush=True)
 
     # ---- per-call drop-in leg -----------------------------------------------
     per_call = None
@@ -321,6 +342,7 @@ def main():
         "per_call": per_call,
         "speedup_vs_per_call": (ctus_per_s / per_call["value"]) if per_call else None,
         "serving": serving,
+        "search_tz": search_tz,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
     }
