@@ -483,6 +483,10 @@ ME_EXPORT int me_plan_run_tz(me_plan_t p, me_ref_t const *refs, const int16_t *m
         me_set_error("me_plan_run_tz: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
         return -1;
     }
+    if (p->cfg.pic_w % 4) {   // the search reads the original 4 aligned samples at a time
+        me_set_error("me_plan_run_tz: picture width must be a multiple of 4", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
     for (int r = 0; r < p->s.nref; r++)
         if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h) {
             me_set_error("me_plan_run_tz: reference size differs from the plan", hipErrorInvalidValue, __FILE__, __LINE__);

@@ -48,15 +48,29 @@ __constant__ static const s8 TZ_DIA16[16][2] = {{-4, 0}, {-3, 1}, {-2, 2}, {-1, 
 __device__ __forceinline__ int wave_sum(int v) { return group_sum(v, 64); }
 
 // SAD of a w x h block, all lanes of the wave; every lane gets the sum.
+// A lane takes 4 horizontally adjacent samples at a time: the original's
+// are aligned (PU x and the picture width are multiples of 4), the
+// prediction's are realigned from two aligned dwords with v_alignbyte.
 __device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred, int sp, int w, int h)
 {
-    const int lane = threadIdx.x & 63, lw = __builtin_ctz((unsigned)w);
-    int s = 0;
-    for (int i = lane; i < w * h; i += 64) {
-        const int xx = i & (w - 1), yy = i >> lw;
-        s += abs((int)org[yy * so + xx] - (int)pred[yy * sp + xx]);
+    const int lane = threadIdx.x & 63, q = w >> 2, lq = __builtin_ctz((unsigned)q);
+    u32 s = 0;
+    for (int i = lane; i < q * h; i += 64) {
+        const int xx = (i & (q - 1)) << 2, yy = i >> lq;
+        const pel *o = org + yy * so + xx, *p = pred + yy * sp + xx;
+        const uintptr_t ad = (uintptr_t)p;
+        const u32 *pq = (const u32 *)(ad & ~(uintptr_t)3);
+        const u32 sh = (u32)(ad & 3);
+#if BIT_DEPTH == 8
+        const u32 pw = __builtin_amdgcn_alignbyte(pq[1], pq[0], sh);
+        s = __builtin_amdgcn_sad_u8(pw, *(const u32 *)o, s);
+#else
+        const u32 p0 = __builtin_amdgcn_alignbyte(pq[1], pq[0], sh), p1 = __builtin_amdgcn_alignbyte(pq[2], pq[1], sh);
+        const uint2 ow = *(const uint2 *)o;
+        s = __builtin_amdgcn_sad_u16(p0, ow.x, __builtin_amdgcn_sad_u16(p1, ow.y, s));
+#endif
     }
-    return (u32)wave_sum(s);
+    return (u32)wave_sum((int)s);
 }
 
 // One normalised com_had tile of type TT (com_cost.c:259, 340, 467, 586,
