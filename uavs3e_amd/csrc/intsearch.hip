@@ -271,9 +271,8 @@ __device__ __forceinline__ int wskew(int row) { return ((row >> 4) & 3) * (4 / (
 // v_qsad_pk_u16_u8 gives the SADs of one 4-sample row at the four
 // displacements dxb..dxb+3 in one instruction (4 x u16).
 __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0, int ccol0,
-                                       int k0, Val L[4][4])
+                                       Val L[4][4])
 {
-    (void)k0;
     u64 acc[4][4];
 #pragma unroll
     for (int r = 0; r < 4; r++)
@@ -297,18 +296,19 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
         for (int c = 0; c < 4; c++) L[r][c] = {(u32)acc[r][c], (u32)(acc[r][c] >> 32)};
 }
 #else
-// v_sad_u16 on sample pairs; odd displacements realign the pairs with
-// v_alignbyte.  One displacement (dxb + k0) per call.
-__device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0, int ccol0,
-                                       int k0, Val L[4][4])
+// v_sad_u16 on sample pairs.  Two displacements dxb + k0 (k0 even: its pairs
+// are dword aligned) and dxb + k0 + 1 (pairs realigned with v_alignbyte)
+// share each window row read: La and Lb receive their 16 leaves.
+__device__ __forceinline__ void leaves2(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0,
+                                        int ccol0, int k0, Val La[4][4], Val Lb[4][4])
 {
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) L[r][c].v = 0;
+        for (int c = 0; c < 4; c++) La[r][c].v = Lb[r][c].v = 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0 + (k0 & ~1));
+        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0 + k0);
         u32 d[9];
 #pragma unroll
         for (int j = 0; j < 9; j++) d[j] = rp[j];
@@ -317,15 +317,11 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
         const u32 cc[8] = {c01.x, c01.y, c01.z, c01.w, c23.x, c23.y, c23.z, c23.w};
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            u32 p0, p1;
-            if (k0 & 1) {
-                p0 = __builtin_amdgcn_alignbyte(d[2 * c + 1], d[2 * c], 2);
-                p1 = __builtin_amdgcn_alignbyte(d[2 * c + 2], d[2 * c + 1], 2);
-            } else {
-                p0 = d[2 * c];
-                p1 = d[2 * c + 1];
-            }
-            L[r >> 2][c].v = __builtin_amdgcn_sad_u16(p0, cc[2 * c], __builtin_amdgcn_sad_u16(p1, cc[2 * c + 1], L[r >> 2][c].v));
+            La[r >> 2][c].v = __builtin_amdgcn_sad_u16(d[2 * c], cc[2 * c],
+                                                       __builtin_amdgcn_sad_u16(d[2 * c + 1], cc[2 * c + 1], La[r >> 2][c].v));
+            const u32 p0 = __builtin_amdgcn_alignbyte(d[2 * c + 1], d[2 * c], 2);
+            const u32 p1 = __builtin_amdgcn_alignbyte(d[2 * c + 2], d[2 * c + 1], 2);
+            Lb[r >> 2][c].v = __builtin_amdgcn_sad_u16(p0, cc[2 * c], __builtin_amdgcn_sad_u16(p1, cc[2 * c + 1], Lb[r >> 2][c].v));
         }
     }
 }
@@ -356,15 +352,14 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
         const u32 ry_rate = a.rate[dyi];
         for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
             if (dxb + 3 < a.lo) continue;
-#pragma unroll
-            for (int k0 = 0; k0 < 4; k0 += NK) {
-                if (NK == 1 && (dxb + k0 < a.lo || dxb + k0 > b.dxe)) continue;
+#if BIT_DEPTH == 8
+            {
                 KeyCtx kc;
                 kc.dy = dy;
                 kc.sh = a.sh;
 #pragma unroll
                 for (int k = 0; k < NK; k++) {
-                    const int dx = dxb + k0 + k;
+                    const int dx = dxb + k;
                     kc.dx[k] = dx;
                     if (dx < a.lo || dx > b.dxe) {
                         kc.khi[k] = KEY_OUT_HI;   // above every valid key, never NaN
@@ -377,7 +372,7 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                     }
                 }
                 Val L[4][4];
-                leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, k0, L);
+                leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
                 Region g;
                 g.build(L);
                 if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
@@ -387,6 +382,38 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                     pass_large<EDGE, SH0>(g, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
                 } else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
             }
+#else
+            // two displacements per window read; each then composed and
+            // compared on its own (one displacement per u32 at 10 bits)
+            auto process = [&](int k0, const Val(&L)[4][4]) {
+                KeyCtx kc;
+                kc.dy = dy;
+                kc.sh = a.sh;
+                const int dx = dxb + k0, dxi = dx + a.R;
+                kc.dx[0] = dx;
+                const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
+                kc.khi[0] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
+                kc.klo[0] = __builtin_amdgcn_readfirstlane((u32)K);
+                Region g;
+                g.build(L);
+                if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+                else if (PASS == 1) pass_large<EDGE, SH0>(g, st, kc, rx, ry, b.cx, b.cy, wlim, hlim);
+                else if (PASS == 3) {
+                    pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+                    pass_large<EDGE, SH0>(g, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
+                } else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+            };
+#pragma unroll
+            for (int k0 = 0; k0 < 4; k0 += 2) {
+                const bool v0 = !(dxb + k0 < a.lo || dxb + k0 > b.dxe);
+                const bool v1 = !(dxb + k0 + 1 < a.lo || dxb + k0 + 1 > b.dxe);
+                if (!v0 && !v1) continue;
+                Val La[4][4], Lb[4][4];
+                leaves2(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, k0, La, Lb);
+                if (v0) process(k0, La);
+                if (v1) process(k0 + 1, Lb);
+            }
+#endif
         }
     }
 }
