@@ -10,9 +10,12 @@
 //                 AMVR 1: half; cost type 1: SAD, then SATD)
 //   final cost    + lambda*(refi_bits+mvr+(mvr<4)+2), (c+2^15)>>16 me.c:560-567
 //
-// One wave per (PU, reference).  The search decisions are made on values
-// every lane holds (uniform control flow); each candidate's SAD or com_had
-// is computed by all lanes of the wave together and summed across them.
+// One group of lanes per (PU, reference): 4 lanes for the smallest shapes up
+// to a whole wave from 32x32 on (area / 16, a power of two).  The search
+// decisions are made on values every lane of the group holds (control flow
+// is uniform within a group; groups of one wave may diverge); each
+// candidate's SAD or com_had is computed by the group's lanes together and
+// summed across them with DPP / swizzle steps that stay inside the group.
 // The running best is replaced only by a strictly cheaper candidate in the
 // reference's evaluation order, exactly as the sequential loops do.
 #include "had.h"
@@ -34,6 +37,7 @@ struct TzArgs {
     int tab_off;
     u32 lambda;
     int npu, nref, n0;       // n0: list-0 size
+    int lpu;                 // lanes per (PU, reference): a power of two, 4..64
     int pw, ph, base, per_ctu, tt, tw, th;
     int gop, msr, adaptive, mvr, cost_type;
     long long cur_poc, ref_poc[ME_MAX_REFS];
@@ -45,17 +49,16 @@ __constant__ static const s8 TZ_DIA8[8][2] = {{-2, 0}, {-1, 1}, {0, 2}, {1, 1}, 
 __constant__ static const s8 TZ_DIA16[16][2] = {{-4, 0}, {-3, 1}, {-2, 2}, {-1, 3}, {0, 4}, {1, 3}, {2, 2}, {3, 1},
                                                 {4, 0}, {3, -1}, {2, -2}, {1, -3}, {0, -4}, {-1, -3}, {-2, -2}, {-3, -1}};
 
-__device__ __forceinline__ int wave_sum(int v) { return group_sum(v, 64); }
 
-// SAD of a w x h block, all lanes of the wave; every lane gets the sum.
+// SAD of a w x h block by the lpu lanes of a group; every lane gets the sum.
 // A lane takes 4 horizontally adjacent samples at a time: the original's
 // are aligned (PU x and the picture width are multiples of 4), the
 // prediction's are realigned from two aligned dwords with v_alignbyte.
-__device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred, int sp, int w, int h)
+__device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred, int sp, int w, int h, int lpu)
 {
-    const int lane = threadIdx.x & 63, q = w >> 2, lq = __builtin_ctz((unsigned)q);
+    const int li = threadIdx.x & (lpu - 1), q = w >> 2, lq = __builtin_ctz((unsigned)q);
     u32 s = 0;
-    for (int i = lane; i < q * h; i += 64) {
+    for (int i = li; i < q * h; i += lpu) {
         const int xx = (i & (q - 1)) << 2, yy = i >> lq;
         const pel *o = org + yy * so + xx, *p = pred + yy * sp + xx;
         const uintptr_t ad = (uintptr_t)p;
@@ -70,7 +73,7 @@ __device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred,
         s = __builtin_amdgcn_sad_u16(p0, ow.x, __builtin_amdgcn_sad_u16(p1, ow.y, s));
 #endif
     }
-    return (u32)wave_sum((int)s);
+    return (u32)group_sum((int)s, lpu);
 }
 
 // One normalised com_had tile of type TT (com_cost.c:259, 340, 467, 586,
@@ -91,13 +94,13 @@ __device__ __forceinline__ int satd_tile_fn(const pel *org, int so, const pel *c
 template <int TT>
 __device__ __noinline__ u32 wave_had(const TzArgs &a, const pel *org, int so, const pel *pred, int sp)
 {
-    const int lane = threadIdx.x & 63, tpr = a.pw / a.tw, T = tpr * (a.ph / a.th);
+    const int li = threadIdx.x & (a.lpu - 1), tpr = a.pw / a.tw, T = tpr * (a.ph / a.th);
     int s = 0;
-    for (int t = lane; t < T; t += 64) {
+    for (int t = li; t < T; t += a.lpu) {
         const int tx = (t % tpr) * a.tw, ty = (t / tpr) * a.th;
         s += satd_tile_fn<TT>(org + ty * so + tx, so, pred + ty * sp + tx, sp);
     }
-    return (u32)(wave_sum(s) >> (BIT_DEPTH - 8));
+    return (u32)(group_sum(s, a.lpu) >> (BIT_DEPTH - 8));
 }
 
 struct TzUnit {
@@ -125,7 +128,7 @@ struct TzUnit {
     __device__ __forceinline__ bool try_ipel(int mx, int my)
     {
         const pel *pred = plane_ptr(*P, r, 0, x + mx, y + my);
-        const u32 sad = wave_sad(org, P->pic_w, pred, P->ref_stride, w, h);
+        const u32 sad = wave_sad(org, P->pic_w, pred, P->ref_stride, w, h, a->lpu);
         const u64 cost = ((u64)sad << shift) + (u64)(ipel_bits(mx, my) * a->lambda);
         if (cost < best) {
             best = cost;
@@ -227,7 +230,7 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
     const int shift = 16 - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
     auto dist = [&](const pel *p) -> u32 {
         return a.cost_type == 0 ? wave_had<TT>(a, u.org, u.P->pic_w, p, u.P->ref_stride)
-                                : wave_sad(u.org, u.P->pic_w, p, u.P->ref_stride, u.w, u.h);
+                                : wave_sad(u.org, u.P->pic_w, p, u.P->ref_stride, u.w, u.h, a.lpu);
     };
     const int cx = u.mv[0], cy = u.mv[1];
     u32 d = dist(plane_ptr(*u.P, u.r, 0, u.x + (cx >> 2), u.y + (cy >> 2)));
@@ -255,13 +258,14 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
     return best;
 }
 
-// grid (ceil(per_ctu / 4), jobs, refs), 4 waves: one (PU, reference) each.
-// TT: the shape's com_had tile type.
+// grid (ceil(per_ctu / (4 * 64 / lpu)), jobs, refs), 4 waves of 64 / lpu
+// (PU, reference) groups each.  TT: the shape's com_had tile type.
 template <int TT>
 __global__ __launch_bounds__(256) void k_tz(TzArgs a)
 {
-    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = blockIdx.x * 4 + wv;
+    const int lg = __builtin_ctz((unsigned)a.lpu), upw = 64 >> lg;
+    const int li = threadIdx.x & (a.lpu - 1), wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = (blockIdx.x * 4 + wv) * upw + ((threadIdx.x & 63) >> lg);
     if (i >= a.per_ctu) return;
     const int job = blockIdx.y, r = blockIdx.z;
     int ctu;
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     u.lo[0] = v[0]; u.lo[1] = v[1]; u.hi[0] = v[2]; u.hi[1] = v[3];
     const bool inside = u.x + u.w <= P.pic_w && u.y + u.h <= P.pic_h;
     if (!inside || u.lo[0] > u.hi[0] || u.lo[1] > u.hi[1]) {
-        if (lane == 0) a.res[slot] = out;
+        if (li == 0) a.res[slot] = out;
         return;
     }
     u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     const u64 cf = best + (u64)(a.lambda * (u32)(rb + mvr + (mvr < 4) + 2));
     out.cost_final = (u32)((cf + (1u << 15)) >> 16);
     out.valid = 1;
-    if (lane == 0) a.res[slot] = out;
+    if (li == 0) a.res[slot] = out;
 }
 
 typedef void (*tz_fn)(TzArgs);
@@ -385,9 +389,12 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
         a.base = s->base[k];
         a.per_ctu = (128 / a.pw) * (128 / a.ph);
         had_tiling(a.pw, a.ph, &a.tt, &a.tw, &a.th);
+        const int area = a.pw * a.ph;
+        a.lpu = area / 16 < 4 ? 4 : (area / 16 > 64 ? 64 : area / 16);
+        const int upb = 4 * (64 / a.lpu);
         tz_fn fn = a.tt == 3 ? me_tz_kernel_16x8(a.tt) : (a.tt == 5 ? me_tz_kernel_8x16(a.tt) : me_tz_kernel_small(a.tt));
         if (!fn) { me_set_error("no search kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
-        hipLaunchKernelGGL(fn, dim3((a.per_ctu + 3) / 4, js.n_jobs, s->nref), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs, s->nref), dim3(256), 0, st, a);
         ME_CHECK(hipGetLastError());
     }
     return 0;
