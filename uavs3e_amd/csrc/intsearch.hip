@@ -513,20 +513,53 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
     pel *curl = win + (size_t)wrows * a.wpitch;
     const int tid = threadIdx.x, nthr = 64 * WAVES;
 
-    // Stage the window: reference samples (cx + dxs + c, cy + dy0 + r), clamped
-    // to the padded plane (it is edge-replicated, so clamping is exact), and
-    // the current CTU clamped to the picture.
+    // Stage the window and the current CTU in LDS, a dword (4 / sizeof(pel)
+    // samples) per thread and step.  Reference samples (cx + dxs + c,
+    // cy + dy0 + r) are clamped to the padded plane (it is edge-replicated, so
+    // clamping is exact); the current CTU to the picture.  Away from the
+    // plane's edges a dword is realigned from two aligned loads.
+    constexpr int SPD = 4 / (int)sizeof(pel);
     const pel *ref0 = P.planes[b.ref] + (size_t)ME_PAD * P.ref_stride + ME_PAD;
     const int xmin = -ME_PAD, xmax = P.pic_w + ME_PAD - 1, ymin = -ME_PAD, ymax = P.pic_h + ME_PAD - 1;
-    for (int i = tid; i < wrows * a.wcols; i += nthr) {
-        int r = i / a.wcols, c = i - r * a.wcols;
-        int x = me_clip3(xmin, xmax, b.cx + b.dxs + c), y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
-        win[r * a.wpitch + wskew(r) + c] = ref0[(ptrdiff_t)y * P.ref_stride + x];
+    {
+        const int wq = a.wcols / SPD, dq = nthr % wq, dr = nthr / wq;
+        int r = tid / wq, q = tid - r * wq;
+        for (; r < wrows; r += dr, q += dq) {
+            if (q >= wq) { q -= wq; r++; if (r >= wrows) break; }
+            const int x0 = b.cx + b.dxs + q * SPD, y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
+            const pel *src = ref0 + (ptrdiff_t)y * P.ref_stride;
+            u32 v;
+            if (x0 >= xmin && x0 + SPD - 1 <= xmax) {
+                const uintptr_t ad = (uintptr_t)(src + x0);
+                const u32 *qa = (const u32 *)(ad & ~(uintptr_t)3);
+                v = __builtin_amdgcn_alignbyte(qa[1], qa[0], (u32)(ad & 3));
+            } else {
+                v = 0;
+#pragma unroll
+                for (int k = 0; k < SPD; k++)
+                    v |= (u32)src[me_clip3(xmin, xmax, x0 + k)] << (8 * (int)sizeof(pel) * k);
+            }
+            *(u32 *)(win + r * a.wpitch + wskew(r) + q * SPD) = v;
+        }
     }
-    for (int i = tid; i < 128 * 128; i += nthr) {
-        int r = i >> 7, c = i & 127;
-        int x = me_clip3(0, P.pic_w - 1, b.cx + c), y = me_clip3(0, P.pic_h - 1, b.cy + r);
-        curl[i] = P.cur[(size_t)y * P.pic_w + x];
+    {
+        constexpr int CQ = 128 / SPD;   // dwords per CTU row
+        const int q = tid % CQ;
+        const bool inner = b.cx + 127 <= P.pic_w - 1;
+        for (int r = tid / CQ; r < 128; r += nthr / CQ) {
+            const int y = me_clip3(0, P.pic_h - 1, b.cy + r), x0 = b.cx + q * SPD;
+            const pel *src = P.cur + (size_t)y * P.pic_w;
+            u32 v;
+            if (inner) {
+                v = *(const u32 *)(src + x0);
+            } else {
+                v = 0;
+#pragma unroll
+                for (int k = 0; k < SPD; k++)
+                    v |= (u32)src[me_clip3(0, P.pic_w - 1, x0 + k)] << (8 * (int)sizeof(pel) * k);
+            }
+            *(u32 *)(curl + r * 128 + q * SPD) = v;
+        }
     }
     __syncthreads();
 
