@@ -13,8 +13,10 @@
 #endif
 // 1: one pass over the displacements updates all 58 node classes of a lane;
 // 0: two passes (33 small, then 25 large classes), recomputing the leaves.
+// One pass is faster: the leaves' SAD instructions (v_qsad_pk_u16_u8 issues
+// at a quarter of the rate of a plain 32-bit op) then run once.
 #ifndef ME_IS_FUSED
-#define ME_IS_FUSED 0
+#define ME_IS_FUSED 1
 #endif
 
 // Values of NK displacements of one node: at 8 bits four displacements are
@@ -45,14 +47,18 @@ __device__ __forceinline__ Val vsel(bool c, Val a, Val b) { return c ? a : b; }
 
 // Uniform key parts of the NK displacements being processed.  A displacement
 // outside the block's range gets a high word that can never win.
+// The key parts are vector values rather than arrays: an array member let
+// the compiler keep them in private memory and reload them as pairs.
+typedef u32 u32xk __attribute__((ext_vector_type(NK == 1 ? 2 : NK)));
 struct KeyCtx {
-    u32 khi[NK], klo[NK];
+    u32xk khi, klo;
     int dx[NK];
     int dy, sh;
 };
 
 
 typedef u32 u32x2 __attribute__((ext_vector_type(2)));
+typedef u32 u32x2a __attribute__((ext_vector_type(2), aligned(4)));
 
 __device__ __forceinline__ bool node_valid(int px, int py, int dx, int dy, int wlim, int hlim)
 {
@@ -87,10 +93,17 @@ __device__ __forceinline__ void upd1(double &st, u32 sad, const KeyCtx &c, int k
 //   xor 8        : DPP row_ror:8             (y pairs)
 //   xor 16       : ds_swizzle xor mode       (y quads)
 //   xor 32       : ds_bpermute               (the other half of the CTU)
-__device__ __forceinline__ u32 px1(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false); }
-__device__ __forceinline__ u32 px2(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false); }
-__device__ __forceinline__ u32 px4(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false); }
-__device__ __forceinline__ u32 py1(u32 v) { return (u32)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false); }
+// Every source lane of these four patterns exists, so bound_ctrl (zero for a
+// missing source) never applies; with it the compiler folds each move into
+// the add that consumes it (v_add_u32_dpp).
+template <int CTRL> __device__ __forceinline__ u32 dppx(u32 v)
+{
+    return (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
+}
+__device__ __forceinline__ u32 px1(u32 v) { return dppx<0xB1>(v); }
+__device__ __forceinline__ u32 px2(u32 v) { return dppx<0x4E>(v); }
+__device__ __forceinline__ u32 px4(u32 v) { return dppx<0x141>(v); }
+__device__ __forceinline__ u32 py1(u32 v) { return dppx<0x128>(v); }
 __device__ __forceinline__ u32 py2(u32 v) { return (u32)__builtin_amdgcn_ds_swizzle((int)v, 0x401F); }
 __device__ __forceinline__ u32 py4(u32 v) { return (u32)__shfl_xor((int)v, 32, 64); }
 #if BIT_DEPTH == 8
@@ -280,15 +293,17 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
         for (int c = 0; c < 4; c++) acc[r][c] = 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
+        // each overlapping dword pair is read on its own: a 64-bit operand
+        // needs an even register pair, so pairs (d1,d2) and (d3,d4) of one
+        // five-dword read would each cost a register copy
         const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0);
-        u32 d[5];
-#pragma unroll
-        for (int j = 0; j < 5; j++) d[j] = rp[j];
         const uint4 cw = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0);
         const u32 cc[4] = {cw.x, cw.y, cw.z, cw.w};
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            acc[r >> 2][j] = __builtin_amdgcn_qsad_pk_u16_u8(((u64)d[j + 1] << 32) | d[j], cc[j], acc[r >> 2][j]);
+        for (int j = 0; j < 4; j++) {
+            const u32x2a p = *(const u32x2a *)(rp + j);
+            acc[r >> 2][j] = __builtin_amdgcn_qsad_pk_u16_u8(((u64)p.y << 32) | p.x, cc[j], acc[r >> 2][j]);
+        }
     }
 #pragma unroll
     for (int r = 0; r < 4; r++)
@@ -490,8 +505,9 @@ __device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, const d
 // One block = (displacement tile, CTU, reference); 64*WAVES threads.  The
 // reference window of the tile and the current CTU are staged in LDS once;
 // then each wave covers the whole CTU (lane = 16x16 region) for every
-// WAVES-th displacement row of the tile, in two passes (small PUs, large
-// PUs) so that each pass's node states fit in registers.
+// WAVES-th displacement row of the tile, updating the node states of all
+// 58 classes per displacement (ME_IS_FUSED, the default) or in two passes
+// (small PUs, then large PUs) with half the live node states.
 template <int MODE, int WAVES>
 // The LDS window allows at most 2 workgroups per CU (8 waves: 2 per SIMD),
 // so the register budget of 2 waves per SIMD costs no occupancy.
