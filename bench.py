@@ -277,8 +277,8 @@ def main():
         # 4 displacement rows per workgroup: short per-batch latency at light load
         cfg = ua.ServerCfg(32, wl.lambda_mv(32), ua.MASK_16x16, 1, 64 << 30, 128, 4, 100, 4, 4, 0)
         host8 = ua.MeHost(8)
-        lat, info, st, _ = host8.serve_load(cfg, sizes, seeds, args.serve_frames, args.serve_rate, arrival_seed=7)
-        lat_ms = lat / 1e3
+        slat, info, st, _ = host8.serve_load(cfg, sizes, seeds, args.serve_frames, args.serve_rate, arrival_seed=7)
+        lat_ms = slat / 1e3
         serving = {"streams": "3x1280x720 + 2x1920x1080 + 1x3840x2160, 8-bit, %d frames each" % args.serve_frames,
                    "job": "one CTU: 16x16 PUs, full search +-32, 1 reference (the previous frame), QP 32, sub-pel",
                    "offered_ctus_per_s": args.serve_rate, "requests": info["requests"],

@@ -45,8 +45,7 @@ def host_lib(bd):
 HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []), ("server.hip", []), ("mc.hip", []), ("tz.hip", ["-DME_TZ_PART=0"]), ("tz.hip", ["-DME_TZ_PART=1"]), ("tz.hip", ["-DME_TZ_PART=2"]),
              ("intsearch.hip", ["-DME_IS_WAVES=1"]), ("intsearch.hip", ["-DME_IS_WAVES=2"]),
              ("intsearch.hip", ["-DME_IS_WAVES=4"]),
-             ("subpel.hip", ["-DME_SP_PART=0"]), ("subpel.hip", ["-DME_SP_PART=1"]),
-             ("subpel.hip", ["-DME_SP_PART=2"])]
+             ("subpel.hip", [])]
 BUILD_DIR = os.path.join(ROOT, "_build")
 
 

@@ -6,7 +6,6 @@
 #include "../../include/uavs3e_me_hip.h"
 
 #define ME_MAX_REFS 4
-#define ME_SIDE_STREAMS 3
 
 // A reference picture resident in device memory: the edge-padded integer
 // plane + 15 quarter-pel planes (com_pic_t / com_subpel_t, com_util.c:62-91,
@@ -109,6 +108,18 @@ int me_int_search_launch_w4(int mode, const IntArgs &a, dim3 grid, size_t lds, h
 // Sub-pel refinement (me_sub_pel_search, me.c:366-500, cost_type 0, bi 0)
 // ---------------------------------------------------------------------------
 
+// One PU shape of the sub-pel launch: its com_had tiling, the lanes per PU
+// and the first workgroup (per job and reference) of its range.
+struct SubShape {
+    int pw, ph, tw, th, tpr;    // PU and tile size, tiles per row
+    int base, per_ctu;          // PU index base in the CTU, PUs per CTU
+    int lpu;                    // lanes per PU position group: tiles, 2 x tiles for 16x8 / 8x16 tiles
+    int G;                      // position groups per PU (lpu * G <= 256)
+    int tt;                     // com_had tile type (had_tiling)
+    int blk0;                   // first workgroup of the shape
+    int pad;
+};
+
 struct SubArgs {
     JobSet js;
     int n0;                     // list-0 size (references [n0, nref) form list 1)
@@ -118,17 +129,12 @@ struct SubArgs {
     const u64 *keys;
     me_pu_result_t *res;
     int npu, nref;
-    // the shape handled by one launch
-    int pw, ph, tw, th, tpr, base, per_ctu;
-    int lpu;                    // lanes per PU: tiles, 2 x tiles for 16x8 / 8x16 tiles
+    const SubShape *shp;        // device table of the searched shapes
+    const u8 *blk_shape;        // workgroup -> index into shp
 };
 
-typedef void (*subpel_fn)(SubArgs);
-// Kernel for a com_had tile type (had_tiling); nullptr for an unknown type.
-subpel_fn me_subpel_kernel(int tt);
-subpel_fn me_subpel_kernel_small(int tt);   // tile types 0, 1, 2, 4
-subpel_fn me_subpel_kernel_16x8(int tt);    // tile type 3
-subpel_fn me_subpel_kernel_8x16(int tt);    // tile type 5
+// Every shape in one launch: grid (blocks, jobs, refs) of 256 lanes.
+int me_subpel_launch(const SubArgs &a, int blocks, int jobs, int refs, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // One search configuration: window geometry, key layout, schedule and the
@@ -145,18 +151,19 @@ struct MeSearch {
     u32 *d_rate;             // [2R+1] integer-pel rate per displacement component
     u8 *d_tab_base;          // MV-bits table, tab_off samples each side of 0
     int tab_off;
-    // side streams on which the sub-pel launches of different shapes run
-    // concurrently (fork/join by events, so graph capture records branches)
-    hipStream_t side[ME_SIDE_STREAMS];
-    hipEvent_t fork, join[ME_SIDE_STREAMS];
+    SubShape *d_shp;         // sub-pel shape table of the searched shapes
+    u8 *d_blk_shape;         // sub-pel workgroup -> shape
+    int sub_blocks;          // sub-pel workgroups per (job, reference)
 };
 
 // max_dim = max(pic_w, pic_h) of the pictures searched (the |mv| <= T - 4
 // limit and the MV-bits table size of inter_search_create).  band_rows,
-// waves and dx_cols are schedule choices (0 = default); results do not
-// depend on them.
+// waves and dx_cols are schedule choices (0 = default); sp_groups is the
+// number of sub-pel search positions a PU evaluates at the same time (1, 2,
+// 4 or 8: 1 does the least work, more cut the latency of small batches).
+// Results do not depend on them.
 int me_search_init(MeSearch *s, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
-                   int waves, int dx_cols);
+                   int waves, int dx_cols, int sp_groups);
 void me_search_free(MeSearch *s);
 // Enqueues key initialisation, integer search, sub-pel refinement and the
 // reference choice for the jobs of js.  keys/res are [job][npu][nref],

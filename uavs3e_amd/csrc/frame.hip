@@ -18,6 +18,7 @@
 #include "frame.h"
 
 #include <string.h>
+#include <algorithm>
 #include <vector>
 
 __global__ void k_fill_keys(u64 *keys, size_t n)
@@ -53,12 +54,6 @@ int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_
     }
 }
 
-subpel_fn me_subpel_kernel(int tt)
-{
-    if (tt == 3) return me_subpel_kernel_16x8(tt);
-    if (tt == 5) return me_subpel_kernel_8x16(tt);
-    return me_subpel_kernel_small(tt);
-}
 
 // ---------------------------------------------------------------------------
 // Search configurations
@@ -79,7 +74,7 @@ void me_mvbits_fill(u8 *base, int off)
 }
 
 int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
-                   int waves, int dx_cols)
+                   int waves, int dx_cols, int sp_groups)
 {
     memset(p, 0, sizeof(*p));
     const uint32_t m = shape_mask & ((1u << ME_NSHAPES) - 1);
@@ -134,15 +129,52 @@ int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int
     me_mvbits_fill(tab.data(), p->tab_off);
     std::vector<u32> rate(p->span);
     for (int i = 0; i < p->span; i++) rate[i] = (u32)tab[p->tab_off + 4 * (i - R)] * lambda;
+    // Sub-pel shape table: every searched shape's workgroups in one launch,
+    // the shapes with the most samples per PU first (their workgroups run
+    // longest, so they should not trail at the end of the launch).
+    std::vector<SubShape> shp;
+    std::vector<u8> blk_shape;
+    {
+        const int G = sp_groups >= 8 ? 8 : sp_groups >= 4 ? 4 : sp_groups >= 2 ? 2 : 1;
+        std::vector<int> order;
+        for (int s = 0; s < ME_NSHAPES; s++)
+            if (p->base[s] >= 0) order.push_back(s);
+        std::stable_sort(order.begin(), order.end(), [](int a, int b) {
+            return ME_SHAPES_HOST[a][0] * ME_SHAPES_HOST[a][1] > ME_SHAPES_HOST[b][0] * ME_SHAPES_HOST[b][1];
+        });
+        for (int s : order) {
+            SubShape q;
+            memset(&q, 0, sizeof(q));
+            q.pw = ME_SHAPES_HOST[s][0];
+            q.ph = ME_SHAPES_HOST[s][1];
+            had_tiling(q.pw, q.ph, &q.tt, &q.tw, &q.th);
+            const int T = (q.pw / q.tw) * (q.ph / q.th);
+            if (T > 64 && T != 256) {
+                me_set_error("me_search_init: no sub-pel tiling for a shape", hipErrorInvalidValue, __FILE__, __LINE__);
+                return -1;
+            }
+            q.tpr = q.pw / q.tw;
+            q.base = p->base[s];
+            q.per_ctu = (128 / q.pw) * (128 / q.ph);
+            q.lpu = q.tt == 3 || q.tt == 5 ? 2 * T : T;   // 1..128, or 256 for 128x128
+            // up to sp_groups search positions of a PU at a time, at most
+            // 256 lanes per PU; lanes per (CTU, reference): a multiple of 256
+            q.G = 256 / q.lpu < G ? 256 / q.lpu : G;
+            q.blk0 = (int)blk_shape.size();
+            const int blocks = q.per_ctu * q.lpu * q.G / 256;
+            for (int b = 0; b < blocks; b++) blk_shape.push_back((u8)shp.size());
+            shp.push_back(q);
+        }
+        p->sub_blocks = (int)blk_shape.size();
+    }
     hipError_t e = hipMalloc((void **)&p->d_rate, p->span * sizeof(u32));
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tab_base, tab.size());
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
-    for (int i = 0; i < ME_SIDE_STREAMS && e == hipSuccess; i++) {
-        e = hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join[i], hipEventDisableTiming);
-    }
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_shp, shp.size() * sizeof(SubShape));
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_blk_shape, blk_shape.size());
     if (e == hipSuccess) e = hipMemcpy(p->d_rate, rate.data(), p->span * sizeof(u32), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_tab_base, tab.data(), tab.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_shp, shp.data(), shp.size() * sizeof(SubShape), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_blk_shape, blk_shape.data(), blk_shape.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         me_set_error("me_search_init", e, __FILE__, __LINE__);
         me_search_free(p);
@@ -155,16 +187,12 @@ void me_search_free(MeSearch *p)
 {
     if (p->d_rate) (void)hipFree(p->d_rate);
     if (p->d_tab_base) (void)hipFree(p->d_tab_base);
+    if (p->d_shp) (void)hipFree(p->d_shp);
+    if (p->d_blk_shape) (void)hipFree(p->d_blk_shape);
     p->d_rate = nullptr;
     p->d_tab_base = nullptr;
-    if (p->fork) (void)hipEventDestroy(p->fork);
-    p->fork = nullptr;
-    for (int i = 0; i < ME_SIDE_STREAMS; i++) {
-        if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
-        if (p->join[i]) (void)hipEventDestroy(p->join[i]);
-        p->side[i] = nullptr;
-        p->join[i] = nullptr;
-    }
+    p->d_shp = nullptr;
+    p->d_blk_shape = nullptr;
 }
 
 int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
@@ -195,44 +223,9 @@ int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_resu
     b.tab = p->d_tab_base + p->tab_off;
     b.keys = keys; b.res = res;
     b.npu = p->npu; b.nref = p->nref;
-    // Each shape's launch fills only a few hundred workgroups, so the shapes
-    // are spread over the caller's stream and the side streams (forked from
-    // and joined back into st by events).
-    int nshapes = 0;
-    for (int s = 0; s < ME_NSHAPES; s++) nshapes += p->base[s] >= 0;
-    const int nst = nshapes > 1 ? 1 + ME_SIDE_STREAMS : 1;
-    if (nst > 1) {
-        ME_CHECK(hipEventRecord(p->fork, st));
-        for (int i = 0; i < ME_SIDE_STREAMS; i++) ME_CHECK(hipStreamWaitEvent(p->side[i], p->fork, 0));
-    }
-    int li = 0;
-    for (int s = 0; s < ME_NSHAPES; s++) {
-        if (p->base[s] < 0) continue;
-        const int w = ME_SHAPES_HOST[s][0], h = ME_SHAPES_HOST[s][1];
-        int tt, tw, th;
-        had_tiling(w, h, &tt, &tw, &th);
-        const int T = (w / tw) * (h / th);
-        subpel_fn fn = me_subpel_kernel(tt);
-        if (!fn || (T > 64 && T != 256)) {
-            me_set_error("no sub-pel kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__);
-            return -1;
-        }
-        b.pw = w; b.ph = h; b.tw = tw; b.th = th; b.tpr = w / tw; b.base = p->base[s];
-        b.per_ctu = (128 / w) * (128 / h);
-        b.lpu = tt == 3 || tt == 5 ? 2 * T : T;   // 1..128, or 256 for 128x128
-        // lanes per (CTU, reference): 256..1024; a unit of more than 64
-        // lanes is a workgroup of its own
-        const int lanes = b.per_ctu * b.lpu, bs = b.lpu > 64 ? b.lpu : (lanes < 256 ? lanes : 256);
-        hipStream_t ls = (li % nst) == 0 ? st : p->side[li % nst - 1];
-        li++;
-        hipLaunchKernelGGL(fn, dim3(lanes / bs, js.n_jobs, p->nref), dim3(bs), 0, ls, b);
-        ME_CHECK(hipGetLastError());
-    }
-    if (nst > 1)
-        for (int i = 0; i < ME_SIDE_STREAMS; i++) {
-            ME_CHECK(hipEventRecord(p->join[i], p->side[i]));
-            ME_CHECK(hipStreamWaitEvent(st, p->join[i], 0));
-        }
+    b.shp = p->d_shp;
+    b.blk_shape = p->d_blk_shape;
+    if (me_subpel_launch(b, p->sub_blocks, js.n_jobs, p->nref, st)) return -1;
     const int npus = js.n_jobs * p->npu;
     hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, res, best, npus, p->nref,
                        p->nref - p->nref_l1, p->nl);
@@ -278,7 +271,7 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     p->n_ctu = p->ctu_cols * p->ctu_rows;
     const int T = cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h;
     if (me_search_init(&p->s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, cfg->num_refs_l1, T,
-                       cfg->band_rows, cfg->waves, cfg->dx_cols)) {
+                       cfg->band_rows, cfg->waves, cfg->dx_cols, 1)) {
         free(p);
         return -1;
     }

@@ -317,9 +317,10 @@ ME_EXPORT int me_server_create(me_server_t *out, const me_server_cfg_t *cfg)
     S->device = me_hip_device();
     memset(&S->stats, 0, sizeof(S->stats));
     // every stream must satisfy R <= max(W,H) - 4, so the window's uniform
-    // part is [-R, R] for all of them (checked at me_server_open_stream)
+    // part is [-R, R] for all of them (checked at me_server_open_stream).
+    // Batches are a few CTUs: sub-pel positions run 4 at a time for latency.
     if (me_search_init(&S->search, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, 0,
-                       cfg->search_range + 4, cfg->band_rows, cfg->waves, cfg->dx_cols)) {
+                       cfg->search_range + 4, cfg->band_rows, cfg->waves, cfg->dx_cols, 4)) {
         delete S;
         return -1;
     }
