@@ -502,6 +502,34 @@ __device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, const d
     if (ex3 && ey3) PUB(S128x128, 128, 128, X, Y, st[24]);
 }
 
+// The waves of a block hold node states for different displacement rows of
+// the same nodes: merge them in LDS (the window is no longer needed), so that
+// one wave publishes per block.  Returns whether this wave publishes.
+template <int WAVES, int NS>
+__device__ __forceinline__ bool merge_waves(double *st, unsigned char *smem)
+{
+    if (WAVES == 1) return true;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    u64 *red = (u64 *)smem;   // [NS][64]
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int i = 0; i < NS; i++) red[i * 64 + lane] = __builtin_bit_cast(u64, st[i]);
+    }
+    __syncthreads();
+    if (wave != 0) {
+#pragma unroll
+        for (int i = 0; i < NS; i++)
+            __hip_atomic_fetch_min(red + i * 64 + lane, __builtin_bit_cast(u64, st[i]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    if (wave != 0) return false;
+#pragma unroll
+    for (int i = 0; i < NS; i++) st[i] = __builtin_bit_cast(double, red[i * 64 + lane]);
+    return true;
+}
+
 // One block = (displacement tile, CTU, reference); 64*WAVES threads.  The
 // reference window of the tile and the current CTU are staged in LDS once;
 // then each wave covers the whole CTU (lane = 16x16 region) for every
@@ -595,7 +623,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         double st[1];
         if (fast) run_pass<2, WAVES, false, true, 1>(a, P, b, win, curl, st);
         else run_pass<2, WAVES, true, false, 1>(a, P, b, win, curl, st);
-        PUB(S16x16, 16, 16, rx * 16, ry * 16, st[0]);
+        if (merge_waves<WAVES, 1>(st, smem)) PUB(S16x16, 16, 16, rx * 16, ry * 16, st[0]);
         return;
     }
 #if ME_IS_FUSED
@@ -603,8 +631,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         double st[N_SMALL + N_LARGE];
         if (fast) run_pass<3, WAVES, false, true, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
         else run_pass<3, WAVES, true, false, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
-        publish_small(a, kb, st, rx * 16, ry * 16);
-        publish_large(a, kb, st + N_SMALL, rx, ry);
+        if (merge_waves<WAVES, N_SMALL + N_LARGE>(st, smem)) {
+            publish_small(a, kb, st, rx * 16, ry * 16);
+            publish_large(a, kb, st + N_SMALL, rx, ry);
+        }
     }
 #else
     {
