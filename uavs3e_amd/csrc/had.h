@@ -82,10 +82,91 @@ __device__ __forceinline__ void fwht(int *v, int step)
             }
 }
 
+#if BIT_DEPTH == 8
+// 8-bit tiles run on packed 16-bit lanes, two samples per operation: every
+// intermediate of an 8x8 transform of residuals in [-510, 510] stays within
+// +-16320 because the last butterfly is never formed: for the last stage's
+// pairs (a, b), |a + b| + |a - b| = 2 max(|a|, |b|).  A Walsh-Hadamard
+// transform's stages commute, so the row pairs (x, x+2) of a dword take that
+// last stage.
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+// NW dwords of the samples starting at p (any alignment).
+template <int NW>
+__device__ __forceinline__ void load_words(const pel *p, u32 *w)
+{
+    const uintptr_t ad = (uintptr_t)p;
+    const u32 *q = (const u32 *)(ad & ~(uintptr_t)3);
+    const u32 sh = (u32)(ad & 3);
+    u32 d[NW + 1];
+#pragma unroll
+    for (int i = 0; i <= NW; i++) d[i] = q[i];
+#pragma unroll
+    for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+}
+
+// The residual org - cur of one row of TW samples (TW = 4, 8) as TW/2 pairs:
+// v[2j] = (r[4j], r[4j+2]), v[2j+1] = (r[4j+1], r[4j+3]).
+template <int TW>
+__device__ __forceinline__ void res_pairs(const pel *org, const pel *cur, s16x2 *v)
+{
+    u32 ow[TW / 4], cw[TW / 4];
+    load_words<TW / 4>(org, ow);
+    load_words<TW / 4>(cur, cw);
+#pragma unroll
+    for (int j = 0; j < TW / 4; j++) {
+        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, ow[j], 0x0C020C00u)) -
+                   __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, cw[j], 0x0C020C00u));
+        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, ow[j], 0x0C030C01u)) -
+                       __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, cw[j], 0x0C030C01u));
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void fwht_pk(s16x2 *v, int step)
+{
+#pragma unroll
+    for (int hs = 1; hs < N; hs <<= 1)
+#pragma unroll
+        for (int i = 0; i < N; i += hs << 1)
+#pragma unroll
+            for (int j = i; j < i + hs; j++) {
+                const s16x2 p = v[j * step], q = v[(j + hs) * step];
+                v[j * step] = p + q;
+                v[(j + hs) * step] = p - q;
+            }
+}
+
+// Sum of |coef| of the TW x TH transform whose rows are given as pairs.
+template <int TW, int TH>
+__device__ __forceinline__ int had_pk_sum(s16x2 *v)
+{
+    constexpr int K = TW / 2;
+#pragma unroll
+    for (int y = 0; y < TH; y++) fwht_pk<K>(v + y * K, 1);
+#pragma unroll
+    for (int k = 0; k < K; k++) fwht_pk<TH>(v + k, K);
+    u32 s = 0;
+#pragma unroll
+    for (int i = 0; i < K * TH; i++) {
+        const u32 a = __builtin_bit_cast(u32, __builtin_elementwise_max(v[i], (s16x2)0 - v[i]));
+        const u32 lo = a & 0xFFFFu, hi = a >> 16;   // both in [0, 16320]
+        s += lo > hi ? lo : hi;
+    }
+    return (int)(2 * s);
+}
+#endif
+
 // Sum of |coef| of the 2-D Walsh-Hadamard transform of a TW x TH residual.
 template <int TW, int TH>
 __device__ __forceinline__ int had_abs(const pel *org, int so, const pel *cur, int sc)
 {
+#if BIT_DEPTH == 8
+    s16x2 v[TH * TW / 2];
+#pragma unroll
+    for (int y = 0; y < TH; y++) res_pairs<TW>(org + y * so, cur + y * sc, v + y * (TW / 2));
+    return had_pk_sum<TW, TH>(v);
+#else
     int m[TW * TH];
 #pragma unroll
     for (int y = 0; y < TH; y++) {
@@ -102,6 +183,7 @@ __device__ __forceinline__ int had_abs(const pel *org, int so, const pel *cur, i
 #pragma unroll
     for (int i = 0; i < TW * TH; i++) s += abs(m[i]);
     return s;
+#endif
 }
 
 // A 16-point WHT is the 8-point WHT of the half sums and of the half
@@ -113,6 +195,19 @@ __device__ __forceinline__ int had_half(const pel *org, int so, const pel *cur, 
 {
     const int sg = half ? -1 : 1;
     const int o2 = TT == 3 ? 8 : 8 * so, c2 = TT == 3 ? 8 : 8 * sc;
+#if BIT_DEPTH == 8
+    s16x2 v[32];
+    const s16x2 sgv = (s16x2)(short)sg;
+#pragma unroll
+    for (int y = 0; y < 8; y++) {
+        s16x2 b[4];
+        res_pairs<8>(org + y * so, cur + y * sc, v + y * 4);
+        res_pairs<8>(org + y * so + o2, cur + y * sc + c2, b);
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[y * 4 + k] += b[k] * sgv;
+    }
+    return had_pk_sum<8, 8>(v);
+#else
     int m[64];
 #pragma unroll
     for (int y = 0; y < 8; y++) {
@@ -131,6 +226,7 @@ __device__ __forceinline__ int had_half(const pel *org, int so, const pel *cur, 
 #pragma unroll
     for (int i = 0; i < 64; i++) s += abs(m[i]);
     return s;
+#endif
 }
 
 // One normalised cost_satd tile (com_cost.c:259, 340, 467, 586, 649, 708)
