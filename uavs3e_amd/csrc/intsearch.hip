@@ -178,6 +178,160 @@ __device__ __forceinline__ void pass_small(const Region &g, double *st, const Ke
     }
 }
 
+#if BIT_DEPTH == 8
+// Key parts of one displacement per lane (k chosen by lane bits).
+struct KeyV {
+    u32 hi, lo;
+    int dx;
+};
+
+template <bool EDGE, bool SH0>
+__device__ __forceinline__ void updv(double &st, u32 sad, const KeyV &v, const KeyCtx &c, int px, int py, int wlim,
+                                     int hlim)
+{
+    u32 hi = (SH0 ? sad : (sad << c.sh)) + v.hi;
+    if (EDGE) {
+        if (!node_valid(px, py, v.dx, c.dy, wlim, hlim)) hi |= KEY_INVALID_HI;
+    }
+    u32x2 key = {v.lo, hi};
+    st = me_vmin64(st, __builtin_bit_cast(double, key));
+}
+
+__device__ __forceinline__ KeyV key_sel(bool b, const KeyV &one, const KeyV &zero)
+{
+    return {b ? one.hi : zero.hi, b ? one.lo : zero.lo, b ? one.dx : zero.dx};
+}
+
+__device__ __forceinline__ KeyV key_k(const KeyCtx &c, int k) { return {c.khi[k], c.klo[k], c.dx[k]}; }
+
+// 16-bit lane sums without carries between lanes: lo + lo, hi + hi.
+__device__ __forceinline__ u32 lo16(u32 v) { return v & 0xFFFFu; }
+__device__ __forceinline__ u32 hi16(u32 v) { return v >> 16; }
+
+// The 19 sparse classes (nodes of 32x16 and larger that are not dense pair
+// slots) with the four displacements spread over the lanes that share a
+// node: a sum over a lane pair hands each lane of the pair the sums of
+// different displacements (a reduce-scatter), so each lane updates each
+// class for one or two displacements instead of four.  Lane bits
+// x0 = (rx ^ rx>>2) & 1, x1 = (rx>>1 ^ rx>>2) & 1, y0 = ry&1, y1 = ry>>1&1:
+// the x bits are taken relative to x2 = rx>>2 because the 128-wide sums pair
+// lane rx with lane 7-rx (DPP row_half_mirror), which then holds the same
+// displacement.  The displacement of a value:
+//   pair by x0 (k = 2x0, 2x0+1): 32x16, 64x8        pair by y0: 16x32, 8x64
+//   k = 2x0 + y0: 32x32 and every class built from it (64x32 ... 128x128)
+//   k = 2x0 + x1: 64x16, 128x16                       k = 2y0 + y1: 16x64, 16x128
+// The lanes holding different displacements of a node are merged before
+// publishing (merge_sparse).
+template <bool EDGE, bool SH0>
+__device__ __forceinline__ void sparse_scatter(const Region &g, const Val *t32x8, const Val *t8x32, double *st,
+                                               const KeyCtx &c, int rx, int ry, int cx, int cy, int wlim, int hlim)
+{
+    const int X = rx * 16, Y = ry * 16, px = cx + X, py = cy + Y;
+    const int x2 = cx + (X & ~31), x4 = cx + (X & ~63), y2 = cy + (Y & ~31), y4 = cy + (Y & ~63);
+    const bool x0 = (rx ^ (rx >> 2)) & 1, x1 = ((rx >> 1) ^ (rx >> 2)) & 1, y0 = ry & 1, y1 = (ry >> 1) & 1;
+    const KeyV kxa = key_sel(x0, key_k(c, 2), key_k(c, 0)), kxb = key_sel(x0, key_k(c, 3), key_k(c, 1));
+    const KeyV kya = key_sel(y0, key_k(c, 2), key_k(c, 0)), kyb = key_sel(y0, key_k(c, 3), key_k(c, 1));
+    const KeyV kxy = key_sel(y0, kxb, kxa), kxx = key_sel(x1, kxb, kxa), kyy = key_sel(y1, kyb, kya);
+    const Val s = g.s16;
+    // x pairs: lane x0 keeps displacements 2x0, 2x0+1 (the lo word holds 0, 1)
+    const u32 sx = px1(x0 ? s.lo : s.hi), kx = x0 ? s.hi : s.lo;
+    const u32 t32x16a = lo16(kx) + lo16(sx), t32x16b = hi16(kx) + hi16(sx);
+    const u32 sy = py1(y0 ? s.lo : s.hi), ky = y0 ? s.hi : s.lo;
+    const u32 t16x32a = lo16(ky) + lo16(sy), t16x32b = hi16(ky) + hi16(sy);
+    updv<EDGE, SH0>(st[6], t32x16a, kxa, c, x2, py, wlim, hlim);
+    updv<EDGE, SH0>(st[6], t32x16b, kxb, c, x2, py, wlim, hlim);
+    updv<EDGE, SH0>(st[7], t16x32a, kya, c, px, y2, wlim, hlim);
+    updv<EDGE, SH0>(st[7], t16x32b, kyb, c, px, y2, wlim, hlim);
+    // 32x32 and up: lane y0 keeps displacement 2x0 + y0
+    const u32 t32x32 = (y0 ? t32x16b : t32x16a) + py1(y0 ? t32x16a : t32x16b);
+    const u32 t64x32 = t32x32 + px2(t32x32), t32x64 = t32x32 + py2(t32x32);
+    const u32 t64x64 = t64x32 + py2(t64x32);
+    const u32 t128x32 = t64x32 + px4(t64x32), t32x128 = t32x64 + py4(t32x64);
+    const u32 t128x64 = t64x64 + px4(t64x64), t64x128 = t64x64 + py4(t64x64);
+    const u32 t128x128 = t128x64 + py4(t128x64);
+    updv<EDGE, SH0>(st[8], t32x32, kxy, c, x2, y2, wlim, hlim);
+    updv<EDGE, SH0>(st[15], t64x32, kxy, c, x4, y2, wlim, hlim);
+    updv<EDGE, SH0>(st[16], t32x64, kxy, c, x2, y4, wlim, hlim);
+    updv<EDGE, SH0>(st[17], t64x64, kxy, c, x4, y4, wlim, hlim);
+    updv<EDGE, SH0>(st[20], t128x32, kxy, c, cx, y2, wlim, hlim);
+    updv<EDGE, SH0>(st[21], t32x128, kxy, c, x2, cy, wlim, hlim);
+    updv<EDGE, SH0>(st[22], t128x64, kxy, c, cx, y4, wlim, hlim);
+    updv<EDGE, SH0>(st[23], t64x128, kxy, c, x4, cy, wlim, hlim);
+    updv<EDGE, SH0>(st[24], t128x128, kxy, c, cx, cy, wlim, hlim);
+    // 64x16, 128x16: lane x1 keeps displacement 2x0 + x1
+    const u32 t64x16 = (x1 ? t32x16b : t32x16a) + px2(x1 ? t32x16a : t32x16b);
+    const u32 t128x16 = t64x16 + px4(t64x16);
+    updv<EDGE, SH0>(st[11], t64x16, kxx, c, x4, py, wlim, hlim);
+    updv<EDGE, SH0>(st[18], t128x16, kxx, c, cx, py, wlim, hlim);
+    // 16x64, 16x128: lane y1 keeps displacement 2y0 + y1
+    const u32 t16x64 = (y1 ? t16x32b : t16x32a) + py2(y1 ? t16x32a : t16x32b);
+    const u32 t16x128 = t16x64 + py4(t16x64);
+    updv<EDGE, SH0>(st[14], t16x64, kyy, c, px, y4, wlim, hlim);
+    updv<EDGE, SH0>(st[19], t16x128, kyy, c, px, cy, wlim, hlim);
+    // 64x8 (from 32x8, equal on both lanes of an x pair) and 8x64
+#pragma unroll
+    for (int r2 = 0; r2 < 2; r2++) {
+        const u32 v = x0 ? t32x8[r2].hi : t32x8[r2].lo, w = px2(v);
+        updv<EDGE, SH0>(st[9 + r2], lo16(v) + lo16(w), kxa, c, x4, py + 8 * r2, wlim, hlim);
+        updv<EDGE, SH0>(st[9 + r2], hi16(v) + hi16(w), kxb, c, x4, py + 8 * r2, wlim, hlim);
+    }
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++) {
+        const u32 v = y0 ? t8x32[c2].hi : t8x32[c2].lo, w = py2(v);
+        updv<EDGE, SH0>(st[12 + c2], lo16(v) + lo16(w), kya, c, px + 8 * c2, y4, wlim, hlim);
+        updv<EDGE, SH0>(st[12 + c2], hi16(v) + hi16(w), kyb, c, px + 8 * c2, y4, wlim, hlim);
+    }
+}
+
+__device__ __forceinline__ double kmin(double a, double b) { return me_vmin64(a, b); }
+__device__ __forceinline__ double dpp64_x1(double v)
+{
+    const u64 u = __builtin_bit_cast(u64, v);
+    return __builtin_bit_cast(double, ((u64)px1((u32)(u >> 32)) << 32) | px1((u32)u));
+}
+__device__ __forceinline__ double dpp64_x2(double v)
+{
+    const u64 u = __builtin_bit_cast(u64, v);
+    return __builtin_bit_cast(double, ((u64)px2((u32)(u >> 32)) << 32) | px2((u32)u));
+}
+__device__ __forceinline__ double dpp64_y1(double v)
+{
+    const u64 u = __builtin_bit_cast(u64, v);
+    return __builtin_bit_cast(double, ((u64)py1((u32)(u >> 32)) << 32) | py1((u32)u));
+}
+__device__ __forceinline__ double dpp64_y2(double v)
+{
+    const u64 u = __builtin_bit_cast(u64, v);
+    return __builtin_bit_cast(double, ((u64)py2((u32)(u >> 32)) << 32) | py2((u32)u));
+}
+
+// Merges the lanes that hold different displacements of one sparse node, so
+// that the node's publishing lane holds the minimum over all four.
+__device__ __forceinline__ void merge_sparse(double *st)
+{
+    st[6] = kmin(st[6], dpp64_x1(st[6]));
+    st[9] = kmin(st[9], dpp64_x1(st[9]));
+    st[10] = kmin(st[10], dpp64_x1(st[10]));
+    st[7] = kmin(st[7], dpp64_y1(st[7]));
+    st[12] = kmin(st[12], dpp64_y1(st[12]));
+    st[13] = kmin(st[13], dpp64_y1(st[13]));
+    const int xy[] = {8, 15, 16, 17, 20, 21, 22, 23, 24};
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        st[xy[i]] = kmin(st[xy[i]], dpp64_x1(st[xy[i]]));
+        st[xy[i]] = kmin(st[xy[i]], dpp64_y1(st[xy[i]]));
+    }
+    st[11] = kmin(st[11], dpp64_x1(st[11]));
+    st[11] = kmin(st[11], dpp64_x2(st[11]));
+    st[18] = kmin(st[18], dpp64_x1(st[18]));
+    st[18] = kmin(st[18], dpp64_x2(st[18]));
+    st[14] = kmin(st[14], dpp64_y1(st[14]));
+    st[14] = kmin(st[14], dpp64_y2(st[14]));
+    st[19] = kmin(st[19], dpp64_y1(st[19]));
+    st[19] = kmin(st[19], dpp64_y2(st[19]));
+}
+#endif
+
 // Pass L: the 609 PUs larger than a 16x16 region.
 // Pairs of regions (x pairs, y pairs) hold 14 nodes between two lanes; each
 // lane keeps half of them (3 "dense" slots per direction, chosen by the lane's
@@ -209,7 +363,6 @@ __device__ __forceinline__ void pass_large(const Region &g, double *st, const Ke
     for (int c2 = 0; c2 < 2; c2++) t8x32[c2] = g.v8x16[c2] + vpy1(g.v8x16[c2]);
     const Val d0 = vsel(xr, t32x4[2], t32x4[0]), d1 = vsel(xr, t32x4[3], t32x4[1]), d2 = vsel(xr, t32x8[1], t32x8[0]);
     const Val d3 = vsel(yr, t4x32[2], t4x32[0]), d4 = vsel(yr, t4x32[3], t4x32[1]), d5 = vsel(yr, t8x32[1], t8x32[0]);
-    // sparse classes are summed one displacement at a time (sums exceed 16 bits)
 #pragma unroll
     for (int k = 0; k < NK; k++) {
         upd1<EDGE, SH0>(st[0], d0.get(k), c, k, x2, py + (xr ? 8 : 0), wlim, hlim);
@@ -218,6 +371,13 @@ __device__ __forceinline__ void pass_large(const Region &g, double *st, const Ke
         upd1<EDGE, SH0>(st[3], d3.get(k), c, k, px + (yr ? 8 : 0), y2, wlim, hlim);
         upd1<EDGE, SH0>(st[4], d4.get(k), c, k, px + (yr ? 12 : 4), y2, wlim, hlim);
         upd1<EDGE, SH0>(st[5], d5.get(k), c, k, px + (yr ? 8 : 0), y2, wlim, hlim);
+    }
+#if BIT_DEPTH == 8
+    sparse_scatter<EDGE, SH0>(g, t32x8, t8x32, st, c, rx, ry, cx, cy, wlim, hlim);
+#else
+    // sparse classes, one displacement at a time
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
         const u32 s = g.s16.get(k);
         const u32 t32x16 = s + px1(s), t16x32 = s + py1(s);
         const u32 t32x32 = t32x16 + py1(t32x16);
@@ -251,6 +411,7 @@ __device__ __forceinline__ void pass_large(const Region &g, double *st, const Ke
         upd1<EDGE, SH0>(st[23], t64x128, c, k, x4, cy, wlim, hlim);
         upd1<EDGE, SH0>(st[24], t128x128, c, k, cx, cy, wlim, hlim);
     }
+#endif
 }
 
 template <bool EDGE, bool SH0>
@@ -632,6 +793,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         if (fast) run_pass<3, WAVES, false, true, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
         else run_pass<3, WAVES, true, false, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
         if (merge_waves<WAVES, N_SMALL + N_LARGE>(st, smem)) {
+#if BIT_DEPTH == 8
+            merge_sparse(st + N_SMALL);
+#endif
             publish_small(a, kb, st, rx * 16, ry * 16);
             publish_large(a, kb, st + N_SMALL, rx, ry);
         }
@@ -647,6 +811,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         double st[N_LARGE];
         if (fast) run_pass<1, WAVES, false, true, N_LARGE>(a, P, b, win, curl, st);
         else run_pass<1, WAVES, true, false, N_LARGE>(a, P, b, win, curl, st);
+#if BIT_DEPTH == 8
+        merge_sparse(st);
+#endif
         publish_large(a, kb, st, rx, ry);
     }
 #endif
