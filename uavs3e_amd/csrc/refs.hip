@@ -39,47 +39,118 @@ __global__ void k_htmp(const pel *__restrict__ p0, int stride, int w, int h, s16
     tmp[(size_t)(dx - 1) * tw * th + (size_t)yi * tw + xi] = (s16)((t + add) >> sh);
 }
 
+// N consecutive samples of type T from any sample-aligned address, read as
+// dwords and realigned with v_alignbyte (reads at most 3 bytes past the last
+// sample: inside the padded planes, whose rows and allocations run further).
+template <typename T, int N>
+__device__ __forceinline__ void load_samples(const T *p, int *out)
+{
+    constexpr int NB = N * (int)sizeof(T), NW = (NB + 3) / 4;
+    const uintptr_t ad = (uintptr_t)p;
+    const u32 *q = (const u32 *)(ad & ~(uintptr_t)3);
+    const u32 sh = (u32)(ad & 3);
+    u32 d[NW + 1], w[NW];
+#pragma unroll
+    for (int i = 0; i <= NW; i++) d[i] = q[i];
+#pragma unroll
+    for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        if (sizeof(T) == 1) out[i] = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+        else out[i] = (int)(T)((w[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
+    }
+}
+
+// One padded sample of plane idx at (X, Y) (plane coordinates).
+__device__ __forceinline__ int plane_sample(const pel *p0, int stride, int w, int h, const s16 *tmp, int idx, int X,
+                                            int Y)
+{
+    const int dx = idx & 3, dy = idx >> 2;
+    const int x = me_clip3(-4, w + 3, X - ME_PAD), y = me_clip3(-4, h + 3, Y - ME_PAD);
+    int t = 0;
+    if (dy == 0) {
+        const pel *s = p0 + (ptrdiff_t)y * stride + x;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[k - 3] * ME_LUMA_TAPS[dx][k];
+        return (t + 32) >> 6;
+    }
+    if (dx == 0) {
+        const pel *s = p0 + (ptrdiff_t)y * stride + x;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * stride] * ME_LUMA_TAPS[dy][k];
+        return (t + 32) >> 6;
+    }
+    const int tw = w + 16, th = h + 16;
+    const s16 *s = tmp + (size_t)(dx - 1) * tw * th + (size_t)(y + 8) * tw + (x + 8);
+#pragma unroll
+    for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * tw] * ME_LUMA_TAPS[dy][k];
+    const int sh2 = 20 - BIT_DEPTH;
+    return (t + (1 << (sh2 - 1))) >> sh2;
+}
+
 // The 15 sub-pel planes of com_if_luma_frame (com_mc.c:760-800) including
 // the final com_img_padding(plane, 1, 4): every padded sample equals the
 // filter output at its position clamped into the [-4, w+4) x [-4, h+4) band.
 //   dy == 0: clip((H8(P0) + 32) >> 6)              com_mc.c:668-702
 //   dx == 0: clip((V8(P0) + 32) >> 6)              com_mc.c:704-729
 //   else:    clip((V8(tmp) + 2^(19-bd)) >> (20-bd)) com_mc.c:731-758
+// A thread writes 4 consecutive samples of a row; where none of them is
+// clamped, the 8-tap inputs are read as whole rows of samples.
 __global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h,
                          const s16 *__restrict__ tmp)
 {
-    int X = blockIdx.x * blockDim.x + threadIdx.x;
-    int Y = blockIdx.y;
-    int idx = blockIdx.z + 1;
-    if (X >= stride || Y >= rows) return;
-    int dx = idx & 3, dy = idx >> 2;
-    int x = me_clip3(-4, w + 3, X - ME_PAD);
-    int y = me_clip3(-4, h + 3, Y - ME_PAD);
+    const int X0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const int Y = blockIdx.y;
+    const int idx = blockIdx.z + 1;
+    if (X0 >= stride || Y >= rows) return;
+    const int dx = idx & 3, dy = idx >> 2;
     const int maxv = (1 << BIT_DEPTH) - 1;
     const pel *p0 = planes + (size_t)ME_PAD * stride + ME_PAD;
-    int v;
-    if (dy == 0) {
-        const pel *s = p0 + (ptrdiff_t)y * stride + x;
-        int t = 0;
+    pel *dst = planes + (size_t)idx * plane_elems + (size_t)Y * stride + X0;
+    const int x0 = X0 - ME_PAD, y = me_clip3(-4, h + 3, Y - ME_PAD);
+    int v[4];
+    if (x0 >= -4 && x0 + 3 <= w + 3 && X0 + 3 < stride) {
+        int acc[4] = {0, 0, 0, 0};
+        if (dy == 0) {
+            int s[11];
+            load_samples<pel, 11>(p0 + (ptrdiff_t)y * stride + x0 - 3, s);
 #pragma unroll
-        for (int k = 0; k < 8; k++) t += (int)s[k - 3] * ME_LUMA_TAPS[dx][k];
-        v = (t + 32) >> 6;
-    } else if (dx == 0) {
-        const pel *s = p0 + (ptrdiff_t)y * stride + x;
-        int t = 0;
+            for (int j = 0; j < 4; j++)
 #pragma unroll
-        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * stride] * ME_LUMA_TAPS[dy][k];
-        v = (t + 32) >> 6;
+                for (int k = 0; k < 8; k++) acc[j] += s[j + k] * ME_LUMA_TAPS[dx][k];
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = (acc[j] + 32) >> 6;
+        } else if (dx == 0) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int s[4];
+                load_samples<pel, 4>(p0 + (ptrdiff_t)(y + k - 3) * stride + x0, s);
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[j] += s[j] * ME_LUMA_TAPS[dy][k];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = (acc[j] + 32) >> 6;
+        } else {
+            const int tw = w + 16, th = h + 16;
+            const s16 *t0 = tmp + (size_t)(dx - 1) * tw * th + (size_t)(y + 8) * tw + (x0 + 8);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int s[4];
+                load_samples<s16, 4>(t0 + (ptrdiff_t)(k - 3) * tw, s);
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[j] += s[j] * ME_LUMA_TAPS[dy][k];
+            }
+            const int sh2 = 20 - BIT_DEPTH;
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = (acc[j] + (1 << (sh2 - 1))) >> sh2;
+        }
     } else {
-        int tw = w + 16, th = h + 16;
-        const s16 *s = tmp + (size_t)(dx - 1) * tw * th + (size_t)(y + 8) * tw + (x + 8);
-        int t = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) t += (int)s[(ptrdiff_t)(k - 3) * tw] * ME_LUMA_TAPS[dy][k];
-        const int sh2 = 20 - BIT_DEPTH, add2 = 1 << (sh2 - 1);
-        v = (t + add2) >> sh2;
+        for (int j = 0; j < 4; j++) v[j] = X0 + j < stride ? plane_sample(p0, stride, w, h, tmp, idx, X0 + j, Y) : 0;
     }
-    planes[(size_t)idx * plane_elems + (size_t)Y * stride + X] = (pel)me_clip3(0, maxv, v);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (X0 + j < stride) dst[j] = (pel)me_clip3(0, maxv, v[j]);
 }
 
 int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
@@ -89,7 +160,7 @@ int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
                        r->planes, r->stride, r->rows);
     hipLaunchKernelGGL(k_htmp, dim3((r->w + 16 + 255) / 256, r->h + 16, 3), b, 0, st,
                        r->planes + (size_t)ME_PAD * r->stride + ME_PAD, r->stride, r->w, r->h, r->tmp);
-    hipLaunchKernelGGL(k_planes, dim3((r->stride + 255) / 256, r->rows, 15), b, 0, st, r->planes, r->plane_elems,
+    hipLaunchKernelGGL(k_planes, dim3((r->stride + 1023) / 1024, r->rows, 15), b, 0, st, r->planes, r->plane_elems,
                        r->stride, r->rows, r->w, r->h, r->tmp);
     ME_CHECK(hipGetLastError());
     return 0;
