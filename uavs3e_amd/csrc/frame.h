@@ -135,6 +135,13 @@ struct SubArgs {
 
 // Every shape in one launch: grid (blocks, jobs, refs) of 256 lanes.
 int me_subpel_launch(const SubArgs &a, int blocks, int jobs, int refs, hipStream_t st);
+// Empty launches on st (no jobs; the kernels return at once): a kernel's
+// first launch in a process, and its first on a queue when it has private
+// memory, otherwise stall for milliseconds.
+int me_subpel_preload(hipStream_t st);
+int me_int_search_preload_w1(int mode, size_t lds, hipStream_t st);
+int me_int_search_preload_w2(int mode, size_t lds, hipStream_t st);
+int me_int_search_preload_w4(int mode, size_t lds, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // One search configuration: window geometry, key layout, schedule and the
@@ -165,6 +172,10 @@ struct MeSearch {
 int me_search_init(MeSearch *s, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
                    int waves, int dx_cols, int sp_groups);
 void me_search_free(MeSearch *s);
+// Makes st ready for the search's launches without delay: loads the
+// kernels' code and creates the stream's device queue (first launches on a
+// new stream otherwise pay for both), then waits for st.
+int me_search_warmup(const MeSearch *s, hipStream_t st);
 // Enqueues key initialisation, integer search, sub-pel refinement and the
 // reference choice for the jobs of js.  keys/res are [job][npu][nref],
 // best is [job][npu][nl].  ev, if given, receives 4 events recorded before the

@@ -197,6 +197,20 @@ void me_search_free(MeSearch *p)
     p->d_blk_shape = nullptr;
 }
 
+int me_search_warmup(const MeSearch *p, hipStream_t st)
+{
+    int rc = p->waves == 1 ? me_int_search_preload_w1(p->mode, p->lds, st)
+                           : (p->waves == 2 ? me_int_search_preload_w2(p->mode, p->lds, st)
+                                            : me_int_search_preload_w4(p->mode, p->lds, st));
+    if (rc || me_subpel_preload(st)) return -1;
+    // n = 0: no access
+    hipLaunchKernelGGL(k_fill_keys, dim3(1), dim3(256), 0, st, (u64 *)nullptr, (size_t)0);
+    hipLaunchKernelGGL(k_select, dim3(1), dim3(256), 0, st, (const me_pu_result_t *)nullptr, (s8 *)nullptr, 0, 1, 1, 1);
+    ME_CHECK(hipGetLastError());
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
 int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
                       hipEvent_t *ev)
 {

@@ -8,6 +8,8 @@
 // that the instances build in parallel.
 #include "frame.h"
 
+#include <string.h>
+
 #ifndef ME_IS_WAVES
 #define ME_IS_WAVES 4
 #endif
@@ -703,6 +705,7 @@ template <int MODE, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_int_search(IntArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    if (a.js.n_jobs == 0) return;   // a warm-up launch
     Block b;
     const int band = blockIdx.x / a.nbx, bandx = blockIdx.x - band * a.nbx, job = blockIdx.y;
     int ctu;
@@ -822,6 +825,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
 
 #define ME_IS_CAT2(a, b) a##b
 #define ME_IS_CAT(a, b) ME_IS_CAT2(a, b)
+
 int ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st)
 {
     const void *fn = mode == 1 ? (const void *)k_int_search<1, ME_IS_WAVES> : (const void *)k_int_search<0, ME_IS_WAVES>;
@@ -830,4 +834,13 @@ int ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(int mode, const IntArgs &a, d
     else hipLaunchKernelGGL((k_int_search<0, ME_IS_WAVES>), grid, dim3(64 * ME_IS_WAVES), lds, st, a);
     ME_CHECK(hipGetLastError());
     return 0;
+}
+
+int ME_IS_CAT(me_int_search_preload_w, ME_IS_WAVES)(int mode, size_t lds, hipStream_t st)
+{
+    // an empty launch (no jobs): the first launch of a kernel in a process
+    // costs milliseconds, which a request should not wait for
+    IntArgs a;
+    memset(&a, 0, sizeof(a));
+    return ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(mode, a, dim3(1), lds, st);
 }

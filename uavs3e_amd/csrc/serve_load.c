@@ -3,7 +3,7 @@
  *
  * Each stream is driven by two threads, as an encoder would be: an uploader
  * that makes the stream's pictures resident at most one frame ahead of the
- * frame being searched (me_server_put_picture), and a submitter that issues
+ * frame being searched (me_server_put_picture; frames 0-2 before the clock), and a submitter that issues
  * the per-CTU ME requests of frame f against frame f-1 at precomputed
  * Poisson arrival times and releases frame f-1 once frame f is submitted.
  * A collector per stream retires the tickets.  A request's latency is its
@@ -43,7 +43,17 @@ typedef struct {
     int searching;                 /* frame whose requests are being submitted */
     int submitted;                 /* requests submitted so far */
     int rc;
+    pthread_barrier_t *go;         /* all threads running; then the schedule is final */
 } stream_t;
+
+/* Every load thread waits here twice: once when it runs, once more after
+ * the main thread has fixed the clock, so that thread start-up (first-touch
+ * of new stacks, symbol binding) never delays the first requests. */
+static void wait_go(stream_t *s)
+{
+    pthread_barrier_wait(s->go);
+    pthread_barrier_wait(s->go);
+}
 
 static void sleep_until(double t_us)
 {
@@ -60,6 +70,7 @@ static void sleep_until(double t_us)
 static void *uploader(void *arg)
 {
     stream_t *s = (stream_t *)arg;
+    wait_go(s);
     for (int f = s->resident + 1; f < s->frames; f++) {
         pthread_mutex_lock(&s->mu);
         while (f > s->searching + 1 && !s->rc) pthread_cond_wait(&s->cv, &s->mu);
@@ -78,6 +89,7 @@ static void *uploader(void *arg)
 static void *submitter(void *arg)
 {
     stream_t *s = (stream_t *)arg;
+    wait_go(s);
     for (int f = 1; f < s->frames; f++) {
         pthread_mutex_lock(&s->mu);
         s->searching = f;
@@ -113,6 +125,7 @@ static void *submitter(void *arg)
 static void *collector(void *arg)
 {
     stream_t *s = (stream_t *)arg;
+    wait_go(s);
     const int n = (s->frames - 1) * s->n_ctu;
     for (int k = 0; k < n; k++) {
         pthread_mutex_lock(&s->mu);
@@ -211,24 +224,28 @@ EXPORT int me_host_serve_load(const me_server_cfg_t *cfg, int nstreams, const in
     clock_gettime(CLOCK_MONOTONIC, &t1);
     if (info) info[2] = (t1.tv_sec - t0.tv_sec) + (t1.tv_nsec - t0.tv_nsec) * 1e-9;
 
-    /* the first two pictures of every stream are resident before the clock
-     * starts (an encoder has them before its first ME request) */
+    /* the first three pictures of every stream are resident before the clock
+     * starts: an encoder has two before its first ME request, and the third
+     * completes the steady-state set (reference, current, next) that the
+     * server's picture pool then recycles, so no picture allocation falls
+     * inside the measured run */
     for (int i = 0; i < nstreams && !rc; i++)
-        for (int f = 0; f < 2 && !rc; f++) {
+        for (int f = 0; f < 3 && f < frames && !rc; f++) {
             if (me_server_put_picture(srv, S[i].sid, f, S[i].pics[f], S[i].w, -1)) rc = -1;
             S[i].resident = f;
         }
     /* arrival schedule: per stream a Poisson process at the stream's share
      * of the rate (its CTUs per frame over all streams'); the superposition
-     * is a Poisson process at the total rate */
-    const double start = me_server_now_us() + 2000.0;
+     * is a Poisson process at the total rate.  Offsets from the start, which
+     * is fixed once every load thread runs. */
+    double start = 0.0;
     uint64_t x = arrival_seed * 0x9E3779B97F4A7C15ull + 1;
     double **arr = (double **)calloc(nstreams, sizeof(double *));
     for (int i = 0; i < nstreams; i++) {
         const int n = S[i].n_ctu * (frames - 1);
         arr[i] = (double *)malloc(n * sizeof(double));
         const double rate_i = rate * S[i].n_ctu / total_ctu;
-        double t = start;
+        double t = 0.0;
         for (int k = 0; k < n; k++) {
             x ^= x << 13; x ^= x >> 7; x ^= x << 17;
             const double u = ((x >> 11) + 0.5) * (1.0 / 9007199254740992.0);
@@ -248,12 +265,23 @@ EXPORT int me_host_serve_load(const me_server_cfg_t *cfg, int nstreams, const in
     }
     pthread_t *th = (pthread_t *)calloc(3 * nstreams, sizeof(pthread_t));
     if (!rc) {
+        pthread_barrier_t go;
+        pthread_barrier_init(&go, NULL, 3 * nstreams + 1);
         for (int i = 0; i < nstreams; i++) {
+            S[i].go = &go;
             pthread_create(&th[3 * i], NULL, uploader, &S[i]);
             pthread_create(&th[3 * i + 1], NULL, submitter, &S[i]);
             pthread_create(&th[3 * i + 2], NULL, collector, &S[i]);
         }
+        pthread_barrier_wait(&go);
+        start = me_server_now_us() + 2000.0;
+        for (int i = 0; i < nstreams; i++) {
+            const int n = S[i].n_ctu * (frames - 1);
+            for (int k = 0; k < n; k++) arr[i][k] += start;
+        }
+        pthread_barrier_wait(&go);
         for (int i = 0; i < 3 * nstreams; i++) pthread_join(th[i], NULL);
+        pthread_barrier_destroy(&go);
     }
     double last = start;
     for (int i = 0; i < nstreams; i++) {

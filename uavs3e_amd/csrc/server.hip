@@ -332,6 +332,20 @@ ME_EXPORT int me_server_create(me_server_t *out, const me_server_cfg_t *cfg)
         hipError_t e = hipStreamCreateWithFlags(&S->upload, hipStreamNonBlocking);
         if (e != hipSuccess) { me_set_error("me_server_create", e, __FILE__, __LINE__); rc = -1; }
     }
+    // the first requests should not pay for kernel loading, queue creation
+    // or the first copies of each direction: a batch's copies, once, on every
+    // slot (the buffers' contents do not matter yet)
+    for (auto &sl : S->slots) {
+        if (!rc) rc = me_search_warmup(&S->search, sl.st);
+        if (rc) break;
+        hipError_t e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, sizeof(u32), hipMemcpyHostToDevice, sl.st);
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.h_res, sl.d_res, sizeof(me_pu_result_t), hipMemcpyDeviceToHost, sl.st);
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.h_best, sl.d_best, 1, hipMemcpyDeviceToHost, sl.st);
+        if (e == hipSuccess) e = hipEventRecord(sl.done, sl.st);
+        if (e == hipSuccess) e = hipEventSynchronize(sl.done);
+        if (e != hipSuccess) { me_set_error("me_server_create: warm-up", e, __FILE__, __LINE__); rc = -1; }
+    }
+    if (!rc) rc = me_search_warmup(&S->search, S->upload);
     if (rc) {
         for (auto &sl : S->slots) slot_free(sl);
         me_search_free(&S->search);

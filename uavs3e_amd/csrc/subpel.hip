@@ -6,6 +6,8 @@
 // and runs the instance of that shape's com_had tile type.
 #include "had.h"
 
+#include <string.h>
+
 // Sub-pel search of one shape.  A unit is one (CTU, PU of this shape,
 // reference).  Its lanes form S.G position groups of S.lpu lanes: in a group
 // one lane per com_had tile (com_cost.c:719-811), two for 16x8 / 8x16 tiles
@@ -167,6 +169,7 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_subpel(SubArgs a)
 {
+    if (a.js.n_jobs == 0) return;   // a warm-up launch (me_subpel_preload)
     // the shape and its parameters are uniform across the workgroup
     const int si = __builtin_amdgcn_readfirstlane(a.blk_shape[blockIdx.x]);
     const SubShape S = a.shp[si];
@@ -184,6 +187,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 int me_subpel_launch(const SubArgs &a, int blocks, int jobs, int refs, hipStream_t st)
 {
     hipLaunchKernelGGL(k_subpel, dim3(blocks, jobs, refs), dim3(256), 0, st, a);
+    ME_CHECK(hipGetLastError());
+    return 0;
+}
+
+int me_subpel_preload(hipStream_t st)
+{
+    // An empty launch: besides loading the code it makes the runtime set up
+    // the kernel's private (spill) memory for st's queue, which otherwise
+    // happens at the first real launch and stalls it for milliseconds.
+    SubArgs a;
+    memset(&a, 0, sizeof(a));
+    hipLaunchKernelGGL(k_subpel, dim3(1), dim3(256), 0, st, a);
     ME_CHECK(hipGetLastError());
     return 0;
 }
