@@ -501,6 +501,14 @@ __device__ __forceinline__ void leaves2(const pel *win, int wcols, int wrow0, in
             const u32 p1 = __builtin_amdgcn_alignbyte(d[2 * c + 2], d[2 * c + 1], 2);
             Lb[r >> 2][c].v = __builtin_amdgcn_sad_u16(p0, cc[2 * c], __builtin_amdgcn_sad_u16(p1, cc[2 * c + 1], Lb[r >> 2][c].v));
         }
+        // keep the compiler from hoisting the LDS reads of all 16 rows (17
+        // dwords each) ahead of their use, which alone exceeds the registers:
+        // a block of 4 rows is summed before the next block is read
+        if ((r & 3) == 3) {
+            Val *a = La[r >> 2], *b = Lb[r >> 2];
+            asm volatile("" : "+v"(a[0].v), "+v"(a[1].v), "+v"(a[2].v), "+v"(a[3].v), "+v"(b[0].v), "+v"(b[1].v),
+                         "+v"(b[2].v), "+v"(b[3].v)::"memory");
+        }
     }
 }
 #endif
