@@ -82,13 +82,15 @@ __device__ __forceinline__ void fwht(int *v, int step)
             }
 }
 
-#if BIT_DEPTH == 8
-// 8-bit tiles run on packed 16-bit lanes, two samples per operation: every
-// intermediate of an 8x8 transform of residuals in [-510, 510] stays within
-// +-16320 because the last butterfly is never formed: for the last stage's
-// pairs (a, b), |a + b| + |a - b| = 2 max(|a|, |b|).  A Walsh-Hadamard
-// transform's stages commute, so the row pairs (x, x+2) of a dword take that
-// last stage.
+// Tiles on packed 16-bit lanes, two samples per operation.  The last
+// butterfly is never formed: for the last stage's pairs (a, b),
+// |a + b| + |a - b| = 2 max(|a|, |b|), and a Walsh-Hadamard transform's
+// stages commute, so the two samples of a dword take that last stage.  Every
+// other intermediate of a transform of N samples is at most N/2 times the
+// largest residual: 8 bits, any tile and the 16x8 / 8x16 halves (residuals
+// within +-510): at most 16320; 10 bits, tiles up to 8x8 (residuals within
+// +-1023): at most 32736 -- both inside a signed 16-bit lane.  (The 10-bit
+// 16x8 / 8x16 halves, residuals to +-2046, stay on 32-bit lanes.)
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 // NW dwords of the samples starting at p (any alignment).
@@ -105,6 +107,7 @@ __device__ __forceinline__ void load_words(const pel *p, u32 *w)
     for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
 }
 
+#if BIT_DEPTH == 8
 // The residual org - cur of one row of TW samples (TW = 4, 8) as TW/2 pairs:
 // v[2j] = (r[4j], r[4j+2]), v[2j+1] = (r[4j+1], r[4j+3]).
 template <int TW>
@@ -121,6 +124,19 @@ __device__ __forceinline__ void res_pairs(const pel *org, const pel *cur, s16x2 
                        __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, cw[j], 0x0C030C01u));
     }
 }
+#else
+// The residual org - cur of one row of TW samples as TW/2 pairs: v[j] =
+// (r[2j], r[2j+1]), straight from the 16-bit samples' dwords.
+template <int TW>
+__device__ __forceinline__ void res_pairs(const pel *org, const pel *cur, s16x2 *v)
+{
+    u32 ow[TW / 2], cw[TW / 2];
+    load_words<TW / 2>(org, ow);
+    load_words<TW / 2>(cur, cw);
+#pragma unroll
+    for (int j = 0; j < TW / 2; j++) v[j] = __builtin_bit_cast(s16x2, ow[j]) - __builtin_bit_cast(s16x2, cw[j]);
+}
+#endif
 
 template <int N>
 __device__ __forceinline__ void fwht_pk(s16x2 *v, int step)
@@ -150,40 +166,20 @@ __device__ __forceinline__ int had_pk_sum(s16x2 *v)
 #pragma unroll
     for (int i = 0; i < K * TH; i++) {
         const u32 a = __builtin_bit_cast(u32, __builtin_elementwise_max(v[i], (s16x2)0 - v[i]));
-        const u32 lo = a & 0xFFFFu, hi = a >> 16;   // both in [0, 16320]
+        const u32 lo = a & 0xFFFFu, hi = a >> 16;   // both in [0, 32767]
         s += lo > hi ? lo : hi;
     }
     return (int)(2 * s);
 }
-#endif
 
 // Sum of |coef| of the 2-D Walsh-Hadamard transform of a TW x TH residual.
 template <int TW, int TH>
 __device__ __forceinline__ int had_abs(const pel *org, int so, const pel *cur, int sc)
 {
-#if BIT_DEPTH == 8
     s16x2 v[TH * TW / 2];
 #pragma unroll
     for (int y = 0; y < TH; y++) res_pairs<TW>(org + y * so, cur + y * sc, v + y * (TW / 2));
     return had_pk_sum<TW, TH>(v);
-#else
-    int m[TW * TH];
-#pragma unroll
-    for (int y = 0; y < TH; y++) {
-        int o[TW], c[TW];
-        load_row<TW>(org + y * so, o);
-        load_row<TW>(cur + y * sc, c);
-#pragma unroll
-        for (int x = 0; x < TW; x++) m[y * TW + x] = o[x] - c[x];
-        fwht<TW>(m + y * TW, 1);
-    }
-#pragma unroll
-    for (int x = 0; x < TW; x++) fwht<TH>(m + x, TW);
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < TW * TH; i++) s += abs(m[i]);
-    return s;
-#endif
 }
 
 // A 16-point WHT is the 8-point WHT of the half sums and of the half
