@@ -107,36 +107,38 @@ __device__ __forceinline__ void load_words(const pel *p, u32 *w)
     for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
 }
 
-#if BIT_DEPTH == 8
-// The residual org - cur of one row of TW samples (TW = 4, 8) as TW/2 pairs:
-// v[2j] = (r[4j], r[4j+2]), v[2j+1] = (r[4j+1], r[4j+3]).
+// One row of TW samples (TW = 4, 8) as TW/2 pairs of 16-bit lanes.  8 bits:
+// v[2j] = (p[4j], p[4j+2]), v[2j+1] = (p[4j+1], p[4j+3]); 10 bits: v[j] =
+// (p[2j], p[2j+1]), straight from the samples' dwords.
 template <int TW>
-__device__ __forceinline__ void res_pairs(const pel *org, const pel *cur, s16x2 *v)
+__device__ __forceinline__ void pairs_row(const pel *p, s16x2 *v)
 {
-    u32 ow[TW / 4], cw[TW / 4];
-    load_words<TW / 4>(org, ow);
-    load_words<TW / 4>(cur, cw);
+#if BIT_DEPTH == 8
+    u32 w[TW / 4];
+    load_words<TW / 4>(p, w);
 #pragma unroll
     for (int j = 0; j < TW / 4; j++) {
-        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, ow[j], 0x0C020C00u)) -
-                   __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, cw[j], 0x0C020C00u));
-        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, ow[j], 0x0C030C01u)) -
-                       __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, cw[j], 0x0C030C01u));
+        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C020C00u));
+        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C030C01u));
     }
-}
 #else
-// The residual org - cur of one row of TW samples as TW/2 pairs: v[j] =
-// (r[2j], r[2j+1]), straight from the 16-bit samples' dwords.
+    u32 w[TW / 2];
+    load_words<TW / 2>(p, w);
+#pragma unroll
+    for (int j = 0; j < TW / 2; j++) v[j] = __builtin_bit_cast(s16x2, w[j]);
+#endif
+}
+
+// The residual org - cur of one row as pairs.
 template <int TW>
 __device__ __forceinline__ void res_pairs(const pel *org, const pel *cur, s16x2 *v)
 {
-    u32 ow[TW / 2], cw[TW / 2];
-    load_words<TW / 2>(org, ow);
-    load_words<TW / 2>(cur, cw);
+    s16x2 o[TW / 2], c[TW / 2];
+    pairs_row<TW>(org, o);
+    pairs_row<TW>(cur, c);
 #pragma unroll
-    for (int j = 0; j < TW / 2; j++) v[j] = __builtin_bit_cast(s16x2, ow[j]) - __builtin_bit_cast(s16x2, cw[j]);
+    for (int j = 0; j < TW / 2; j++) v[j] = o[j] - c[j];
 }
-#endif
 
 template <int N>
 __device__ __forceinline__ void fwht_pk(s16x2 *v, int step)
@@ -235,5 +237,75 @@ __device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, in
     if (TT == 1) return (had_abs<8, 8>(org, so, cur, sc) + 2) >> 2;
     if (TT == 2) return (int)((double)had_abs<8, 4>(org, so, cur, sc) / SQRT32 * 2.0);
     return (int)((double)had_abs<4, 8>(org, so, cur, sc) / SQRT32 * 2.0);
+}
+
+// The same costs with the current block's pairs made once (a sub-pel search
+// compares one current block with 17 predictions): ORG_PAIRS<TT> pairs per
+// lane, from org_pairs<TT>, then satd_pre<TT> / had_half_pre<TT> per
+// prediction.  The 10-bit 16x8 / 8x16 halves keep had_half.
+template <int TT> struct tile_dims { static constexpr int W = TT == 0 || TT == 4 ? 4 : 8, H = TT == 0 || TT == 2 ? 4 : 8; };
+template <int TT> constexpr int org_pairs_n() { return TT == 3 || TT == 5 ? 32 : tile_dims<TT>::W * tile_dims<TT>::H / 2; }
+
+// For TT 3/5 (8 bits) the pairs of the tile's half sum (half 0) or
+// difference (half 1) of its two 8x8 halves.
+template <int TT>
+__device__ __forceinline__ void org_pairs(const pel *org, int so, int half, s16x2 *o)
+{
+    if (TT == 3 || TT == 5) {
+        const s16x2 sgv = (s16x2)(short)(half ? -1 : 1);
+        const int o2 = TT == 3 ? 8 : 8 * so;
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            s16x2 b[4];
+            pairs_row<8>(org + y * so, o + y * 4);
+            pairs_row<8>(org + y * so + o2, b);
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[y * 4 + k] += b[k] * sgv;
+        }
+    } else {
+        constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
+#pragma unroll
+        for (int y = 0; y < H; y++) pairs_row<W>(org + y * so, o + y * (W / 2));
+    }
+}
+
+template <int TT>
+__device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, int half)
+{
+    s16x2 v[org_pairs_n<TT>()];
+    if (TT == 3 || TT == 5) {
+        const s16x2 sgv = (s16x2)(short)(half ? -1 : 1);
+        const int c2 = TT == 3 ? 8 : 8 * sc;
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            s16x2 a[4], b[4];
+            pairs_row<8>(cur + y * sc, a);
+            pairs_row<8>(cur + y * sc + c2, b);
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[y * 4 + k] = o[y * 4 + k] - (a[k] + b[k] * sgv);
+        }
+        return had_pk_sum<8, 8>(v);
+    } else {
+        constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
+#pragma unroll
+        for (int y = 0; y < H; y++) {
+            s16x2 c[W / 2];
+            pairs_row<W>(cur + y * sc, c);
+#pragma unroll
+            for (int k = 0; k < W / 2; k++) v[y * (W / 2) + k] = o[y * (W / 2) + k] - c[k];
+        }
+        return had_pk_sum<W, H>(v);
+    }
+}
+
+// satd_t (TT 0, 1, 2, 4) from the precomputed pairs.
+template <int TT>
+__device__ __forceinline__ int satd_pre(const s16x2 *o, const pel *cur, int sc)
+{
+    const double SQRT32 = 0x1.6a09e667f3bcdp+2;
+    const int s = had_pre<TT>(o, cur, sc, 0);
+    if (TT == 0) return (s + 1) >> 1;
+    if (TT == 1) return (s + 2) >> 2;
+    return (int)((double)s / SQRT32 * 2.0);
 }
 

@@ -66,6 +66,11 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
     const int tx = (t % S.tpr) * S.tw, ty = (t / S.tpr) * S.th;
     const int toff_o = ty * P.pic_w + tx, toff_p = ty * P.ref_stride + tx;
     const int red_w = LPU < 64 ? LPU : 64;
+    // the lane's current tile as 16-bit pairs, made once for all 17
+    // positions (the 10-bit 16x8 / 8x16 halves read it per position)
+    constexpr bool PRE = !SPLIT || BIT_DEPTH == 8;
+    s16x2 opairs[PRE ? org_pairs_n<TT>() : 1];
+    if (PRE && active) org_pairs<TT>(org + toff_o, P.pic_w, li & 1, opairs);
     // com_had of the unit at pred, in every lane of the group; run = 0 skips
     // the sample reads (the sum is then 0) but not the barriers
     auto had = [&](const pel *pred, bool run) -> u32 {
@@ -75,12 +80,13 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
                 // the two lanes of a tile hold its sum and difference halves;
                 // the tile is normalised (com_cost.c:467, 586) before tiles
                 // are summed
-                const int hs = had_half<TT>(org + toff_o, P.pic_w, pred + toff_p, P.ref_stride, li & 1);
+                const int hs = PRE ? had_pre<TT>(opairs, pred + toff_p, P.ref_stride, li & 1)
+                                   : had_half<TT>(org + toff_o, P.pic_w, pred + toff_p, P.ref_stride, li & 1);
                 const int s = hs + __builtin_amdgcn_mov_dpp(hs, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
                 const double SQRT128 = 0x1.6a09e667f3bcdp+3;
                 v = (li & 1) ? 0 : (int)((double)s / SQRT128 * 2.0);
             } else {
-                v = satd_t<TT>(org + toff_o, P.pic_w, pred + toff_p, P.ref_stride);
+                v = satd_pre<TT>(opairs, pred + toff_p, P.ref_stride);
             }
         }
         v = group_sum(v, red_w);
