@@ -126,6 +126,9 @@ CASES = [
     (8, 136, 136, 128, 1 << SHAPES.index((128, 128)), 1),
     (10, 384, 264, 8, MASK_ALL, 2),
     (10, 200, 136, 24, MASK_16x16, 2),
+    # a width that is not a multiple of 4: rows of the 8-bit planes are not
+    # dword-aligned to each other (the sub-pel kernel's general path)
+    (8, 202, 136, 8, MASK_ALL, 1),
 ]
 
 

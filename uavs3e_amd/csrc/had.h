@@ -129,6 +129,31 @@ __device__ __forceinline__ void pairs_row(const pel *p, s16x2 *v)
 #endif
 }
 
+// The same from a dword-aligned global address q plus a byte shift sh (the
+// rows of a block at a stride of whole dwords share sh): one address add
+// per row and global (not flat) loads.
+typedef const __attribute__((address_space(1))) u32 *gu32p;
+template <int TW>
+__device__ __forceinline__ void pairs_row_al(gu32p q, u32 sh, s16x2 *v)
+{
+    constexpr int NW = TW * (int)sizeof(pel) / 4;
+    u32 d[NW + 1], w[NW];
+#pragma unroll
+    for (int i = 0; i <= NW; i++) d[i] = q[i];
+#pragma unroll
+    for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+#if BIT_DEPTH == 8
+#pragma unroll
+    for (int j = 0; j < TW / 4; j++) {
+        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C020C00u));
+        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C030C01u));
+    }
+#else
+#pragma unroll
+    for (int j = 0; j < TW / 2; j++) v[j] = __builtin_bit_cast(s16x2, w[j]);
+#endif
+}
+
 // The residual org - cur of one row as pairs.
 template <int TW>
 __device__ __forceinline__ void res_pairs(const pel *org, const pel *cur, s16x2 *v)
@@ -273,6 +298,36 @@ template <int TT>
 __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, int half)
 {
     s16x2 v[org_pairs_n<TT>()];
+    if ((sc * (int)sizeof(pel)) % 4 == 0) {
+        // rows a whole number of dwords apart: one alignment for all rows
+        const uintptr_t ad = (uintptr_t)cur;
+        const u32 sh = (u32)(ad & 3);
+        const gu32p q0 = (gu32p)(ad & ~(uintptr_t)3);
+        const int sw = sc * (int)sizeof(pel) / 4;
+        if (TT == 3 || TT == 5) {
+            const s16x2 sgv = (s16x2)(short)(half ? -1 : 1);
+            const int c2 = TT == 3 ? 8 * (int)sizeof(pel) / 4 : 8 * sw;
+#pragma unroll
+            for (int y = 0; y < 8; y++) {
+                s16x2 a[4], b[4];
+                pairs_row_al<8>(q0 + y * sw, sh, a);
+                pairs_row_al<8>(q0 + y * sw + c2, sh, b);
+#pragma unroll
+                for (int k = 0; k < 4; k++) v[y * 4 + k] = o[y * 4 + k] - (a[k] + b[k] * sgv);
+            }
+            return had_pk_sum<8, 8>(v);
+        } else {
+            constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
+#pragma unroll
+            for (int y = 0; y < H; y++) {
+                s16x2 c[W / 2];
+                pairs_row_al<W>(q0 + y * sw, sh, c);
+#pragma unroll
+                for (int k = 0; k < W / 2; k++) v[y * (W / 2) + k] = o[y * (W / 2) + k] - c[k];
+            }
+            return had_pk_sum<W, H>(v);
+        }
+    }
     if (TT == 3 || TT == 5) {
         const s16x2 sgv = (s16x2)(short)(half ? -1 : 1);
         const int c2 = TT == 3 ? 8 : 8 * sc;
