@@ -277,14 +277,14 @@ def main():
         # 4 displacement rows per workgroup: short per-batch latency at light load
         cfg = ua.ServerCfg(32, wl.lambda_mv(32), ua.MASK_16x16, 1, 64 << 30, 128, 4, 100, 4, 4, 0)
         host8 = ua.MeHost(8)
-        # a short unmeasured run first: the first server run in a process
-        # stalls once for several milliseconds after its first batch
-        host8.serve_load(cfg, sizes, seeds, 3, args.serve_rate, arrival_seed=8)
+        # one unmeasured run of the same load first: the first server run in
+        # a process stalls once for several milliseconds after its first batch
+        host8.serve_load(cfg, sizes, seeds, args.serve_frames, args.serve_rate, arrival_seed=8)
         slat, info, st, _ = host8.serve_load(cfg, sizes, seeds, args.serve_frames, args.serve_rate, arrival_seed=7)
         lat_ms = slat / 1e3
         serving = {"streams": "3x1280x720 + 2x1920x1080 + 1x3840x2160, 8-bit, %d frames each" % args.serve_frames,
                    "job": "one CTU: 16x16 PUs, full search +-32, 1 reference (the previous frame), QP 32, sub-pel",
-                   "warmup": "one unmeasured 3-frame run of the same load first",
+                   "warmup": "one unmeasured run of the same load first (other arrival seed)",
                    "offered_ctus_per_s": args.serve_rate, "requests": info["requests"],
                    "throughput_ctus_per_s": info["requests"] / (info["wall_us"] * 1e-6),
                    "latency_ms": {"p50": pct(lat_ms, 50), "p99": pct(lat_ms, 99), "max": float(lat_ms.max())},
