@@ -304,3 +304,41 @@ def test_search_tz_vs_oracle(bd, mvr, cost_type):
                         checked += 1
                     pu += 1
     assert checked > 1000
+
+
+@pytest.mark.gpu
+def test_bad_arguments_are_rejected():
+    """Invalid configurations and arguments fail with an error (and leave
+    the library usable): window out of range, no shapes, too many or no
+    references, a list-1 count that leaves list 0 empty, a reference of
+    another size, a server stream smaller than the window."""
+    hip = ua.MeHip(8)
+    lam = wl.lambda_mv(32)
+    for args in [(256, 136, 256, lam, ua.MASK_ALL, 1),    # R > 255
+                 (256, 136, 16, lam, 0, 1),               # no shape
+                 (256, 136, 16, lam, ua.MASK_ALL, 0),     # no reference
+                 (256, 136, 16, lam, ua.MASK_ALL, 5)]:    # more than 4
+        with pytest.raises(ua.MeError):
+            hip.plan_create(*args)
+    with pytest.raises(ua.MeError):
+        hip.plan_create(256, 136, 16, lam, ua.MASK_ALL, 2, num_refs_l1=2)
+    work = wl.hd_allpu().scaled(256, 136, 8)
+    work.ref_shifts_qpel = work.ref_shifts_qpel[:1]
+    cur, refs = work.frames()
+    other = hip.ref_create(np.zeros((136, 264), hip.dt))
+    plan = hip.plan_create(256, 136, 8, lam, ua.MASK_16x16, 1)
+    hip.plan_set_cur(plan, cur)
+    with pytest.raises(ua.MeError):
+        hip.plan_run(plan, [other])                        # reference of another size
+    good = hip.ref_create(refs[0])
+    hip.plan_run(plan, [good])                             # the plan still runs
+    res, best = hip.plan_results(plan, 1)
+    assert (res["valid"] == 1).sum() > 0
+    hip.plan_destroy(plan)
+    hip.ref_destroy(good)
+    hip.ref_destroy(other)
+    srv = ua.MeServer(hip, 32, lam, ua.MASK_16x16, 1)
+    with pytest.raises(ua.MeError):
+        srv.open_stream(24, 24)                            # max(W, H) - 4 < R
+    srv.close()
+    assert hip.error_count() == 0
