@@ -292,6 +292,7 @@ def main():
                               "band_rows": 4,
                               "batches": st["batches"], "mean_batch": info["requests"] / max(1, st["batches"]),
                               "mixed_batches": st["mixed_batches"], "max_inflight_seen": st["max_inflight_seen"],
+                              "max_launch_us": st["max_launch_us"], "max_batch_us": st["max_batch_us"],
                               "peak_picture_bytes_end": st["picture_bytes"]},
                    "frame_synthesis_s": info["synth_s"]}
         print("[bench] serving: p50 %.3f ms p99 %.3f ms, %.0f CTUs/s" % (

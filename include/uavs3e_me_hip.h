@@ -307,6 +307,8 @@ typedef struct me_server_stats_t {
     uint64_t mixed_batches;       /* batches holding CTUs of more than one stream */
     uint64_t picture_bytes;       /* device bytes held by pictures now (live + pooled) */
     uint64_t admission_waits;     /* me_server_put_picture calls that had to wait for budget */
+    uint64_t max_launch_us;       /* longest enqueue of one batch by the dispatcher (us) */
+    uint64_t max_batch_us;        /* longest time from a batch's enqueue to its results on the host (us) */
 } me_server_stats_t;
 
 typedef struct me_server_s *me_server_t;

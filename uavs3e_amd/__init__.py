@@ -109,7 +109,7 @@ class ServerCfg(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_uint64), ("batches", C.c_uint64), ("max_batch_seen", C.c_uint64),
                 ("max_inflight_seen", C.c_uint64), ("mixed_batches", C.c_uint64), ("picture_bytes", C.c_uint64),
-                ("admission_waits", C.c_uint64)]
+                ("admission_waits", C.c_uint64), ("max_launch_us", C.c_uint64), ("max_batch_us", C.c_uint64)]
 
 
 class MeError(RuntimeError):

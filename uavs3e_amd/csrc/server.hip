@@ -70,9 +70,11 @@ struct Slot {
     u64 *d_keys = nullptr;
     me_pu_result_t *d_res = nullptr, *h_res = nullptr;
     s8 *d_best = nullptr, *h_best = nullptr;
+    void *hv_res = nullptr, *hv_best = nullptr;   // device views of h_res, h_best
     std::vector<Job> jobs;
     int rc = 0;
     bool busy = false;
+    double launch_us = 0;   // when the dispatcher began enqueuing the batch
 };
 
 struct Ticket {
@@ -141,10 +143,31 @@ static int slot_alloc(Slot &s, int max_jobs, int npu, int nref)
     ME_CHECK(hipHostMalloc((void **)&s.h_pics, max_jobs * sizeof(PicDesc), hipHostMallocDefault));
     ME_CHECK(hipHostMalloc((void **)&s.h_res, nres * sizeof(me_pu_result_t), hipHostMallocDefault));
     ME_CHECK(hipHostMalloc((void **)&s.h_best, (size_t)max_jobs * npu, hipHostMallocDefault));
+    ME_CHECK(hipHostGetDevicePointer(&s.hv_res, s.h_res, 0));
+    ME_CHECK(hipHostGetDevicePointer(&s.hv_best, s.h_best, 0));
     return 0;
 }
 
 // Enqueues one batch on its slot (called by the dispatcher without the lock).
+// A batch's results to the slot's pinned host buffers, written by the GPU
+// itself: a runtime device-to-host copy above 16 KB now and then took
+// milliseconds to enqueue, which every request of the batch then waited for.
+__global__ void k_copy_out(uint4 *__restrict__ hres, const uint4 *__restrict__ dres, size_t n16,
+                           u8 *__restrict__ hbest, const u8 *__restrict__ dbest, size_t nb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x, step = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n16; k += step) hres[k] = dres[k];
+    for (size_t k = i; k < nb; k += step) hbest[k] = dbest[k];
+}
+
+static void copy_out(Slot &sl, size_t nres, size_t nbest)
+{
+    const size_t n16 = nres * sizeof(me_pu_result_t) / 16;
+    const size_t blocks = std::min<size_t>(64, (std::max(n16, nbest) + 255) / 256);
+    hipLaunchKernelGGL(k_copy_out, dim3((unsigned)std::max<size_t>(blocks, 1)), dim3(256), 0, sl.st, (uint4 *)sl.hv_res,
+                       (const uint4 *)sl.d_res, n16, (u8 *)sl.hv_best, (const u8 *)sl.d_best, nbest);
+}
+
 static int launch_batch(me_server_s *S, Slot &sl)
 {
     const int n = (int)sl.jobs.size();
@@ -185,8 +208,8 @@ static int launch_batch(me_server_s *S, Slot &sl)
     js.n_jobs = n;
     if (me_search_enqueue(&S->search, js, sl.d_keys, sl.d_res, sl.d_best, sl.st, nullptr)) return -1;
     const size_t nres = (size_t)n * S->search.npu * nref;
-    ME_CHECK(hipMemcpyAsync(sl.h_res, sl.d_res, nres * sizeof(me_pu_result_t), hipMemcpyDeviceToHost, sl.st));
-    ME_CHECK(hipMemcpyAsync(sl.h_best, sl.d_best, (size_t)n * S->search.npu, hipMemcpyDeviceToHost, sl.st));
+    copy_out(sl, nres, (size_t)n * S->search.npu);
+    ME_CHECK(hipGetLastError());
     ME_CHECK(hipEventRecord(sl.done, sl.st));
     return 0;
 }
@@ -230,8 +253,11 @@ static void dispatch_loop(me_server_s *S)
             mixed |= sl.jobs[j].stream != sl.jobs[0].stream;
         }
         lk.unlock();
+        sl.launch_us = me_server_now_us();
         sl.rc = launch_batch(S, sl);
+        const double enq_us = me_server_now_us() - sl.launch_us;
         lk.lock();
+        S->stats.max_launch_us = std::max<uint64_t>(S->stats.max_launch_us, (uint64_t)enq_us);
         S->inflight.push_back(slot);
         S->stats.batches++;
         S->stats.mixed_batches += mixed ? 1 : 0;
@@ -281,6 +307,7 @@ static void complete_loop(me_server_s *S)
         }
         const double now = me_server_now_us();
         lk.lock();
+        S->stats.max_batch_us = std::max<uint64_t>(S->stats.max_batch_us, (uint64_t)(now - sl.launch_us));
         S->inflight.pop_front();
         for (const Job &jb : sl.jobs) {
             auto it = S->tickets.find(jb.ticket);
@@ -333,14 +360,19 @@ ME_EXPORT int me_server_create(me_server_t *out, const me_server_cfg_t *cfg)
         if (e != hipSuccess) { me_set_error("me_server_create", e, __FILE__, __LINE__); rc = -1; }
     }
     // the first requests should not pay for kernel loading, queue creation
-    // or the first copies of each direction: a batch's copies, once, on every
-    // slot (the buffers' contents do not matter yet)
+    // or the first copies: a batch's copies, once, on every slot (the
+    // buffers' contents do not matter yet)
     for (auto &sl : S->slots) {
         if (!rc) rc = me_search_warmup(&S->search, sl.st);
         if (rc) break;
-        hipError_t e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, sizeof(u32), hipMemcpyHostToDevice, sl.st);
-        if (e == hipSuccess) e = hipMemcpyAsync(sl.h_res, sl.d_res, sizeof(me_pu_result_t), hipMemcpyDeviceToHost, sl.st);
-        if (e == hipSuccess) e = hipMemcpyAsync(sl.h_best, sl.d_best, 1, hipMemcpyDeviceToHost, sl.st);
+        // whole buffers: copies of different sizes take different paths
+        const size_t mj = (size_t)cfg->max_batch_ctus, np = S->search.npu, nres = mj * np * S->search.nref;
+        hipError_t e = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, mj * sizeof(u32), hipMemcpyHostToDevice, sl.st);
+        if (e == hipSuccess) e = hipMemcpyAsync(sl.d_pics, sl.h_pics, mj * sizeof(PicDesc), hipMemcpyHostToDevice, sl.st);
+        if (e == hipSuccess) {
+            copy_out(sl, nres, mj * np);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess) e = hipEventRecord(sl.done, sl.st);
         if (e == hipSuccess) e = hipEventSynchronize(sl.done);
         if (e != hipSuccess) { me_set_error("me_server_create: warm-up", e, __FILE__, __LINE__); rc = -1; }
