@@ -147,6 +147,55 @@ def test_frame_plan_vs_oracle(case):
     assert res["valid"].sum() > 0
 
 
+def _spoil_planes(hip, ref, w, h):
+    """Overwrites the 16 padded planes of a device reference (not its luma)
+    through the HIP runtime the library itself links, so that only a rebuild
+    inside the frame can restore them."""
+    plane, stride, pad = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_int()
+    fill = hip.lib.hipMemset
+    fill.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    for i in range(16):
+        assert hip.lib.me_ref_plane(ref, i, ctypes.byref(plane), ctypes.byref(stride), ctypes.byref(pad)) == 0
+        item = np.dtype(hip.dt).itemsize
+        base = plane.value - (pad.value * stride.value + pad.value) * item
+        assert fill(base, 0x5A, stride.value * (h + 2 * pad.value) * item) == 0
+
+
+@pytest.mark.parametrize("bd,nref,mask", [(8, 2, 1), (8, 3, 3), (10, 2, 1), (8, 2, 3), (8, 2, 2)])
+def test_frame_plan_rebuilds_references(bd, nref, mask):
+    """References named in the prepare mask are rebuilt inside the frame
+    graph before their search: a leading range of them on a second stream
+    beside the search of the others, any other mask in order on one stream.
+    Spoiled planes of those references must not reach the results."""
+    base = wl.hd_allpu() if bd == 8 else wl.uhd10_allpu()
+    w, h, R = 256, 200, 12
+    work = base.scaled(w, h, R)
+    work.ref_shifts_qpel = (work.ref_shifts_qpel * 2)[:nref]
+    cur, refs = work.frames()
+    hip, orc = ua.MeHip(bd), Oracle(bd)
+    lam = wl.lambda_mv(32)
+    dref = [hip.ref_create(r) for r in refs]
+    plan = hip.plan_create(w, h, R, lam, MASK_ALL, nref)
+    hip.plan_set_cur(plan, cur)
+    ores, obest = orc.me_frame(cur, [orc.make_planes(r) for r in refs], R, lam, MASK_ALL)
+    for _ in range(2):                       # a capture, then a replay of the same graph
+        for r in range(nref):
+            if mask >> r & 1:
+                _spoil_planes(hip, dref[r], w, h)
+        hip.plan_run(plan, dref, mask)
+        res, best = hip.plan_results(plan, nref)
+        _compare(res, best, ores, obest)
+    # the same handle twice: rebuilt in order, never beside its own search
+    hip.plan_run(plan, [dref[0]] * nref, 1)
+    res, best = hip.plan_results(plan, nref)
+    o2, b2 = orc.me_frame(cur, [orc.make_planes(refs[0])] * nref, R, lam, MASK_ALL)
+    _compare(res, best, o2, b2)
+    hip.plan_destroy(plan)
+    for r in dref:
+        hip.ref_destroy(r)
+    assert hip.error_count() == 0
+
+
 @pytest.mark.parametrize("bd,nref", [(8, 2), (10, 1)])
 def test_frame_plan_high_contrast(bd, nref):
     """Independent uniform-noise frames: mean |difference| of about a third of

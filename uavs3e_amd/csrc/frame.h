@@ -85,6 +85,7 @@ __device__ __forceinline__ const PicDesc *job_decode(const JobSet &js, int job, 
 
 struct IntArgs {
     JobSet js;
+    int ref0;                // reference of blockIdx.z == 0
     int R;                   // window half size
     int lo, hi;              // uniform displacement clip: [max(-R,-T+4), min(R,T-4)]
     int IB, sh;              // index bits; sad shift into the high word
@@ -180,6 +181,11 @@ int me_search_warmup(const MeSearch *s, hipStream_t st);
 // reference choice for the jobs of js.  keys/res are [job][npu][nref],
 // best is [job][npu][nl].  ev, if given, receives 4 events recorded before the
 // key fill, before the integer search, before sub-pel and at the end.
+// The same in three parts, so that the integer search of references [r0, r1)
+// can run on its own stream: key fill, integer search, sub-pel + choice.
+int me_search_fill(const MeSearch *s, const JobSet &js, u64 *keys, hipStream_t st);
+int me_search_int(const MeSearch *s, const JobSet &js, u64 *keys, int r0, int r1, hipStream_t st);
+int me_search_finish(const MeSearch *s, const JobSet &js, const u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st);
 int me_search_enqueue(const MeSearch *s, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
                       hipEvent_t *ev);
 // The MV-bits table of inter_search_create (inter.c:2029-2058) in closed

@@ -211,26 +211,29 @@ int me_search_warmup(const MeSearch *p, hipStream_t st)
     return 0;
 }
 
-int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
-                      hipEvent_t *ev)
+int me_search_fill(const MeSearch *p, const JobSet &js, u64 *keys, hipStream_t st)
 {
-    if (js.n_jobs <= 0) return 0;
-    if (ev) ME_CHECK(hipEventRecord(ev[0], st));
     const size_t nkeys = (size_t)js.n_jobs * p->npu * p->nref;
     hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, st, keys, nkeys);
     ME_CHECK(hipGetLastError());
-    if (ev) ME_CHECK(hipEventRecord(ev[1], st));
+    return 0;
+}
 
+int me_search_int(const MeSearch *p, const JobSet &js, u64 *keys, int r0, int r1, hipStream_t st)
+{
     IntArgs a;
     memset(&a, 0, sizeof(a));
     a.js = js;
+    a.ref0 = r0;
     a.R = p->R; a.lo = p->lo; a.hi = p->hi; a.IB = p->IB; a.sh = p->sh;
     a.rate = p->d_rate; a.keys = keys; a.npu = p->npu; a.nref = p->nref;
     a.band_rows = p->band_rows; a.wcols = p->wcols; a.dx_cols = p->dx_cols; a.nbx = p->nbx; a.wpitch = p->wpitch;
     memcpy(a.base, p->base, sizeof(a.base));
-    if (me_int_search_launch(p->mode, p->waves, a, dim3(p->nbands * p->nbx, js.n_jobs, p->nref), p->lds, st)) return -1;
-    if (ev) ME_CHECK(hipEventRecord(ev[2], st));
+    return me_int_search_launch(p->mode, p->waves, a, dim3(p->nbands * p->nbx, js.n_jobs, r1 - r0), p->lds, st);
+}
 
+int me_search_finish(const MeSearch *p, const JobSet &js, const u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st)
+{
     SubArgs b;
     memset(&b, 0, sizeof(b));
     b.js = js;
@@ -246,6 +249,19 @@ int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_resu
     hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, res, best, npus, p->nref,
                        p->nref - p->nref_l1, p->nl);
     ME_CHECK(hipGetLastError());
+    return 0;
+}
+
+int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
+                      hipEvent_t *ev)
+{
+    if (js.n_jobs <= 0) return 0;
+    if (ev) ME_CHECK(hipEventRecord(ev[0], st));
+    if (me_search_fill(p, js, keys, st)) return -1;
+    if (ev) ME_CHECK(hipEventRecord(ev[1], st));
+    if (me_search_int(p, js, keys, 0, p->nref, st)) return -1;
+    if (ev) ME_CHECK(hipEventRecord(ev[2], st));
+    if (me_search_finish(p, js, keys, res, best, st)) return -1;
     if (ev) ME_CHECK(hipEventRecord(ev[3], st));
     return 0;
 }
@@ -267,6 +283,8 @@ struct me_plan_s {
     me_pu_result_t *d_res;
     s8 *d_best;
     hipStream_t own;
+    hipStream_t side;     // rebuilds references while the others are searched
+    hipEvent_t fork, join;
     hipGraphExec_t exec;
     me_ref_t graph_refs[ME_MAX_REFS];
     u32 graph_mask;
@@ -301,6 +319,9 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     ALLOC(p->d_best, (size_t)p->n_ctu * p->s.npu * p->s.nl);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&p->own);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         me_set_error("me_plan_create", e, __FILE__, __LINE__);
         me_plan_destroy(p);
@@ -362,9 +383,36 @@ static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prep
     return me_search_enqueue(&p->s, js, p->d_keys, p->d_res, p->d_best, st, ev ? ev + 1 : nullptr);
 }
 
+// The frame graph.  When the references to rebuild are a leading range
+// [0, m) of fewer than all of them, they are rebuilt on a side stream while
+// the integer search of [m, nref) runs (it does not read them); their own
+// search follows on the side stream, and the two branches join before the
+// sub-pel stage.  The results are those of the serial order: each reference's
+// keys are written only by its own integer search.
 static int plan_enqueue(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
 {
-    return plan_enqueue_stages(p, refs, prepare_mask, st, nullptr);
+    const int nref = p->s.nref;
+    int m = 0;
+    while (m < nref && (prepare_mask >> m & 1)) m++;
+    bool split = m > 0 && m < nref && (prepare_mask >> m) == 0;
+    // a handle listed twice would be read while it is rebuilt
+    for (int r = m; split && r < nref; r++)
+        for (int q = 0; q < m; q++) split = split && refs[r] != refs[q];
+    if (!split) return plan_enqueue_stages(p, refs, prepare_mask, st, nullptr);
+    JobSet js;
+    js.pics = p->d_pic;
+    js.jobs = nullptr;
+    js.n_jobs = p->n_ctu;
+    if (me_search_fill(&p->s, js, p->d_keys, st)) return -1;
+    ME_CHECK(hipEventRecord(p->fork, st));
+    ME_CHECK(hipStreamWaitEvent(p->side, p->fork, 0));
+    for (int r = 0; r < m; r++)
+        if (me_ref_prepare_async(refs[r], p->side)) return -1;
+    if (me_search_int(&p->s, js, p->d_keys, 0, m, p->side)) return -1;
+    ME_CHECK(hipEventRecord(p->join, p->side));
+    if (me_search_int(&p->s, js, p->d_keys, m, nref, st)) return -1;
+    ME_CHECK(hipStreamWaitEvent(st, p->join, 0));
+    return me_search_finish(&p->s, js, p->d_keys, p->d_res, p->d_best, st);
 }
 
 ME_EXPORT int me_plan_run(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, void *stream)
@@ -586,6 +634,9 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->own) (void)hipStreamDestroy(p->own);
+    if (p->side) (void)hipStreamDestroy(p->side);
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    if (p->join) (void)hipEventDestroy(p->join);
     me_search_free(&p->s);
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);

@@ -718,7 +718,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
     const int band = blockIdx.x / a.nbx, bandx = blockIdx.x - band * a.nbx, job = blockIdx.y;
     int ctu;
     const PicDesc &P = *job_decode(a.js, job, ctu);
-    b.ref = blockIdx.z;
+    b.ref = a.ref0 + blockIdx.z;
     b.cx = (ctu % P.ctu_cols) * 128;
     b.cy = (ctu / P.ctu_cols) * 128;
     b.dy0 = -a.R + band * a.band_rows;
