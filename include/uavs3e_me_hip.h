@@ -243,6 +243,27 @@ typedef struct me_tz_cfg_t {
 int me_plan_run_tz(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_ipel_prev,
                    const me_tz_cfg_t *tz, void *stream);
 
+/* The inter part of the lookahead's cost estimator, loka_estimate_coding_cost
+ * (lookahead.c:82-208), for every whole 16x16 unit of the plan's current
+ * picture: each reference (the plan's list-0 references, then list 1) is
+ * searched by me_search_tz from MVP 0 at integer precision (curr_mvr 2, no
+ * adaptive raster range, lambda_mv floor(65536*sqrt(1.43631*2^4)) = 314169,
+ * lookahead.c:85-110; gop_size, max_search_range and the POCs from tz, its
+ * mvr / cost_type / adaptive_raster_range ignored); the unit's cost is the
+ * least com_had of (a) each reference's prediction at its MV, (b) with both
+ * lists, the average of each list's first cheapest prediction and (c) the
+ * average of both lists' first reference at MV 0 (lookahead.c:130-208).
+ * is_padding: inter_search_init's flag (inter.c:2015-2027) -- 1 lets windows
+ * reach 127 samples outside the picture (the picture-encoding pinter,
+ * uavs3e.c:185), 0 keeps every block inside it (the lookahead thread's,
+ * uavs3e.c:1326).  The intra part of the estimator (lookahead.c:210-231) is
+ * the caller's: the picture's cost is the sum over units of the minimum of
+ * this and the intra cost.  unit_cost: host array of (pic_w/16)*(pic_h/16)
+ * u32 in raster order, written before the call returns.  The plan needs the
+ * 16x16 shape; its 16x16 results are overwritten. */
+int me_plan_lookahead(me_plan_t plan, me_ref_t const *refs, const me_tz_cfg_t *tz, int is_padding,
+                      uint32_t *unit_cost, void *stream);
+
 /* Waits for the last run and copies results to host memory:
  * res[(ctu*npu + pu)*num_refs + r], best_ref[(ctu*npu + pu)*L + l] with L = 1
  * (single list) or 2 (num_refs_l1 > 0): the reference index (into refs) of

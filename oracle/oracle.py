@@ -105,8 +105,8 @@ class Oracle(_Lib):
         return c, (ptrs, tab, cur, planes)
 
     def me_search_tz(self, cur, planes, x, y, w, h, refi, num_refp, cur_poc, ref_poc, gop, msr, mvr, cost_type,
-                     adaptive, lambda_mv, mvp):
-        c, keep = self.search_ctx(cur, x, y, planes, lambda_mv, mvr, cost_type)
+                     adaptive, lambda_mv, mvp, is_padding=1):
+        c, keep = self.search_ctx(cur, x, y, planes, lambda_mv, mvr, cost_type, is_padding)
         t = OrcTz(gop, msr, adaptive, cur_poc, ref_poc, num_refp)
         mvp_a = np.array(mvp, np.int16)
         mv = np.zeros(2, np.int16)
@@ -123,6 +123,50 @@ class Oracle(_Lib):
         mv = np.array([imv[0] * 4, imv[1] * 4], np.int16)
         r = self.lib.orc_subpel(C.byref(c), ptr(mvp_a), x, y, w, h, ptr(mv))
         return (int(mv[0]), int(mv[1])), r
+
+    def loka_inter_cost(self, cur, planes_l0, planes_l1, cur_poc, poc_l0, poc_l1, gop, msr, is_padding, pad=PAD):
+        """The inter part of loka_estimate_coding_cost (lookahead.c:82-208)
+        for every whole 16x16 unit of `cur`, uint32 [h // 16][w // 16]:
+        me_search_tz from MVP 0 with curr_mvr 2 and the lookahead's lambda
+        (lookahead.c:85-110, 150-152), com_mc_blk_luma at the integer MV
+        (153), com_had (155), the first cheapest per list (156-170), then
+        with both lists the pel_avrg of the two kept predictions (172-179)
+        and of both lists' first reference at MV 0 (181-207)."""
+        h, w = cur.shape
+        stride = w + 2 * pad
+        lam = int(np.floor(65536.0 * np.sqrt(1.43631 * 2.0 ** ((32 - 16.0) / 4.0))))
+        lists = [(planes_l0, poc_l0), (planes_l1, poc_l1)]
+        is_bi = len(planes_l0) > 0 and len(planes_l1) > 0
+        out = np.zeros((h // 16, w // 16), np.uint32)
+        org_all = np.ascontiguousarray(cur)
+
+        def block(planes, bx, by):
+            return np.ascontiguousarray(planes[0][pad + by:pad + by + 16, pad + bx:pad + bx + 16])
+
+        for uy in range(h // 16):
+            for ux in range(w // 16):
+                x, y = ux * 16, uy * 16
+                org = np.ascontiguousarray(org_all[y:y + 16, x:x + 16])
+                best = (1 << 64) - 1
+                kept = [None, None]
+                for lidx, (pl, pocs) in enumerate(lists):
+                    best_l = (1 << 64) - 1
+                    for refi, planes in enumerate(pl):
+                        mv, _ = self.me_search_tz(cur, planes, x, y, 16, 16, refi, len(pl), cur_poc, pocs[refi],
+                                                  gop, msr, 2, 0, 0, lam, (0, 0), is_padding)
+                        pred = block(planes, x + (mv[0] >> 2), y + (mv[1] >> 2))
+                        c = self.had(16, 16, org, 16, pred, 16)
+                        if c < best_l:
+                            best_l = c
+                            best = min(best, c)
+                            kept[lidx] = pred
+                if is_bi:
+                    c = self.had(16, 16, org, 16, self.pel_avrg(16, 16, kept[0], kept[1]), 16)
+                    best = min(best, c)
+                    z = self.pel_avrg(16, 16, block(planes_l0[0], x, y), block(planes_l1[0], x, y))
+                    best = min(best, self.had(16, 16, org, 16, z, 16))
+                out[uy, ux] = min(best, 0xFFFFFFFF)
+        return out
 
     # -- block costs --------------------------------------------------------
     def sad(self, w, h, org, i_org, pred, i_pred, org_off=0, pred_off=0):
@@ -291,6 +335,9 @@ class RefLib(_Lib):
         L.refh_pic_free.argtypes = [C.c_void_p]
         L.refh_luma_coef.restype = C.c_void_p
         L.refh_chroma_coef.restype = C.c_void_p
+        L.refh_loka_cost.restype = C.c_double
+        L.refh_loka_cost.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_longlong, C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_int, C.c_int, C.c_int, C.c_int]
         L.refh_me_search_tz.restype = C.c_uint64
         L.refh_me_search_tz.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_int, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -390,6 +437,16 @@ class RefLib(_Lib):
         self.lib.refh_mc_cu(x, y, pic_w, pic_h, w, h, C.c_void_p(p0) if p0 else None, C.c_void_p(p1) if p1 else None,
                             ptr(m0), ptr(m1), 0 if chroma else 1, ptr(py), ptr(pu), ptr(pv))
         return py, (pu if chroma else None), (pv if chroma else None)
+
+    def loka_cost(self, cur, pics_l0, pics_l1, cur_poc, gop, msr, is_padding):
+        """loka_estimate_coding_cost (lookahead.c:82-290) of the compiled
+        reference: the picture's summed least inter / intra unit cost."""
+        h, w = cur.shape
+        cur = np.ascontiguousarray(cur, self.dt)
+        a0 = (C.c_void_p * 4)(*pics_l0)
+        a1 = (C.c_void_p * 4)(*pics_l1)
+        return self.lib.refh_loka_cost(ptr(cur), w, h, cur_poc, a0, len(pics_l0), a1, len(pics_l1), gop, msr,
+                                       is_padding)
 
     def me_frame(self, cur, pics, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, threads=1, ctu_ms=None):
         h, w = cur.shape

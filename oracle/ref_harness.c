@@ -106,6 +106,7 @@ RH_EXPORT void *refh_pic_create(const pel *luma, int w, int h, long long poc)
     r->pic->subpel = com_subpel_create(w, h, PIC_PAD_SIZE_L, PIC_PAD_SIZE_C, &err);
     r->pic->subpel->imgs[0][0] = r->pic->img;
     r->pic->ptr = poc;
+    r->pic->img->ptr = poc;   /* what the lookahead reads (lookahead.c:143) */
     int ts = w + 16;
     s16 *tmp[3];
     /* The SIMD frame filters write whole vectors past the end of a row (the
@@ -450,4 +451,35 @@ RH_EXPORT double refh_me_frame(int pic_w, int pic_h, const pel *cur, int i_cur, 
     free(tid);
     pthread_mutex_destroy(&j.lock);
     return t;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Lookahead                                                                */
+/* ---------------------------------------------------------------------- */
+
+/* loka_estimate_coding_cost (lookahead.c:82-290) of `cur` against pictures
+ * made by refh_pic_create (l0[0..n0), l1[0..n1)), with an inter_search_t set
+ * up by inter_search_init(.., is_padding) (inter.c:2015-2027) as the encoder
+ * does (uavs3e.c:185, 1326); no icost / chroma / dqp outputs.  Returns the
+ * picture cost: the sum over whole 16x16 units of the least inter or intra
+ * com_had. */
+RH_EXPORT double refh_loka_cost(const pel *cur, int w, int h, long long cur_poc, void **l0, int n0, void **l1,
+                                int n1, int gop_size, int max_search_range, int is_padding)
+{
+    int err = 0;
+    com_pic_t *org = com_pic_create(w, h, PIC_PAD_SIZE_L, PIC_PAD_SIZE_C, &err);
+    for (int y = 0; y < h; y++) memcpy(org->y + y * org->stride_luma, cur + y * w, sizeof(pel) * w);
+    org->img->ptr = cur_poc;
+    com_img_t *ref_l0[4], *ref_l1[4];
+    for (int i = 0; i < n0; i++) ref_l0[i] = ((refh_pic_t *)l0[i])->pic->img;
+    for (int i = 0; i < n1; i++) ref_l1[i] = ((refh_pic_t *)l1[i])->pic->img;
+    refh_search_t s;
+    search_setup(&s, w, h, 0, 0, is_padding);
+    s.pi.gop_size = gop_size;
+    s.pi.max_search_range = max_search_range;
+    int num_ref[2] = {n0, n1};
+    double c = loka_estimate_coding_cost(&s.pi, org->img, ref_l0, ref_l1, num_ref, BIT_DEPTH, NULL, NULL, NULL);
+    search_free(&s);
+    com_pic_destroy(org);
+    return c;
 }

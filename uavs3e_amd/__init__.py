@@ -159,6 +159,7 @@ class MeHip:
         L.me_plan_mc_size.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_mc_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_run_tz.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.me_plan_lookahead.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
@@ -314,6 +315,15 @@ class MeHip:
         mvi = np.ascontiguousarray(mvi_prev, np.int16) if mvi_prev is not None else None
         self.check(self.lib.me_plan_run_tz(plan, arr, _p(mvp), _p(mvi) if mvi is not None else None, C.byref(tz),
                                            stream), "me_plan_run_tz")
+
+    def plan_lookahead(self, plan, refs, tz, is_padding, pic_w, pic_h, stream=None):
+        """me_plan_lookahead: the lookahead's inter cost of each whole 16x16
+        unit, as uint32 [pic_h // 16][pic_w // 16]."""
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        out = np.zeros((pic_h // 16, pic_w // 16), np.uint32)
+        self.check(self.lib.me_plan_lookahead(plan, arr, C.byref(tz), int(is_padding), _p(out), stream),
+                   "me_plan_lookahead")
+        return out
 
     def plan_mc(self, plan, refs, stream=None):
         """me_plan_mc into the plan's own device buffers."""

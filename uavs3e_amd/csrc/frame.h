@@ -208,8 +208,15 @@ void me_mc_sizes(const MeSearch *s, size_t *luma_per_ctu, size_t *chroma_per_ctu
 // me_search_tz of every PU from per-PU MVPs
 // ---------------------------------------------------------------------------
 
-// Enqueues the reference's search (me.c:529-568) for the jobs of js: mvp and
-// mvi_prev (optional) are [job][npu][nref][2] device arrays; results go to
-// res [job][npu][nref].
+// Enqueues the reference's search (me.c:529-568) for the jobs of js: mvp
+// (null: all zero) and mvi_prev (optional) are [job][npu][nref][2] device
+// arrays; results go to res [job][npu][nref].  only_shape >= 0 searches that
+// shape alone; pad_win is inter_search_t.is_padding; lambda 0 takes the
+// search's own.
 int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mvi_prev,
-                  const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st);
+                  const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st, int only_shape = -1, int pad_win = 1,
+                  u32 lambda = 0);
+// The lookahead's inter cost of every whole 16x16 unit of the picture, in
+// raster order, from the 16x16 results of me_tz_enqueue (device out).
+int me_loka_enqueue(const MeSearch *s, const PicDesc *pic, const me_pu_result_t *res, int pic_w, int pic_h, u32 *out,
+                    hipStream_t st);

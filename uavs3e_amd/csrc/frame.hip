@@ -17,6 +17,7 @@
 //   ref choice    smallest final cost, lower index on ties         inter.c:901-904
 #include "frame.h"
 
+#include <math.h>
 #include <string.h>
 #include <algorithm>
 #include <vector>
@@ -282,6 +283,7 @@ struct me_plan_s {
     u64 *d_keys;
     me_pu_result_t *d_res;
     s8 *d_best;
+    u32 *d_loka;          // lookahead unit costs (allocated on first use)
     hipStream_t own;
     hipStream_t side;     // rebuilds references while the others are searched
     hipEvent_t fork, join;
@@ -569,6 +571,47 @@ ME_EXPORT int me_plan_run_tz(me_plan_t p, me_ref_t const *refs, const int16_t *m
     return 0;
 }
 
+// loka_estimate_coding_cost (lookahead.c:82-208), the inter part: every
+// whole 16x16 unit searched by me_search_tz from MVP 0 at integer precision
+// with the lookahead's lambda (lookahead.c:85-110), then its com_had costs.
+ME_EXPORT int me_plan_lookahead(me_plan_t p, me_ref_t const *refs, const me_tz_cfg_t *tz, int is_padding,
+                                uint32_t *unit_cost, void *stream)
+{
+    if (!p || !refs || !tz || !unit_cost || tz->gop_size < 1) {
+        me_set_error("me_plan_lookahead: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (p->s.base[S16x16] < 0 || p->cfg.pic_w % 4 || p->cfg.pic_w < 32 || p->cfg.pic_h < 32) {
+        me_set_error("me_plan_lookahead: the plan needs the 16x16 shape and a picture of at least 32x32 with a "
+                     "width that is a multiple of 4", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    for (int r = 0; r < p->s.nref; r++)
+        if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h) {
+            me_set_error("me_plan_lookahead: reference size differs from the plan", hipErrorInvalidValue, __FILE__,
+                         __LINE__);
+            return -1;
+        }
+    hipStream_t st = stream ? (hipStream_t)stream : (p->graph_stream ? p->graph_stream : p->own);
+    const int units = (p->cfg.pic_w / 16) * (p->cfg.pic_h / 16);
+    if (!p->d_loka) ME_CHECK(hipMalloc((void **)&p->d_loka, (size_t)units * sizeof(u32)));
+    if (plan_set_pic(p, refs, st)) return -1;
+    me_tz_cfg_t t = *tz;
+    t.mvr = 2;                       // curr_mvr = 2: integer MVs, no sub-pel
+    t.adaptive_raster_range = 0;
+    t.cost_type = 0;
+    const u32 lambda = (u32)floor(65536.0 * sqrt(1.43631 * pow(2.0, (32 - 16.0) / 4.0)));
+    JobSet js;
+    js.pics = p->d_pic;
+    js.jobs = nullptr;
+    js.n_jobs = p->n_ctu;
+    if (me_tz_enqueue(&p->s, js, nullptr, nullptr, &t, p->d_res, st, S16x16, is_padding ? 1 : 0, lambda)) return -1;
+    if (me_loka_enqueue(&p->s, p->d_pic, p->d_res, p->cfg.pic_w, p->cfg.pic_h, p->d_loka, st)) return -1;
+    ME_CHECK(hipMemcpyAsync(unit_cost, p->d_loka, (size_t)units * sizeof(u32), hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
 ME_EXPORT int me_plan_mc_size(me_plan_t p, uint64_t *luma_elems, uint64_t *chroma_elems)
 {
     if (!p) return -1;
@@ -638,6 +681,7 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     if (p->fork) (void)hipEventDestroy(p->fork);
     if (p->join) (void)hipEventDestroy(p->join);
     me_search_free(&p->s);
+    (void)hipFree(p->d_loka);
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);
     (void)hipFree(p->d_pred);

@@ -40,6 +40,7 @@ struct TzArgs {
     int lpu;                 // lanes per (PU, reference): a power of two, 4..64
     int pw, ph, base, per_ctu, tt, tw, th;
     int gop, msr, adaptive, mvr, cost_type;
+    int pad_win;             // inter_search_t.is_padding: windows may leave the picture
     long long cur_poc, ref_poc[ME_MAX_REFS];
 };
 
@@ -285,13 +286,18 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     out.cost_int = out.cost_sub = ~0ull;
     out.cost_final = 0xFFFFFFFFu;
     out.valid = 0;
-    u.mvpx = a.mvp[slot * 2];
-    u.mvpy = a.mvp[slot * 2 + 1];
-    // get_max_search_range (me.c:502-527), is_padding = 1
+    u.mvpx = a.mvp ? a.mvp[slot * 2] : 0;
+    u.mvpy = a.mvp ? a.mvp[slot * 2 + 1] : 0;
+    // get_max_search_range (me.c:502-527)
     const int mvr = a.mvr;
     const int T = (a.tab_off << mvr) >> 2;
     const int px = u.mvpx >> 2, py = u.mvpy >> 2;
-    int lox = -128 + 1 - u.x, loy = -128 + 1 - u.y, hix = P.pic_w - 1 - u.x, hiy = P.pic_h - 1 - u.y;
+    int lox, loy, hix, hiy;
+    if (a.pad_win) {
+        lox = -128 + 1 - u.x; loy = -128 + 1 - u.y; hix = P.pic_w - 1 - u.x; hiy = P.pic_h - 1 - u.y;
+    } else {
+        lox = -u.x; loy = -u.y; hix = P.pic_w - 1 - u.x - u.w; hiy = P.pic_h - 1 - u.y - u.h;
+    }
     if (px - T + 4 > lox) lox = px - T + 4;
     if (py - T + 4 > loy) loy = py - T + 4;
     if (px + T - 4 < hix) hix = px + T - 4;
@@ -360,8 +366,115 @@ tz_fn me_tz_kernel_8x16(int tt) { return tt == 5 ? k_tz<5> : nullptr; }
 tz_fn me_tz_kernel_16x8(int tt);
 tz_fn me_tz_kernel_8x16(int tt);
 
+// The lookahead's inter cost of each 16x16 unit (loka_estimate_coding_cost,
+// lookahead.c:130-208) from the searched MVs: per list, the com_had of the
+// unit against each reference at its MV (com_mc_blk_luma at an integer MV is
+// the integer plane at x + mv, y + mv, com_mc.c:41-86), the list's first
+// cheapest kept; then the average (pel_avrg, com_util.c:3088-3105) of the
+// two lists' kept predictions, and of both lists' first reference at MV 0.
+// out[unit] is the cheapest (u32 max when nothing was searched).  A group of
+// 4 lanes per unit, one 8x8 com_had tile per lane; the averaged tiles go
+// through LDS.
+#if ME_TZ_PART == 0
+struct LokaArgs {
+    const PicDesc *pic;
+    const me_pu_result_t *res;
+    u32 *out;
+    int ucols, units, npu, nref, n0, base16;
+};
+
+__device__ __forceinline__ u32 loka_had(const pel *org, int so, const pel *pred, int sp)
+{
+    const int li = threadIdx.x & 3, tx = (li & 1) * 8, ty = (li >> 1) * 8;
+    const int s = satd_t<1>(org + ty * so + tx, so, pred + ty * sp + tx, sp);
+    return (u32)(group_sum(s, 4) >> (BIT_DEPTH - 8));
+}
+
+// this lane's 8x8 tile of (p + q + 1) >> 1 into t (stride 16: the group's
+// block).  Stored as whole dwords: com_had reads the tile back as dwords
+// (load_row), and 16-bit stores would not be ordered before those loads.
+__device__ __forceinline__ void loka_avg_tile(pel *t, const pel *p, const pel *q, int sp, int sq)
+{
+    constexpr int PW = 4 / (int)sizeof(pel);   // samples per dword
+    const int li = threadIdx.x & 3, tx = (li & 1) * 8, ty = (li >> 1) * 8;
+    for (int y = 0; y < 8; y++) {
+        u32 *d = (u32 *)(t + (ty + y) * 16 + tx);
+        const pel *a = p + (ty + y) * sp + tx, *b = q + (ty + y) * sq + tx;
+        for (int k = 0; k < 8 / PW; k++) {
+            u32 v = 0;
+            for (int j = 0; j < PW; j++)
+                v |= (u32)((a[k * PW + j] + b[k * PW + j] + 1) >> 1) << (8 * (int)sizeof(pel) * j);
+            d[k] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_loka(LokaArgs a)
+{
+    // 64 groups' blocks, + 8 samples of slack: tile rows are read as dwords
+    __shared__ __align__(16) pel avg[64 * 256 + 8];
+    const int g = threadIdx.x >> 2, unit = blockIdx.x * 64 + g;
+    if (unit >= a.units) return;
+    const PicDesc &P = *a.pic;
+    const int x = (unit % a.ucols) * 16, y = (unit / a.ucols) * 16;
+    const int ctu = (y >> 7) * P.ctu_cols + (x >> 7), i = ((y & 127) >> 4) * 8 + ((x & 127) >> 4);
+    const size_t slot = ((size_t)ctu * a.npu + a.base16 + i) * a.nref;
+    const pel *org = P.cur + (size_t)y * P.pic_w + x;
+    u32 best = 0xFFFFFFFFu;
+    const pel *kept[2] = {nullptr, nullptr};
+    for (int l = 0; l < 2; l++) {
+        const int r0 = l ? a.n0 : 0, r1 = l ? a.nref : a.n0;
+        u32 best_l = 0xFFFFFFFFu;
+        for (int r = r0; r < r1; r++) {
+            const me_pu_result_t &q = a.res[slot + r];
+            if (!q.valid) continue;
+            const pel *pred = plane_ptr(P, r, 0, x + (q.mv_sub[0] >> 2), y + (q.mv_sub[1] >> 2));
+            const u32 c = loka_had(org, P.pic_w, pred, P.ref_stride);
+            if (c < best_l) {
+                best_l = c;
+                best = c < best ? c : best;
+                kept[l] = pred;
+            }
+        }
+    }
+    if (a.n0 > 0 && a.nref > a.n0) {
+        pel *t = avg + g * 256;
+        if (kept[0] && kept[1]) {
+            loka_avg_tile(t, kept[0], kept[1], P.ref_stride, P.ref_stride);
+            const u32 c = loka_had(org, P.pic_w, t, 16);
+            best = c < best ? c : best;
+        }
+        loka_avg_tile(t, plane_ptr(P, 0, 0, x, y), plane_ptr(P, a.n0, 0, x, y), P.ref_stride, P.ref_stride);
+        const u32 c = loka_had(org, P.pic_w, t, 16);
+        best = c < best ? c : best;
+    }
+    if ((threadIdx.x & 3) == 0) a.out[unit] = best;
+}
+
+int me_loka_enqueue(const MeSearch *s, const PicDesc *pic, const me_pu_result_t *res, int pic_w, int pic_h, u32 *out,
+                    hipStream_t st)
+{
+    LokaArgs a;
+    memset(&a, 0, sizeof(a));
+    a.pic = pic;
+    a.res = res;
+    a.out = out;
+    a.ucols = pic_w / 16;
+    a.units = a.ucols * (pic_h / 16);
+    a.npu = s->npu;
+    a.nref = s->nref;
+    a.n0 = s->nref - s->nref_l1;
+    a.base16 = s->base[S16x16];
+    if (a.units <= 0) return 0;
+    hipLaunchKernelGGL(k_loka, dim3((a.units + 63) / 64), dim3(256), 0, st, a);
+    ME_CHECK(hipGetLastError());
+    return 0;
+}
+#endif
+
 int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mvi_prev,
-                  const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st)
+                  const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st, int only_shape, int pad_win,
+                  u32 lambda)
 {
     TzArgs a;
     memset(&a, 0, sizeof(a));
@@ -371,7 +484,8 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
     a.res = res;
     a.tab = s->d_tab_base + s->tab_off;
     a.tab_off = s->tab_off;
-    a.lambda = s->lambda;
+    a.lambda = lambda ? lambda : s->lambda;
+    a.pad_win = pad_win;
     a.npu = s->npu;
     a.nref = s->nref;
     a.n0 = s->nref - s->nref_l1;
@@ -383,7 +497,7 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
     a.cur_poc = tz->cur_poc;
     for (int r = 0; r < ME_MAX_REFS; r++) a.ref_poc[r] = tz->ref_poc[r];
     for (int k = 0; k < ME_NSHAPES; k++) {
-        if (s->base[k] < 0) continue;
+        if (s->base[k] < 0 || (only_shape >= 0 && k != only_shape)) continue;
         a.pw = ME_SHAPES_HOST[k][0];
         a.ph = ME_SHAPES_HOST[k][1];
         a.base = s->base[k];
