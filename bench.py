@@ -195,6 +195,43 @@ def main():
                      "pus_searched": int((tres["valid"] == 1).sum())}
         print("[bench] me_search_tz: %.2f ms/frame" % tz_ms, file=sys.stderr, flush=True)
 
+    # ---- lookahead cost estimator (inter part) -------------------------------
+    lookahead = None
+    if not args.no_tz and hip.lib.me_plan_num_pus(plan) > 0:
+        import ctypes
+        n0 = work.num_refs - work.num_refs_l1
+        pocs = [4, 0, 2, 6][:n0] + [12, 16][:work.num_refs_l1]
+        # random access: GOP 16, SEARCH_RANGE_IPEL_RA 384 (com_define.h:137), padded references
+        ltz = ua.TzCfg(16, 384, 0, 2, 0, 8, (ctypes.c_int64 * 4)(*(pocs + [0] * 4)[:4]))
+        if work.mask >> 13 & 1:   # the 16x16 shape
+            lcost = hip.plan_lookahead(plan, dref, ltz, 1, work.w, work.h)   # warm-up
+            t0 = time.perf_counter()
+            for _ in range(5):
+                lcost = hip.plan_lookahead(plan, dref, ltz, 1, work.w, work.h)
+            l_ms = (time.perf_counter() - t0) * 1e3 / 5
+            lookahead = {"ms_per_frame": l_ms, "units_per_s": lcost.size / (l_ms * 1e-3), "units": int(lcost.size),
+                         "note": "me_plan_lookahead: me_search_tz of every whole 16x16 unit from MVP 0 at "
+                                 "integer precision against every reference, then com_had of each "
+                                 "prediction (and the bi averages with two lists); includes the copy of "
+                                 "the unit costs to the host"}
+            from oracle.oracle import RefLib
+            if RefLib.available(bd) and not args.no_cpu:
+                rl = RefLib(bd, level=2)
+                p0 = [rl.pic_create(refs[i], pocs[i]) for i in range(n0)]
+                p1 = [rl.pic_create(refs[i], pocs[i]) for i in range(n0, work.num_refs)]
+                t0 = time.perf_counter()
+                rl.loka_cost(cur, p0, p1, 8, 16, 384, 1)
+                c_ms = (time.perf_counter() - t0) * 1e3
+                for q in p0 + p1:
+                    rl.pic_free(q)
+                lookahead["cpu_reference_ms"] = c_ms
+                lookahead["cpu_note"] = ("loka_estimate_coding_cost of the compiled reference (C+SSE4.2+AVX2), "
+                                         "one thread as the encoder runs it; includes its intra part "
+                                         "(11 modes per unit)")
+            print("[bench] lookahead: %.3f ms/frame (%d units)%s" % (
+                l_ms, lcost.size, "; reference CPU %.1f ms" % lookahead["cpu_reference_ms"]
+                if "cpu_reference_ms" in lookahead else ""), file=sys.stderr, flush=True)
+
     # ---- per-call drop-in leg -----------------------------------------------
     per_call = None
     if not args.no_percall:
@@ -348,6 +385,7 @@ def main():
         "speedup_vs_per_call": (ctus_per_s / per_call["value"]) if per_call else None,
         "serving": serving,
         "search_tz": search_tz,
+        "lookahead": lookahead,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
     }
