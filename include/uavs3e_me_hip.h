@@ -243,6 +243,37 @@ typedef struct me_tz_cfg_t {
 int me_plan_run_tz(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_ipel_prev,
                    const me_tz_cfg_t *tz, void *stream);
 
+/* Bi-prediction ME of analyze_bi (inter.c:972-1033), the search part, for
+ * every PU of the plan that has references in both lists: starting from the
+ * uni-prediction results, up to two rounds of -- the prediction of the fixed
+ * list (com_mc_cu luma, com_mc.c:140-165) makes the target 2*org - pred
+ * clipped (create_bi_org, inter.c:55-68); every reference of the other list
+ * is searched against it by me_search_tz with bi = 1 (core square around
+ * the previous MV, halved distortion, the fixed list's mot_bits in the
+ * final cost, me.c:199-355, 366-500, 529-568); a strictly cheaper cost than
+ * the best so far takes that list's reference, MV and mot_bits (get_mv_bits,
+ * inter.c:50-53); the searched list becomes the fixed one, and a round
+ * without a change ends the search.  The RDO after the search
+ * (inter.c:1035-1070) is the caller's.
+ * Per PU (pus[ctu*npu + pu]): refi, mv, lidx_ref and mot_bits on entry are
+ * analyze_bi's refi_L0L1, mv_L0L1, lidx_ref and pi->mot_bits; on return refi,
+ * mv and mot_bits are cur_info->refi / mv and pi->mot_bits after the search
+ * and cost is best_mecost (u32 max when nothing was searched).  active = 0
+ * skips a PU.  mvp and mv_scale are [(ctu*npu + pu)*num_refs + r][2] quarter
+ * pel (pi->mvp_scale / mv_scale, list 0 first); mv_scale is updated as the
+ * reference updates it.  tz: mvr and cost_type.  Blocking (results on the
+ * host when the call returns); the plan's results are overwritten. */
+typedef struct me_bi_pu_t {
+    int8_t refi[2];
+    int8_t lidx_ref;
+    int8_t active;
+    int16_t mv[2][2];
+    int32_t mot_bits[2];
+    uint32_t cost;
+} me_bi_pu_t;
+int me_plan_run_bi(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, int16_t *mv_scale, me_bi_pu_t *pus,
+                   const me_tz_cfg_t *tz, void *stream);
+
 /* The inter part of the lookahead's cost estimator, loka_estimate_coding_cost
  * (lookahead.c:82-208), for every whole 16x16 unit of the plan's current
  * picture: each reference (the plan's list-0 references, then list 1) is

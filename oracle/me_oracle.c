@@ -364,6 +364,7 @@ typedef struct {
     int is_padding;
     int mvr;                        /* AMVR index; the batched configs use 0 */
     int cost_type;                  /* 0 = SATD, 1 = SAD then SATD at winner */
+    int bi;                         /* me_search_tz's bi: org is the bi-prediction target, costs halved */
 } orc_ctx_t;
 
 EXPORT int orc_ctx_size(void) { return (int)sizeof(orc_ctx_t); }
@@ -484,7 +485,7 @@ EXPORT u64 orc_subpel(const orc_ctx_t *c, const s16 mvp[2], int x, int y, int w,
 {
     static const int HP[8][2] = {{-2, 0}, {-2, 2}, {0, 2}, {2, 2}, {2, 0}, {2, -2}, {0, -2}, {-2, -2}};
     static const int QP[8][2] = {{-1, 0}, {0, 1}, {1, 0}, {0, -1}, {-1, 1}, {1, 1}, {-1, -1}, {1, -1}};
-    int shift = 16 - (c->cost_type == 0 ? 0 : (c->bd - 8));
+    int shift = 16 - c->bi - (c->cost_type == 0 ? 0 : (c->bd - 8));
     int cx = mv[0], cy = mv[1];
     u32 d = subpel_dist(c, w, h, ipel_ptr(c, x, y, cx >> 2, cy >> 2));
     u64 best = (u64)(c->lambda_mv * qpel_bits(c, mvp, cx, cy)) + ((u64)d << shift);
@@ -505,7 +506,7 @@ EXPORT u64 orc_subpel(const orc_ctx_t *c, const s16 mvp[2], int x, int y, int w,
     }
     if (c->cost_type) {
         d = orc_had(w, h, c->org, c->i_org, orc_subpel_ptr(c, mv[0] + (x << 2), mv[1] + (y << 2)), c->i_ref, c->bd);
-        best = (u64)(c->lambda_mv * qpel_bits(c, mvp, mv[0], mv[1])) + ((u64)d << 16);
+        best = (u64)(c->lambda_mv * qpel_bits(c, mvp, mv[0], mv[1])) + ((u64)d << (16 - c->bi));
     }
     return best;
 }
@@ -531,6 +532,7 @@ typedef struct {
     long long ptr, ref_ptr;     /* POC of the current and the reference picture */
     int num_refp;
     s16 mv_ipel[2];             /* previous integer result for this list/ref (adaptive range) */
+    int bi_bits;                /* bi: mot_bits of the other list + 1 (me.c:561) */
 } orc_tz_t;
 
 EXPORT int orc_tz_size(void) { return (int)sizeof(orc_tz_t); }
@@ -615,9 +617,15 @@ static int search_diamond(const orc_ctx_t *c, const s16 lo[2], const s16 hi[2], 
         {{-4, 0}, {-3, 1}, {-2, 2}, {-1, 3}, {0, 4}, {1, 3}, {2, 2}, {3, 1},
          {4, 0}, {3, -1}, {2, -2}, {1, -3}, {0, -4}, {-1, -3}, {-2, -2}, {-3, -1}}};
     int best_step = 0;
-    int shift = 16 - (c->bd - 8);
-    int core = 2, cstep = 1;
+    int shift = 16 - c->bi - (c->bd - 8);
+    int core = c->bi ? 5 : 2, cstep = 1;
     int cx = mv[0], cy = mv[1];
+    if (c->bi) {   /* the core square spans the distance to the rounded MVP, 2..5 (6 above integer AMVR) */
+        if (c->mvr > 2) core++;
+        int dx = abs(((mvp[0] + 2) >> 2) - cx), dy = abs(((mvp[1] + 2) >> 2) - cy);
+        int d = (dx > dy ? dx : dy) >> (c->mvr > 2 ? c->mvr - 2 : 0);
+        core = CLIP3(2, core, d);
+    }
     if (c->mvr > 2) { core <<= c->mvr - 2; cstep <<= c->mvr - 2; }
     int x0 = cx - core, y0 = cy - core, x1 = cx + core, y1 = cy + core;
     while (x0 <= lo[0]) x0 += cstep;
@@ -630,6 +638,7 @@ static int search_diamond(const orc_ctx_t *c, const s16 lo[2], const s16 hi[2], 
             for (int mx = x0; mx <= x1; mx += cstep) try_ipel(c, mvp, x, y, w, h, shift, mx, my, best, mv);
     }
     if (mv[0] != cx || mv[1] != cy) best_step = 2;
+    if (c->bi) return best_step;   /* no multilayer diamond (me.c:317-319) */
     int scale = c->mvr - 2 > 0 ? c->mvr - 2 : 0;
     for (int step = 4, round = 0; step < max_step; step <<= 1, round++) {
         int ss = (step << scale) >> 1;
@@ -644,30 +653,35 @@ static int search_diamond(const orc_ctx_t *c, const s16 lo[2], const s16 hi[2], 
     return best_step;
 }
 
-/* me_search_tz (me.c:529-568) for uni-prediction (bi = 0).  Returns the u64
- * that the reference returns: UINT64_MAX for an empty window, otherwise the
- * rounded u32 cost.  mv_out is in quarter-pel; mv_ipel_out receives the
- * integer-pel result (me.c:553); *cost64 the unrounded cost. */
+/* me_search_tz (me.c:529-568).  Returns the u64 that the reference
+ * returns: UINT64_MAX for an empty window (mv_out untouched), otherwise the
+ * rounded u32 cost.  mv_out is in quarter-pel; with bi = 0 the search starts
+ * at the MVP and mv_ipel_out receives the integer-pel result (me.c:553);
+ * with bi = 1 it starts at mv_out's value on entry, runs the core square
+ * only and leaves mv_ipel_out alone; *cost64 is the unrounded cost. */
 EXPORT u64 orc_me_search_tz(const orc_ctx_t *c, orc_tz_t *t, int x, int y, int w, int h, int refi,
                             const s16 mvp[2], s16 mv_out[2], s16 mv_ipel_out[2], u64 *cost64)
 {
     s16 lo[2], hi[2];
     if (!orc_window(c, mvp, x, y, w, h, lo, hi)) { *cost64 = UINT64_MAX; return UINT64_MAX; }
     s16 mv[2];
-    mv[0] = (s16)CLIP3(lo[0], hi[0], (mvp[0] + 2) >> 2);
-    mv[1] = (s16)CLIP3(lo[1], hi[1], (mvp[1] + 2) >> 2);
+    const s16 *start = c->bi ? mv_out : mvp;
+    mv[0] = (s16)CLIP3(lo[0], hi[0], (start[0] + 2) >> 2);
+    mv[1] = (s16)CLIP3(lo[1], hi[1], (start[1] + 2) >> 2);
     u64 best = UINT64_MAX;
     int bs = search_diamond(c, lo, hi, x, y, w, h, mvp, mv, &best, 32);
-    if (abs(mvp[0] - (mv[0] << 2)) > 2 || abs(mvp[1] - (mv[1] << 2)) > 2) {
+    if (!c->bi && (abs(mvp[0] - (mv[0] << 2)) > 2 || abs(mvp[1] - (mv[1] << 2)) > 2)) {
         if (bs > 5) search_raster(c, t, lo, hi, x, y, w, h, refi, mvp, mv, &best);
         search_diamond(c, lo, hi, x, y, w, h, mvp, mv, &best, 8);
     }
-    t->mv_ipel[0] = mv[0]; t->mv_ipel[1] = mv[1];
-    mv_ipel_out[0] = mv[0]; mv_ipel_out[1] = mv[1];
+    if (!c->bi) {
+        t->mv_ipel[0] = mv[0]; t->mv_ipel[1] = mv[1];
+        mv_ipel_out[0] = mv[0]; mv_ipel_out[1] = mv[1];
+    }
     mv[0] = (s16)(mv[0] * 4); mv[1] = (s16)(mv[1] * 4);
     if (c->mvr < 2) best = orc_subpel(c, mvp, x, y, w, h, mv);
     mv_out[0] = mv[0]; mv_out[1] = mv[1];
-    int extra = orc_refi_bits(t->num_refp, refi) + c->mvr + (c->mvr < 4) + 2;
+    int extra = orc_refi_bits(t->num_refp, refi) + c->mvr + (c->mvr < 4) + (c->bi ? t->bi_bits : 2);
     best += (u64)(c->lambda_mv * (u32)extra);
     *cost64 = best;
     return (u32)((best + (1u << 15)) >> 16);
@@ -757,6 +771,7 @@ EXPORT void orc_me_frame_lists(int pic_w, int pic_h, int bd, const pel *cur, int
                         c.planes = ref_planes + 16 * r; c.i_ref = i_ref;
                         c.tab_mvbits = tab + off; c.tab_off = off;
                         c.lambda_mv = lambda_mv; c.is_padding = 1; c.mvr = 0; c.cost_type = cost_type;
+                        c.bi = 0;
                         s16 mv[2];
                         u64 ci = orc_full_search(&c, mvp, x, y, w, h, R, mv);
                         o->mv_int[0] = mv[0]; o->mv_int[1] = mv[1];

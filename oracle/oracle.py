@@ -63,13 +63,15 @@ class OrcCtx(C.Structure):
     """orc_ctx_t of me_oracle.c."""
     _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("bd", C.c_int), ("org", C.c_void_p), ("i_org", C.c_int),
                 ("planes", C.c_void_p), ("i_ref", C.c_int), ("tab_mvbits", C.c_void_p), ("tab_off", C.c_int),
-                ("lambda_mv", C.c_uint32), ("is_padding", C.c_int), ("mvr", C.c_int), ("cost_type", C.c_int)]
+                ("lambda_mv", C.c_uint32), ("is_padding", C.c_int), ("mvr", C.c_int), ("cost_type", C.c_int),
+                ("bi", C.c_int)]
 
 
 class OrcTz(C.Structure):
     """orc_tz_t of me_oracle.c."""
     _fields_ = [("gop_size", C.c_int), ("max_search_range", C.c_int), ("adaptive_raster_range", C.c_int),
-                ("ptr", C.c_longlong), ("ref_ptr", C.c_longlong), ("num_refp", C.c_int), ("mv_ipel", C.c_int16 * 2)]
+                ("ptr", C.c_longlong), ("ref_ptr", C.c_longlong), ("num_refp", C.c_int), ("mv_ipel", C.c_int16 * 2),
+                ("bi_bits", C.c_int)]
 
 
 class Oracle(_Lib):
@@ -93,23 +95,31 @@ class Oracle(_Lib):
         assert L.orc_pu_res_size() == PU_RES_DTYPE.itemsize
         assert L.orc_ctx_size() == C.sizeof(OrcCtx) and L.orc_tz_size() == C.sizeof(OrcTz)
 
-    def search_ctx(self, cur, x, y, planes, lambda_mv, mvr=0, cost_type=0, is_padding=1, pad=PAD):
-        """An orc_ctx_t for the PU at (x, y) of `cur` against padded `planes`.
+    def search_ctx(self, cur, x, y, planes, lambda_mv, mvr=0, cost_type=0, is_padding=1, pad=PAD, org_bi=None):
+        """An orc_ctx_t for the PU at (x, y) of `cur` against padded `planes`
+        (with org_bi, an h x w bi-prediction target, a bi = 1 search of it).
         Returns (ctx, keepalive)."""
         h, w = cur.shape
         stride = w + 2 * pad
         ptrs = (C.c_void_p * 16)(*[planes[i].ctypes.data + (pad * stride + pad) * planes.itemsize for i in range(16)])
         off, tab = self.mvbits(w, h)
-        c = OrcCtx(w, h, self.bd, cur.ctypes.data + (y * w + x) * cur.itemsize, w, C.cast(ptrs, C.c_void_p).value,
-                   stride, tab.ctypes.data + off, off, lambda_mv, is_padding, mvr, cost_type)
-        return c, (ptrs, tab, cur, planes)
+        if org_bi is None:
+            org, i_org = cur.ctypes.data + (y * w + x) * cur.itemsize, w
+        else:
+            org_bi = np.ascontiguousarray(org_bi, self.dt)
+            org, i_org = org_bi.ctypes.data, org_bi.shape[1]
+        c = OrcCtx(w, h, self.bd, org, i_org, C.cast(ptrs, C.c_void_p).value,
+                   stride, tab.ctypes.data + off, off, lambda_mv, is_padding, mvr, cost_type, int(org_bi is not None))
+        return c, (ptrs, tab, cur, planes, org_bi)
 
     def me_search_tz(self, cur, planes, x, y, w, h, refi, num_refp, cur_poc, ref_poc, gop, msr, mvr, cost_type,
-                     adaptive, lambda_mv, mvp, is_padding=1):
-        c, keep = self.search_ctx(cur, x, y, planes, lambda_mv, mvr, cost_type, is_padding)
-        t = OrcTz(gop, msr, adaptive, cur_poc, ref_poc, num_refp)
+                     adaptive, lambda_mv, mvp, is_padding=1, org_bi=None, mv_start=(0, 0), bi_bits=0):
+        """me_search_tz (me.c:529-568); with org_bi the bi = 1 search from
+        mv_start (quarter pel) with the other list's mot_bits + 1 = bi_bits."""
+        c, keep = self.search_ctx(cur, x, y, planes, lambda_mv, mvr, cost_type, is_padding, org_bi=org_bi)
+        t = OrcTz(gop, msr, adaptive, cur_poc, ref_poc, num_refp, (C.c_int16 * 2)(0, 0), bi_bits)
         mvp_a = np.array(mvp, np.int16)
-        mv = np.zeros(2, np.int16)
+        mv = np.array(mv_start, np.int16)
         mvi = np.zeros(2, np.int16)
         c64 = C.c_uint64()
         r = self.lib.orc_me_search_tz(C.byref(c), C.byref(t), x, y, w, h, refi, ptr(mvp_a), ptr(mv), ptr(mvi),
@@ -123,6 +133,46 @@ class Oracle(_Lib):
         mv = np.array([imv[0] * 4, imv[1] * 4], np.int16)
         r = self.lib.orc_subpel(C.byref(c), ptr(mvp_a), x, y, w, h, ptr(mv))
         return (int(mv[0]), int(mv[1])), r
+
+    def analyze_bi(self, cur, planes, n0, x, y, w, h, mvp, mv_scale, refi, mv, lidx_ref, mot_bits, mvr, cost_type,
+                   lambda_mv):
+        """The search part of analyze_bi (inter.c:972-1033) for the PU at
+        (x, y): planes per reference (list 0 first, n0 of them); mvp and
+        mv_scale [nref][2] quarter pel (mv_scale updated in place as
+        me_search_tz updates pi->mv_scale); refi [2] (index in each list),
+        mv [2][2], mot_bits [2] updated in place as cur_info->refi / mv and
+        pi->mot_bits.  Returns best_mecost (u64)."""
+        pic_h, pic_w = cur.shape
+        nref = len(planes)
+        nl, base = [n0, nref - n0], [0, n0]
+        top = (1 << self.bd) - 1
+        off, tab = self.mvbits(pic_w, pic_h)
+        org = cur[y:y + h, x:x + w].astype(np.int32)
+        best = (1 << 64) - 1
+        fixed = lidx_ref                       # the list whose prediction makes the target
+        for _ in range(2):
+            # com_mc_cu of the fixed list alone, then create_bi_org (inter.c:998-999)
+            pred, _, _ = self.mc_pu(pic_w, pic_h, [planes[base[fixed] + refi[fixed]]], None, x, y, w, h,
+                                    [tuple(mv[fixed])])
+            org_bi = np.clip(2 * org - pred.astype(np.int32), 0, top).astype(self.dt)
+            s = 1 - fixed                      # SWAP (inter.c:1001-1002)
+            changed = False
+            for rc in range(nl[s]):
+                r = base[s] + rc
+                m, c = self.me_search_tz(cur, planes[r], x, y, w, h, rc, nl[s], 0, 0, 16, 64, mvr, cost_type, 0,
+                                         lambda_mv, tuple(mvp[r]), org_bi=org_bi, mv_start=tuple(mv_scale[r]),
+                                         bi_bits=int(mot_bits[fixed]) + 1)
+                mv_scale[r] = m[:2]
+                if c < best:
+                    best, changed = c, True
+                    refi[s] = rc
+                    mv[s] = m[:2]
+                    mot_bits[s] = (self.lib.orc_refi_bits(nl[s], rc) + int(tab[off + ((m[0] - mvp[r][0]) >> mvr)]) +
+                                   int(tab[off + ((m[1] - mvp[r][1]) >> mvr)]))
+            if not changed:
+                break
+            fixed = s
+        return best
 
     def loka_inter_cost(self, cur, planes_l0, planes_l1, cur_poc, poc_l0, poc_l1, gop, msr, is_padding, pad=PAD):
         """The inter part of loka_estimate_coding_cost (lookahead.c:82-208)
@@ -335,6 +385,10 @@ class RefLib(_Lib):
         L.refh_pic_free.argtypes = [C.c_void_p]
         L.refh_luma_coef.restype = C.c_void_p
         L.refh_chroma_coef.restype = C.c_void_p
+        L.refh_me_search_tz_bi.restype = C.c_uint64
+        L.refh_me_search_tz_bi.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                           C.c_void_p, C.c_void_p]
         L.refh_loka_cost.restype = C.c_double
         L.refh_loka_cost.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_longlong, C.c_void_p, C.c_int, C.c_void_p,
                                      C.c_int, C.c_int, C.c_int, C.c_int]
@@ -437,6 +491,17 @@ class RefLib(_Lib):
         self.lib.refh_mc_cu(x, y, pic_w, pic_h, w, h, C.c_void_p(p0) if p0 else None, C.c_void_p(p1) if p1 else None,
                             ptr(m0), ptr(m1), 0 if chroma else 1, ptr(py), ptr(pu), ptr(pv))
         return py, (pu if chroma else None), (pv if chroma else None)
+
+    def me_search_tz_bi(self, pic, org_bi, x, y, refi, num_refp, lidx, mot_bits_other, mvr, cost_type, lambda_mv,
+                        mvp, mv):
+        """me_search_tz(bi = 1) of the compiled reference; returns (cost, mv)."""
+        org_bi = np.ascontiguousarray(org_bi, self.dt)
+        h, w = org_bi.shape
+        mvp_a = np.array(mvp, np.int16)
+        mv_a = np.array(mv, np.int16)
+        c = self.lib.refh_me_search_tz_bi(pic, ptr(org_bi), w, x, y, w, h, refi, num_refp, lidx, mot_bits_other,
+                                          mvr, cost_type, lambda_mv, ptr(mvp_a), ptr(mv_a))
+        return c, (int(mv_a[0]), int(mv_a[1]))
 
     def loka_cost(self, cur, pics_l0, pics_l1, cur_poc, gop, msr, is_padding):
         """loka_estimate_coding_cost (lookahead.c:82-290) of the compiled

@@ -239,6 +239,31 @@ RH_EXPORT u64 refh_me_search_tz(void *ref, const pel *org, int i_org, int x, int
     return c;
 }
 
+/* me_search_tz with bi = 1 as analyze_bi calls it (inter.c:1006-1011): org
+ * is the bi-prediction target (create_bi_org, inter.c:55-68) with stride
+ * i_org, the search starts at mv (quarter pel, updated in place) for list
+ * lidx, and the other list's mot_bits enter the final cost. */
+RH_EXPORT u64 refh_me_search_tz_bi(void *ref, const pel *org, int i_org, int x, int y, int w, int h,
+                                   int refi, int num_refp, int lidx, int mot_bits_other, int mvr, int cost_type,
+                                   u32 lambda_mv, const s16 *mvp, s16 *mv)
+{
+    refh_pic_t *r = (refh_pic_t *)ref;
+    refh_search_t s;
+    search_setup(&s, r->w, r->h, lambda_mv, cost_type, 1);
+    inter_search_t *pi = &s.pi;
+    pi->ref_pic = r->pic;
+    pi->num_refp = (u8)num_refp;
+    pi->curr_mvr = (u8)mvr;
+    pi->org = (pel *)org;
+    pi->i_org = i_org;
+    pi->adaptive_raster_range = 0;
+    pi->mot_bits[1 - lidx] = mot_bits_other;
+    s16 mvp_l[2] = {mvp[0], mvp[1]};
+    u64 c = me_search_tz(pi, x, y, w, h, r->w, r->h, (s8)refi, lidx, mvp_l, mv, 1);
+    search_free(&s);
+    return c;
+}
+
 /* me_sub_pel_search (me.c:366-500) from a given integer MV (quarter-pel
  * units in mv, refined on return), bi = 0, caches cleared. */
 RH_EXPORT u64 refh_subpel(void *ref, const pel *org, int i_org, int x, int y, int w, int h,

@@ -112,6 +112,11 @@ class ServerStats(C.Structure):
                 ("admission_waits", C.c_uint64), ("max_launch_us", C.c_uint64), ("max_batch_us", C.c_uint64)]
 
 
+# me_bi_pu_t
+BI_PU = np.dtype([("refi", np.int8, 2), ("lidx_ref", np.int8), ("active", np.int8), ("mv", np.int16, (2, 2)),
+                  ("mot_bits", np.int32, 2), ("cost", np.uint32)], align=True)
+
+
 class MeError(RuntimeError):
     pass
 
@@ -160,6 +165,8 @@ class MeHip:
         L.me_plan_mc_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_run_tz.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.me_plan_lookahead.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.me_plan_run_bi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
         L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
@@ -315,6 +322,16 @@ class MeHip:
         mvi = np.ascontiguousarray(mvi_prev, np.int16) if mvi_prev is not None else None
         self.check(self.lib.me_plan_run_tz(plan, arr, _p(mvp), _p(mvi) if mvi is not None else None, C.byref(tz),
                                            stream), "me_plan_run_tz")
+
+    def plan_run_bi(self, plan, refs, mvp, mv_scale, pus, tz, stream=None):
+        """me_plan_run_bi: mvp, mv_scale int16 [ctu][pu][ref][2] (mv_scale
+        updated in place), pus a BI_PU array [ctu][pu] (updated in place)."""
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        mvp = np.ascontiguousarray(mvp, np.int16)
+        assert mv_scale.dtype == np.int16 and mv_scale.flags.c_contiguous
+        assert pus.dtype == BI_PU and pus.flags.c_contiguous and BI_PU.itemsize == 24
+        self.check(self.lib.me_plan_run_bi(plan, arr, _p(mvp), _p(mv_scale), _p(pus), C.byref(tz), stream),
+                   "me_plan_run_bi")
 
     def plan_lookahead(self, plan, refs, tz, is_padding, pic_w, pic_h, stream=None):
         """me_plan_lookahead: the lookahead's inter cost of each whole 16x16

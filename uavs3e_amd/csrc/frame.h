@@ -213,9 +213,21 @@ void me_mc_sizes(const MeSearch *s, size_t *luma_per_ctu, size_t *chroma_per_ctu
 // arrays; results go to res [job][npu][nref].  only_shape >= 0 searches that
 // shape alone; pad_win is inter_search_t.is_padding; lambda 0 takes the
 // search's own.
+struct BiState;
+struct BiBufs {
+    me_bi_pu_t *pus;         // [job][npu]
+    BiState *state;          // [job][npu]
+    s16 *mv_scale;           // [job][npu][nref][2]
+    pel *orgbi;              // [job][selected shape][16384]
+};
 int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mvi_prev,
                   const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st, int only_shape = -1, int pad_win = 1,
-                  u32 lambda = 0);
+                  u32 lambda = 0, const BiBufs *bi = nullptr);
+// analyze_bi's search (two rounds of target, search and choice) for the jobs
+// of js; bi.pus holds the inputs and receives the results.
+int me_bi_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const me_tz_cfg_t *tz,
+                  me_pu_result_t *res, const BiBufs &bi, hipStream_t st);
+size_t me_bi_state_bytes(size_t n);
 // The lookahead's inter cost of every whole 16x16 unit of the picture, in
 // raster order, from the 16x16 results of me_tz_enqueue (device out).
 int me_loka_enqueue(const MeSearch *s, const PicDesc *pic, const me_pu_result_t *res, int pic_w, int pic_h, u32 *out,

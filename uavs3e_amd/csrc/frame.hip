@@ -284,6 +284,9 @@ struct me_plan_s {
     me_pu_result_t *d_res;
     s8 *d_best;
     u32 *d_loka;          // lookahead unit costs (allocated on first use)
+    me_bi_pu_t *d_bipu;   // analyze_bi per-PU inputs / results (allocated on first use)
+    void *d_bist;         // analyze_bi per-PU state
+    pel *d_orgbi;         // analyze_bi targets, [ctu][selected shape][16384]
     hipStream_t own;
     hipStream_t side;     // rebuilds references while the others are searched
     hipEvent_t fork, join;
@@ -571,6 +574,66 @@ ME_EXPORT int me_plan_run_tz(me_plan_t p, me_ref_t const *refs, const int16_t *m
     return 0;
 }
 
+// analyze_bi's search (inter.c:972-1033) for every PU with both lists.
+ME_EXPORT int me_plan_run_bi(me_plan_t p, me_ref_t const *refs, const int16_t *mvp, int16_t *mv_scale,
+                             me_bi_pu_t *pus, const me_tz_cfg_t *tz, void *stream)
+{
+    if (!p || !refs || !mvp || !mv_scale || !pus || !tz || tz->mvr < 0 || tz->mvr > 4 || tz->cost_type < 0 ||
+        tz->cost_type > 1) {
+        me_set_error("me_plan_run_bi: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (p->s.nref_l1 < 1 || p->s.nref_l1 >= p->s.nref || p->cfg.pic_w % 4) {
+        me_set_error("me_plan_run_bi: the plan needs references in both lists and a width that is a multiple of 4",
+                     hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    const int n0 = p->s.nref - p->s.nref_l1;
+    const size_t npus = (size_t)p->n_ctu * p->s.npu;
+    for (size_t k = 0; k < npus; k++) {
+        const me_bi_pu_t &q = pus[k];
+        if (q.active && (q.lidx_ref < 0 || q.lidx_ref > 1 || q.refi[0] < 0 || q.refi[0] >= n0 || q.refi[1] < 0 ||
+                         q.refi[1] >= p->s.nref_l1)) {
+            me_set_error("me_plan_run_bi: PU reference index or list out of range", hipErrorInvalidValue, __FILE__,
+                         __LINE__);
+            return -1;
+        }
+    }
+    for (int r = 0; r < p->s.nref; r++)
+        if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h) {
+            me_set_error("me_plan_run_bi: reference size differs from the plan", hipErrorInvalidValue, __FILE__,
+                         __LINE__);
+            return -1;
+        }
+    hipStream_t st = stream ? (hipStream_t)stream : (p->graph_stream ? p->graph_stream : p->own);
+    const size_t n = npus * p->s.nref * 2;
+    int nsel = 0;
+    for (int k = 0; k < ME_NSHAPES; k++) nsel += p->s.base[k] >= 0;
+    if (!p->d_mvp) ME_CHECK(hipMalloc((void **)&p->d_mvp, 2 * n * sizeof(s16)));
+    if (!p->d_bipu) ME_CHECK(hipMalloc((void **)&p->d_bipu, npus * sizeof(me_bi_pu_t)));
+    if (!p->d_bist) ME_CHECK(hipMalloc(&p->d_bist, me_bi_state_bytes(npus)));
+    // +64 B: com_had reads tile rows as dwords, up to 3 bytes past a row's end
+    if (!p->d_orgbi) ME_CHECK(hipMalloc((void **)&p->d_orgbi, (size_t)p->n_ctu * nsel * 16384 * sizeof(pel) + 64));
+    if (plan_set_pic(p, refs, st)) return -1;
+    ME_CHECK(hipMemcpyAsync(p->d_mvp, mvp, n * sizeof(s16), hipMemcpyHostToDevice, st));
+    ME_CHECK(hipMemcpyAsync(p->d_mvp + n, mv_scale, n * sizeof(s16), hipMemcpyHostToDevice, st));
+    ME_CHECK(hipMemcpyAsync(p->d_bipu, pus, npus * sizeof(me_bi_pu_t), hipMemcpyHostToDevice, st));
+    JobSet js;
+    js.pics = p->d_pic;
+    js.jobs = nullptr;
+    js.n_jobs = p->n_ctu;
+    BiBufs bb;
+    bb.pus = p->d_bipu;
+    bb.state = (BiState *)p->d_bist;
+    bb.mv_scale = p->d_mvp + n;
+    bb.orgbi = p->d_orgbi;
+    if (me_bi_enqueue(&p->s, js, p->d_mvp, tz, p->d_res, bb, st)) return -1;
+    ME_CHECK(hipMemcpyAsync(pus, p->d_bipu, npus * sizeof(me_bi_pu_t), hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipMemcpyAsync(mv_scale, p->d_mvp + n, n * sizeof(s16), hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
 // loka_estimate_coding_cost (lookahead.c:82-208), the inter part: every
 // whole 16x16 unit searched by me_search_tz from MVP 0 at integer precision
 // with the lookahead's lambda (lookahead.c:85-110), then its com_had costs.
@@ -682,6 +745,9 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     if (p->join) (void)hipEventDestroy(p->join);
     me_search_free(&p->s);
     (void)hipFree(p->d_loka);
+    (void)hipFree(p->d_bipu);
+    (void)hipFree(p->d_bist);
+    (void)hipFree(p->d_orgbi);
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);
     (void)hipFree(p->d_pred);

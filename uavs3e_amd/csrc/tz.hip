@@ -21,6 +21,7 @@
 #include "had.h"
 
 #include <string.h>
+#include <algorithm>
 
 // compiled once per group of com_had tile types, like subpel.hip (ME_TZ_PART
 // 0: 4x4, 8x8, 8x4, 4x8 tiles and the launcher; 1: 16x8; 2: 8x16)
@@ -42,6 +43,22 @@ struct TzArgs {
     int gop, msr, adaptive, mvr, cost_type;
     int pad_win;             // inter_search_t.is_padding: windows may leave the picture
     long long cur_poc, ref_poc[ME_MAX_REFS];
+    // bi = 1 (analyze_bi): per PU, the list searched is the one not fixed in
+    // its BiState, the target is its bi-prediction original and the search
+    // starts from (and writes back) mv_scale
+    int bi;
+    const pel *orgbi;        // [job][selected shape][16384]
+    int orgbi_off, nsel;     // this shape's rank * 16384; selected shapes
+    s16 *mv_scale;           // [job][npu][nref][2], quarter pel
+    const struct BiState *bst;   // [job][npu]
+    const me_bi_pu_t *bpu;       // [job][npu]
+};
+
+// analyze_bi's running state of one PU (inter.c:986-1033)
+struct BiState {
+    u64 best;                // best_mecost
+    int fixed;               // the list whose prediction makes the target
+    int done;                // no change in the last round, or not a bi PU
 };
 
 // Diamond patterns of search_diamond (me.c:330-333): 8 points of radius 2,
@@ -109,6 +126,7 @@ struct TzUnit {
     const PicDesc *P;
     int r, x, y, w, h;
     const pel *org;
+    int so;                  // org stride
     int mvpx, mvpy;
     int lo[2], hi[2];
     u64 best;
@@ -129,7 +147,7 @@ struct TzUnit {
     __device__ __forceinline__ bool try_ipel(int mx, int my)
     {
         const pel *pred = plane_ptr(*P, r, 0, x + mx, y + my);
-        const u32 sad = wave_sad(org, P->pic_w, pred, P->ref_stride, w, h, a->lpu);
+        const u32 sad = wave_sad(org, so, pred, P->ref_stride, w, h, a->lpu);
         const u64 cost = ((u64)sad << shift) + (u64)(ipel_bits(mx, my) * a->lambda);
         if (cost < best) {
             best = cost;
@@ -141,14 +159,20 @@ struct TzUnit {
     }
 };
 
-// search_diamond (me.c:199-355) for bi = 0; returns the step of the last
-// improvement (0: centre, 2: core square, else the diamond step).
+// search_diamond (me.c:199-355); returns the step of the last improvement
+// (0: centre, 2: core square, else the diamond step).  With bi the core
+// square spans the distance to the rounded MVP (2..5, 6 above integer AMVR)
+// and the multilayer diamond is skipped (me.c:221-231, 317-319).
 __device__ __noinline__ int tz_diamond(TzUnit &u, int max_step)
 {
     const int mvr = u.a->mvr;
     int best_step = 0;
     int core = 2, cstep = 1;
     const int cx = u.mv[0], cy = u.mv[1];
+    if (u.a->bi) {
+        const int dx = abs(((u.mvpx + 2) >> 2) - cx), dy = abs(((u.mvpy + 2) >> 2) - cy);
+        core = me_clip3(2, mvr > 2 ? 6 : 5, (dx > dy ? dx : dy) >> (mvr > 2 ? mvr - 2 : 0));
+    }
     if (mvr > 2) { core <<= mvr - 2; cstep <<= mvr - 2; }
     int x0 = cx - core, y0 = cy - core, x1 = cx + core, y1 = cy + core;
     while (x0 <= u.lo[0]) x0 += cstep;
@@ -160,6 +184,7 @@ __device__ __noinline__ int tz_diamond(TzUnit &u, int max_step)
         for (int my = y0; my <= y1; my += cstep)
             for (int mx = x0; mx <= x1; mx += cstep) u.try_ipel(mx, my);
     if (u.mv[0] != cx || u.mv[1] != cy) best_step = 2;
+    if (u.a->bi) return best_step;
     // multilayer diamonds: 4 points at step 4, 8 at step 8, 16 beyond
     // (me.c:330-333), scaled by the AMVR precision
     const int scale = mvr - 2 > 0 ? mvr - 2 : 0;
@@ -222,16 +247,16 @@ __device__ __noinline__ void tz_raster(TzUnit &u, int refi, int mvi_prev_x, int 
     }
 }
 
-// me_sub_pel_search (me.c:366-500) for bi = 0; u.mv holds the quarter-pel
+// me_sub_pel_search (me.c:366-500); u.mv holds the quarter-pel
 // integer result on entry and the refined MV on return.
 template <int TT>
 __device__ __noinline__ u64 tz_subpel(TzUnit &u)
 {
     const TzArgs &a = *u.a;
-    const int shift = 16 - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
+    const int shift = 16 - a.bi - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
     auto dist = [&](const pel *p) -> u32 {
-        return a.cost_type == 0 ? wave_had<TT>(a, u.org, u.P->pic_w, p, u.P->ref_stride)
-                                : wave_sad(u.org, u.P->pic_w, p, u.P->ref_stride, u.w, u.h, a.lpu);
+        return a.cost_type == 0 ? wave_had<TT>(a, u.org, u.so, p, u.P->ref_stride)
+                                : wave_sad(u.org, u.so, p, u.P->ref_stride, u.w, u.h, a.lpu);
     };
     const int cx = u.mv[0], cy = u.mv[1];
     u32 d = dist(plane_ptr(*u.P, u.r, 0, u.x + (cx >> 2), u.y + (cy >> 2)));
@@ -252,9 +277,9 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
         }
     }
     if (a.cost_type) {
-        d = wave_had<TT>(a, u.org, u.P->pic_w, qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)),
+        d = wave_had<TT>(a, u.org, u.so, qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)),
                      u.P->ref_stride);
-        best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << 16);
+        best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << (16 - a.bi));
     }
     return best;
 }
@@ -268,7 +293,16 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     const int li = threadIdx.x & (a.lpu - 1), wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = (blockIdx.x * 4 + wv) * upw + ((threadIdx.x & 63) >> lg);
     if (i >= a.per_ctu) return;
-    const int job = blockIdx.y, r = blockIdx.z;
+    const int job = blockIdx.y;
+    const int n0 = a.n0, n1 = a.nref - a.n0;
+    int r = blockIdx.z, blist = 0;
+    if (a.bi) {   // the searched list's reference blockIdx.z
+        const BiState &bs = a.bst[(size_t)job * a.npu + a.base + i];
+        if (bs.done) return;
+        blist = 1 - bs.fixed;
+        if (r >= (blist ? n1 : n0)) return;
+        r += blist ? n0 : 0;
+    }
     int ctu;
     const PicDesc &P = *job_decode(a.js, job, ctu);
     const int cols = 128 / a.pw;
@@ -313,14 +347,21 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
         if (li == 0) a.res[slot] = out;
         return;
     }
-    u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
-    u.shift = 16 - (BIT_DEPTH - 8);
+    if (a.bi) {
+        u.org = a.orgbi + ((size_t)job * a.nsel * 16384 + a.orgbi_off) + (size_t)i * a.pw * a.ph;
+        u.so = a.pw;
+    } else {
+        u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
+        u.so = P.pic_w;
+    }
+    u.shift = 16 - a.bi - (BIT_DEPTH - 8);
     u.best = ~0ull;
-    u.mv[0] = me_clip3(u.lo[0], u.hi[0], (u.mvpx + 2) >> 2);
-    u.mv[1] = me_clip3(u.lo[1], u.hi[1], (u.mvpy + 2) >> 2);
+    const int sx = a.bi ? a.mv_scale[slot * 2] : u.mvpx, sy = a.bi ? a.mv_scale[slot * 2 + 1] : u.mvpy;
+    u.mv[0] = me_clip3(u.lo[0], u.hi[0], (sx + 2) >> 2);
+    u.mv[1] = me_clip3(u.lo[1], u.hi[1], (sy + 2) >> 2);
     const int refi = r < a.n0 ? r : r - a.n0, nlist = r < a.n0 ? a.n0 : a.nref - a.n0;
     const int bs = tz_diamond(u, 32);
-    if (abs(u.mvpx - (u.mv[0] << 2)) > 2 || abs(u.mvpy - (u.mv[1] << 2)) > 2) {
+    if (!a.bi && (abs(u.mvpx - (u.mv[0] << 2)) > 2 || abs(u.mvpy - (u.mv[1] << 2)) > 2)) {
         if (bs > 5) {
             const int pvx = a.mvi_prev ? a.mvi_prev[slot * 2] : 0, pvy = a.mvi_prev ? a.mvi_prev[slot * 2 + 1] : 0;
             tz_raster(u, refi, pvx, pvy);
@@ -338,10 +379,18 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     out.mv_sub[1] = (s16)u.mv[1];
     out.cost_sub = best;
     const int rb = nlist <= 1 ? 0 : (refi < nlist - 1 ? refi + 1 : nlist - 1);   // tbl_refi_bits
-    const u64 cf = best + (u64)(a.lambda * (u32)(rb + mvr + (mvr < 4) + 2));
+    // bi: the fixed list's mot_bits + 1 (me.c:560-561)
+    const int last = a.bi ? a.bpu[(size_t)job * a.npu + a.base + i].mot_bits[1 - blist] + 1 : 2;
+    const u64 cf = best + (u64)(a.lambda * (u32)(rb + mvr + (mvr < 4) + last));
     out.cost_final = (u32)((cf + (1u << 15)) >> 16);
     out.valid = 1;
-    if (li == 0) a.res[slot] = out;
+    if (li == 0) {
+        a.res[slot] = out;
+        if (a.bi) {
+            a.mv_scale[slot * 2] = (s16)u.mv[0];
+            a.mv_scale[slot * 2 + 1] = (s16)u.mv[1];
+        }
+    }
 }
 
 typedef void (*tz_fn)(TzArgs);
@@ -472,9 +521,82 @@ int me_loka_enqueue(const MeSearch *s, const PicDesc *pic, const me_pu_result_t 
 }
 #endif
 
+// ---------------------------------------------------------------------------
+// analyze_bi (inter.c:972-1033): target, search of the other list, choice
+// ---------------------------------------------------------------------------
+
+#if ME_TZ_PART == 0
+size_t me_bi_state_bytes(size_t n) { return n * sizeof(BiState); }
+
+__global__ void k_bi_init(const me_bi_pu_t *pu, BiState *st, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    st[k].best = ~0ull;
+    st[k].fixed = pu[k].lidx_ref;
+    st[k].done = pu[k].active ? 0 : 1;
+}
+
+// create_bi_org (inter.c:55-68) of one shape's PUs from the fixed list's
+// prediction (com_mc_blk_luma at its quarter-pel MV: the plane [qy&3][qx&3]
+// at the clipped integer position, com_mc.c:41-86, 140-165).  Block (256):
+// one PU per block, 4 samples per thread and step.  grid (per_ctu, jobs).
+__global__ __launch_bounds__(256) void k_bi_org(TzArgs a)
+{
+    const int i = blockIdx.x, job = blockIdx.y;
+    const size_t pk = (size_t)job * a.npu + a.base + i;
+    if (a.bst[pk].done) return;
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    const int cols = 128 / a.pw;
+    const int x = (ctu % P.ctu_cols) * 128 + (i % cols) * a.pw, y = (ctu / P.ctu_cols) * 128 + (i / cols) * a.ph;
+    if (x + a.pw > P.pic_w || y + a.ph > P.pic_h) return;
+    const me_bi_pu_t &q = a.bpu[pk];
+    const int f = a.bst[pk].fixed, r = (f ? a.n0 : 0) + q.refi[f];
+    const pel *pred = qpel_ptr(P, r, (x << 2) + q.mv[f][0], (y << 2) + q.mv[f][1]);
+    const pel *org = P.cur + (size_t)y * P.pic_w + x;
+    pel *dst = (pel *)a.orgbi + ((size_t)job * a.nsel * 16384 + a.orgbi_off) + (size_t)i * a.pw * a.ph;
+    const int maxv = (1 << BIT_DEPTH) - 1;
+    for (int k = threadIdx.x; k < a.pw * a.ph; k += blockDim.x) {
+        const int yy = k / a.pw, xx = k - yy * a.pw;
+        const int v = 2 * (int)org[yy * P.pic_w + xx] - (int)pred[yy * P.ref_stride + xx];
+        dst[k] = (pel)me_clip3(0, maxv, v);
+    }
+}
+
+// The reference loop over the searched list (inter.c:1006-1021): a strictly
+// cheaper me_search_tz cost takes the reference, its MV and get_mv_bits.
+__global__ void k_bi_select(TzArgs a, me_bi_pu_t *pu, BiState *st, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || st[k].done) return;
+    const int s = 1 - st[k].fixed, nl = s ? a.nref - a.n0 : a.n0, r0 = s ? a.n0 : 0;
+    bool changed = false;
+    for (int rc = 0; rc < nl; rc++) {
+        const size_t slot = (size_t)k * a.nref + r0 + rc;
+        const me_pu_result_t &q = a.res[slot];
+        const u64 c = q.valid ? (u64)q.cost_final : ~0ull;
+        if (c < st[k].best) {
+            st[k].best = c;
+            changed = true;
+            const int mx = a.mv_scale[slot * 2], my = a.mv_scale[slot * 2 + 1];
+            const int px = a.mvp[slot * 2], py = a.mvp[slot * 2 + 1];
+            pu[k].refi[s] = (s8)rc;
+            pu[k].mv[s][0] = (s16)mx;
+            pu[k].mv[s][1] = (s16)my;
+            const int rb = nl <= 1 ? 0 : (rc < nl - 1 ? rc + 1 : nl - 1);
+            pu[k].mot_bits[s] = rb + a.tab[(mx - px) >> a.mvr] + a.tab[(my - py) >> a.mvr];
+        }
+    }
+    if (changed) st[k].fixed = s;
+    else st[k].done = 1;
+    pu[k].cost = st[k].best > 0xFFFFFFFFull ? 0xFFFFFFFFu : (u32)st[k].best;
+}
+#endif
+
 int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mvi_prev,
                   const me_tz_cfg_t *tz, me_pu_result_t *res, hipStream_t st, int only_shape, int pad_win,
-                  u32 lambda)
+                  u32 lambda, const BiBufs *bi)
 {
     TzArgs a;
     memset(&a, 0, sizeof(a));
@@ -496,8 +618,18 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
     a.cost_type = tz->cost_type;
     a.cur_poc = tz->cur_poc;
     for (int r = 0; r < ME_MAX_REFS; r++) a.ref_poc[r] = tz->ref_poc[r];
+    if (bi) {
+        a.bi = 1;
+        a.orgbi = bi->orgbi;
+        a.mv_scale = bi->mv_scale;
+        a.bst = bi->state;
+        a.bpu = bi->pus;
+        for (int k = 0; k < ME_NSHAPES; k++) a.nsel += s->base[k] >= 0;
+    }
+    int rank = 0;
     for (int k = 0; k < ME_NSHAPES; k++) {
         if (s->base[k] < 0 || (only_shape >= 0 && k != only_shape)) continue;
+        a.orgbi_off = 16384 * rank++;
         a.pw = ME_SHAPES_HOST[k][0];
         a.ph = ME_SHAPES_HOST[k][1];
         a.base = s->base[k];
@@ -508,9 +640,27 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
         const int upb = 4 * (64 / a.lpu);
         tz_fn fn = a.tt == 3 ? me_tz_kernel_16x8(a.tt) : (a.tt == 5 ? me_tz_kernel_8x16(a.tt) : me_tz_kernel_small(a.tt));
         if (!fn) { me_set_error("no search kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
-        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs, s->nref), dim3(256), 0, st, a);
+        if (bi) hipLaunchKernelGGL(k_bi_org, dim3(a.per_ctu, js.n_jobs), dim3(256), 0, st, a);
+        const int nz = bi ? std::max(a.n0, a.nref - a.n0) : s->nref;
+        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs, nz), dim3(256), 0, st, a);
         ME_CHECK(hipGetLastError());
     }
+    if (bi) {
+        const int n = js.n_jobs * s->npu;
+        hipLaunchKernelGGL(k_bi_select, dim3((n + 255) / 256), dim3(256), 0, st, a, bi->pus, bi->state, n);
+        ME_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+int me_bi_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const me_tz_cfg_t *tz,
+                  me_pu_result_t *res, const BiBufs &bi, hipStream_t st)
+{
+    const int n = js.n_jobs * s->npu;
+    hipLaunchKernelGGL(k_bi_init, dim3((n + 255) / 256), dim3(256), 0, st, bi.pus, bi.state, n);
+    ME_CHECK(hipGetLastError());
+    for (int round = 0; round < 2; round++)
+        if (me_tz_enqueue(s, js, mvp, nullptr, tz, res, st, -1, 1, 0, &bi)) return -1;
     return 0;
 }
 #endif
