@@ -195,6 +195,56 @@ def main():
                      "pus_searched": int((tres["valid"] == 1).sum())}
         print("[bench] me_search_tz: %.2f ms/frame" % tz_ms, file=sys.stderr, flush=True)
 
+    # ---- bi-prediction ME (analyze_bi's search) on a two-list frame ---------
+    bi_search = None
+    if not args.no_tz and bi:
+        import ctypes
+        n0 = work.num_refs - work.num_refs_l1
+        mvp = np.zeros((n_ctu, npu, work.num_refs, 2), np.int16)
+        tz = ua.TzCfg(16, work.R, 0, 0, 0, 8, (ctypes.c_int64 * 4)(*([7, 6, 9, 10][:work.num_refs] +
+                                                                        [0] * (4 - work.num_refs))))
+        hip.plan_run_tz(plan, dref, mvp, tz)     # the uni-prediction searches analyze_bi starts from
+        ures, ubest = hip.plan_results(plan, work.num_refs)
+        pus = np.zeros((n_ctu, npu), ua.BI_PU)
+        ok = (ubest[..., 0] >= 0) & (ubest[..., 1] >= 0)
+        pus["active"] = ok
+        idx = np.arange(n_ctu)[:, None], np.arange(npu)[None, :]
+
+        def mvbits(v):   # the MV-bits table of inter_search_create in closed form
+            a = np.abs(v.astype(np.int64))
+            return np.where(a == 0, 1, 2 * (np.floor(np.log2(a + 1)).astype(np.int64) + 1))
+
+        costs = []
+        for l in range(2):
+            r = np.where(ok, ubest[..., l], 0).astype(np.int64)
+            nl = n0 if l == 0 else work.num_refs_l1
+            pus["refi"][..., l] = np.where(ok, r - (n0 if l else 0), 0)
+            m = ures["mv_sub"][idx[0], idx[1], r]
+            pus["mv"][..., l, :] = m
+            ri = r - (n0 if l else 0)
+            rb = np.where(nl <= 1, 0, np.where(ri < nl - 1, ri + 1, nl - 1))
+            pus["mot_bits"][..., l] = rb + mvbits(m[..., 0]) + mvbits(m[..., 1])
+            costs.append(ures["cost_final"][idx[0], idx[1], r])
+        pus["lidx_ref"] = (costs[1] < costs[0]).astype(np.int8)
+        ms0 = np.ascontiguousarray(ures["mv_sub"])
+        pus0 = pus.copy()
+        ms = ms0.copy()
+        hip.plan_run_bi(plan, dref, mvp, ms, pus, tz)   # warm-up
+        t_bi = []
+        for _ in range(3):
+            ms, pus = ms0.copy(), pus0.copy()
+            t0 = time.perf_counter()
+            hip.plan_run_bi(plan, dref, mvp, ms, pus, tz)
+            t_bi.append((time.perf_counter() - t0) * 1e3)
+        b_ms = float(np.median(t_bi))
+        bi_search = {"ms_per_frame": b_ms, "pus": int(ok.sum()), "pus_per_s": int(ok.sum()) / (b_ms * 1e-3),
+                     "note": "me_plan_run_bi: analyze_bi's search (bi-prediction target, me_search_tz bi = 1 "
+                             "over the other list's references, choice; up to two rounds) for every PU with "
+                             "both lists, from the frame's uni-prediction me_search_tz results; includes the "
+                             "host<->device copies of the per-PU state (%.0f MB)" % (
+                                 (pus.nbytes + 2 * ms.nbytes) / 1e6)}
+        print("[bench] bi search: %.2f ms/frame (%d PUs)" % (b_ms, int(ok.sum())), file=sys.stderr, flush=True)
+
     # ---- lookahead cost estimator (inter part) -------------------------------
     lookahead = None
     if not args.no_tz and hip.lib.me_plan_num_pus(plan) > 0:
@@ -386,6 +436,7 @@ def main():
         "serving": serving,
         "search_tz": search_tz,
         "lookahead": lookahead,
+        "bi_search": bi_search,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
     }
