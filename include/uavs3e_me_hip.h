@@ -274,6 +274,27 @@ typedef struct me_bi_pu_t {
 int me_plan_run_bi(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, int16_t *mv_scale, me_bi_pu_t *pus,
                    const me_tz_cfg_t *tz, void *stream);
 
+/* Symmetric-MVD search of analyze_smvd (inter.c:1194-1255) with
+ * smvd_refine (inter.c:1083-1191) for every PU of the plan: list 0 and
+ * list 1 reference 0, list 1's MVD the negation of list 0's.  Candidates:
+ * both MVPs; list 0's uni MV mirrored; then smvd_refine's diamond (8 rounds)
+ * and cross (1 round) around list 0's MV, offsets scaled by the AMVR shift;
+ * each costs get_smvd_cost of the bi-average prediction (com_mc_cu) --
+ * com_had, or SAD >> (bd - 8) with cost_type 1, then com_had of the winner --
+ * plus MV_COST of the MVD bits.  pus[ctu*npu + pu]: active selects the PU;
+ * mv receives the two MVs (quarter pel) and cost the u32 mecost.  mvp and
+ * mv_scale as in me_plan_run_bi (reference 0 of each list is read).  tz:
+ * mvr and cost_type.  The RDO after the search (inter.c:1257-1275) is the
+ * caller's.  Blocking. */
+typedef struct me_smvd_pu_t {
+    int8_t active;
+    int8_t pad[3];
+    int16_t mv[2][2];
+    uint32_t cost;
+} me_smvd_pu_t;
+int me_plan_run_smvd(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_scale,
+                     me_smvd_pu_t *pus, const me_tz_cfg_t *tz, void *stream);
+
 /* The inter part of the lookahead's cost estimator, loka_estimate_coding_cost
  * (lookahead.c:82-208), for every whole 16x16 unit of the plan's current
  * picture: each reference (the plan's list-0 references, then list 1) is

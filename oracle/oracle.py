@@ -174,6 +174,67 @@ class Oracle(_Lib):
             fixed = s
         return best
 
+    def analyze_smvd(self, cur, planes, n0, x, y, w, h, mvp, mv_scale, mvr, cost_type, lambda_mv):
+        """The search part of analyze_smvd (inter.c:1194-1255) with
+        smvd_refine (inter.c:1083-1191) for the PU at (x, y): planes per
+        reference (list 0 first, n0 of them; reference 0 of each list is
+        used), mvp / mv_scale [nref][2] quarter pel.  Returns (mv [2][2],
+        mecost)."""
+        pic_h, pic_w = cur.shape
+        off, tab = self.mvbits(pic_w, pic_h)
+        org = np.ascontiguousarray(cur[y:y + h, x:x + w])
+        p0, p1 = [int(v) for v in mvp[0]], [int(v) for v in mvp[n0]]
+        M = 0xFFFFFFFF
+
+        def bits(v):   # tab_mvbits[v >> mvr] (GET_MVBITS_X, util.h:41)
+            return int(tab[off + max(-off, min(off, v >> mvr))])
+
+        def mv_cost(b):   # MV_COST (inter.c:41), u32 arithmetic
+            return (((lambda_mv * b) & M) + 32768 & M) >> 16
+
+        def s16(v):
+            return (v + 32768) % 65536 - 32768
+
+        def cost(m0, m1, had):   # get_smvd_cost of com_mc_cu's bi-average (inter.c:1071-1081)
+            pred, _, _ = self.mc_pu(pic_w, pic_h, [planes[0], planes[n0]], None, x, y, w, h, [tuple(m0), tuple(m1)])
+            if had:
+                return self.had(w, h, org, w, pred, w)
+            return self.sad(w, h, org, w, pred, w) >> (self.bd - 8)
+
+        had = cost_type == 0
+        mv = [list(p0), list(p1)]
+        best = (cost(mv[0], mv[1], had) + mv_cost(bits(0) * 2 + mvr + (mvr < 4))) & M
+        m0 = [int(v) for v in mv_scale[0]]
+        d = [m0[0] - p0[0], m0[1] - p0[1]]
+        m1 = [s16(p1[0] - d[0]), s16(p1[1] - d[1])]
+        cb = (cost(m0, m1, had) + mv_cost(bits(d[0]) + bits(d[1]) + mvr + (mvr < 4))) & M
+        if cb < best:
+            best, mv = cb, [m0, m1]
+        cross = [(0, 1), (1, 0), (0, -1), (-1, 0)]
+        diamond = [(0, 2), (1, 1), (2, 0), (1, -1), (0, -2), (-1, -1), (-2, 0), (-1, 1)]
+        for pattern, rounds in ((2, 8), (0, 1)):
+            offs, de, rnd, mask = (cross, 3, 4, 3) if pattern == 0 else (diamond, 7, 8, 7)
+            ds = 0
+            for _ in range(rounds):
+                c = list(mv[0])
+                best_dir = -1
+                for idx in range(ds, de + 1):
+                    dr = (idx + rnd) & mask
+                    c0 = [s16(c[0] + s16(offs[dr][0] << mvr)), s16(c[1] + s16(offs[dr][1] << mvr))]
+                    dd = [c0[0] - p0[0], c0[1] - p0[1]]
+                    c1 = [s16(p1[0] - dd[0]), s16(p1[1] - dd[1])]
+                    t = (cost(c0, c1, had) + mv_cost(mvr + (mvr < 4) + bits(dd[0]) + bits(dd[1]))) & M
+                    if t < best:
+                        best, mv, best_dir = t, [c0, c1], dr
+                if best_dir < 0:
+                    break
+                step = 2 - (best_dir & 1) if pattern == 2 else 1
+                ds, de = best_dir - step, best_dir + step
+        if not had:
+            dd = [mv[0][0] - p0[0], mv[0][1] - p0[1]]
+            best = (cost(mv[0], mv[1], True) + mv_cost(bits(dd[0]) + bits(dd[1]) + mvr + (mvr < 4))) & M
+        return mv, best
+
     def loka_inter_cost(self, cur, planes_l0, planes_l1, cur_poc, poc_l0, poc_l1, gop, msr, is_padding, pad=PAD):
         """The inter part of loka_estimate_coding_cost (lookahead.c:82-208)
         for every whole 16x16 unit of `cur`, uint32 [h // 16][w // 16]:

@@ -117,6 +117,11 @@ BI_PU = np.dtype([("refi", np.int8, 2), ("lidx_ref", np.int8), ("active", np.int
                   ("mot_bits", np.int32, 2), ("cost", np.uint32)], align=True)
 
 
+# me_smvd_pu_t
+SMVD_PU = np.dtype([("active", np.int8), ("pad", np.int8, 3), ("mv", np.int16, (2, 2)), ("cost", np.uint32)],
+                   align=True)
+
+
 class MeError(RuntimeError):
     pass
 
@@ -167,6 +172,8 @@ class MeHip:
         L.me_plan_lookahead.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.me_plan_run_bi.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_void_p]
+        L.me_plan_run_smvd.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]
         L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
@@ -332,6 +339,16 @@ class MeHip:
         assert pus.dtype == BI_PU and pus.flags.c_contiguous and BI_PU.itemsize == 24
         self.check(self.lib.me_plan_run_bi(plan, arr, _p(mvp), _p(mv_scale), _p(pus), C.byref(tz), stream),
                    "me_plan_run_bi")
+
+    def plan_run_smvd(self, plan, refs, mvp, mv_scale, pus, tz, stream=None):
+        """me_plan_run_smvd: mvp, mv_scale int16 [ctu][pu][ref][2]; pus an
+        SMVD_PU array [ctu][pu] (active in, mv / cost out)."""
+        arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
+        mvp = np.ascontiguousarray(mvp, np.int16)
+        mv_scale = np.ascontiguousarray(mv_scale, np.int16)
+        assert pus.dtype == SMVD_PU and pus.flags.c_contiguous and SMVD_PU.itemsize == 16
+        self.check(self.lib.me_plan_run_smvd(plan, arr, _p(mvp), _p(mv_scale), _p(pus), C.byref(tz), stream),
+                   "me_plan_run_smvd")
 
     def plan_lookahead(self, plan, refs, tz, is_padding, pic_w, pic_h, stream=None):
         """me_plan_lookahead: the lookahead's inter cost of each whole 16x16

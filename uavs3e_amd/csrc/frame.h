@@ -228,6 +228,9 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
 int me_bi_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const me_tz_cfg_t *tz,
                   me_pu_result_t *res, const BiBufs &bi, hipStream_t st);
 size_t me_bi_state_bytes(size_t n);
+// analyze_smvd's search for the jobs of js; out.active selects PUs.
+int me_smvd_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mv_scale,
+                    const me_tz_cfg_t *tz, me_smvd_pu_t *out, hipStream_t st);
 // The lookahead's inter cost of every whole 16x16 unit of the picture, in
 // raster order, from the 16x16 results of me_tz_enqueue (device out).
 int me_loka_enqueue(const MeSearch *s, const PicDesc *pic, const me_pu_result_t *res, int pic_w, int pic_h, u32 *out,

@@ -287,6 +287,7 @@ struct me_plan_s {
     me_bi_pu_t *d_bipu;   // analyze_bi per-PU inputs / results (allocated on first use)
     void *d_bist;         // analyze_bi per-PU state
     pel *d_orgbi;         // analyze_bi targets, [ctu][selected shape][16384]
+    me_smvd_pu_t *d_smvd; // analyze_smvd per-PU results (allocated on first use)
     hipStream_t own;
     hipStream_t side;     // rebuilds references while the others are searched
     hipEvent_t fork, join;
@@ -634,6 +635,45 @@ ME_EXPORT int me_plan_run_bi(me_plan_t p, me_ref_t const *refs, const int16_t *m
     return 0;
 }
 
+// analyze_smvd's search (inter.c:1083-1255) for every PU.
+ME_EXPORT int me_plan_run_smvd(me_plan_t p, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_scale,
+                               me_smvd_pu_t *pus, const me_tz_cfg_t *tz, void *stream)
+{
+    if (!p || !refs || !mvp || !mv_scale || !pus || !tz || tz->mvr < 0 || tz->mvr > 4 || tz->cost_type < 0 ||
+        tz->cost_type > 1) {
+        me_set_error("me_plan_run_smvd: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (p->s.nref_l1 < 1 || p->s.nref_l1 >= p->s.nref) {
+        me_set_error("me_plan_run_smvd: the plan needs references in both lists", hipErrorInvalidValue, __FILE__,
+                     __LINE__);
+        return -1;
+    }
+    for (int r = 0; r < p->s.nref; r++)
+        if (!refs[r] || refs[r]->w != p->cfg.pic_w || refs[r]->h != p->cfg.pic_h) {
+            me_set_error("me_plan_run_smvd: reference size differs from the plan", hipErrorInvalidValue, __FILE__,
+                         __LINE__);
+            return -1;
+        }
+    hipStream_t st = stream ? (hipStream_t)stream : (p->graph_stream ? p->graph_stream : p->own);
+    const size_t npus = (size_t)p->n_ctu * p->s.npu;
+    const size_t n = npus * p->s.nref * 2;
+    if (!p->d_mvp) ME_CHECK(hipMalloc((void **)&p->d_mvp, 2 * n * sizeof(s16)));
+    if (!p->d_smvd) ME_CHECK(hipMalloc((void **)&p->d_smvd, npus * sizeof(me_smvd_pu_t)));
+    if (plan_set_pic(p, refs, st)) return -1;
+    ME_CHECK(hipMemcpyAsync(p->d_mvp, mvp, n * sizeof(s16), hipMemcpyHostToDevice, st));
+    ME_CHECK(hipMemcpyAsync(p->d_mvp + n, mv_scale, n * sizeof(s16), hipMemcpyHostToDevice, st));
+    ME_CHECK(hipMemcpyAsync(p->d_smvd, pus, npus * sizeof(me_smvd_pu_t), hipMemcpyHostToDevice, st));
+    JobSet js;
+    js.pics = p->d_pic;
+    js.jobs = nullptr;
+    js.n_jobs = p->n_ctu;
+    if (me_smvd_enqueue(&p->s, js, p->d_mvp, p->d_mvp + n, tz, p->d_smvd, st)) return -1;
+    ME_CHECK(hipMemcpyAsync(pus, p->d_smvd, npus * sizeof(me_smvd_pu_t), hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
 // loka_estimate_coding_cost (lookahead.c:82-208), the inter part: every
 // whole 16x16 unit searched by me_search_tz from MVP 0 at integer precision
 // with the lookahead's lambda (lookahead.c:85-110), then its com_had costs.
@@ -748,6 +788,7 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     (void)hipFree(p->d_bipu);
     (void)hipFree(p->d_bist);
     (void)hipFree(p->d_orgbi);
+    (void)hipFree(p->d_smvd);
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);
     (void)hipFree(p->d_pred);

@@ -393,6 +393,192 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// analyze_smvd (inter.c:1194-1243) with smvd_refine (inter.c:1083-1191):
+// symmetric MVD search of list 0 / list 1 reference 0 for every PU.
+// ---------------------------------------------------------------------------
+
+// smvd_refine's patterns (inter.c:1097-1098): cross, then diamond
+__constant__ static const s8 SMVD_CROSS[4][2] = {{0, 1}, {1, 0}, {0, -1}, {-1, 0}};
+__constant__ static const s8 SMVD_DIAMOND[8][2] = {{0, 2}, {1, 1}, {2, 0}, {1, -1}, {0, -2}, {-1, -1}, {-2, 0}, {-1, 1}};
+
+// LDS tile of one lane: the bi-average of a com_had tile (up to 16x8)
+#define SMVD_TILE 136
+
+struct SmvdUnit {
+    const TzArgs *a;
+    const PicDesc *P;
+    int x, y;
+    const pel *org;
+    pel *tile;               // this lane's LDS tile
+    int mvp[2][2];
+    __device__ u32 mvbits(int v) const   // tab_mvbits[v >> mvr], the table held to its range
+    {
+        const int i = v >> a->mvr;
+        return a->tab[i < -a->tab_off ? -a->tab_off : (i > a->tab_off ? a->tab_off : i)];
+    }
+    __device__ u32 mv_cost(u32 bits) const { return (a->lambda * bits + 32768u) >> 16; }   // MV_COST (inter.c:41)
+};
+
+// get_smvd_cost (inter.c:1071-1081) of the bi-average of list 0 ref 0 at m0
+// and list 1 ref 0 at m1 (com_mc_cu, com_mc.c:140-200: the rounding
+// average of two quarter-pel plane blocks); `had` forces com_had.
+template <int TT>
+__device__ u32 smvd_cost(const SmvdUnit &u, int m0x, int m0y, int m1x, int m1y, bool had)
+{
+    const TzArgs &a = *u.a;
+    const PicDesc &P = *u.P;
+    const pel *p0 = qpel_ptr(P, 0, (u.x << 2) + m0x, (u.y << 2) + m0y);
+    const pel *p1 = qpel_ptr(P, a.n0, (u.x << 2) + m1x, (u.y << 2) + m1y);
+    const int li = threadIdx.x & (a.lpu - 1);
+    int s = 0;
+    if (!had) {   // SAD >> (bd - 8)
+        const int q = a.pw >> 2;
+        for (int i = li; i < q * a.ph; i += a.lpu) {
+            const int xx = (i % q) * 4, yy = i / q;
+            int v0[4], v1[4];
+            load_row<4>(p0 + yy * P.ref_stride + xx, v0);
+            load_row<4>(p1 + yy * P.ref_stride + xx, v1);
+            const pel *o = u.org + yy * P.pic_w + xx;
+            for (int k = 0; k < 4; k++) s += abs((int)o[k] - ((v0[k] + v1[k] + 1) >> 1));
+        }
+        return (u32)(group_sum(s, a.lpu) >> (BIT_DEPTH - 8));
+    }
+    constexpr int PW = 4 / (int)sizeof(pel);
+    const int tpr = a.pw / a.tw, T = tpr * (a.ph / a.th);
+    for (int t = li; t < T; t += a.lpu) {
+        const int tx = (t % tpr) * a.tw, ty = (t / tpr) * a.th;
+        // the tile's average, stored as whole dwords (com_had reads dwords)
+        for (int yy = 0; yy < a.th; yy++) {
+            u32 *d = (u32 *)(u.tile + yy * a.tw);
+            const pel *r0 = p0 + (ty + yy) * P.ref_stride + tx, *r1 = p1 + (ty + yy) * P.ref_stride + tx;
+            for (int k = 0; k < a.tw / PW; k++) {
+                u32 v = 0;
+                for (int j = 0; j < PW; j++)
+                    v |= (u32)((r0[k * PW + j] + r1[k * PW + j] + 1) >> 1) << (8 * (int)sizeof(pel) * j);
+                d[k] = v;
+            }
+        }
+        s += satd_tile_fn<TT>(u.org + ty * P.pic_w + tx, P.pic_w, u.tile, a.tw);
+    }
+    return (u32)(group_sum(s, a.lpu) >> (BIT_DEPTH - 8));
+}
+
+// smvd_refine (inter.c:1083-1191) for the patterns analyze_smvd uses:
+// diamond (pattern 2) and cross (pattern 0), offsets scaled by the AMVR shift.
+template <int TT>
+__device__ u32 smvd_refine(const SmvdUnit &u, int mv[2][2], u32 cost, int pattern, int rounds)
+{
+    const TzArgs &a = *u.a;
+    const int mvr = a.mvr;
+    const u32 ref_mvr_bits = (u32)(mvr + (mvr < 4));   // tbl_refi_bits[0][0] = 0
+    int ds = 0, de = pattern == 0 ? 3 : 7;
+    const int rnd = pattern == 0 ? 4 : 8, mask = pattern == 0 ? 3 : 7;
+    for (int round = 0; round < rounds; round++) {
+        const int cx = mv[0][0], cy = mv[0][1];
+        int best_dir = -1;
+        for (int idx = ds; idx <= de; idx++) {
+            const int dir = (idx + rnd) & mask;
+            const int ox = pattern == 0 ? SMVD_CROSS[dir][0] : SMVD_DIAMOND[dir][0];
+            const int oy = pattern == 0 ? SMVD_CROSS[dir][1] : SMVD_DIAMOND[dir][1];
+            const int c0x = (s16)(cx + (s16)(ox << mvr)), c0y = (s16)(cy + (s16)(oy << mvr));
+            const int dx = c0x - u.mvp[0][0], dy = c0y - u.mvp[0][1];
+            const int c1x = (s16)(u.mvp[1][0] - dx), c1y = (s16)(u.mvp[1][1] - dy);
+            const u32 bits = ref_mvr_bits + u.mvbits(dx) + u.mvbits(dy);
+            const u32 t = smvd_cost<TT>(u, c0x, c0y, c1x, c1y, a.cost_type == 0) + u.mv_cost(bits);
+            if (t < cost) {
+                cost = t;
+                mv[0][0] = c0x; mv[0][1] = c0y; mv[1][0] = c1x; mv[1][1] = c1y;
+                best_dir = dir;
+            }
+        }
+        if (best_dir < 0) break;
+        const int step = pattern == 2 ? 2 - (best_dir & 1) : 1;
+        ds = best_dir - step;
+        de = best_dir + step;
+    }
+    return cost;
+}
+
+// grid (ceil(per_ctu / (2 * 64 / lpu)), jobs), 2 waves of 64 / lpu PU groups.
+// out[job][npu]: .active selects the PU; mv and cost are written.
+template <int TT>
+__global__ __launch_bounds__(128) void k_smvd(TzArgs a, me_smvd_pu_t *out)
+{
+    __shared__ __align__(16) pel tiles[128 * SMVD_TILE];
+    const int lg = __builtin_ctz((unsigned)a.lpu), upw = 64 >> lg;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = (blockIdx.x * 2 + wv) * upw + ((threadIdx.x & 63) >> lg);
+    if (i >= a.per_ctu) return;
+    const int job = blockIdx.y;
+    const size_t pk = (size_t)job * a.npu + a.base + i;
+    if (!out[pk].active) return;
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    const int cols = 128 / a.pw;
+    SmvdUnit u;
+    u.a = &a;
+    u.P = &P;
+    u.x = (ctu % P.ctu_cols) * 128 + (i % cols) * a.pw;
+    u.y = (ctu / P.ctu_cols) * 128 + (i / cols) * a.ph;
+    if (u.x + a.pw > P.pic_w || u.y + a.ph > P.pic_h) return;
+    u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
+    u.tile = tiles + threadIdx.x * SMVD_TILE;
+    const size_t s0 = pk * a.nref, s1 = pk * a.nref + a.n0;   // list 0 / list 1 reference 0
+    u.mvp[0][0] = a.mvp[s0 * 2]; u.mvp[0][1] = a.mvp[s0 * 2 + 1];
+    u.mvp[1][0] = a.mvp[s1 * 2]; u.mvp[1][1] = a.mvp[s1 * 2 + 1];
+    const int mvr = a.mvr;
+    const bool had = a.cost_type == 0;
+    // MV = MVP in both lists (inter.c:1219-1225)
+    int mv[2][2] = {{u.mvp[0][0], u.mvp[0][1]}, {u.mvp[1][0], u.mvp[1][1]}};
+    u32 cost = smvd_cost<TT>(u, mv[0][0], mv[0][1], mv[1][0], mv[1][1], had) +
+               u.mv_cost(u.mvbits(0) * 2 + mvr + (mvr < 4));
+    // list 0's uni MV, mirrored into list 1 (inter.c:1227-1243)
+    {
+        const int m0x = a.mv_scale[s0 * 2], m0y = a.mv_scale[s0 * 2 + 1];
+        const int dx = m0x - u.mvp[0][0], dy = m0y - u.mvp[0][1];
+        const int m1x = (s16)(u.mvp[1][0] - dx), m1y = (s16)(u.mvp[1][1] - dy);
+        const u32 cb = smvd_cost<TT>(u, m0x, m0y, m1x, m1y, had) +
+                       u.mv_cost(u.mvbits(dx) + u.mvbits(dy) + mvr + (mvr < 4));
+        if (cb < cost) {
+            cost = cb;
+            mv[0][0] = m0x; mv[0][1] = m0y; mv[1][0] = m1x; mv[1][1] = m1y;
+        }
+    }
+    cost = smvd_refine<TT>(u, mv, cost, 2, 8);
+    cost = smvd_refine<TT>(u, mv, cost, 0, 1);
+    if (!had) {   // SAD search: the winner's com_had (inter.c:1248-1255)
+        const int dx = mv[0][0] - u.mvp[0][0], dy = mv[0][1] - u.mvp[0][1];
+        cost = smvd_cost<TT>(u, mv[0][0], mv[0][1], mv[1][0], mv[1][1], true) +
+               u.mv_cost(u.mvbits(dx) + u.mvbits(dy) + mvr + (mvr < 4));
+    }
+    if ((threadIdx.x & (a.lpu - 1)) == 0) {
+        for (int l = 0; l < 2; l++) {
+            out[pk].mv[l][0] = (s16)mv[l][0];
+            out[pk].mv[l][1] = (s16)mv[l][1];
+        }
+        out[pk].cost = cost;
+    }
+}
+
+typedef void (*smvd_fn)(TzArgs, me_smvd_pu_t *);
+#if ME_TZ_PART == 0
+smvd_fn me_smvd_kernel_small(int tt)
+{
+    switch (tt) {
+    case 0: return k_smvd<0>;
+    case 1: return k_smvd<1>;
+    case 2: return k_smvd<2>;
+    case 4: return k_smvd<4>;
+    default: return nullptr;
+    }
+}
+#elif ME_TZ_PART == 1
+smvd_fn me_smvd_kernel_16x8(int tt) { return tt == 3 ? k_smvd<3> : nullptr; }
+#else
+smvd_fn me_smvd_kernel_8x16(int tt) { return tt == 5 ? k_smvd<5> : nullptr; }
+#endif
+
 typedef void (*tz_fn)(TzArgs);
 #if ME_TZ_PART == 0
 tz_fn me_tz_kernel_small(int tt)
@@ -648,6 +834,44 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
     if (bi) {
         const int n = js.n_jobs * s->npu;
         hipLaunchKernelGGL(k_bi_select, dim3((n + 255) / 256), dim3(256), 0, st, a, bi->pus, bi->state, n);
+        ME_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+smvd_fn me_smvd_kernel_16x8(int tt);
+smvd_fn me_smvd_kernel_8x16(int tt);
+
+int me_smvd_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const int16_t *mv_scale,
+                    const me_tz_cfg_t *tz, me_smvd_pu_t *out, hipStream_t st)
+{
+    TzArgs a;
+    memset(&a, 0, sizeof(a));
+    a.js = js;
+    a.mvp = mvp;
+    a.mv_scale = (s16 *)mv_scale;
+    a.tab = s->d_tab_base + s->tab_off;
+    a.tab_off = s->tab_off;
+    a.lambda = s->lambda;
+    a.npu = s->npu;
+    a.nref = s->nref;
+    a.n0 = s->nref - s->nref_l1;
+    a.mvr = tz->mvr;
+    a.cost_type = tz->cost_type;
+    for (int k = 0; k < ME_NSHAPES; k++) {
+        if (s->base[k] < 0) continue;
+        a.pw = ME_SHAPES_HOST[k][0];
+        a.ph = ME_SHAPES_HOST[k][1];
+        a.base = s->base[k];
+        a.per_ctu = (128 / a.pw) * (128 / a.ph);
+        had_tiling(a.pw, a.ph, &a.tt, &a.tw, &a.th);
+        const int area = a.pw * a.ph;
+        a.lpu = area / 16 < 4 ? 4 : (area / 16 > 64 ? 64 : area / 16);
+        const int upb = 2 * (64 / a.lpu);
+        smvd_fn fn = a.tt == 3 ? me_smvd_kernel_16x8(a.tt)
+                               : (a.tt == 5 ? me_smvd_kernel_8x16(a.tt) : me_smvd_kernel_small(a.tt));
+        if (!fn) { me_set_error("no SMVD kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs), dim3(128), 0, st, a, out);
         ME_CHECK(hipGetLastError());
     }
     return 0;
