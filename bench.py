@@ -244,6 +244,21 @@ def main():
                              "host<->device copies of the per-PU state (%.0f MB)" % (
                                  (pus.nbytes + 2 * ms.nbytes) / 1e6)}
         print("[bench] bi search: %.2f ms/frame (%d PUs)" % (b_ms, int(ok.sum())), file=sys.stderr, flush=True)
+        # symmetric MVD (analyze_smvd) over the same PUs
+        spus = np.zeros((n_ctu, npu), ua.SMVD_PU)
+        spus["active"] = ok
+        hip.plan_run_smvd(plan, dref, mvp, ms0, spus, tz)   # warm-up
+        t_s = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            hip.plan_run_smvd(plan, dref, mvp, ms0, spus, tz)
+            t_s.append((time.perf_counter() - t0) * 1e3)
+        s_ms = float(np.median(t_s))
+        bi_search["smvd_ms_per_frame"] = s_ms
+        bi_search["smvd_note"] = ("me_plan_run_smvd: analyze_smvd's search (MVP pair, mirrored uni MV, diamond x8 "
+                                  "and cross refinement of bi-average com_had costs) for the same PUs; includes "
+                                  "the host<->device copies")
+        print("[bench] smvd search: %.2f ms/frame" % s_ms, file=sys.stderr, flush=True)
 
     # ---- lookahead cost estimator (inter part) -------------------------------
     lookahead = None
