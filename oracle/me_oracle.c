@@ -165,6 +165,18 @@ static const s8 CHROMA_TAPS[8][4] = {
     {-4, 36, 36, -4}, {-3, 25, 47, -5}, {-1, 15, 56, -6}, {0, 6, 62, -4}};
 
 EXPORT const s8 *orc_luma_taps(int frac) { return LUMA_TAPS[frac & 3]; }
+
+/* The 1/16-pel luma filter of affine motion compensation
+ * (com_tbl_mc_l_coeff_hp, com_tables.c:130-147; the standard's table). */
+static const s8 LUMA_TAPS_HP[16][8] = {
+    {0, 0, 0, 64, 0, 0, 0, 0},        {0, 1, -3, 63, 4, -2, 1, 0},      {-1, 2, -5, 62, 8, -3, 1, 0},
+    {-1, 3, -8, 60, 13, -4, 1, 0},    {-1, 4, -10, 58, 17, -5, 1, 0},   {-1, 4, -11, 52, 26, -8, 3, -1},
+    {-1, 3, -9, 47, 31, -10, 4, -1},  {-1, 4, -11, 45, 34, -10, 4, -1}, {-1, 4, -11, 40, 40, -11, 4, -1},
+    {-1, 4, -10, 34, 45, -11, 4, -1}, {-1, 4, -10, 31, 47, -9, 3, -1},  {-1, 3, -8, 26, 52, -11, 4, -1},
+    {0, 1, -5, 17, 58, -10, 4, -1},   {0, 1, -4, 13, 60, -8, 3, -1},    {0, 1, -3, 8, 62, -5, 2, -1},
+    {0, 1, -2, 4, 63, -3, 1, 0}};
+
+EXPORT const s8 *orc_luma_taps_hp(int frac) { return LUMA_TAPS_HP[frac & 15]; }
 EXPORT const s8 *orc_chroma_taps(int frac) { return CHROMA_TAPS[frac & 7]; }
 
 /* n-tap dot product centred so that tap index (n/2-1) hits src[0]. */
@@ -932,4 +944,51 @@ EXPORT void orc_mc_frame(int pic_w, int pic_h, int bd, int num_refs, int num_ref
             rank_c += has_c;
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Affine motion compensation, luma                                          */
+/* ------------------------------------------------------------------------ */
+
+static int log2i(int v) { int l = 0; while ((1 << l) < v) l++; return l; }
+
+/* Rounds a value with 7 fraction bits to an integer, half away from zero
+ * (com_mv_rounding_s32 with right shift 7, left shift 0, com_util.h:189-194). */
+static int round7(int v) { return v >= 0 ? (v + 64) >> 7 : -((-v + 64) >> 7); }
+
+/* com_mc_blk_affine_luma (com_mc.c:202-250): 4-parameter model from the two
+ * control-point MVs cpmv (1/16 pel); each sub_w x sub_h sub-block takes the
+ * MV at its centre (the top-left sub-block the MV at the CU's top-left
+ * corner, the top-right one at its top-right corner), rounded to 1/16 pel
+ * and clipped to 18 bits, and is predicted by com_mc_blk_luma with hp_flag:
+ * position clipped to [-132, pic + 4] (com_mc.c:61-62), then an integer
+ * copy, a one-direction 8-tap filter or the two-pass filter with the 1/16
+ * table.  (com_mc.c:65's test for the half-pel planes can never hold, so
+ * the filters always run.)  planes0: the padded integer plane at (0,0). */
+EXPORT void orc_affine_luma(int x, int y, int pic_w, int pic_h, int cu_w, int cu_h, const int cpmv[2][2],
+                            const pel *plane0, int i_ref, pel *pred, int sub_w, int sub_h, int bd)
+{
+    const int lw = log2i(cu_w);
+    const int base_x = cpmv[0][0] * 128, base_y = cpmv[0][1] * 128;
+    const int dhx = ((cpmv[1][0] - cpmv[0][0]) * 128) >> lw, dhy = ((cpmv[1][1] - cpmv[0][1]) * 128) >> lw;
+    const int dvx = -dhy, dvy = dhx;
+    const int max_val = (1 << bd) - 1;
+    for (int h = 0; h < cu_h; h += sub_h)
+        for (int w = 0; w < cu_w; w += sub_w) {
+            int px = w + sub_w / 2, py = h + sub_h / 2;
+            if (w == 0 && h == 0) { px = 0; py = 0; }
+            else if (w + sub_w == cu_w && h == 0) { px = cu_w; py = 0; }
+            int mx = round7(base_x + dhx * px + dvx * py), my = round7(base_y + dhy * px + dvy * py);
+            mx = CLIP3(-131072, 131071, mx);
+            my = CLIP3(-131072, 131071, my);
+            const int gx = ((x + w) << 4) + mx, gy = ((y + h) << 4) + my;
+            const int fx = gx & 15, fy = gy & 15;
+            const int ix = CLIP3(-128 - 4, pic_w + 4, gx >> 4), iy = CLIP3(-128 - 4, pic_h + 4, gy >> 4);
+            const pel *src = plane0 + iy * i_ref + ix;
+            pel *dst = pred + h * cu_w + w;
+            if (fx == 0 && fy == 0) orc_ipcpy(src, i_ref, dst, cu_w, sub_w, sub_h);
+            else if (fy == 0) orc_ipflt(0, src, i_ref, dst, cu_w, sub_w, sub_h, LUMA_TAPS_HP[fx], max_val);
+            else if (fx == 0) orc_ipflt(2, src, i_ref, dst, cu_w, sub_w, sub_h, LUMA_TAPS_HP[fy], max_val);
+            else orc_ipflt_ext(0, src, i_ref, dst, cu_w, sub_w, sub_h, LUMA_TAPS_HP[fx], LUMA_TAPS_HP[fy], max_val);
+        }
 }

@@ -84,6 +84,7 @@ class Oracle(_Lib):
         L.orc_lambda_mv.restype = C.c_uint32
         L.orc_luma_taps.restype = C.c_void_p
         L.orc_chroma_taps.restype = C.c_void_p
+        L.orc_luma_taps_hp.restype = C.c_void_p
         L.orc_full_search.restype = C.c_uint64
         L.orc_subpel.restype = C.c_uint64
         L.orc_ipel_cost.restype = C.c_uint64
@@ -173,6 +174,20 @@ class Oracle(_Lib):
                 break
             fixed = s
         return best
+
+    def affine_luma(self, planes, pic_w, pic_h, x, y, cu_w, cu_h, cpmv, sub_w, sub_h, pad=PAD):
+        """com_mc_blk_affine_luma (com_mc.c:202-250) from padded planes; cpmv
+        [[x0, y0], [x1, y1]] in 1/16 pel.  Returns the (cu_h, cu_w) block."""
+        stride = pic_w + 2 * pad
+        cp = (C.c_int * 4)(*[int(v) for v in np.asarray(cpmv).reshape(-1)])
+        out = np.zeros((cu_h, cu_w), self.dt)
+        self.lib.orc_affine_luma(x, y, pic_w, pic_h, cu_w, cu_h, cp,
+                                 C.c_void_p(planes[0].ctypes.data + (pad * stride + pad) * planes.itemsize), stride,
+                                 ptr(out), sub_w, sub_h, self.bd)
+        return out
+
+    def luma_taps_hp(self, frac):
+        return np.ctypeslib.as_array(C.cast(self.lib.orc_luma_taps_hp(frac), C.POINTER(C.c_int8)), (8,)).copy()
 
     def analyze_smvd(self, cur, planes, n0, x, y, w, h, mvp, mv_scale, mvr, cost_type, lambda_mv):
         """The search part of analyze_smvd (inter.c:1194-1255) with
@@ -552,6 +567,13 @@ class RefLib(_Lib):
         self.lib.refh_mc_cu(x, y, pic_w, pic_h, w, h, C.c_void_p(p0) if p0 else None, C.c_void_p(p1) if p1 else None,
                             ptr(m0), ptr(m1), 0 if chroma else 1, ptr(py), ptr(pu), ptr(pv))
         return py, (pu if chroma else None), (pv if chroma else None)
+
+    def affine_luma(self, pic, x, y, cu_w, cu_h, cpmv, sub_w, sub_h):
+        """com_mc_blk_affine_luma of the compiled reference."""
+        cp = np.ascontiguousarray(np.asarray(cpmv, np.int32).reshape(-1))
+        out = np.zeros((cu_h, cu_w), self.dt)
+        self.lib.refh_affine_luma(C.c_void_p(pic), x, y, cu_w, cu_h, ptr(cp), sub_w, sub_h, ptr(out))
+        return out
 
     def me_search_tz_bi(self, pic, org_bi, x, y, refi, num_refp, lidx, mot_bits_other, mvr, cost_type, lambda_mv,
                         mvp, mv):

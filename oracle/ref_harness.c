@@ -508,3 +508,22 @@ RH_EXPORT double refh_loka_cost(const pel *cur, int w, int h, long long cur_poc,
     com_pic_destroy(org);
     return c;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Affine motion compensation                                               */
+/* ---------------------------------------------------------------------- */
+
+/* com_mc_blk_affine_luma (com_mc.c:202-250) of a cu_w x cu_h CU at (x, y)
+ * from the two control-point MVs (1/16 pel) into pred (stride cu_w). */
+RH_EXPORT void refh_affine_luma(void *ref, int x, int y, int cu_w, int cu_h, const int *cpmv, int sub_w, int sub_h,
+                                pel *pred)
+{
+    refh_pic_t *r = (refh_pic_t *)ref;
+    CPMV mv[VER_NUM][MV_D];
+    memset(mv, 0, sizeof(mv));
+    mv[0][0] = cpmv[0]; mv[0][1] = cpmv[1];
+    mv[1][0] = cpmv[2]; mv[1][1] = cpmv[3];
+    ALIGNED_32(pel buf[MAX_CU_DIM]);
+    com_mc_blk_affine_luma(x, y, r->w, r->h, cu_w, cu_h, mv, r->pic, buf, sub_w, sub_h, BIT_DEPTH);
+    memcpy(pred, buf, sizeof(pel) * (size_t)cu_w * cu_h);
+}

@@ -1,0 +1,104 @@
+"""Affine luma motion compensation (com_mc_blk_affine_luma,
+com_mc.c:202-250): the oracle pinned against the compiled reference, and the
+GPU path (me_affine_mc_luma) against the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, RefLib
+import uavs3e_amd as ua
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _golden(bd):
+    return np.load(os.path.join(GOLD, "affine_bd%d.npz" % bd), allow_pickle=False)
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_affine_oracle_golden(bd):
+    g, o = _golden(bd), Oracle(bd)
+    pic = g["pic"]
+    h, w = pic.shape
+    planes = o.make_planes(pic)
+    for c in g["cases"]:
+        x, y, cw, ch, sw, sh = (int(v) for v in c[:6])
+        got = o.affine_luma(planes, w, h, x, y, cw, ch, c[6:10].reshape(2, 2), sw, sh)
+        off = int(c[10])
+        assert np.array_equal(got.reshape(-1), g["preds"][off:off + cw * ch]), c.tolist()
+
+
+@pytest.mark.skipif(not RefLib.available(8), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("bd", [8, 10])
+def test_affine_oracle_live(bd):
+    o, r = Oracle(bd), RefLib(bd, 2)
+    rng = np.random.default_rng(bd + 3)
+    h, w = 96, 160
+    pic = rng.integers(0, 1 << bd, (h, w)).astype(o.dt)
+    planes, rp = o.make_planes(pic), r.pic_create(pic, 0)
+    for k in range(60):
+        cw, ch = (16, 32, 64, 128)[k % 4], (16, 32, 64, 128)[(k // 4) % 4]
+        x, y = int(rng.integers(-16, w)) // 4 * 4, int(rng.integers(-16, h)) // 4 * 4
+        sw, sh = (4, 8)[k % 2], (4, 8)[(k // 2) % 2]
+        cp = rng.integers(-3000, 3000, (2, 2))
+        assert np.array_equal(o.affine_luma(planes, w, h, x, y, cw, ch, cp, sw, sh),
+                              r.affine_luma(rp, x, y, cw, ch, cp, sw, sh)), (k, x, y, cw, ch, sw, sh, cp.tolist())
+    r.pic_free(rp)
+
+
+def test_affine_oracle_special_cases():
+    """Equal control points at a whole-sample MV copy the displaced block;
+    at a quarter-sample MV every sub-block equals the 1/16 filter at phase 4;
+    the 1/16 table's phase 0 is the identity and every phase sums to 64."""
+    o = Oracle(8)
+    rng = np.random.default_rng(2)
+    h, w = 64, 96
+    pic = rng.integers(0, 256, (h, w)).astype(np.uint8)
+    planes = o.make_planes(pic)
+    got = o.affine_luma(planes, w, h, 16, 16, 32, 16, [[32, -16], [32, -16]], 4, 4)
+    assert np.array_equal(got, pic[15:31, 18:50])
+    for f in range(16):
+        t = o.luma_taps_hp(f)
+        assert int(t.sum()) == 64
+    assert o.luma_taps_hp(0).tolist() == [0, 0, 0, 64, 0, 0, 0, 0]
+    got = o.affine_luma(planes, w, h, 16, 16, 16, 16, [[4, 0], [4, 0]], 8, 8)
+    want = o.ipflt(0, planes[0], w + 2 * 144, (144 + 16) * (w + 2 * 144) + 144 + 16, 16, 16, o.luma_taps_hp(4), 255)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bd", [8, 10])
+def test_affine_gpu_vs_oracle(bd):
+    """The golden cases plus random CUs with large control-point MVs (reaching
+    the clipped picture border), every CU size, all four sub-block shapes."""
+    g, o = _golden(bd), Oracle(bd)
+    hip = ua.MeHip(bd)
+    pic = g["pic"]
+    h, w = pic.shape
+    planes = o.make_planes(pic)
+    rng = np.random.default_rng(bd)
+    rows = [list(c[:10]) for c in g["cases"]]
+    for k in range(64):
+        cw, ch = (16, 32, 64, 128)[k % 4], (16, 32, 64, 128)[(k // 4) % 4]
+        rows.append([int(rng.integers(0, w)) // 4 * 4, int(rng.integers(0, h)) // 4 * 4, cw, ch, (4, 8)[k % 2],
+                     (4, 8)[(k // 2) % 2]] + rng.integers(-6000, 6000, 4).tolist())
+    cus = np.zeros(len(rows), ua.AFFINE_CU)
+    off = 0
+    for i, r in enumerate(rows):
+        cus[i]["x"], cus[i]["y"], cus[i]["w"], cus[i]["h"], cus[i]["sub_w"], cus[i]["sub_h"] = r[:6]
+        cus[i]["cpmv"] = np.array(r[6:10]).reshape(2, 2)
+        cus[i]["pred_offset"] = off
+        off += r[2] * r[3]
+    pred = np.zeros(off + 7, hip.dt)
+    pred[-7:] = 77
+    ref = hip.ref_create(pic)
+    hip.affine_mc_luma(ref, cus, pred)
+    hip.ref_destroy(ref)
+    for c in cus:
+        o_ = int(c["pred_offset"])
+        want = o.affine_luma(planes, w, h, int(c["x"]), int(c["y"]), int(c["w"]), int(c["h"]), c["cpmv"],
+                             int(c["sub_w"]), int(c["sub_h"]))
+        assert np.array_equal(pred[o_:o_ + c["w"] * c["h"]], want.reshape(-1)), c
+    assert (pred[-7:] == 77).all()
+    assert hip.error_count() == 0

@@ -117,6 +117,10 @@ BI_PU = np.dtype([("refi", np.int8, 2), ("lidx_ref", np.int8), ("active", np.int
                   ("mot_bits", np.int32, 2), ("cost", np.uint32)], align=True)
 
 
+# me_affine_cu_t
+AFFINE_CU = np.dtype([("x", np.int32), ("y", np.int32), ("w", np.int32), ("h", np.int32), ("sub_w", np.int32),
+                      ("sub_h", np.int32), ("cpmv", np.int32, (2, 2)), ("pred_offset", np.uint64)], align=True)
+
 # me_smvd_pu_t
 SMVD_PU = np.dtype([("active", np.int8), ("pad", np.int8, 3), ("mv", np.int16, (2, 2)), ("cost", np.uint32)],
                    align=True)
@@ -174,6 +178,7 @@ class MeHip:
                                      C.c_void_p]
         L.me_plan_run_smvd.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p]
+        L.me_affine_mc_luma.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]
         L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
@@ -339,6 +344,15 @@ class MeHip:
         assert pus.dtype == BI_PU and pus.flags.c_contiguous and BI_PU.itemsize == 24
         self.check(self.lib.me_plan_run_bi(plan, arr, _p(mvp), _p(mv_scale), _p(pus), C.byref(tz), stream),
                    "me_plan_run_bi")
+
+    def affine_mc_luma(self, ref, cus, pred):
+        """me_affine_mc_luma: cus an AFFINE_CU array, pred a flat array of
+        samples (updated in place at each CU's pred_offset)."""
+        assert cus.dtype == AFFINE_CU and AFFINE_CU.itemsize == 48 and pred.dtype == self.dt
+        cus = np.ascontiguousarray(cus)
+        assert pred.flags.c_contiguous
+        self.check(self.lib.me_affine_mc_luma(ref, _p(cus), len(cus), _p(pred), pred.size, None),
+                   "me_affine_mc_luma")
 
     def plan_run_smvd(self, plan, refs, mvp, mv_scale, pus, tz, stream=None):
         """me_plan_run_smvd: mvp, mv_scale int16 [ctu][pu][ref][2]; pus an

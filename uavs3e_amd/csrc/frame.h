@@ -17,6 +17,11 @@ struct me_ref_s {
     pel *luma;               // unpadded w x h copy (source for re-preparation)
     s16 *tmp;                // 3 horizontal s16 planes over [-8, w+8) x [-8, h+8)
     pel *chroma;             // optional 4:2:0 u then v, (w/2) x (h/2) each, unpadded
+    // affine motion compensation buffers (grown on demand)
+    pel *aff_pred;
+    size_t aff_pred_cap;
+    me_affine_cu_t *aff_cus;
+    int aff_cus_cap;
 };
 
 // Pads the integer plane of r from r->luma and rebuilds its 15 sub-pel

@@ -246,6 +246,8 @@ ME_EXPORT void me_ref_destroy(me_ref_t r)
     (void)hipFree(r->luma);
     (void)hipFree(r->tmp);
     (void)hipFree(r->chroma);
+    (void)hipFree(r->aff_pred);
+    (void)hipFree(r->aff_cus);
     free(r);
 }
 
