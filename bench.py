@@ -260,6 +260,33 @@ def main():
                                   "the host<->device copies")
         print("[bench] smvd search: %.2f ms/frame" % s_ms, file=sys.stderr, flush=True)
 
+    # ---- affine luma motion compensation --------------------------------------
+    affine = None
+    if not args.no_tz:
+        # the picture tiled by 32x32 CUs with 8x8 sub-blocks (4x4 in every other
+        # CU), control-point MVs of a small zoom / rotation around the global motion
+        arng = np.random.default_rng(12)
+        xs, ys = np.meshgrid(np.arange(0, work.w - 31, 32), np.arange(0, work.h - 31, 32))
+        nc = xs.size
+        cus = np.zeros(nc, ua.AFFINE_CU)
+        cus["x"], cus["y"], cus["w"], cus["h"] = xs.reshape(-1), ys.reshape(-1), 32, 32
+        cus["sub_w"] = cus["sub_h"] = np.where(np.arange(nc) % 2 == 0, 8, 4)
+        cus["cpmv"][:, 0, :] = arng.integers(-64, 65, (nc, 2))
+        cus["cpmv"][:, 1, :] = cus["cpmv"][:, 0, :] + arng.integers(-12, 13, (nc, 2))
+        cus["pred_offset"] = np.arange(nc, dtype=np.uint64) * 1024
+        apred = np.zeros(nc * 1024, hip.dt)
+        hip.affine_mc_luma(dref[0], cus, apred)   # warm-up
+        t_a = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            hip.affine_mc_luma(dref[0], cus, apred)
+            t_a.append((time.perf_counter() - t0) * 1e3)
+        a_ms = float(np.median(t_a))
+        affine = {"ms_per_frame": a_ms, "cus": int(nc), "samples_per_s": nc * 1024 / (a_ms * 1e-3),
+                  "note": "me_affine_mc_luma: every 32x32 CU of the picture, 8x8 / 4x4 sub-blocks, 1/16-pel "
+                          "8-tap interpolation; includes the host<->device copies of the predictions"}
+        print("[bench] affine MC: %.3f ms/frame (%d CUs)" % (a_ms, nc), file=sys.stderr, flush=True)
+
     # ---- lookahead cost estimator (inter part) -------------------------------
     lookahead = None
     if not args.no_tz and hip.lib.me_plan_num_pus(plan) > 0:
@@ -452,6 +479,7 @@ def main():
         "search_tz": search_tz,
         "lookahead": lookahead,
         "bi_search": bi_search,
+        "affine_mc": affine,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
     }

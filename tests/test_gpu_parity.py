@@ -164,9 +164,8 @@ def _spoil_planes(hip, ref, w, h):
 @pytest.mark.parametrize("bd,nref,mask", [(8, 2, 1), (8, 3, 3), (10, 2, 1), (8, 2, 3), (8, 2, 2)])
 def test_frame_plan_rebuilds_references(bd, nref, mask):
     """References named in the prepare mask are rebuilt inside the frame
-    graph before their search: a leading range of them on a second stream
-    beside the search of the others, any other mask in order on one stream.
-    Spoiled planes of those references must not reach the results."""
+    graph before the search, for any mask and on a capture as on a replay:
+    spoiled planes of those references must not reach the results."""
     base = wl.hd_allpu() if bd == 8 else wl.uhd10_allpu()
     w, h, R = 256, 200, 12
     work = base.scaled(w, h, R)
@@ -185,7 +184,7 @@ def test_frame_plan_rebuilds_references(bd, nref, mask):
         hip.plan_run(plan, dref, mask)
         res, best = hip.plan_results(plan, nref)
         _compare(res, best, ores, obest)
-    # the same handle twice: rebuilt in order, never beside its own search
+    # the same handle twice
     hip.plan_run(plan, [dref[0]] * nref, 1)
     res, best = hip.plan_results(plan, nref)
     o2, b2 = orc.me_frame(cur, [orc.make_planes(refs[0])] * nref, R, lam, MASK_ALL)

@@ -289,8 +289,6 @@ struct me_plan_s {
     pel *d_orgbi;         // analyze_bi targets, [ctu][selected shape][16384]
     me_smvd_pu_t *d_smvd; // analyze_smvd per-PU results (allocated on first use)
     hipStream_t own;
-    hipStream_t side;     // rebuilds references while the others are searched
-    hipEvent_t fork, join;
     hipGraphExec_t exec;
     me_ref_t graph_refs[ME_MAX_REFS];
     u32 graph_mask;
@@ -325,9 +323,6 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     ALLOC(p->d_best, (size_t)p->n_ctu * p->s.npu * p->s.nl);
 #undef ALLOC
     if (e == hipSuccess) e = hipStreamCreate(&p->own);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         me_set_error("me_plan_create", e, __FILE__, __LINE__);
         me_plan_destroy(p);
@@ -389,36 +384,9 @@ static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prep
     return me_search_enqueue(&p->s, js, p->d_keys, p->d_res, p->d_best, st, ev ? ev + 1 : nullptr);
 }
 
-// The frame graph.  When the references to rebuild are a leading range
-// [0, m) of fewer than all of them, they are rebuilt on a side stream while
-// the integer search of [m, nref) runs (it does not read them); their own
-// search follows on the side stream, and the two branches join before the
-// sub-pel stage.  The results are those of the serial order: each reference's
-// keys are written only by its own integer search.
-static int plan_enqueue(me_plan_s *p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
+static int plan_enqueue(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
 {
-    const int nref = p->s.nref;
-    int m = 0;
-    while (m < nref && (prepare_mask >> m & 1)) m++;
-    bool split = m > 0 && m < nref && (prepare_mask >> m) == 0;
-    // a handle listed twice would be read while it is rebuilt
-    for (int r = m; split && r < nref; r++)
-        for (int q = 0; q < m; q++) split = split && refs[r] != refs[q];
-    if (!split) return plan_enqueue_stages(p, refs, prepare_mask, st, nullptr);
-    JobSet js;
-    js.pics = p->d_pic;
-    js.jobs = nullptr;
-    js.n_jobs = p->n_ctu;
-    if (me_search_fill(&p->s, js, p->d_keys, st)) return -1;
-    ME_CHECK(hipEventRecord(p->fork, st));
-    ME_CHECK(hipStreamWaitEvent(p->side, p->fork, 0));
-    for (int r = 0; r < m; r++)
-        if (me_ref_prepare_async(refs[r], p->side)) return -1;
-    if (me_search_int(&p->s, js, p->d_keys, 0, m, p->side)) return -1;
-    ME_CHECK(hipEventRecord(p->join, p->side));
-    if (me_search_int(&p->s, js, p->d_keys, m, nref, st)) return -1;
-    ME_CHECK(hipStreamWaitEvent(st, p->join, 0));
-    return me_search_finish(&p->s, js, p->d_keys, p->d_res, p->d_best, st);
+    return plan_enqueue_stages(p, refs, prepare_mask, st, nullptr);
 }
 
 ME_EXPORT int me_plan_run(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, void *stream)
@@ -780,9 +748,6 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->own) (void)hipStreamDestroy(p->own);
-    if (p->side) (void)hipStreamDestroy(p->side);
-    if (p->fork) (void)hipEventDestroy(p->fork);
-    if (p->join) (void)hipEventDestroy(p->join);
     me_search_free(&p->s);
     (void)hipFree(p->d_loka);
     (void)hipFree(p->d_bipu);
