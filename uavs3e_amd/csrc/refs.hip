@@ -61,6 +61,28 @@ __device__ __forceinline__ void load_samples(const T *p, int *out)
     }
 }
 
+// 4 consecutive clipped samples at dst: one dword (8 bits) or two (10 bits)
+// when dst is aligned to the 4 samples' size, else sample by sample.
+__device__ __forceinline__ void store4(pel *dst, const int *v, int maxv, bool aligned)
+{
+    if (aligned) {
+        if (sizeof(pel) == 1) {
+            u32 d = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) d |= (u32)me_clip3(0, maxv, v[j]) << (8 * j);
+            *(u32 *)dst = d;
+        } else {
+            uint2 d;
+            d.x = (u32)me_clip3(0, maxv, v[0]) | (u32)me_clip3(0, maxv, v[1]) << 16;
+            d.y = (u32)me_clip3(0, maxv, v[2]) | (u32)me_clip3(0, maxv, v[3]) << 16;
+            *(uint2 *)dst = d;
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) dst[j] = (pel)me_clip3(0, maxv, v[j]);
+}
+
 // One padded sample of plane idx at (X, Y) (plane coordinates).
 __device__ __forceinline__ int plane_sample(const pel *p0, int stride, int w, int h, const s16 *tmp, int idx, int X,
                                             int Y)
@@ -148,9 +170,10 @@ __global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int strid
 #pragma unroll
         for (int j = 0; j < 4; j++) v[j] = X0 + j < stride ? plane_sample(p0, stride, w, h, tmp, idx, X0 + j, Y) : 0;
     }
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-        if (X0 + j < stride) dst[j] = (pel)me_clip3(0, maxv, v[j]);
+    if (X0 + 3 < stride) store4(dst, v, maxv, (stride & 3) == 0 && (plane_elems & 3) == 0);
+    else
+        for (int j = 0; j < 4; j++)
+            if (X0 + j < stride) dst[j] = (pel)me_clip3(0, maxv, v[j]);
 }
 
 int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
