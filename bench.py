@@ -271,6 +271,7 @@ def main():
         cus = np.zeros(nc, ua.AFFINE_CU)
         cus["x"], cus["y"], cus["w"], cus["h"] = xs.reshape(-1), ys.reshape(-1), 32, 32
         cus["sub_w"] = cus["sub_h"] = np.where(np.arange(nc) % 2 == 0, 8, 4)
+        cus["vertex_num"] = 2
         cus["cpmv"][:, 0, :] = arng.integers(-64, 65, (nc, 2))
         cus["cpmv"][:, 1, :] = cus["cpmv"][:, 0, :] + arng.integers(-12, 13, (nc, 2))
         cus["pred_offset"] = np.arange(nc, dtype=np.uint64) * 1024

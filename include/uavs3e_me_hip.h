@@ -243,20 +243,25 @@ typedef struct me_tz_cfg_t {
 int me_plan_run_tz(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, const int16_t *mv_ipel_prev,
                    const me_tz_cfg_t *tz, void *stream);
 
-/* Affine motion compensation of luma, com_mc_blk_affine_luma
- * (com_mc.c:202-250), for a batch of CUs against one reference: the
- * 4-parameter model from two control-point MVs (CPMV, 1/16 pel); every
- * sub_w x sub_h sub-block (4 or 8) takes the MV at its centre (the top-left
- * and top-right sub-blocks the MVs at the CU's corners), rounded to 1/16 pel
+/* Affine motion compensation of luma for a batch of CUs against one
+ * reference: com_mc_blk_affine_luma (com_mc.c:202-250) with two control
+ * points, the luma part of com_mc_blk_affine (com_mc.c:252-333, through
+ * com_mc_cu_affine, com_mc.c:365-385) with two or three (vertex_num).  The
+ * model comes from the control-point MVs (CPMV, 1/16 pel: top-left,
+ * top-right, bottom-left); every sub_w x sub_h sub-block (4 or 8) takes the
+ * MV at its centre (the top-left, top-right and, with three control points,
+ * bottom-left sub-blocks the MVs at the CU's corners), rounded to 1/16 pel
  * and clipped to 18 bits, and is interpolated from the integer plane with
  * the 1/16-pel 8-tap filters (com_tables.c:130-147) as com_mc_blk_luma does
- * with hp_flag (position clipped to [-132, pic + 4]).  CU sizes 16..128 (powers
- * of two).  pred (host memory, pred_elems samples) receives each CU's
- * w x h prediction row by row at its pred_offset.  Blocking. */
+ * with hp_flag (position clipped to [-132, pic + 4]).  CU sizes 16..128
+ * (powers of two).  pred (host memory, pred_elems samples) receives each
+ * CU's w x h prediction row by row at its pred_offset.  Blocking. */
 typedef struct me_affine_cu_t {
     int32_t x, y, w, h;
     int32_t sub_w, sub_h;
-    int32_t cpmv[2][2];
+    int32_t vertex_num;       /* 2 or 3 control points */
+    int32_t reserved;
+    int32_t cpmv[3][2];
     uint64_t pred_offset;
 } me_affine_cu_t;
 int me_affine_mc_luma(me_ref_t ref, const me_affine_cu_t *cus, int n, me_pel *pred, uint64_t pred_elems,

@@ -956,28 +956,35 @@ static int log2i(int v) { int l = 0; while ((1 << l) < v) l++; return l; }
  * (com_mv_rounding_s32 with right shift 7, left shift 0, com_util.h:189-194). */
 static int round7(int v) { return v >= 0 ? (v + 64) >> 7 : -((-v + 64) >> 7); }
 
-/* com_mc_blk_affine_luma (com_mc.c:202-250): 4-parameter model from the two
- * control-point MVs cpmv (1/16 pel); each sub_w x sub_h sub-block takes the
- * MV at its centre (the top-left sub-block the MV at the CU's top-left
- * corner, the top-right one at its top-right corner), rounded to 1/16 pel
+/* com_mc_blk_affine_luma (com_mc.c:202-250) and the luma part of
+ * com_mc_blk_affine (com_mc.c:252-333): 4-parameter model from two
+ * control-point MVs, 6-parameter from three (cp_num; cpmv in 1/16 pel); each
+ * sub_w x sub_h sub-block takes the MV at its centre (the top-left sub-block
+ * the MV at the CU's top-left corner, the top-right one at its top-right
+ * corner, with three points the bottom-left one at its bottom-left corner),
+ * rounded to 1/16 pel
  * and clipped to 18 bits, and is predicted by com_mc_blk_luma with hp_flag:
  * position clipped to [-132, pic + 4] (com_mc.c:61-62), then an integer
  * copy, a one-direction 8-tap filter or the two-pass filter with the 1/16
  * table.  (com_mc.c:65's test for the half-pel planes can never hold, so
  * the filters always run.)  planes0: the padded integer plane at (0,0). */
-EXPORT void orc_affine_luma(int x, int y, int pic_w, int pic_h, int cu_w, int cu_h, const int cpmv[2][2],
-                            const pel *plane0, int i_ref, pel *pred, int sub_w, int sub_h, int bd)
+EXPORT void orc_affine_luma(int x, int y, int pic_w, int pic_h, int cu_w, int cu_h, const int cpmv[3][2],
+                            int cp_num, const pel *plane0, int i_ref, pel *pred, int sub_w, int sub_h, int bd)
 {
-    const int lw = log2i(cu_w);
+    const int lw = log2i(cu_w), lh = log2i(cu_h);
     const int base_x = cpmv[0][0] * 128, base_y = cpmv[0][1] * 128;
     const int dhx = ((cpmv[1][0] - cpmv[0][0]) * 128) >> lw, dhy = ((cpmv[1][1] - cpmv[0][1]) * 128) >> lw;
-    const int dvx = -dhy, dvy = dhx;
+    /* the 6-parameter model takes the vertical gradient from the bottom-left
+     * control point (com_mc.c:273-279) */
+    const int dvx = cp_num == 3 ? ((cpmv[2][0] - cpmv[0][0]) * 128) >> lh : -dhy;
+    const int dvy = cp_num == 3 ? ((cpmv[2][1] - cpmv[0][1]) * 128) >> lh : dhx;
     const int max_val = (1 << bd) - 1;
     for (int h = 0; h < cu_h; h += sub_h)
         for (int w = 0; w < cu_w; w += sub_w) {
             int px = w + sub_w / 2, py = h + sub_h / 2;
             if (w == 0 && h == 0) { px = 0; py = 0; }
             else if (w + sub_w == cu_w && h == 0) { px = cu_w; py = 0; }
+            else if (w == 0 && h + sub_h == cu_h && cp_num == 3) { px = 0; py = cu_h; }
             int mx = round7(base_x + dhx * px + dvx * py), my = round7(base_y + dhy * px + dvy * py);
             mx = CLIP3(-131072, 131071, mx);
             my = CLIP3(-131072, 131071, my);

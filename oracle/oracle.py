@@ -176,12 +176,16 @@ class Oracle(_Lib):
         return best
 
     def affine_luma(self, planes, pic_w, pic_h, x, y, cu_w, cu_h, cpmv, sub_w, sub_h, pad=PAD):
-        """com_mc_blk_affine_luma (com_mc.c:202-250) from padded planes; cpmv
-        [[x0, y0], [x1, y1]] in 1/16 pel.  Returns the (cu_h, cu_w) block."""
+        """Affine luma MC from padded planes (com_mc_blk_affine_luma,
+        com_mc.c:202-250, or com_mc_blk_affine's luma with three points);
+        cpmv [[x0, y0], [x1, y1]] or with [x2, y2] too, 1/16 pel.  Returns
+        the (cu_h, cu_w) block."""
         stride = pic_w + 2 * pad
-        cp = (C.c_int * 4)(*[int(v) for v in np.asarray(cpmv).reshape(-1)])
+        cpa = np.asarray(cpmv).reshape(-1, 2)
+        cp_num = len(cpa)
+        cp = (C.c_int * 6)(*[int(v) for v in np.concatenate([cpa.reshape(-1), np.zeros(6 - cpa.size, int)])])
         out = np.zeros((cu_h, cu_w), self.dt)
-        self.lib.orc_affine_luma(x, y, pic_w, pic_h, cu_w, cu_h, cp,
+        self.lib.orc_affine_luma(x, y, pic_w, pic_h, cu_w, cu_h, cp, cp_num,
                                  C.c_void_p(planes[0].ctypes.data + (pad * stride + pad) * planes.itemsize), stride,
                                  ptr(out), sub_w, sub_h, self.bd)
         return out
@@ -567,6 +571,17 @@ class RefLib(_Lib):
         self.lib.refh_mc_cu(x, y, pic_w, pic_h, w, h, C.c_void_p(p0) if p0 else None, C.c_void_p(p1) if p1 else None,
                             ptr(m0), ptr(m1), 0 if chroma else 1, ptr(py), ptr(pu), ptr(pv))
         return py, (pu if chroma else None), (pv if chroma else None)
+
+    def affine_cu_luma(self, pic, x, y, cu_w, cu_h, cpmv, sub8):
+        """Luma of com_mc_cu_affine (com_mc.c:365-385) of the compiled
+        reference, one list, two or three control points; sub8 selects 8x8
+        sub-blocks (affine_subblk_size_idx)."""
+        cpa = np.asarray(cpmv, np.int32).reshape(-1, 2)
+        cp = np.zeros(8, np.int32)
+        cp[:cpa.size] = cpa.reshape(-1)
+        out = np.zeros((cu_h, cu_w), self.dt)
+        self.lib.refh_affine_cu_luma(C.c_void_p(pic), x, y, cu_w, cu_h, ptr(cp), len(cpa), int(sub8), ptr(out))
+        return out
 
     def affine_luma(self, pic, x, y, cu_w, cu_h, cpmv, sub_w, sub_h):
         """com_mc_blk_affine_luma of the compiled reference."""

@@ -527,3 +527,28 @@ RH_EXPORT void refh_affine_luma(void *ref, int x, int y, int cu_w, int cu_h, con
     com_mc_blk_affine_luma(x, y, r->w, r->h, cu_w, cu_h, mv, r->pic, buf, sub_w, sub_h, BIT_DEPTH);
     memcpy(pred, buf, sizeof(pel) * (size_t)cu_w * cu_h);
 }
+
+/* Luma of com_mc_cu_affine (com_mc.c:365-385) for list 0 alone, with two or
+ * three control points (1/16 pel) and 4x4 or 8x8 sub-blocks
+ * (affine_subblk_size_idx); the chroma it also predicts is not returned. */
+RH_EXPORT void refh_affine_cu_luma(void *ref, int x, int y, int cu_w, int cu_h, const int *cpmv, int vertex_num,
+                                   int sub8, pel *pred)
+{
+    refh_pic_t *r = (refh_pic_t *)ref;
+    static com_ref_pic_t refp[MAX_REFS][REFP_NUM];
+    memset(refp, 0, sizeof(refp));
+    refp[0][REFP_0].pic = r->pic;
+    s8 refi[REFP_NUM] = {0, REFI_INVALID};
+    CPMV mv[REFP_NUM][VER_NUM][MV_D];
+    memset(mv, 0, sizeof(mv));
+    for (int i = 0; i < vertex_num; i++) {
+        mv[REFP_0][i][MV_X] = cpmv[2 * i];
+        mv[REFP_0][i][MV_Y] = cpmv[2 * i + 1];
+    }
+    com_pic_header_t sh;
+    memset(&sh, 0, sizeof(sh));
+    sh.affine_subblk_size_idx = sub8;
+    static pel buf[N_C][MAX_CU_DIM];
+    com_mc_cu_affine(x, y, r->w, r->h, cu_w, cu_h, refi, mv, refp, buf, vertex_num, &sh, BIT_DEPTH);
+    memcpy(pred, buf[Y_C], sizeof(pel) * (size_t)cu_w * cu_h);
+}

@@ -1,6 +1,7 @@
 """Affine luma motion compensation (com_mc_blk_affine_luma,
-com_mc.c:202-250): the oracle pinned against the compiled reference, and the
-GPU path (me_affine_mc_luma) against the oracle."""
+com_mc.c:202-250, and the luma of com_mc_cu_affine with three control
+points): the oracle pinned against the compiled reference, and the GPU path
+(me_affine_mc_luma) against the oracle."""
 import os
 
 import numpy as np
@@ -27,6 +28,41 @@ def test_affine_oracle_golden(bd):
         got = o.affine_luma(planes, w, h, x, y, cw, ch, c[6:10].reshape(2, 2), sw, sh)
         off = int(c[10])
         assert np.array_equal(got.reshape(-1), g["preds"][off:off + cw * ch]), c.tolist()
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_affine3_oracle_golden(bd):
+    """Three control points: the luma of com_mc_cu_affine."""
+    g, o = _golden(bd), Oracle(bd)
+    pic = g["pic"]
+    h, w = pic.shape
+    planes = o.make_planes(pic)
+    for c in g["cases3"]:
+        x, y, cw, ch, s8 = (int(v) for v in c[:5])
+        sub = 8 if s8 else 4
+        got = o.affine_luma(planes, w, h, x, y, cw, ch, c[5:11].reshape(3, 2), sub, sub)
+        off = int(c[11])
+        assert np.array_equal(got.reshape(-1), g["preds3"][off:off + cw * ch]), c.tolist()
+
+
+@pytest.mark.skipif(not RefLib.available(8), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("bd", [8, 10])
+def test_affine3_oracle_live(bd):
+    o, r = Oracle(bd), RefLib(bd, 2)
+    rng = np.random.default_rng(bd + 5)
+    h, w = 96, 160
+    pic = rng.integers(0, 1 << bd, (h, w)).astype(o.dt)
+    planes, rp = o.make_planes(pic), r.pic_create(pic, 0)
+    r.pic_set_chroma(rp, pic[::2, ::2].copy(), pic[1::2, 1::2].copy())
+    for k in range(40):
+        cw, ch = (16, 32, 64, 128)[k % 4], (16, 32, 64, 128)[(k // 4) % 4]
+        x, y = int(rng.integers(0, w)) // 4 * 4, int(rng.integers(0, h)) // 4 * 4
+        s8, ncp = k % 2, 2 + (k // 2) % 2
+        cp = rng.integers(-2000, 2000, (ncp, 2))
+        sub = 8 if s8 else 4
+        assert np.array_equal(o.affine_luma(planes, w, h, x, y, cw, ch, cp, sub, sub),
+                              r.affine_cu_luma(rp, x, y, cw, ch, cp, s8)), (k, x, y, cw, ch, s8, cp.tolist())
+    r.pic_free(rp)
 
 
 @pytest.mark.skipif(not RefLib.available(8), reason="oracle/_ref not built (reference tree absent)")
@@ -78,16 +114,20 @@ def test_affine_gpu_vs_oracle(bd):
     h, w = pic.shape
     planes = o.make_planes(pic)
     rng = np.random.default_rng(bd)
-    rows = [list(c[:10]) for c in g["cases"]]
+    # (x, y, w, h, sub_w, sub_h, control points)
+    rows = [[int(v) for v in c[:6]] + [c[6:10].reshape(2, 2)] for c in g["cases"]]
+    rows += [[int(c[0]), int(c[1]), int(c[2]), int(c[3])] + [8 if c[4] else 4] * 2 + [c[5:11].reshape(3, 2)]
+             for c in g["cases3"]]
     for k in range(64):
         cw, ch = (16, 32, 64, 128)[k % 4], (16, 32, 64, 128)[(k // 4) % 4]
         rows.append([int(rng.integers(0, w)) // 4 * 4, int(rng.integers(0, h)) // 4 * 4, cw, ch, (4, 8)[k % 2],
-                     (4, 8)[(k // 2) % 2]] + rng.integers(-6000, 6000, 4).tolist())
+                     (4, 8)[(k // 2) % 2], rng.integers(-6000, 6000, (2 + (k // 8) % 2, 2))])
     cus = np.zeros(len(rows), ua.AFFINE_CU)
     off = 0
     for i, r in enumerate(rows):
         cus[i]["x"], cus[i]["y"], cus[i]["w"], cus[i]["h"], cus[i]["sub_w"], cus[i]["sub_h"] = r[:6]
-        cus[i]["cpmv"] = np.array(r[6:10]).reshape(2, 2)
+        cus[i]["vertex_num"] = len(r[6])
+        cus[i]["cpmv"][:len(r[6])] = r[6]
         cus[i]["pred_offset"] = off
         off += r[2] * r[3]
     pred = np.zeros(off + 7, hip.dt)
@@ -97,8 +137,8 @@ def test_affine_gpu_vs_oracle(bd):
     hip.ref_destroy(ref)
     for c in cus:
         o_ = int(c["pred_offset"])
-        want = o.affine_luma(planes, w, h, int(c["x"]), int(c["y"]), int(c["w"]), int(c["h"]), c["cpmv"],
-                             int(c["sub_w"]), int(c["sub_h"]))
+        want = o.affine_luma(planes, w, h, int(c["x"]), int(c["y"]), int(c["w"]), int(c["h"]),
+                             c["cpmv"][:int(c["vertex_num"])], int(c["sub_w"]), int(c["sub_h"]))
         assert np.array_equal(pred[o_:o_ + c["w"] * c["h"]], want.reshape(-1)), c
     assert (pred[-7:] == 77).all()
     assert hip.error_count() == 0

@@ -119,7 +119,8 @@ BI_PU = np.dtype([("refi", np.int8, 2), ("lidx_ref", np.int8), ("active", np.int
 
 # me_affine_cu_t
 AFFINE_CU = np.dtype([("x", np.int32), ("y", np.int32), ("w", np.int32), ("h", np.int32), ("sub_w", np.int32),
-                      ("sub_h", np.int32), ("cpmv", np.int32, (2, 2)), ("pred_offset", np.uint64)], align=True)
+                      ("sub_h", np.int32), ("vertex_num", np.int32), ("reserved", np.int32),
+                      ("cpmv", np.int32, (3, 2)), ("pred_offset", np.uint64)], align=True)
 
 # me_smvd_pu_t
 SMVD_PU = np.dtype([("active", np.int8), ("pad", np.int8, 3), ("mv", np.int16, (2, 2)), ("cost", np.uint32)],
@@ -348,7 +349,7 @@ class MeHip:
     def affine_mc_luma(self, ref, cus, pred):
         """me_affine_mc_luma: cus an AFFINE_CU array, pred a flat array of
         samples (updated in place at each CU's pred_offset)."""
-        assert cus.dtype == AFFINE_CU and AFFINE_CU.itemsize == 48 and pred.dtype == self.dt
+        assert cus.dtype == AFFINE_CU and AFFINE_CU.itemsize == 64 and pred.dtype == self.dt
         cus = np.ascontiguousarray(cus)
         assert pred.flags.c_contiguous
         self.check(self.lib.me_affine_mc_luma(ref, _p(cus), len(cus), _p(pred), pred.size, None),

@@ -243,13 +243,17 @@ __global__ __launch_bounds__(256) void k_affine_luma(const me_affine_cu_t *cus, 
     if (k >= c.w * c.h) return;
     const int sx = k % c.w, sy = k / c.w;
     const int w0 = sx - sx % c.sub_w, h0 = sy - sy % c.sub_h;
-    const int lw = __builtin_ctz((unsigned)c.w);
+    const int lw = __builtin_ctz((unsigned)c.w), lh = __builtin_ctz((unsigned)c.h);
     const int dhx = ((c.cpmv[1][0] - c.cpmv[0][0]) * 128) >> lw, dhy = ((c.cpmv[1][1] - c.cpmv[0][1]) * 128) >> lw;
+    const bool three = c.vertex_num == 3;
+    const int dvx = three ? ((c.cpmv[2][0] - c.cpmv[0][0]) * 128) >> lh : -dhy;
+    const int dvy = three ? ((c.cpmv[2][1] - c.cpmv[0][1]) * 128) >> lh : dhx;
     int px = w0 + c.sub_w / 2, py = h0 + c.sub_h / 2;
     if (w0 == 0 && h0 == 0) { px = 0; py = 0; }
     else if (w0 + c.sub_w == c.w && h0 == 0) { px = c.w; py = 0; }
-    int mx = aff_round7(c.cpmv[0][0] * 128 + dhx * px - dhy * py);
-    int my = aff_round7(c.cpmv[0][1] * 128 + dhy * px + dhx * py);
+    else if (w0 == 0 && h0 + c.sub_h == c.h && three) { px = 0; py = c.h; }
+    int mx = aff_round7(c.cpmv[0][0] * 128 + dhx * px + dvx * py);
+    int my = aff_round7(c.cpmv[0][1] * 128 + dhy * px + dvy * py);
     mx = me_clip3(-131072, 131071, mx);
     my = me_clip3(-131072, 131071, my);
     const int gx = ((c.x + w0) << 4) + mx, gy = ((c.y + h0) << 4) + my;
@@ -291,8 +295,9 @@ ME_EXPORT int me_affine_mc_luma(me_ref_t r, const me_affine_cu_t *cus, int n, me
         const me_affine_cu_t &c = cus[i];
         const bool pow2 = c.w >= 16 && c.w <= 128 && c.h >= 16 && c.h <= 128 && !(c.w & (c.w - 1)) && !(c.h & (c.h - 1));
         const bool sub = (c.sub_w == 4 || c.sub_w == 8) && (c.sub_h == 4 || c.sub_h == 8);
-        if (!pow2 || !sub || c.pred_offset + (uint64_t)c.w * c.h > pred_elems) {
-            me_set_error("me_affine_mc_luma: CU size, sub-block size or prediction offset out of range",
+        const bool cps = c.vertex_num == 2 || c.vertex_num == 3;
+        if (!pow2 || !sub || !cps || c.pred_offset + (uint64_t)c.w * c.h > pred_elems) {
+            me_set_error("me_affine_mc_luma: CU size, sub-block size, control points or prediction offset out of range",
                          hipErrorInvalidValue, __FILE__, __LINE__);
             return -1;
         }
