@@ -263,9 +263,21 @@ typedef struct me_affine_cu_t {
     int32_t reserved;
     int32_t cpmv[3][2];
     uint64_t pred_offset;
+    uint64_t chroma_offset;   /* me_affine_mc_chroma: where the (w/2) x (h/2) u and v blocks go */
 } me_affine_cu_t;
 int me_affine_mc_luma(me_ref_t ref, const me_affine_cu_t *cus, int n, me_pel *pred, uint64_t pred_elems,
                       void *stream);
+/* The chroma of com_mc_blk_affine (com_mc.c:335-360) for the same CUs
+ * (square 4x4 or 8x8 sub-blocks, as com_mc_cu_affine uses): each 8x8 luma
+ * block's 4x4 chroma takes the rounded average of its four 4x4 luma units'
+ * sub-block MVs, read as a 1/32-pel chroma position and interpolated with
+ * the 4-tap 1/32-pel filters (com_tables.c:156-189) as com_mc_blk_chroma
+ * does with hp_flag (position clipped to [-66, (pic + 4) / 2]; reads outside
+ * the chroma planes see the nearest edge sample, as in the reference's
+ * padded planes).  Needs me_ref_load_chroma.  pred_u / pred_v (host memory,
+ * chroma_elems samples each) receive each CU's blocks at chroma_offset. */
+int me_affine_mc_chroma(me_ref_t ref, const me_affine_cu_t *cus, int n, me_pel *pred_u, me_pel *pred_v,
+                        uint64_t chroma_elems, void *stream);
 
 /* Bi-prediction ME of analyze_bi (inter.c:972-1033), the search part, for
  * every PU of the plan that has references in both lists: starting from the

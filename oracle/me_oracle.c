@@ -999,3 +999,62 @@ EXPORT void orc_affine_luma(int x, int y, int pic_w, int pic_h, int cu_w, int cu
             else orc_ipflt_ext(0, src, i_ref, dst, cu_w, sub_w, sub_h, LUMA_TAPS_HP[fx], LUMA_TAPS_HP[fy], max_val);
         }
 }
+
+/* The 1/32-pel chroma filter of affine MC (com_tbl_mc_c_coeff_hp,
+ * com_tables.c:156-189; the standard's table). */
+static const s8 CHROMA_TAPS_HP[32][4] = {
+    {0, 64, 0, 0},    {-1, 63, 2, 0},   {-2, 62, 4, 0},   {-2, 60, 7, -1},  {-2, 58, 10, -2}, {-3, 57, 12, -2},
+    {-4, 56, 14, -2}, {-4, 55, 15, -2}, {-4, 54, 16, -2}, {-5, 53, 18, -2}, {-6, 52, 20, -2}, {-6, 49, 24, -3},
+    {-6, 46, 28, -4}, {-5, 44, 29, -4}, {-4, 42, 30, -4}, {-4, 39, 33, -4}, {-4, 36, 36, -4}, {-4, 33, 39, -4},
+    {-4, 30, 42, -4}, {-4, 29, 44, -5}, {-4, 28, 46, -6}, {-3, 24, 49, -6}, {-2, 20, 52, -6}, {-2, 18, 53, -5},
+    {-2, 16, 54, -4}, {-2, 15, 55, -4}, {-2, 14, 56, -4}, {-2, 12, 57, -3}, {-2, 10, 58, -2}, {-1, 7, 60, -2},
+    {0, 4, 62, -2},   {0, 2, 63, -1}};
+
+/* The chroma of com_mc_blk_affine (com_mc.c:290-360) for square sub-blocks
+ * (sub = 4 or 8): the luma sub-block MVs as above, kept per 4x4 luma unit
+ * (mv_save); each 8x8 luma block's 4x4 chroma uses (sum of its four units'
+ * MVs + 2) >> 2 as a 1/32-pel chroma position, clipped to [-66, (pic+4)/2],
+ * and com_mc_blk_chroma's copy / 4-tap / two-pass filters with the 1/32
+ * table.  u0 / v0: padded chroma planes at (0,0), stride i_c. */
+EXPORT void orc_affine_chroma(int x, int y, int pic_w, int pic_h, int cu_w, int cu_h, const int cpmv[3][2],
+                              int cp_num, int sub, const pel *u0, const pel *v0, int i_c, pel *pred_u, pel *pred_v,
+                              int bd)
+{
+    const int lw = log2i(cu_w), lh = log2i(cu_h);
+    const int dhx = ((cpmv[1][0] - cpmv[0][0]) * 128) >> lw, dhy = ((cpmv[1][1] - cpmv[0][1]) * 128) >> lw;
+    const int dvx = cp_num == 3 ? ((cpmv[2][0] - cpmv[0][0]) * 128) >> lh : -dhy;
+    const int dvy = cp_num == 3 ? ((cpmv[2][1] - cpmv[0][1]) * 128) >> lh : dhx;
+    static int save[32][32][2];
+    for (int h = 0; h < cu_h; h += sub)
+        for (int w = 0; w < cu_w; w += sub) {
+            int px = w + sub / 2, py = h + sub / 2;
+            if (w == 0 && h == 0) { px = 0; py = 0; }
+            else if (w + sub == cu_w && h == 0) { px = cu_w; py = 0; }
+            else if (w == 0 && h + sub == cu_h && cp_num == 3) { px = 0; py = cu_h; }
+            int mx = CLIP3(-131072, 131071, round7(cpmv[0][0] * 128 + dhx * px + dvx * py));
+            int my = CLIP3(-131072, 131071, round7(cpmv[0][1] * 128 + dhy * px + dvy * py));
+            for (int j = 0; j < sub / 4; j++)
+                for (int i = 0; i < sub / 4; i++) {
+                    save[w / 4 + i][h / 4 + j][0] = mx;
+                    save[w / 4 + i][h / 4 + j][1] = my;
+                }
+        }
+    const int max_val = (1 << bd) - 1;
+    for (int h = 0; h < cu_h; h += 8)
+        for (int w = 0; w < cu_w; w += 8) {
+            const int a = w / 4, b = h / 4;
+            const int mx = (save[a][b][0] + save[a + 1][b][0] + save[a][b + 1][0] + save[a + 1][b + 1][0] + 2) >> 2;
+            const int my = (save[a][b][1] + save[a + 1][b][1] + save[a][b + 1][1] + save[a + 1][b + 1][1] + 2) >> 2;
+            const int gx = ((x + w) << 4) + mx, gy = ((y + h) << 4) + my;
+            const int fx = gx & 31, fy = gy & 31;
+            const int ix = CLIP3(-66, (pic_w + 4) >> 1, gx >> 5), iy = CLIP3(-66, (pic_h + 4) >> 1, gy >> 5);
+            for (int pl = 0; pl < 2; pl++) {
+                const pel *src = (pl ? v0 : u0) + iy * i_c + ix;
+                pel *dst = (pl ? pred_v : pred_u) + (h / 2) * (cu_w / 2) + w / 2;
+                if (fx == 0 && fy == 0) orc_ipcpy(src, i_c, dst, cu_w / 2, 4, 4);
+                else if (fy == 0) orc_ipflt(1, src, i_c, dst, cu_w / 2, 4, 4, CHROMA_TAPS_HP[fx], max_val);
+                else if (fx == 0) orc_ipflt(3, src, i_c, dst, cu_w / 2, 4, 4, CHROMA_TAPS_HP[fy], max_val);
+                else orc_ipflt_ext(1, src, i_c, dst, cu_w / 2, 4, 4, CHROMA_TAPS_HP[fx], CHROMA_TAPS_HP[fy], max_val);
+            }
+        }
+}

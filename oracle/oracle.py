@@ -190,6 +190,21 @@ class Oracle(_Lib):
                                  ptr(out), sub_w, sub_h, self.bd)
         return out
 
+    def affine_chroma(self, chroma_uv, pic_w, pic_h, x, y, cu_w, cu_h, cpmv, sub, cpad=PAD // 2):
+        """The chroma of com_mc_blk_affine (com_mc.c:290-360) from padded
+        chroma planes (make_chroma); square sub-blocks sub = 4 or 8.
+        Returns (u, v) blocks of (cu_h/2, cu_w/2)."""
+        cs = pic_w // 2 + 2 * cpad
+        cpa = np.asarray(cpmv).reshape(-1, 2)
+        cp = (C.c_int * 6)(*[int(v) for v in np.concatenate([cpa.reshape(-1), np.zeros(6 - cpa.size, int)])])
+        u = np.zeros((cu_h // 2, cu_w // 2), self.dt)
+        v = np.zeros((cu_h // 2, cu_w // 2), self.dt)
+        off = (cpad * cs + cpad) * self.dt().itemsize
+        self.lib.orc_affine_chroma(x, y, pic_w, pic_h, cu_w, cu_h, cp, len(cpa), sub,
+                                   C.c_void_p(chroma_uv[0].ctypes.data + off), C.c_void_p(chroma_uv[1].ctypes.data + off),
+                                   cs, ptr(u), ptr(v), self.bd)
+        return u, v
+
     def luma_taps_hp(self, frac):
         return np.ctypeslib.as_array(C.cast(self.lib.orc_luma_taps_hp(frac), C.POINTER(C.c_int8)), (8,)).copy()
 
@@ -580,8 +595,20 @@ class RefLib(_Lib):
         cp = np.zeros(8, np.int32)
         cp[:cpa.size] = cpa.reshape(-1)
         out = np.zeros((cu_h, cu_w), self.dt)
-        self.lib.refh_affine_cu_luma(C.c_void_p(pic), x, y, cu_w, cu_h, ptr(cp), len(cpa), int(sub8), ptr(out))
+        self.lib.refh_affine_cu(C.c_void_p(pic), x, y, cu_w, cu_h, ptr(cp), len(cpa), int(sub8), ptr(out), None, None)
         return out
+
+    def affine_cu(self, pic, x, y, cu_w, cu_h, cpmv, sub8):
+        """com_mc_cu_affine of the compiled reference: (luma, u, v)."""
+        cpa = np.asarray(cpmv, np.int32).reshape(-1, 2)
+        cp = np.zeros(8, np.int32)
+        cp[:cpa.size] = cpa.reshape(-1)
+        out = np.zeros((cu_h, cu_w), self.dt)
+        u = np.zeros((cu_h // 2, cu_w // 2), self.dt)
+        v = np.zeros((cu_h // 2, cu_w // 2), self.dt)
+        self.lib.refh_affine_cu(C.c_void_p(pic), x, y, cu_w, cu_h, ptr(cp), len(cpa), int(sub8), ptr(out), ptr(u),
+                                ptr(v))
+        return out, u, v
 
     def affine_luma(self, pic, x, y, cu_w, cu_h, cpmv, sub_w, sub_h):
         """com_mc_blk_affine_luma of the compiled reference."""

@@ -528,11 +528,12 @@ RH_EXPORT void refh_affine_luma(void *ref, int x, int y, int cu_w, int cu_h, con
     memcpy(pred, buf, sizeof(pel) * (size_t)cu_w * cu_h);
 }
 
-/* Luma of com_mc_cu_affine (com_mc.c:365-385) for list 0 alone, with two or
- * three control points (1/16 pel) and 4x4 or 8x8 sub-blocks
- * (affine_subblk_size_idx); the chroma it also predicts is not returned. */
-RH_EXPORT void refh_affine_cu_luma(void *ref, int x, int y, int cu_w, int cu_h, const int *cpmv, int vertex_num,
-                                   int sub8, pel *pred)
+/* com_mc_cu_affine (com_mc.c:365-385) for list 0 alone, with two or three
+ * control points (1/16 pel) and 4x4 or 8x8 sub-blocks
+ * (affine_subblk_size_idx): luma into pred, chroma into pred_u / pred_v
+ * (either may be NULL). */
+RH_EXPORT void refh_affine_cu(void *ref, int x, int y, int cu_w, int cu_h, const int *cpmv, int vertex_num,
+                              int sub8, pel *pred, pel *pred_u, pel *pred_v)
 {
     refh_pic_t *r = (refh_pic_t *)ref;
     static com_ref_pic_t refp[MAX_REFS][REFP_NUM];
@@ -551,4 +552,6 @@ RH_EXPORT void refh_affine_cu_luma(void *ref, int x, int y, int cu_w, int cu_h, 
     static pel buf[N_C][MAX_CU_DIM];
     com_mc_cu_affine(x, y, r->w, r->h, cu_w, cu_h, refi, mv, refp, buf, vertex_num, &sh, BIT_DEPTH);
     memcpy(pred, buf[Y_C], sizeof(pel) * (size_t)cu_w * cu_h);
+    if (pred_u) memcpy(pred_u, buf[U_C], sizeof(pel) * (size_t)(cu_w / 2) * (cu_h / 2));
+    if (pred_v) memcpy(pred_v, buf[V_C], sizeof(pel) * (size_t)(cu_w / 2) * (cu_h / 2));
 }

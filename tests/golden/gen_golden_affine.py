@@ -12,9 +12,10 @@ compiled it into oracle/_ref:
 For each bit depth it writes tests/golden/affine_bd{8,10}.npz: the reference
 picture (160x96), the cases (x, y, cu_w, cu_h, sub_w, sub_h, the two CPMVs,
 offset of the prediction) and the reference's predictions concatenated; and
-cases3 / preds3, the luma of com_mc_cu_affine (com_mc.c:365-385) with three
-control points (x, y, cu_w, cu_h, 8x8 sub-blocks or not, the three CPMVs,
-offset)."""
+cases3 / preds3 / preds3_u / preds3_v, com_mc_cu_affine (com_mc.c:365-385)
+with two or three control points, luma and chroma (x, y, cu_w, cu_h, 8x8
+sub-blocks or not, number of control points, three CPMVs, luma offset,
+chroma offset); the chroma planes are two subsampled phases of the picture."""
 import os
 import sys
 
@@ -45,16 +46,24 @@ def make_cases(seed):
 
 
 def make_cases3(seed):
-    """Three-control-point cases (com_mc_cu_affine luma): x, y, cu_w, cu_h,
-    8x8 sub-blocks or not, the three CPMVs, offset."""
+    """com_mc_cu_affine cases (luma and chroma): x, y, cu_w, cu_h, 8x8
+    sub-blocks or not, control points (2 or 3), the three CPMVs (the third
+    unused with 2), luma offset, chroma offset."""
     rng = np.random.default_rng(seed)
-    rows, off = [], 0
+    rows, off, coff = [], 0, 0
     for k in range(NCASES // 2):
         cw, ch = (16, 16, 32, 32, 64, 128)[rng.integers(0, 6)], (16, 16, 32, 64, 128)[rng.integers(0, 5)]
         x, y = int(rng.integers(0, W)) // 4 * 4, int(rng.integers(0, H)) // 4 * 4
-        rows.append([x, y, cw, ch, int(rng.integers(0, 2))] + rng.integers(-700, 700, 6).tolist() + [off])
+        rows.append([x, y, cw, ch, int(rng.integers(0, 2)), 2 + k % 2] + rng.integers(-700, 700, 6).tolist() +
+                    [off, coff])
         off += cw * ch
+        coff += cw * ch // 4
     return np.array(rows, np.int32)
+
+
+def chroma_planes(pic):
+    """The 4:2:0 chroma planes of the golden picture: two subsampled phases."""
+    return pic[::2, ::2].copy(), pic[1::2, 1::2].copy()
 
 
 def main():
@@ -70,13 +79,17 @@ def main():
         rp = ref.pic_create(pic, 0)
         preds = [ref.affine_luma(rp, *[int(v) for v in c[:4]], c[6:10].reshape(2, 2), int(c[4]), int(c[5])).reshape(-1)
                  for c in cases]
-        ref.pic_set_chroma(rp, pic[::2, ::2].copy(), pic[1::2, 1::2].copy())   # com_mc_cu_affine also predicts chroma
+        u, v = chroma_planes(pic)
+        ref.pic_set_chroma(rp, u, v)
         cases3 = make_cases3(200 + bd)
-        preds3 = [ref.affine_cu_luma(rp, *[int(v) for v in c[:4]], c[5:11].reshape(3, 2), int(c[4])).reshape(-1)
-                  for c in cases3]
+        out3 = [ref.affine_cu(rp, *[int(q) for q in c[:4]], c[6:12].reshape(3, 2)[:int(c[5])], int(c[4]))
+                for c in cases3]
         ref.pic_free(rp)
         np.savez_compressed(os.path.join(OUT, "affine_bd%d.npz" % bd), pic=pic, cases=cases,
-                            preds=np.concatenate(preds), cases3=cases3, preds3=np.concatenate(preds3))
+                            preds=np.concatenate(preds), cases3=cases3,
+                            preds3=np.concatenate([o[0].reshape(-1) for o in out3]),
+                            preds3_u=np.concatenate([o[1].reshape(-1) for o in out3]),
+                            preds3_v=np.concatenate([o[2].reshape(-1) for o in out3]))
         print("bd %d: %d cases, %d samples" % (bd, len(cases), sum(p.size for p in preds)))
 
 

@@ -120,7 +120,8 @@ BI_PU = np.dtype([("refi", np.int8, 2), ("lidx_ref", np.int8), ("active", np.int
 # me_affine_cu_t
 AFFINE_CU = np.dtype([("x", np.int32), ("y", np.int32), ("w", np.int32), ("h", np.int32), ("sub_w", np.int32),
                       ("sub_h", np.int32), ("vertex_num", np.int32), ("reserved", np.int32),
-                      ("cpmv", np.int32, (3, 2)), ("pred_offset", np.uint64)], align=True)
+                      ("cpmv", np.int32, (3, 2)), ("pred_offset", np.uint64), ("chroma_offset", np.uint64)],
+                     align=True)
 
 # me_smvd_pu_t
 SMVD_PU = np.dtype([("active", np.int8), ("pad", np.int8, 3), ("mv", np.int16, (2, 2)), ("cost", np.uint32)],
@@ -180,6 +181,8 @@ class MeHip:
         L.me_plan_run_smvd.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_void_p]
         L.me_affine_mc_luma.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.me_affine_mc_chroma.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64,
+                                          C.c_void_p]
         L.me_plan_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p]
         L.me_plan_destroy.argtypes = [C.c_void_p]
@@ -349,11 +352,20 @@ class MeHip:
     def affine_mc_luma(self, ref, cus, pred):
         """me_affine_mc_luma: cus an AFFINE_CU array, pred a flat array of
         samples (updated in place at each CU's pred_offset)."""
-        assert cus.dtype == AFFINE_CU and AFFINE_CU.itemsize == 64 and pred.dtype == self.dt
+        assert cus.dtype == AFFINE_CU and AFFINE_CU.itemsize == 72 and pred.dtype == self.dt
         cus = np.ascontiguousarray(cus)
         assert pred.flags.c_contiguous
         self.check(self.lib.me_affine_mc_luma(ref, _p(cus), len(cus), _p(pred), pred.size, None),
                    "me_affine_mc_luma")
+
+    def affine_mc_chroma(self, ref, cus, pred_u, pred_v):
+        """me_affine_mc_chroma: pred_u / pred_v flat sample arrays updated in
+        place at each CU's chroma_offset."""
+        assert cus.dtype == AFFINE_CU and pred_u.dtype == self.dt and pred_v.dtype == self.dt
+        assert pred_u.size == pred_v.size and pred_u.flags.c_contiguous and pred_v.flags.c_contiguous
+        cus = np.ascontiguousarray(cus)
+        self.check(self.lib.me_affine_mc_chroma(ref, _p(cus), len(cus), _p(pred_u), _p(pred_v), pred_u.size, None),
+                   "me_affine_mc_chroma")
 
     def plan_run_smvd(self, plan, refs, mvp, mv_scale, pus, tz, stream=None):
         """me_plan_run_smvd: mvp, mv_scale int16 [ctu][pu][ref][2]; pus an

@@ -231,6 +231,85 @@ __constant__ static const s8 AFF_TAPS[16][8] = {
 
 __device__ __forceinline__ int aff_round7(int v) { return v >= 0 ? (v + 64) >> 7 : -((-v + 64) >> 7); }
 
+// The 1/32-pel chroma interpolation filter (com_tbl_mc_c_coeff_hp,
+// com_tables.c:156-189): the standard's table.
+__constant__ static const s8 AFF_CTAPS[32][4] = {
+    {0, 64, 0, 0},    {-1, 63, 2, 0},   {-2, 62, 4, 0},   {-2, 60, 7, -1},  {-2, 58, 10, -2}, {-3, 57, 12, -2},
+    {-4, 56, 14, -2}, {-4, 55, 15, -2}, {-4, 54, 16, -2}, {-5, 53, 18, -2}, {-6, 52, 20, -2}, {-6, 49, 24, -3},
+    {-6, 46, 28, -4}, {-5, 44, 29, -4}, {-4, 42, 30, -4}, {-4, 39, 33, -4}, {-4, 36, 36, -4}, {-4, 33, 39, -4},
+    {-4, 30, 42, -4}, {-4, 29, 44, -5}, {-4, 28, 46, -6}, {-3, 24, 49, -6}, {-2, 20, 52, -6}, {-2, 18, 53, -5},
+    {-2, 16, 54, -4}, {-2, 15, 55, -4}, {-2, 14, 56, -4}, {-2, 12, 57, -3}, {-2, 10, 58, -2}, {-1, 7, 60, -2},
+    {0, 4, 62, -2},   {0, 2, 63, -1}};
+
+// The sub-block MV (1/16 pel, 18 bits) of the sub-block at (w0, h0) of CU c
+// (com_mc.c:224-241, 290-310).
+__device__ __forceinline__ void aff_sub_mv(const me_affine_cu_t &c, int w0, int h0, int &mx, int &my)
+{
+    const int lw = __builtin_ctz((unsigned)c.w), lh = __builtin_ctz((unsigned)c.h);
+    const int dhx = ((c.cpmv[1][0] - c.cpmv[0][0]) * 128) >> lw, dhy = ((c.cpmv[1][1] - c.cpmv[0][1]) * 128) >> lw;
+    const bool three = c.vertex_num == 3;
+    const int dvx = three ? ((c.cpmv[2][0] - c.cpmv[0][0]) * 128) >> lh : -dhy;
+    const int dvy = three ? ((c.cpmv[2][1] - c.cpmv[0][1]) * 128) >> lh : dhx;
+    int px = w0 + c.sub_w / 2, py = h0 + c.sub_h / 2;
+    if (w0 == 0 && h0 == 0) { px = 0; py = 0; }
+    else if (w0 + c.sub_w == c.w && h0 == 0) { px = c.w; py = 0; }
+    else if (w0 == 0 && h0 + c.sub_h == c.h && three) { px = 0; py = c.h; }
+    mx = me_clip3(-131072, 131071, aff_round7(c.cpmv[0][0] * 128 + dhx * px + dvx * py));
+    my = me_clip3(-131072, 131071, aff_round7(c.cpmv[0][1] * 128 + dhy * px + dvy * py));
+}
+
+// One thread per chroma sample of u and v; grid (16, n): block b of CU c
+// covers its chroma samples [256 b, 256 b + 256).
+__global__ __launch_bounds__(256) void k_affine_chroma(const me_affine_cu_t *cus, const pel *chroma, int pic_w,
+                                                       int pic_h, pel *pred_u, pel *pred_v)
+{
+    const me_affine_cu_t c = cus[blockIdx.y];
+    const int cwid = c.w >> 1, k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= cwid * (c.h >> 1)) return;
+    const int cx = k % cwid, cy = k / cwid;
+    const int w8 = (cx >> 2) << 3, h8 = (cy >> 2) << 3;   // the 8x8 luma block
+    // its four 4x4 luma units' MVs (mv_save, com_mc.c:312-325), averaged (com_mc.c:346-347)
+    int sx = 0, sy = 0;
+    for (int u = 0; u < 4; u++) {
+        const int lx = w8 + (u & 1) * 4, ly = h8 + (u >> 1) * 4;
+        int mx, my;
+        aff_sub_mv(c, lx - lx % c.sub_w, ly - ly % c.sub_h, mx, my);
+        sx += mx;
+        sy += my;
+    }
+    const int gx = ((c.x + w8) << 4) + ((sx + 2) >> 2), gy = ((c.y + h8) << 4) + ((sy + 2) >> 2);
+    const int fx = gx & 31, fy = gy & 31;
+    const int ix = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_w + 4) >> 1, gx >> 5) + (cx & 3);
+    const int iy = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_h + 4) >> 1, gy >> 5) + (cy & 3);
+    const int cw = pic_w >> 1, chh = pic_h >> 1, maxv = (1 << BIT_DEPTH) - 1;
+    for (int pl = 0; pl < 2; pl++) {
+        const pel *P = chroma + (size_t)pl * cw * chh;
+        auto S = [&](int xx, int yy) -> int {
+            return (int)P[(size_t)me_clip3(0, chh - 1, yy) * cw + me_clip3(0, cw - 1, xx)];
+        };
+        int v;
+        if (fx == 0 && fy == 0) {
+            v = S(ix, iy);
+        } else if (fy == 0 || fx == 0) {
+            int t = 0;
+            for (int j = 0; j < 4; j++)
+                t += (fy == 0 ? AFF_CTAPS[fx][j] * S(ix - 1 + j, iy) : AFF_CTAPS[fy][j] * S(ix, iy - 1 + j));
+            v = me_clip3(0, maxv, (t + 32) >> 6);
+        } else {
+            const int shift1 = BIT_DEPTH == 8 ? 0 : 2, shift2 = BIT_DEPTH == 8 ? 12 : 10;
+            const int add1 = (1 << shift1) >> 1, add2 = 1 << (shift2 - 1);
+            int t2 = 0;
+            for (int j = 0; j < 4; j++) {
+                int t = 0;
+                for (int i = 0; i < 4; i++) t += AFF_CTAPS[fx][i] * S(ix - 1 + i, iy - 1 + j);
+                t2 += AFF_CTAPS[fy][j] * (int)(s16)((t + add1) >> shift1);
+            }
+            v = me_clip3(0, maxv, (t2 + add2) >> shift2);
+        }
+        (pl ? pred_v : pred_u)[c.chroma_offset + k] = (pel)v;
+    }
+}
+
 // One output sample per thread; grid (64, n): block b of CU c covers its
 // samples [256 b, 256 b + 256).  The sub-block MV (com_mc.c:224-241) is
 // recomputed per sample; the 1/16 phase picks a copy, a one-direction 8-tap
@@ -324,6 +403,57 @@ ME_EXPORT int me_affine_mc_luma(me_ref_t r, const me_affine_cu_t *cus, int n, me
                        r->planes + (size_t)ME_PAD * r->stride + ME_PAD, r->stride, r->w, r->h, r->aff_pred);
     ME_CHECK(hipGetLastError());
     ME_CHECK(hipMemcpyAsync(pred, r->aff_pred, pred_elems * sizeof(pel), hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+ME_EXPORT int me_affine_mc_chroma(me_ref_t r, const me_affine_cu_t *cus, int n, me_pel *pred_u, me_pel *pred_v,
+                                  uint64_t chroma_elems, void *stream)
+{
+    if (!r || !cus || !pred_u || !pred_v || n < 0) {
+        me_set_error("me_affine_mc_chroma: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (!r->chroma) {
+        me_set_error("me_affine_mc_chroma: the reference has no chroma (me_ref_load_chroma)", hipErrorInvalidValue,
+                     __FILE__, __LINE__);
+        return -1;
+    }
+    for (int i = 0; i < n; i++) {
+        const me_affine_cu_t &c = cus[i];
+        const bool pow2 = c.w >= 16 && c.w <= 128 && c.h >= 16 && c.h <= 128 && !(c.w & (c.w - 1)) && !(c.h & (c.h - 1));
+        const bool sub = (c.sub_w == 4 || c.sub_w == 8) && c.sub_h == c.sub_w;
+        const bool cps = c.vertex_num == 2 || c.vertex_num == 3;
+        if (!pow2 || !sub || !cps || c.chroma_offset + (uint64_t)(c.w / 2) * (c.h / 2) > chroma_elems) {
+            me_set_error("me_affine_mc_chroma: CU size, sub-block size (4x4 or 8x8), control points or offset out of "
+                         "range", hipErrorInvalidValue, __FILE__, __LINE__);
+            return -1;
+        }
+    }
+    if (n == 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (r->aff_cus_cap < n) {
+        (void)hipFree(r->aff_cus);
+        r->aff_cus = nullptr;
+        r->aff_cus_cap = 0;
+        ME_CHECK(hipMalloc((void **)&r->aff_cus, (size_t)n * sizeof(me_affine_cu_t)));
+        r->aff_cus_cap = n;
+    }
+    if (r->aff_pred_cap < 2 * chroma_elems) {
+        (void)hipFree(r->aff_pred);
+        r->aff_pred = nullptr;
+        r->aff_pred_cap = 0;
+        ME_CHECK(hipMalloc((void **)&r->aff_pred, 2 * chroma_elems * sizeof(pel)));
+        r->aff_pred_cap = 2 * chroma_elems;
+    }
+    pel *du = r->aff_pred, *dv = r->aff_pred + chroma_elems;
+    ME_CHECK(hipMemcpyAsync(r->aff_cus, cus, (size_t)n * sizeof(me_affine_cu_t), hipMemcpyHostToDevice, st));
+    ME_CHECK(hipMemcpyAsync(du, pred_u, chroma_elems * sizeof(pel), hipMemcpyHostToDevice, st));
+    ME_CHECK(hipMemcpyAsync(dv, pred_v, chroma_elems * sizeof(pel), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_affine_chroma, dim3(16, n), dim3(256), 0, st, r->aff_cus, r->chroma, r->w, r->h, du, dv);
+    ME_CHECK(hipGetLastError());
+    ME_CHECK(hipMemcpyAsync(pred_u, du, chroma_elems * sizeof(pel), hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipMemcpyAsync(pred_v, dv, chroma_elems * sizeof(pel), hipMemcpyDeviceToHost, st));
     ME_CHECK(hipStreamSynchronize(st));
     return 0;
 }
