@@ -172,3 +172,35 @@ def test_affine_gpu_vs_oracle(bd):
         assert np.array_equal(pu[co:co + ca], wu.reshape(-1)) and np.array_equal(pv[co:co + ca], wv.reshape(-1)), c
     assert len(sq) > 40
     assert hip.error_count() == 0
+
+
+@pytest.mark.gpu
+def test_affine_rejects_bad_arguments():
+    """Unsupported CU / sub-block sizes, control-point counts, offsets past
+    the buffer and chroma without chroma planes are errors; the reference
+    stays usable."""
+    hip = ua.MeHip(8)
+    pic = np.random.default_rng(1).integers(0, 256, (64, 96)).astype(np.uint8)
+    ref = hip.ref_create(pic)
+    good = np.zeros(1, ua.AFFINE_CU)
+    good[0]["w"] = good[0]["h"] = 16
+    good[0]["sub_w"] = good[0]["sub_h"] = 4
+    good[0]["vertex_num"] = 2
+    pred = np.zeros(256, np.uint8)
+    for field, value in (("w", 8), ("h", 24), ("sub_w", 2), ("vertex_num", 4), ("pred_offset", 1)):
+        bad = good.copy()
+        bad[0][field] = value
+        with pytest.raises(ua.MeError):
+            hip.affine_mc_luma(ref, bad, pred)
+    pu, pv = np.zeros(64, np.uint8), np.zeros(64, np.uint8)
+    with pytest.raises(ua.MeError):
+        hip.affine_mc_chroma(ref, good, pu, pv)          # no chroma loaded
+    hip.ref_load_chroma(ref, pic[::2, ::2].copy(), pic[1::2, 1::2].copy())
+    bad = good.copy()
+    bad[0]["sub_w"] = 8                                   # chroma needs square sub-blocks
+    with pytest.raises(ua.MeError):
+        hip.affine_mc_chroma(ref, bad, pu, pv)
+    hip.affine_mc_luma(ref, good, pred)
+    hip.affine_mc_chroma(ref, good, pu, pv)
+    assert np.array_equal(pred.reshape(16, 16), pic[:16, :16])
+    hip.ref_destroy(ref)

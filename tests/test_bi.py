@@ -184,3 +184,37 @@ def test_bi_gpu_vs_oracle(bd, n0, n1, mvr, ct):
                 pu += 1
     assert checked > 300
     assert hip.error_count() == 0
+
+
+@pytest.mark.gpu
+def test_bi_and_smvd_reject_bad_plans():
+    """Both searches need references in both lists, and PU reference
+    indices inside their lists."""
+    hip = ua.MeHip(8)
+    w, h = 128, 64
+    cur = np.zeros((h, w), np.uint8)
+    refs = [hip.ref_create(cur) for _ in range(2)]
+    tz = ua.TzCfg(16, 64, 0, 0, 0, 8, (ctypes.c_int64 * 4)(0, 0, 0, 0))
+    plan = hip.plan_create(w, h, 8, LAMBDA, 1 << SHAPES.index((16, 16)), 2)   # one list only
+    hip.plan_set_cur(plan, cur)
+    n_ctu, npu = hip.lib.me_plan_num_ctus(plan), hip.lib.me_plan_num_pus(plan)
+    mvp = np.zeros((n_ctu, npu, 2, 2), np.int16)
+    ms = mvp.copy()
+    with pytest.raises(ua.MeError):
+        hip.plan_run_bi(plan, refs, mvp, ms, np.zeros((n_ctu, npu), ua.BI_PU), tz)
+    with pytest.raises(ua.MeError):
+        hip.plan_run_smvd(plan, refs, mvp, ms, np.zeros((n_ctu, npu), ua.SMVD_PU), tz)
+    hip.plan_destroy(plan)
+    plan = hip.plan_create(w, h, 8, LAMBDA, 1 << SHAPES.index((16, 16)), 2, num_refs_l1=1)
+    hip.plan_set_cur(plan, cur)
+    pus = np.zeros((n_ctu, npu), ua.BI_PU)
+    pus["active"] = 1
+    pus["refi"][0, 0, 1] = 1                               # list 1 has one reference
+    with pytest.raises(ua.MeError):
+        hip.plan_run_bi(plan, refs, mvp, ms, pus, tz)
+    pus["refi"][0, 0, 1] = 0
+    hip.plan_run_bi(plan, refs, mvp, ms, pus, tz)          # identical pictures: both MVs stay 0
+    assert (pus["mv"] == 0).all()
+    hip.plan_destroy(plan)
+    for r in refs:
+        hip.ref_destroy(r)
