@@ -19,6 +19,7 @@
 #include "frame.h"
 
 #include <string.h>
+#include <algorithm>
 
 struct McArgs {
     JobSet js;
@@ -258,8 +259,8 @@ __device__ __forceinline__ void aff_sub_mv(const me_affine_cu_t &c, int w0, int 
     my = me_clip3(-131072, 131071, aff_round7(c.cpmv[0][1] * 128 + dhy * px + dvy * py));
 }
 
-// One thread per chroma sample of u and v; grid (16, n): block b of CU c
-// covers its chroma samples [256 b, 256 b + 256).
+// One thread per chroma sample of u and v; grid (blocks for the batch's
+// largest CU, n): block b of CU c covers its chroma samples [256 b, 256 b + 256).
 __global__ __launch_bounds__(256) void k_affine_chroma(const me_affine_cu_t *cus, const pel *chroma, int pic_w,
                                                        int pic_h, pel *pred_u, pel *pred_v)
 {
@@ -310,8 +311,8 @@ __global__ __launch_bounds__(256) void k_affine_chroma(const me_affine_cu_t *cus
     }
 }
 
-// One output sample per thread; grid (64, n): block b of CU c covers its
-// samples [256 b, 256 b + 256).  The sub-block MV (com_mc.c:224-241) is
+// One output sample per thread; grid (blocks for the batch's largest CU, n):
+// block b of CU c covers its samples [256 b, 256 b + 256).  The sub-block MV (com_mc.c:224-241) is
 // recomputed per sample; the 1/16 phase picks a copy, a one-direction 8-tap
 // filter, or the two-pass filter (as ipflt / ipflt_ext, com_mc.c:461-657).
 __global__ __launch_bounds__(256) void k_affine_luma(const me_affine_cu_t *cus, const pel *plane0, int stride,
@@ -399,7 +400,9 @@ ME_EXPORT int me_affine_mc_luma(me_ref_t r, const me_affine_cu_t *cus, int n, me
     }
     ME_CHECK(hipMemcpyAsync(r->aff_cus, cus, (size_t)n * sizeof(me_affine_cu_t), hipMemcpyHostToDevice, st));
     ME_CHECK(hipMemcpyAsync(r->aff_pred, pred, pred_elems * sizeof(pel), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_affine_luma, dim3(64, n), dim3(256), 0, st, r->aff_cus,
+    int max_area = 0;
+    for (int i = 0; i < n; i++) max_area = std::max(max_area, cus[i].w * cus[i].h);
+    hipLaunchKernelGGL(k_affine_luma, dim3((max_area + 255) / 256, n), dim3(256), 0, st, r->aff_cus,
                        r->planes + (size_t)ME_PAD * r->stride + ME_PAD, r->stride, r->w, r->h, r->aff_pred);
     ME_CHECK(hipGetLastError());
     ME_CHECK(hipMemcpyAsync(pred, r->aff_pred, pred_elems * sizeof(pel), hipMemcpyDeviceToHost, st));
@@ -450,7 +453,10 @@ ME_EXPORT int me_affine_mc_chroma(me_ref_t r, const me_affine_cu_t *cus, int n, 
     ME_CHECK(hipMemcpyAsync(r->aff_cus, cus, (size_t)n * sizeof(me_affine_cu_t), hipMemcpyHostToDevice, st));
     ME_CHECK(hipMemcpyAsync(du, pred_u, chroma_elems * sizeof(pel), hipMemcpyHostToDevice, st));
     ME_CHECK(hipMemcpyAsync(dv, pred_v, chroma_elems * sizeof(pel), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_affine_chroma, dim3(16, n), dim3(256), 0, st, r->aff_cus, r->chroma, r->w, r->h, du, dv);
+    int max_area = 0;
+    for (int i = 0; i < n; i++) max_area = std::max(max_area, (cus[i].w / 2) * (cus[i].h / 2));
+    hipLaunchKernelGGL(k_affine_chroma, dim3((max_area + 255) / 256, n), dim3(256), 0, st, r->aff_cus, r->chroma, r->w,
+                       r->h, du, dv);
     ME_CHECK(hipGetLastError());
     ME_CHECK(hipMemcpyAsync(pred_u, du, chroma_elems * sizeof(pel), hipMemcpyDeviceToHost, st));
     ME_CHECK(hipMemcpyAsync(pred_v, dv, chroma_elems * sizeof(pel), hipMemcpyDeviceToHost, st));
