@@ -255,7 +255,10 @@ int me_plan_run_tz(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, con
  * the 1/16-pel 8-tap filters (com_tables.c:130-147) as com_mc_blk_luma does
  * with hp_flag (position clipped to [-132, pic + 4]).  CU sizes 16..128
  * (powers of two).  pred (host memory, pred_elems samples) receives each
- * CU's w x h prediction row by row at its pred_offset.  Blocking. */
+ * CU's w x h prediction row by row at its pred_offset; samples outside
+ * every CU's block are left as they were.  Blocking.  The reference holds
+ * the device staging buffers of both affine calls: calls on one reference
+ * must not overlap. */
 typedef struct me_affine_cu_t {
     int32_t x, y, w, h;
     int32_t sub_w, sub_h;
