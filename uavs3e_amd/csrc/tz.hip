@@ -726,10 +726,13 @@ __global__ void k_bi_init(const me_bi_pu_t *pu, BiState *st, int n)
 // create_bi_org (inter.c:55-68) of one shape's PUs from the fixed list's
 // prediction (com_mc_blk_luma at its quarter-pel MV: the plane [qy&3][qx&3]
 // at the clipped integer position, com_mc.c:41-86, 140-165).  Block (256):
-// one PU per block, 4 samples per thread and step.  grid (per_ctu, jobs).
+// min(256, area) threads per PU, 256 / that PUs per block, one sample per
+// thread and step.  grid (ceil(per_ctu / PUs per block), jobs).
 __global__ __launch_bounds__(256) void k_bi_org(TzArgs a)
 {
-    const int i = blockIdx.x, job = blockIdx.y;
+    const int area = a.pw * a.ph, tpp = area < 256 ? area : 256;
+    const int i = blockIdx.x * (256 / tpp) + threadIdx.x / tpp, job = blockIdx.y;
+    if (i >= a.per_ctu) return;
     const size_t pk = (size_t)job * a.npu + a.base + i;
     if (a.bst[pk].done) return;
     int ctu;
@@ -743,7 +746,7 @@ __global__ __launch_bounds__(256) void k_bi_org(TzArgs a)
     const pel *org = P.cur + (size_t)y * P.pic_w + x;
     pel *dst = (pel *)a.orgbi + ((size_t)job * a.nsel * 16384 + a.orgbi_off) + (size_t)i * a.pw * a.ph;
     const int maxv = (1 << BIT_DEPTH) - 1;
-    for (int k = threadIdx.x; k < a.pw * a.ph; k += blockDim.x) {
+    for (int k = threadIdx.x % tpp; k < area; k += tpp) {
         const int yy = k / a.pw, xx = k - yy * a.pw;
         const int v = 2 * (int)org[yy * P.pic_w + xx] - (int)pred[yy * P.ref_stride + xx];
         dst[k] = (pel)me_clip3(0, maxv, v);
@@ -826,7 +829,10 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
         const int upb = 4 * (64 / a.lpu);
         tz_fn fn = a.tt == 3 ? me_tz_kernel_16x8(a.tt) : (a.tt == 5 ? me_tz_kernel_8x16(a.tt) : me_tz_kernel_small(a.tt));
         if (!fn) { me_set_error("no search kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
-        if (bi) hipLaunchKernelGGL(k_bi_org, dim3(a.per_ctu, js.n_jobs), dim3(256), 0, st, a);
+        if (bi) {
+            const int per_block = 256 / std::min(256, area);
+            hipLaunchKernelGGL(k_bi_org, dim3((a.per_ctu + per_block - 1) / per_block, js.n_jobs), dim3(256), 0, st, a);
+        }
         const int nz = bi ? std::max(a.n0, a.nref - a.n0) : s->nref;
         hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs, nz), dim3(256), 0, st, a);
         ME_CHECK(hipGetLastError());
