@@ -2,6 +2,8 @@
 // 15 quarter-pel planes of com_if_luma_frame, plus the PU shape API.
 #include "frame.h"
 
+#include <algorithm>
+
 #include <string.h>
 
 // ---------------------------------------------------------------------------
@@ -118,13 +120,16 @@ __device__ __forceinline__ int plane_sample(const pel *p0, int stride, int w, in
 //   else:    clip((V8(tmp) + 2^(19-bd)) >> (20-bd)) com_mc.c:731-758
 // A thread writes 4 consecutive samples of a row; where none of them is
 // clamped, the 8-tap inputs are read as whole rows of samples.
+// Launched over the band [-4, w+4) x [-4, h+4) only (k_pad_planes copies the
+// band's edges outwards); a thread's 4 samples may run past the band's
+// right edge, where they get the clamped values the copy would give.
 __global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h,
                          const s16 *__restrict__ tmp)
 {
-    const int X0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
-    const int Y = blockIdx.y;
+    const int X0 = ME_PAD - 4 + 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+    const int Y = ME_PAD - 4 + blockIdx.y;
     const int idx = blockIdx.z + 1;
-    if (X0 >= stride || Y >= rows) return;
+    if (X0 >= ME_PAD + w + 4 || Y >= rows) return;
     const int dx = idx & 3, dy = idx >> 2;
     const int maxv = (1 << BIT_DEPTH) - 1;
     const pel *p0 = planes + (size_t)ME_PAD * stride + ME_PAD;
@@ -176,6 +181,45 @@ __global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int strid
             if (X0 + j < stride) dst[j] = (pel)me_clip3(0, maxv, v[j]);
 }
 
+// com_img_padding(plane, 1, 4) of the 15 sub-pel planes: every sample
+// outside the band [-4, w+4) x [-4, h+4) takes the band sample at its
+// clamped position.  grid.y: the rows above and below the band (whole rows,
+// 4 samples per thread), then the band's rows (the columns left and right
+// of it, one sample per thread); grid.z: the plane.  Reads stay inside the
+// band, writes outside it.
+__global__ void k_pad_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h)
+{
+    const int bx0 = ME_PAD - 4, bx1 = ME_PAD + w + 3, by0 = ME_PAD - 4, by1 = ME_PAD + h + 3;
+    const int ntop = by0, nbot = rows - 1 - by1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int maxv = (1 << BIT_DEPTH) - 1;
+    const bool rows_aligned = (stride & 3) == 0 && (plane_elems & 3) == 0;
+    pel *pl = planes + (size_t)(blockIdx.z + 1) * plane_elems;
+    int Y = blockIdx.y, X, end;
+    const pel *src;
+    if (Y < ntop + nbot) {   // 4 samples of a row outside the band
+        Y = Y < ntop ? Y : by1 + 1 + (Y - ntop);
+        X = 4 * t;
+        end = stride;
+        src = pl + (size_t)me_clip3(by0, by1, Y) * stride;
+    } else {                 // 4 samples left or right of the band in one of its rows
+        Y = by0 + (Y - ntop - nbot);
+        const int gl = (bx0 + 3) / 4, gr = (stride - 1 - bx1 + 3) / 4;
+        if (t >= gl + gr) return;
+        X = t < gl ? 4 * t : bx1 + 1 + 4 * (t - gl);
+        end = t < gl ? bx0 : stride;
+        src = pl + (size_t)Y * stride;
+    }
+    if (X >= end) return;
+    pel *dst = pl + (size_t)Y * stride + X;
+    int v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = src[me_clip3(bx0, bx1, X + j)];
+    if (X + 3 < end) store4(dst, v, maxv, rows_aligned && (X & 3) == 0);
+    else
+        for (int j = 0; X + j < end; j++) dst[j] = (pel)v[j];
+}
+
 int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
 {
     dim3 b(256);
@@ -183,8 +227,12 @@ int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
                        r->planes, r->stride, r->rows);
     hipLaunchKernelGGL(k_htmp, dim3((r->w + 16 + 255) / 256, r->h + 16, 3), b, 0, st,
                        r->planes + (size_t)ME_PAD * r->stride + ME_PAD, r->stride, r->w, r->h, r->tmp);
-    hipLaunchKernelGGL(k_planes, dim3((r->stride + 1023) / 1024, r->rows, 15), b, 0, st, r->planes, r->plane_elems,
+    hipLaunchKernelGGL(k_planes, dim3((r->w + 8 + 1023) / 1024, r->h + 8, 15), b, 0, st, r->planes, r->plane_elems,
                        r->stride, r->rows, r->w, r->h, r->tmp);
+    const int pad_rows = r->rows - (r->h + 8);
+    const int pad_threads = std::max((r->stride + 3) / 4, (ME_PAD - 4 + 3) / 4 + (r->stride - (ME_PAD + r->w + 4) + 3) / 4);
+    hipLaunchKernelGGL(k_pad_planes, dim3((pad_threads + 255) / 256, pad_rows + r->h + 8, 15), b, 0, st, r->planes,
+                       r->plane_elems, r->stride, r->rows, r->w, r->h);
     ME_CHECK(hipGetLastError());
     return 0;
 }
