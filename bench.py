@@ -61,8 +61,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs the ME path on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the ME path runs on a single MI355X "
+              "(sharding across GPUs is designed, not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
