@@ -183,11 +183,12 @@ __global__ void k_planes(pel *__restrict__ planes, size_t plane_elems, int strid
 
 // com_img_padding(plane, 1, 4) of the 15 sub-pel planes: every sample
 // outside the band [-4, w+4) x [-4, h+4) takes the band sample at its
-// clamped position.  grid.y: the rows above and below the band (whole rows,
-// 4 samples per thread), then the band's rows (the columns left and right
-// of it, one sample per thread); grid.z: the plane.  Reads stay inside the
-// band, writes outside it.
-__global__ void k_pad_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h)
+// clamped position.  sides = 0: grid.y over the rows above and below the
+// band (whole rows); sides = 1: grid.y over the band's rows (the columns
+// left and right of it); 4 samples per thread; grid.z: the plane.  Reads
+// stay inside the band, writes outside it.
+__global__ void k_pad_planes(pel *__restrict__ planes, size_t plane_elems, int stride, int rows, int w, int h,
+                             int sides)
 {
     const int bx0 = ME_PAD - 4, bx1 = ME_PAD + w + 3, by0 = ME_PAD - 4, by1 = ME_PAD + h + 3;
     const int ntop = by0, nbot = rows - 1 - by1;
@@ -195,7 +196,7 @@ __global__ void k_pad_planes(pel *__restrict__ planes, size_t plane_elems, int s
     const int maxv = (1 << BIT_DEPTH) - 1;
     const bool rows_aligned = (stride & 3) == 0 && (plane_elems & 3) == 0;
     pel *pl = planes + (size_t)(blockIdx.z + 1) * plane_elems;
-    int Y = blockIdx.y, X, end;
+    int Y = blockIdx.y + (sides ? ntop + nbot : 0), X, end;
     const pel *src;
     if (Y < ntop + nbot) {   // 4 samples of a row outside the band
         Y = Y < ntop ? Y : by1 + 1 + (Y - ntop);
@@ -230,9 +231,11 @@ int me_ref_prepare_async(me_ref_s *r, hipStream_t st)
     hipLaunchKernelGGL(k_planes, dim3((r->w + 8 + 1023) / 1024, r->h + 8, 15), b, 0, st, r->planes, r->plane_elems,
                        r->stride, r->rows, r->w, r->h, r->tmp);
     const int pad_rows = r->rows - (r->h + 8);
-    const int pad_threads = std::max((r->stride + 3) / 4, (ME_PAD - 4 + 3) / 4 + (r->stride - (ME_PAD + r->w + 4) + 3) / 4);
-    hipLaunchKernelGGL(k_pad_planes, dim3((pad_threads + 255) / 256, pad_rows + r->h + 8, 15), b, 0, st, r->planes,
-                       r->plane_elems, r->stride, r->rows, r->w, r->h);
+    const int side_threads = (ME_PAD - 4 + 3) / 4 + (r->stride - (ME_PAD + r->w + 4) + 3) / 4;
+    hipLaunchKernelGGL(k_pad_planes, dim3((r->stride + 3) / 4 / 256 + 1, pad_rows, 15), b, 0, st, r->planes,
+                       r->plane_elems, r->stride, r->rows, r->w, r->h, 0);
+    hipLaunchKernelGGL(k_pad_planes, dim3((side_threads + 255) / 256, r->h + 8, 15), b, 0, st, r->planes,
+                       r->plane_elems, r->stride, r->rows, r->w, r->h, 1);
     ME_CHECK(hipGetLastError());
     return 0;
 }
