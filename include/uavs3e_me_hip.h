@@ -130,6 +130,13 @@ void me_hip_ip_flt_y_ver(const me_pel *src, int i_src, me_pel *dst[3], int i_dst
 void me_hip_ip_flt_y_ver_ext(const int16_t *src, int i_src, me_pel *dst[3], int i_dst, int width, int height,
                              int8_t (*coeff)[8], int bit_depth);                                          /* com_mc.c:731-758 */
 
+/* Self-check of the path's one floating-point step (no counterpart in the
+ * reference): out[i] = (int)((s0 + i) / sqrt(c) * 2.0) for i < n, evaluated
+ * on the GPU by the function every SATD kernel of this library normalises
+ * with -- c = 32 (big = 0: the 8x4 / 4x8 tiles, com_cost.c:649, 708) or 128
+ * (big = 1: 16x8 / 8x16, com_cost.c:467, 586).  out: host memory. */
+int me_hip_satd_norm(int big, uint32_t s0, uint32_t n, uint32_t *out);
+
 /* ------------------------------------------------------------------------ */
 /* 2. Batched frame ME (no counterpart in the reference)                    */
 /* ------------------------------------------------------------------------ */

@@ -80,7 +80,9 @@ def test_filters(libs):
     assert hip.error_count() == 0
 
 
-@pytest.mark.parametrize("size", [(72, 40), (200, 136)])
+# widths 202, 73 and 75 leave 2, 1 and 3 samples past the last whole dword
+# of a row: per-sample stores, and right-side padding from an unaligned column
+@pytest.mark.parametrize("size", [(72, 40), (200, 136), (202, 40), (73, 41), (75, 36)])
 def test_planes_device_and_table(libs, size):
     """Device reference planes and com_if_luma_frame through the per-call
     frame filters both equal the oracle's 16 padded planes."""
@@ -191,6 +193,49 @@ def test_frame_plan_rebuilds_references(bd, nref, mask):
     _compare(res, best, o2, b2)
     hip.plan_destroy(plan)
     for r in dref:
+        hip.ref_destroy(r)
+    assert hip.error_count() == 0
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_plan_graph_follows_other_entry_points(bd):
+    """The other entry points of a plan point its picture descriptor at their
+    own references; a later me_plan_run with the references of its cached
+    graph must search those again.  And a reference destroyed and created
+    anew (whose handle address may come back) must not replay a graph that
+    holds the old one's buffers."""
+    base = wl.hd_allpu() if bd == 8 else wl.uhd10_allpu()
+    w, h, R = 200, 136, 8
+    work = base.scaled(w, h, R)
+    work.ref_shifts_qpel = work.ref_shifts_qpel[:2]
+    cur, refs = work.frames()
+    other = [np.ascontiguousarray(np.flipud(r)) for r in refs]
+    hip, orc = ua.MeHip(bd), Oracle(bd)
+    lam = wl.lambda_mv(32)
+    mask = sum(1 << SHAPES.index(s) for s in [(8, 8), (16, 16), (32, 32)])
+    want = orc.me_frame(cur, [orc.make_planes(r) for r in refs], R, lam, mask)
+    want_b = orc.me_frame(cur, [orc.make_planes(r) for r in other], R, lam, mask)
+    da = [hip.ref_create(r) for r in refs]
+    db = [hip.ref_create(r) for r in other]
+    plan = hip.plan_create(w, h, R, lam, mask, 2)
+    hip.plan_set_cur(plan, cur)
+    hip.plan_run(plan, da)
+    _compare(*hip.plan_results(plan, 2), *want)
+    npu = hip.lib.me_plan_num_pus(plan)
+    tz = ua.TzCfg(8, 64, 1, 0, 0, 8, (C_int64 * 4)(7, 4, 0, 0))
+    hip.plan_run_tz(plan, db, np.zeros((hip.lib.me_plan_num_ctus(plan), npu, 2, 2), np.int16), tz)
+    hip.plan_run(plan, da)                     # the cached graph, descriptor back at da
+    _compare(*hip.plan_results(plan, 2), *want)
+    hip.plan_run(plan, db)
+    _compare(*hip.plan_results(plan, 2), *want_b)
+    for r in da:
+        hip.ref_destroy(r)
+    dc = [hip.ref_create(r) for r in refs]     # new buffers, possibly at da's addresses
+    for _ in range(2):
+        hip.plan_run(plan, dc, 1)              # rebuilds reference 0 inside the graph
+        _compare(*hip.plan_results(plan, 2), *want)
+    hip.plan_destroy(plan)
+    for r in db + dc:
         hip.ref_destroy(r)
     assert hip.error_count() == 0
 
@@ -365,7 +410,8 @@ def test_bad_arguments_are_rejected():
     for args in [(256, 136, 256, lam, ua.MASK_ALL, 1),    # R > 255
                  (256, 136, 16, lam, 0, 1),               # no shape
                  (256, 136, 16, lam, ua.MASK_ALL, 0),     # no reference
-                 (256, 136, 16, lam, ua.MASK_ALL, 5)]:    # more than 4
+                 (256, 136, 16, lam, ua.MASK_ALL, 5),     # more than 4
+                 (64, 40, 61, lam, ua.MASK_ALL, 1)]:      # R > max(W, H) - 4
         with pytest.raises(ua.MeError):
             hip.plan_create(*args)
     with pytest.raises(ua.MeError):

@@ -455,3 +455,20 @@ def test_live_reference_mc_sweep(bd):
     finally:
         for p in pics:
             ref.pic_free(p)
+
+
+def test_orc_par_equals_whole_frame():
+    """The threaded per-(shape, CTU) split of the oracle's frame search that
+    the GPU configuration tests use returns exactly the whole-frame call."""
+    import orc_par
+    from uavs3e_amd import workloads as wl
+    work = wl.fuhd10_bi().scaled(136, 72, 6)
+    cur, refs = work.frames()
+    orc = Oracle(10)
+    planes = [orc.make_planes(r) for r in refs]
+    lam = wl.lambda_mv(32)
+    mask = MASK_ALL & ~(1 << SHAPES.index((8, 8)))
+    for l1 in (0, 2):
+        a = orc.me_frame(cur, planes, 6, lam, mask, num_refs_l1=l1)
+        b = orc_par.me_frame(orc, cur, planes, 6, lam, mask, num_refs_l1=l1)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

@@ -63,6 +63,15 @@ __constant__ static const s8 ME_CHROMA_TAPS[8][4] = {
     {0, 64, 0, 0}, {-4, 62, 6, 0}, {-6, 56, 15, -1}, {-5, 47, 25, -3},
     {-4, 36, 36, -4}, {-3, 25, 47, -5}, {-1, 15, 56, -6}, {0, 6, 62, -4}};
 
+// The normalisation of the 8x4 / 4x8 (32 coefficients) and 16x8 / 8x16 (128
+// coefficients) cost_satd tiles: (int)(s / com_tbl_sqrt[i] * 2.0) in IEEE
+// double (com_cost.c:467, 586, 649, 708; com_tbl_sqrt = {sqrt(32.0),
+// sqrt(128.0)}, com_tables.c:276-277).  sqrt(32) = 4 sqrt(2) and sqrt(128) =
+// 8 sqrt(2) are the correctly rounded sqrt(2) times a power of two.  Every
+// kernel normalises through these two functions.
+__host__ __device__ __forceinline__ int me_satd_norm32(int s) { return (int)((double)s / 0x1.6a09e667f3bcdp+2 * 2.0); }
+__host__ __device__ __forceinline__ int me_satd_norm128(int s) { return (int)((double)s / 0x1.6a09e667f3bcdp+3 * 2.0); }
+
 __device__ __forceinline__ int me_clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // One wave64 min of two u64 keys held as the bit patterns of positive normal

@@ -11,6 +11,7 @@
 // plane + 15 quarter-pel planes (com_pic_t / com_subpel_t, com_util.c:62-91,
 // 251-284), plus the sources from which they are rebuilt.
 struct me_ref_s {
+    u64 id;                  // unique per me_ref_create in this process (never reused)
     int w, h, stride, rows;
     size_t plane_elems;      // stride * rows
     pel *planes;             // 16 planes, plane i at planes + i * plane_elems

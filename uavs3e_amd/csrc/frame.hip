@@ -290,7 +290,7 @@ struct me_plan_s {
     me_smvd_pu_t *d_smvd; // analyze_smvd per-PU results (allocated on first use)
     hipStream_t own;
     hipGraphExec_t exec;
-    me_ref_t graph_refs[ME_MAX_REFS];
+    u64 graph_refs[ME_MAX_REFS];   // me_ref_s::id of the references the graph was captured with
     u32 graph_mask;
     hipStream_t graph_stream;
 };
@@ -308,6 +308,12 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
     p->ctu_rows = (cfg->pic_h + 127) / 128;
     p->n_ctu = p->ctu_cols * p->ctu_rows;
     const int T = cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h;
+    if (cfg->search_range > T - 4) {   // the window and the MV-bits table (4*T+3 each side) hold |mv| <= T-4
+        me_set_error("me_plan_create: search range exceeds max(pic_w, pic_h) - 4", hipErrorInvalidValue, __FILE__,
+                     __LINE__);
+        free(p);
+        return -1;
+    }
     if (me_search_init(&p->s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, cfg->num_refs_l1, T,
                        cfg->band_rows, cfg->waves, cfg->dx_cols, 1)) {
         free(p);
@@ -398,21 +404,29 @@ ME_EXPORT int me_plan_run(me_plan_t p, me_ref_t const *refs, uint32_t prepare_ma
             return -1;
         }
     hipStream_t st = stream ? (hipStream_t)stream : p->own;
+    // The other entry points point the shared descriptor at their own
+    // references, so it is set for every run (a no-op when unchanged).
+    if (plan_set_pic(p, refs, st)) return -1;
+    // The graph's preparation kernels hold the references' buffers, so the
+    // cache is keyed on their ids: a destroyed reference whose handle address
+    // comes back from a later me_ref_create does not match.
     bool same = p->exec && p->graph_mask == prepare_mask && p->graph_stream == st;
-    for (int r = 0; same && r < p->s.nref; r++) same = p->graph_refs[r] == refs[r];
+    for (int r = 0; same && r < p->s.nref; r++) same = p->graph_refs[r] == refs[r]->id;
     if (!same) {
         if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
-        if (plan_set_pic(p, refs, st)) return -1;
-        hipGraph_t g;
+        hipGraph_t g = nullptr;
         ME_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int rc = plan_enqueue(p, refs, prepare_mask, st);
         hipError_t e = hipStreamEndCapture(st, &g);
-        if (rc) return -1;
-        if (e != hipSuccess) { me_set_error("hipStreamEndCapture", e, __FILE__, __LINE__); return -1; }
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (!rc) me_set_error("hipStreamEndCapture", e, __FILE__, __LINE__);
+            return -1;
+        }
         e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) { me_set_error("hipGraphInstantiate", e, __FILE__, __LINE__); p->exec = nullptr; return -1; }
-        for (int r = 0; r < p->s.nref; r++) p->graph_refs[r] = refs[r];
+        for (int r = 0; r < p->s.nref; r++) p->graph_refs[r] = refs[r]->id;
         p->graph_mask = prepare_mask;
         p->graph_stream = st;
     }

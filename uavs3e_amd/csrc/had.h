@@ -257,11 +257,10 @@ __device__ __forceinline__ int had_half(const pel *org, int so, const pel *cur, 
 template <int TT>
 __device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, int sc)
 {
-    const double SQRT32 = 0x1.6a09e667f3bcdp+2;
     if (TT == 0) return (had_abs<4, 4>(org, so, cur, sc) + 1) >> 1;
     if (TT == 1) return (had_abs<8, 8>(org, so, cur, sc) + 2) >> 2;
-    if (TT == 2) return (int)((double)had_abs<8, 4>(org, so, cur, sc) / SQRT32 * 2.0);
-    return (int)((double)had_abs<4, 8>(org, so, cur, sc) / SQRT32 * 2.0);
+    if (TT == 2) return me_satd_norm32(had_abs<8, 4>(org, so, cur, sc));
+    return me_satd_norm32(had_abs<4, 8>(org, so, cur, sc));
 }
 
 // The same costs with the current block's pairs made once (a sub-pel search
@@ -357,10 +356,9 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 template <int TT>
 __device__ __forceinline__ int satd_pre(const s16x2 *o, const pel *cur, int sc)
 {
-    const double SQRT32 = 0x1.6a09e667f3bcdp+2;
     const int s = had_pre<TT>(o, cur, sc, 0);
     if (TT == 0) return (s + 1) >> 1;
     if (TT == 1) return (s + 2) >> 2;
-    return (int)((double)s / SQRT32 * 2.0);
+    return me_satd_norm32(s);
 }
 

@@ -15,6 +15,7 @@
 #include <atomic>
 #include <mutex>
 #include <string.h>
+#include <vector>
 
 // ---------------------------------------------------------------------------
 // Library state
@@ -26,7 +27,11 @@ static std::atomic<int> g_fallback{0};
 static std::mutex g_init_mu;
 static int g_init_rc = 1;   // 1 = not tried
 static int g_device = 0;
-static me_dispatch_t g_prev;  // slots replaced by me_funs_init_hip
+// Slots replaced by me_funs_init_hip (the fallback of a failing entry).  The
+// reference has one process-wide table (com_util.c:40); a slot that already
+// holds a HIP entry when the table is initialised again keeps the previous
+// non-HIP slot recorded here, so a fallback never calls back into HIP.
+static me_dispatch_t g_prev;
 
 void me_set_error(const char *what, hipError_t e, const char *file, int line)
 {
@@ -71,25 +76,46 @@ ME_EXPORT int me_hip_init(int device)
 // Per-thread staging context
 // ---------------------------------------------------------------------------
 
+// A thread's stream and staging buffers.  Contexts are never freed: a thread
+// that ends hands its context back to a pool (no HIP call, since thread-exit
+// destructors can run after the HIP runtime has been torn down at process
+// exit), and the next thread that calls an entry takes it over.  The pool
+// holds at most as many contexts as threads ever called entries at once.
 struct CallCtx {
     hipStream_t st = nullptr;
     unsigned char *h = nullptr, *d = nullptr;
     size_t cap = 0;
     bool ok = false;
-    ~CallCtx()
+};
+static std::mutex g_ctx_mu;
+static std::vector<CallCtx *> g_ctx_pool;
+
+struct CtxSlot {
+    CallCtx *c = nullptr;
+    CallCtx &get()
     {
-        if (st) hipStreamDestroy(st);
-        if (h) hipHostFree(h);
-        if (d) hipFree(d);
+        if (!c) {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            if (!g_ctx_pool.empty()) { c = g_ctx_pool.back(); g_ctx_pool.pop_back(); }
+            else c = new CallCtx();
+        }
+        return *c;
+    }
+    ~CtxSlot()
+    {
+        if (!c) return;
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_ctx_pool.push_back(c);
     }
 };
-static thread_local CallCtx t_ctx;
+static thread_local CtxSlot t_slot;
+static CallCtx &ctx() { return t_slot.get(); }
 
 // Ensures this thread has a stream and at least `bytes` of pinned + device
 // staging.  Returns 0 on success.
 static int ctx_reserve(size_t bytes)
 {
-    CallCtx &c = t_ctx;
+    CallCtx &c = ctx();
     if (!c.ok) {
         if (me_hip_init(-1)) return -1;
         ME_CHECK(hipSetDevice(g_device));
@@ -99,8 +125,8 @@ static int ctx_reserve(size_t bytes)
     if (bytes > c.cap) {
         size_t n = c.cap ? c.cap : (1 << 20);
         while (n < bytes) n <<= 1;
-        if (c.h) { hipHostFree(c.h); c.h = nullptr; }
-        if (c.d) { hipFree(c.d); c.d = nullptr; }
+        if (c.h) { (void)hipHostFree(c.h); c.h = nullptr; }
+        if (c.d) { (void)hipFree(c.d); c.d = nullptr; }
         c.cap = 0;
         ME_CHECK(hipHostMalloc((void **)&c.h, n, hipHostMallocDefault));
         ME_CHECK(hipMalloc((void **)&c.d, n));
@@ -137,13 +163,13 @@ static void scatter(void *dst, const void *src, int rows, int cols, int stride, 
 
 static int ctx_upload(size_t bytes)
 {
-    ME_CHECK(hipMemcpyAsync(t_ctx.d, t_ctx.h, bytes, hipMemcpyHostToDevice, t_ctx.st));
+    ME_CHECK(hipMemcpyAsync(ctx().d, ctx().h, bytes, hipMemcpyHostToDevice, ctx().st));
     return 0;
 }
 static int ctx_download(size_t off, size_t bytes)
 {
-    ME_CHECK(hipMemcpyAsync(t_ctx.h + off, t_ctx.d + off, bytes, hipMemcpyDeviceToHost, t_ctx.st));
-    ME_CHECK(hipStreamSynchronize(t_ctx.st));
+    ME_CHECK(hipMemcpyAsync(ctx().h + off, ctx().d + off, bytes, hipMemcpyDeviceToHost, ctx().st));
+    ME_CHECK(hipStreamSynchronize(ctx().st));
     return 0;
 }
 
@@ -300,14 +326,14 @@ static int sad_n(int w, const pel *org, int i_org, pel *const *p, int n, int i_p
     size_t o_pred = sg.add(blk * 4);
     size_t o_out = sg.add(16);
     if (ctx_reserve(sg.used)) return -1;
-    gather(t_ctx.h + o_org, org, h, w, i_org, sizeof(pel));
-    for (int k = 0; k < n; k++) gather(t_ctx.h + o_pred + k * blk, p[k], h, w, i_pred, sizeof(pel));
+    gather(ctx().h + o_org, org, h, w, i_org, sizeof(pel));
+    for (int k = 0; k < n; k++) gather(ctx().h + o_pred + k * blk, p[k], h, w, i_pred, sizeof(pel));
     if (ctx_upload(o_out)) return -1;
-    hipLaunchKernelGGL(k_sad_n, dim3(1), dim3(256), 0, t_ctx.st, (const pel *)(t_ctx.d + o_org),
-                       (const pel *)(t_ctx.d + o_pred), (size_t)w * h, w, h, n, (u32 *)(t_ctx.d + o_out));
+    hipLaunchKernelGGL(k_sad_n, dim3(1), dim3(256), 0, ctx().st, (const pel *)(ctx().d + o_org),
+                       (const pel *)(ctx().d + o_pred), (size_t)w * h, w, h, n, (u32 *)(ctx().d + o_out));
     ME_CHECK(hipGetLastError());
     if (ctx_download(o_out, n * sizeof(u32))) return -1;
-    memcpy(out, t_ctx.h + o_out, n * sizeof(u32));
+    memcpy(out, ctx().h + o_out, n * sizeof(u32));
     return 0;
 }
 
@@ -350,14 +376,14 @@ static int satd_call(int wl, int hl, const pel *org, int i_org, const pel *cur, 
     Stage sg;
     size_t oo = sg.add(blk), oc = sg.add(blk), ou = sg.add(16);
     if (ctx_reserve(sg.used)) return -1;
-    gather(t_ctx.h + oo, org, th, tw, i_org, sizeof(pel));
-    gather(t_ctx.h + oc, cur, th, tw, i_cur, sizeof(pel));
+    gather(ctx().h + oo, org, th, tw, i_org, sizeof(pel));
+    gather(ctx().h + oc, cur, th, tw, i_cur, sizeof(pel));
     if (ctx_upload(ou)) return -1;
-    hipLaunchKernelGGL(k_satd, dim3(1), dim3(64), 0, t_ctx.st, tt, (const pel *)(t_ctx.d + oo), tw,
-                       (const pel *)(t_ctx.d + oc), tw, (u32 *)(t_ctx.d + ou));
+    hipLaunchKernelGGL(k_satd, dim3(1), dim3(64), 0, ctx().st, tt, (const pel *)(ctx().d + oo), tw,
+                       (const pel *)(ctx().d + oc), tw, (u32 *)(ctx().d + ou));
     ME_CHECK(hipGetLastError());
     if (ctx_download(ou, 4)) return -1;
-    memcpy(out, t_ctx.h + ou, 4);
+    memcpy(out, ctx().h + ou, 4);
     return 0;
 }
 
@@ -377,19 +403,19 @@ ME_EXPORT uint32_t me_hip_had(int w, int h, me_pel *org, int s_org, me_pel *cur,
     u32 v = 0;
     int rc = ctx_reserve(sg.used);
     if (!rc) {
-        gather(t_ctx.h + oo, org, h, w, s_org, sizeof(pel));
-        gather(t_ctx.h + oc, cur, h, w, s_cur, sizeof(pel));
+        gather(ctx().h + oo, org, h, w, s_org, sizeof(pel));
+        gather(ctx().h + oc, cur, h, w, s_cur, sizeof(pel));
         rc = ctx_upload(ou);
     }
     if (!rc) {
-        hipLaunchKernelGGL(k_had, dim3(1), dim3(256), 0, t_ctx.st, w, h, (const pel *)(t_ctx.d + oo), w,
-                           (const pel *)(t_ctx.d + oc), w, bd, (u32 *)(t_ctx.d + ou));
+        hipLaunchKernelGGL(k_had, dim3(1), dim3(256), 0, ctx().st, w, h, (const pel *)(ctx().d + oo), w,
+                           (const pel *)(ctx().d + oc), w, bd, (u32 *)(ctx().d + ou));
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) { me_set_error("k_had", e, __FILE__, __LINE__); rc = -1; }
     }
     if (!rc) rc = ctx_download(ou, 4);
     if (rc) { entry_failed("com_had", false); return 0; }
-    memcpy(&v, t_ctx.h + ou, 4);
+    memcpy(&v, ctx().h + ou, 4);
     return v;
 }
 
@@ -397,11 +423,11 @@ static int ipcpy_call(const pel *src, int i_src, pel *dst, int i_dst, int w, int
 {
     const size_t blk = (size_t)w * h * sizeof(pel);
     if (ctx_reserve(blk)) return -1;
-    gather(t_ctx.h, src, h, w, i_src, sizeof(pel));
+    gather(ctx().h, src, h, w, i_src, sizeof(pel));
     if (ctx_upload(blk)) return -1;
     // the copy itself is the kernel-free device round trip of the block
     if (ctx_download(0, blk)) return -1;
-    scatter(dst, t_ctx.h, h, w, i_dst, sizeof(pel));
+    scatter(dst, ctx().h, h, w, i_dst, sizeof(pel));
     return 0;
 }
 
@@ -425,19 +451,19 @@ static int ipflt_call(int type, int ext, const pel *src, int i_src, pel *dst, in
     Stage sg;
     size_t os = sg.add((size_t)rw * rh * sizeof(pel)), od = sg.add((size_t)w * h * sizeof(pel));
     if (ctx_reserve(sg.used)) return -1;
-    gather(t_ctx.h + os, src + (ptrdiff_t)y0 * i_src + x0, rh, rw, i_src, sizeof(pel));
+    gather(ctx().h + os, src + (ptrdiff_t)y0 * i_src + x0, rh, rw, i_src, sizeof(pel));
     if (ctx_upload(od)) return -1;
     Taps8 a, b;
     memset(&a, 0, sizeof(a));
     memset(&b, 0, sizeof(b));
     for (int k = 0; k < n; k++) { a.c[k] = cx[k]; if (cy) b.c[k] = cy[k]; }
-    const pel *s0 = (const pel *)(t_ctx.d + os) + (ptrdiff_t)(-y0) * rw + (-x0);
+    const pel *s0 = (const pel *)(ctx().d + os) + (ptrdiff_t)(-y0) * rw + (-x0);
     dim3 g((w + 63) / 64, h);
-    if (ext) hipLaunchKernelGGL(k_ipflt_ext, g, dim3(64), 0, t_ctx.st, s0, rw, (pel *)(t_ctx.d + od), w, h, n, a, b, max_val);
-    else hipLaunchKernelGGL(k_ipflt, g, dim3(64), 0, t_ctx.st, s0, rw, (pel *)(t_ctx.d + od), w, h, n, (int)ver, a, max_val);
+    if (ext) hipLaunchKernelGGL(k_ipflt_ext, g, dim3(64), 0, ctx().st, s0, rw, (pel *)(ctx().d + od), w, h, n, a, b, max_val);
+    else hipLaunchKernelGGL(k_ipflt, g, dim3(64), 0, ctx().st, s0, rw, (pel *)(ctx().d + od), w, h, n, (int)ver, a, max_val);
     ME_CHECK(hipGetLastError());
     if (ctx_download(od, (size_t)w * h * sizeof(pel))) return -1;
-    scatter(dst, t_ctx.h + od, h, w, i_dst, sizeof(pel));
+    scatter(dst, ctx().h + od, h, w, i_dst, sizeof(pel));
     return 0;
 }
 
@@ -463,14 +489,14 @@ static int avrg_call(int w, pel *dst, int i_dst, const pel *s1, const pel *s2, i
     Stage sg;
     size_t oa = sg.add(blk), ob = sg.add(blk), od = sg.add(blk);
     if (ctx_reserve(sg.used)) return -1;
-    memcpy(t_ctx.h + oa, s1, blk);
-    memcpy(t_ctx.h + ob, s2, blk);
+    memcpy(ctx().h + oa, s1, blk);
+    memcpy(ctx().h + ob, s2, blk);
     if (ctx_upload(od)) return -1;
-    hipLaunchKernelGGL(k_avrg, dim3((w * h + 255) / 256), dim3(256), 0, t_ctx.st, (const pel *)(t_ctx.d + oa),
-                       (const pel *)(t_ctx.d + ob), (pel *)(t_ctx.d + od), w * h);
+    hipLaunchKernelGGL(k_avrg, dim3((w * h + 255) / 256), dim3(256), 0, ctx().st, (const pel *)(ctx().d + oa),
+                       (const pel *)(ctx().d + ob), (pel *)(ctx().d + od), w * h);
     ME_CHECK(hipGetLastError());
     if (ctx_download(od, blk)) return -1;
-    scatter(dst, t_ctx.h + od, h, w, i_dst, sizeof(pel));
+    scatter(dst, ctx().h + od, h, w, i_dst, sizeof(pel));
     return 0;
 }
 
@@ -497,16 +523,16 @@ static int flt_hor_call(const pel *src, int i_src, pel *dst[3], int i_dst, s16 *
     size_t od = sg.add((size_t)3 * w * h * sizeof(pel));
     size_t ot = sg.add((size_t)3 * w * h * sizeof(s16));
     if (ctx_reserve(sg.used)) return -1;
-    gather(t_ctx.h + os, src - 3, h, rw, i_src, sizeof(pel));
+    gather(ctx().h + os, src - 3, h, rw, i_src, sizeof(pel));
     if (ctx_upload(od)) return -1;
-    hipLaunchKernelGGL(k_flt_hor, dim3((w + 255) / 256, h, 3), dim3(256), 0, t_ctx.st,
-                       (const pel *)(t_ctx.d + os) + 3, rw, (pel *)(t_ctx.d + od), (s16 *)(t_ctx.d + ot), w, h,
+    hipLaunchKernelGGL(k_flt_hor, dim3((w + 255) / 256, h, 3), dim3(256), 0, ctx().st,
+                       (const pel *)(ctx().d + os) + 3, rw, (pel *)(ctx().d + od), (s16 *)(ctx().d + ot), w, h,
                        coef3(coeff), bd);
     ME_CHECK(hipGetLastError());
     if (ctx_download(od, sg.used - od)) return -1;
     for (int p = 0; p < 3; p++) {
-        scatter(dst[p], t_ctx.h + od + (size_t)p * w * h * sizeof(pel), h, w, i_dst, sizeof(pel));
-        scatter(dt[p], t_ctx.h + ot + (size_t)p * w * h * sizeof(s16), h, w, i_dt, sizeof(s16));
+        scatter(dst[p], ctx().h + od + (size_t)p * w * h * sizeof(pel), h, w, i_dst, sizeof(pel));
+        scatter(dt[p], ctx().h + ot + (size_t)p * w * h * sizeof(s16), h, w, i_dt, sizeof(s16));
     }
     return 0;
 }
@@ -527,17 +553,17 @@ static int flt_ver_call(const void *src, int es, int i_src, pel *dst[3], int i_d
     size_t os = sg.add((size_t)w * rh * es);
     size_t od = sg.add((size_t)3 * w * h * sizeof(pel));
     if (ctx_reserve(sg.used)) return -1;
-    gather(t_ctx.h + os, (const unsigned char *)src - (ptrdiff_t)3 * i_src * es, rh, w, i_src, es);
+    gather(ctx().h + os, (const unsigned char *)src - (ptrdiff_t)3 * i_src * es, rh, w, i_src, es);
     if (ctx_upload(od)) return -1;
     if (ext)
-        hipLaunchKernelGGL(k_flt_ver_ext, dim3((w + 255) / 256, h, 3), dim3(256), 0, t_ctx.st,
-                           (const s16 *)(t_ctx.d + os) + 3 * w, w, (pel *)(t_ctx.d + od), w, h, coef3(coeff), bd);
+        hipLaunchKernelGGL(k_flt_ver_ext, dim3((w + 255) / 256, h, 3), dim3(256), 0, ctx().st,
+                           (const s16 *)(ctx().d + os) + 3 * w, w, (pel *)(ctx().d + od), w, h, coef3(coeff), bd);
     else
-        hipLaunchKernelGGL(k_flt_ver, dim3((w + 255) / 256, h, 3), dim3(256), 0, t_ctx.st,
-                           (const pel *)(t_ctx.d + os) + 3 * w, w, (pel *)(t_ctx.d + od), w, h, coef3(coeff), bd);
+        hipLaunchKernelGGL(k_flt_ver, dim3((w + 255) / 256, h, 3), dim3(256), 0, ctx().st,
+                           (const pel *)(ctx().d + os) + 3 * w, w, (pel *)(ctx().d + od), w, h, coef3(coeff), bd);
     ME_CHECK(hipGetLastError());
     if (ctx_download(od, (size_t)3 * w * h * sizeof(pel))) return -1;
-    for (int p = 0; p < 3; p++) scatter(dst[p], t_ctx.h + od + (size_t)p * w * h * sizeof(pel), h, w, i_dst, sizeof(pel));
+    for (int p = 0; p < 3; p++) scatter(dst[p], ctx().h + od + (size_t)p * w * h * sizeof(pel), h, w, i_dst, sizeof(pel));
     return 0;
 }
 
@@ -557,6 +583,35 @@ ME_EXPORT void me_hip_ip_flt_y_ver_ext(const int16_t *src, int i_src, me_pel *ds
         g_prev.ip_flt_y_ver_ext(src, i_src, dst, i_dst, w, h, coeff, bd);
 }
 
+// ---------------------------------------------------------------------------
+// Self-check of the one floating-point step
+// ---------------------------------------------------------------------------
+
+__global__ void k_satd_norm(int big, u32 s0, u32 n, u32 *out)
+{
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = (int)(s0 + i);
+    out[i] = (u32)(big ? me_satd_norm128(s) : me_satd_norm32(s));
+}
+
+ME_EXPORT int me_hip_satd_norm(int big, uint32_t s0, uint32_t n, uint32_t *out)
+{
+    if (!out || n == 0 || (uint64_t)s0 + n > 0x80000000ull) {
+        me_set_error("me_hip_satd_norm: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (me_hip_init(-1)) return -1;
+    u32 *d = nullptr;
+    ME_CHECK(hipMalloc((void **)&d, (size_t)n * sizeof(u32)));
+    hipLaunchKernelGGL(k_satd_norm, dim3((n + 255) / 256), dim3(256), 0, 0, big, s0, n, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(out, d, (size_t)n * sizeof(u32), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) { me_set_error("me_hip_satd_norm", e, __FILE__, __LINE__); return -1; }
+    return 0;
+}
+
 // Width-specialised wrappers: the reference slots of SAD and pel_avrg carry
 // the block width in their table index only.
 template <int W> static u32 sad_w(me_pel *o, int io, me_pel *p, int ip, int h) { return me_hip_sad(W, o, io, p, ip, h); }
@@ -571,11 +626,29 @@ template <int T> static void fltx_w(const me_pel *s, int is, me_pel *d, int id, 
 { me_hip_ipflt_ext(T, s, is, d, id, w, h, a, b, mv); }
 template <int WL, int HL> static u32 satd_w(me_pel *o, int io, me_pel *p, int ip) { return me_hip_satd(WL, HL, o, io, p, ip); }
 
+static void fill_hip_slots(me_dispatch_t *t);
+
 ME_EXPORT int me_funs_init_hip(me_dispatch_t *t)
 {
     if (!t) return -1;
     if (me_hip_init(-1)) return -1;
-    g_prev = *t;
+    me_dispatch_t hip;
+    memset(&hip, 0, sizeof(hip));
+    fill_hip_slots(&hip);
+    {
+        // Slot by slot: remember what the table held unless it is already ours.
+        std::lock_guard<std::mutex> lk(g_init_mu);
+        void **slot = (void **)t, **ours = (void **)&hip, **prev = (void **)&g_prev;
+        for (size_t i = 0; i < sizeof(me_dispatch_t) / sizeof(void *); i++)
+            if (ours[i] && slot[i] != ours[i]) prev[i] = slot[i];
+    }
+    fill_hip_slots(t);
+    return 0;
+}
+
+// Writes every HIP entry into its slot of t.
+static void fill_hip_slots(me_dispatch_t *t)
+{
     t->cost_sad[0] = sad_w<4>; t->cost_sad[1] = sad_w<8>; t->cost_sad[2] = sad_w<16>;
     t->cost_sad[3] = sad_w<32>; t->cost_sad[4] = sad_w<64>; t->cost_sad[5] = sad_w<128>;
     t->cost_sad_x3[0] = sad3_w<4>; t->cost_sad_x3[1] = sad3_w<8>; t->cost_sad_x3[2] = sad3_w<16>;
@@ -594,5 +667,4 @@ ME_EXPORT int me_funs_init_hip(me_dispatch_t *t)
     t->ip_flt_y_ver_ext = me_hip_ip_flt_y_ver_ext;
     t->cost_satd[0][0] = satd_w<0, 0>; t->cost_satd[1][0] = satd_w<1, 0>; t->cost_satd[0][1] = satd_w<0, 1>;
     t->cost_satd[1][1] = satd_w<1, 1>; t->cost_satd[2][1] = satd_w<2, 1>; t->cost_satd[1][2] = satd_w<1, 2>;
-    return 0;
 }

@@ -2,6 +2,8 @@
 // 15 quarter-pel planes of com_if_luma_frame, plus the PU shape API.
 #include "frame.h"
 
+#include <atomic>
+
 #include <algorithm>
 
 #include <string.h>
@@ -244,7 +246,9 @@ ME_EXPORT int me_ref_create(me_ref_t *out, int pic_w, int pic_h)
 {
     if (me_hip_init(-1)) return -1;
     if (!out || pic_w <= 0 || pic_h <= 0) { me_set_error("me_ref_create: bad size", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    static std::atomic<u64> next_id{1};
     me_ref_s *r = (me_ref_s *)calloc(1, sizeof(me_ref_s));
+    r->id = next_id.fetch_add(1);
     r->w = pic_w; r->h = pic_h;
     r->stride = pic_w + 2 * ME_PAD;
     r->rows = pic_h + 2 * ME_PAD;

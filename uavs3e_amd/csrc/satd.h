@@ -5,8 +5,7 @@
 
 // Sum of |coef| of the 2-D Walsh-Hadamard transform of a TW x TH residual,
 // then the per-shape normalisation of cost_satd (com_cost.c:259, 340, 467,
-// 586, 649, 708).  sqrt(32) and sqrt(128) are 4*sqrt(2) and 8*sqrt(2), i.e.
-// the correctly rounded sqrt(2) scaled by a power of two.
+// 586, 649, 708).
 template <int TW, int TH>
 __device__ __forceinline__ int satd_tile(const pel *org, int s_org, const pel *cur, int s_cur)
 {
@@ -42,11 +41,10 @@ __device__ __forceinline__ int satd_tile(const pel *org, int s_org, const pel *c
     int s = 0;
 #pragma unroll
     for (int i = 0; i < TW * TH; i++) s += abs(m[i]);
-    const double SQRT32 = 0x1.6a09e667f3bcdp+2, SQRT128 = 0x1.6a09e667f3bcdp+3;
     if (TW == 4 && TH == 4) return (s + 1) >> 1;
     if (TW == 8 && TH == 8) return (s + 2) >> 2;
-    if (TW * TH == 128) return (int)((double)s / SQRT128 * 2.0);
-    return (int)((double)s / SQRT32 * 2.0);
+    if (TW * TH == 128) return me_satd_norm128(s);
+    return me_satd_norm32(s);
 }
 
 __device__ __forceinline__ int satd_any(int tt, const pel *org, int s_org, const pel *cur, int s_cur)

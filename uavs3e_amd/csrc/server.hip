@@ -24,6 +24,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <set>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -49,6 +50,7 @@ struct Picture {
 struct Stream {
     int w, h, ctu_cols, n_ctu;
     std::map<int64_t, Picture *> pics;
+    std::set<int64_t> loading;   // POCs reserved by a put that is still uploading
 };
 
 struct Job {
@@ -409,15 +411,32 @@ ME_EXPORT int me_server_open_stream(me_server_t S, int pic_w, int pic_h, int *st
     return 0;
 }
 
+static int put_picture_locked(me_server_s *S, int sid, int64_t poc, const me_pel *luma, int stride, int timeout_ms,
+                              std::unique_lock<std::mutex> &lk);
+
 ME_EXPORT int me_server_put_picture(me_server_t S, int sid, int64_t poc, const me_pel *luma, int stride, int timeout_ms)
 {
     if (!S || !luma) return -1;
     std::unique_lock<std::mutex> lk(S->mu);
-    if (sid < 0 || sid >= (int)S->streams.size() || stride < S->streams[sid].w || S->streams[sid].pics.count(poc)) {
+    if (sid < 0 || sid >= (int)S->streams.size() || stride < S->streams[sid].w || S->streams[sid].pics.count(poc) ||
+        S->streams[sid].loading.count(poc)) {
         me_set_error("me_server_put_picture: bad stream, stride or duplicate picture", hipErrorInvalidValue, __FILE__,
                      __LINE__);
         return -1;
     }
+    // The POC stays reserved while the lock is dropped (allocation, upload),
+    // so a concurrent put of the same picture fails instead of replacing it.
+    S->streams[sid].loading.insert(poc);
+    const int rc = put_picture_locked(S, sid, poc, luma, stride, timeout_ms, lk);
+    S->streams[sid].loading.erase(poc);
+    return rc;
+}
+
+// The body of me_server_put_picture with the POC reserved; lk is held on
+// entry and on return.
+static int put_picture_locked(me_server_s *S, int sid, int64_t poc, const me_pel *luma, int stride, int timeout_ms,
+                              std::unique_lock<std::mutex> &lk)
+{
     const int w = S->streams[sid].w, h = S->streams[sid].h;
     const size_t need = picture_bytes(w, h);
     const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms);

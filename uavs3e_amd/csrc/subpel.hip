@@ -83,8 +83,7 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
                 const int hs = PRE ? had_pre<TT>(opairs, pred + toff_p, P.ref_stride, li & 1)
                                    : had_half<TT>(org + toff_o, P.pic_w, pred + toff_p, P.ref_stride, li & 1);
                 const int s = hs + __builtin_amdgcn_mov_dpp(hs, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-                const double SQRT128 = 0x1.6a09e667f3bcdp+3;
-                v = (li & 1) ? 0 : (int)((double)s / SQRT128 * 2.0);
+                v = (li & 1) ? 0 : me_satd_norm128(s);
             } else {
                 v = satd_pre<TT>(opairs, pred + toff_p, P.ref_stride);
             }

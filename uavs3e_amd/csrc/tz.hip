@@ -100,8 +100,7 @@ template <int TT>
 __device__ __forceinline__ int satd_tile_fn(const pel *org, int so, const pel *cur, int sc)
 {
     if (TT == 3 || TT == 5) {
-        const double SQRT128 = 0x1.6a09e667f3bcdp+3;
-        return (int)((double)(had_half<TT>(org, so, cur, sc, 0) + had_half<TT>(org, so, cur, sc, 1)) / SQRT128 * 2.0);
+        return me_satd_norm128(had_half<TT>(org, so, cur, sc, 0) + had_half<TT>(org, so, cur, sc, 1));
     }
     return satd_t<TT>(org, so, cur, sc);
 }
