@@ -362,6 +362,25 @@ int me_plan_run_smvd(me_plan_t plan, me_ref_t const *refs, const int16_t *mvp, c
 int me_plan_lookahead(me_plan_t plan, me_ref_t const *refs, const me_tz_cfg_t *tz, int is_padding,
                       uint32_t *unit_cost, void *stream);
 
+/* The frame search of me_plan_create + me_plan_run + me_plan_results in one
+ * call that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The picture
+ * descriptor, the rate tables and the per-(CTU, PU, reference) keys live in
+ * a fixed 8 MiB device array of the library (one per GPU, held until the
+ * process ends); a frame whose keys and tables need more (8 bytes per PU and
+ * reference, e.g. 1080p with all 29 shapes and 2 references takes 5.9 MB)
+ * is rejected with an error.  cfg as for me_plan_create (band_rows, waves
+ * and dx_cols choose the schedule); cur: device memory, pic_h rows of pic_w
+ * samples at stride pic_w followed by at least 64 readable bytes; refs: the
+ * cfg->num_refs references, already loaded; res / best_ref: device memory
+ * laid out as me_plan_results writes them.  Enqueued on `stream` (NULL: the
+ * null stream); the results are complete when the stream reaches this point.
+ * Every call on a GPU uses the same array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller. */
+int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *cur, me_ref_t const *refs, me_pu_result_t *res,
+                            int8_t *best_ref, void *stream);
+
 /* Waits for the last run and copies results to host memory:
  * res[(ctu*npu + pu)*num_refs + r], best_ref[(ctu*npu + pu)*L + l] with L = 1
  * (single list) or 2 (num_refs_l1 > 0): the reference index (into refs) of

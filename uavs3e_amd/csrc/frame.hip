@@ -20,6 +20,7 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 __global__ void k_fill_keys(u64 *keys, size_t n)
@@ -74,8 +75,11 @@ void me_mvbits_fill(u8 *base, int off)
     }
 }
 
-int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
-                   int waves, int dx_cols, int sp_groups)
+// The host part of a search configuration: window geometry, key layout,
+// schedule, and the sub-pel shape table (shp[0 .. *n_shp), in launch order).
+// Allocates nothing.
+static int search_layout(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim,
+                         int band_rows, int waves, int dx_cols, int sp_groups, SubShape shp[ME_NSHAPES], int *n_shp)
 {
     memset(p, 0, sizeof(*p));
     const uint32_t m = shape_mask & ((1u << ME_NSHAPES) - 1);
@@ -125,58 +129,77 @@ int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int
                      __FILE__, __LINE__);
         return -1;
     }
-
-    // rate[i] = bits(4*(i-R)) * lambda (GET_MVBITS_IPEL, util.h:43-44)
     p->tab_off = (max_dim << 2) + 3;
+    // Sub-pel shape table: every searched shape's workgroups in one launch,
+    // the shapes with the most samples per PU first (their workgroups run
+    // longest, so they should not trail at the end of the launch).
+    const int G = sp_groups >= 8 ? 8 : sp_groups >= 4 ? 4 : sp_groups >= 2 ? 2 : 1;
+    int order[ME_NSHAPES], no = 0;
+    for (int s = 0; s < ME_NSHAPES; s++)
+        if (p->base[s] >= 0) order[no++] = s;
+    std::stable_sort(order, order + no, [](int a, int b) {
+        return ME_SHAPES_HOST[a][0] * ME_SHAPES_HOST[a][1] > ME_SHAPES_HOST[b][0] * ME_SHAPES_HOST[b][1];
+    });
+    int blocks_total = 0;
+    for (int k = 0; k < no; k++) {
+        const int s = order[k];
+        SubShape &q = shp[k];
+        memset(&q, 0, sizeof(q));
+        q.pw = ME_SHAPES_HOST[s][0];
+        q.ph = ME_SHAPES_HOST[s][1];
+        had_tiling(q.pw, q.ph, &q.tt, &q.tw, &q.th);
+        const int Tn = (q.pw / q.tw) * (q.ph / q.th);
+        if (Tn > 64 && Tn != 256) {
+            me_set_error("me_search_init: no sub-pel tiling for a shape", hipErrorInvalidValue, __FILE__, __LINE__);
+            return -1;
+        }
+        q.tpr = q.pw / q.tw;
+        q.base = p->base[s];
+        q.per_ctu = (128 / q.pw) * (128 / q.ph);
+        q.lpu = q.tt == 3 || q.tt == 5 ? 2 * Tn : Tn;   // 1..128, or 256 for 128x128
+        // up to sp_groups search positions of a PU at a time, at most
+        // 256 lanes per PU; lanes per (CTU, reference): a multiple of 256
+        q.G = 256 / q.lpu < G ? 256 / q.lpu : G;
+        q.blk0 = blocks_total;
+        blocks_total += q.per_ctu * q.lpu * q.G / 256;
+    }
+    p->sub_blocks = blocks_total;
+    *n_shp = no;
+    return 0;
+}
+
+// Workgroup -> shape of the sub-pel launch: shape k covers the workgroups
+// [shp[k].blk0, shp[k + 1].blk0).
+static void blk_shape_fill(const SubShape *shp, int n_shp, int sub_blocks, u8 *out)
+{
+    for (int k = 0; k < n_shp; k++) {
+        const int end = k + 1 < n_shp ? shp[k + 1].blk0 : sub_blocks;
+        for (int b = shp[k].blk0; b < end; b++) out[b] = (u8)k;
+    }
+}
+
+int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int nref_l1, int max_dim, int band_rows,
+                   int waves, int dx_cols, int sp_groups)
+{
+    SubShape shp[ME_NSHAPES];
+    int n_shp = 0;
+    if (search_layout(p, R, lambda, shape_mask, nref, nref_l1, max_dim, band_rows, waves, dx_cols, sp_groups, shp,
+                      &n_shp))
+        return -1;
+    // rate[i] = bits(4*(i-R)) * lambda (GET_MVBITS_IPEL, util.h:43-44)
     std::vector<u8> tab(2 * p->tab_off + 1);
     me_mvbits_fill(tab.data(), p->tab_off);
     std::vector<u32> rate(p->span);
     for (int i = 0; i < p->span; i++) rate[i] = (u32)tab[p->tab_off + 4 * (i - R)] * lambda;
-    // Sub-pel shape table: every searched shape's workgroups in one launch,
-    // the shapes with the most samples per PU first (their workgroups run
-    // longest, so they should not trail at the end of the launch).
-    std::vector<SubShape> shp;
-    std::vector<u8> blk_shape;
-    {
-        const int G = sp_groups >= 8 ? 8 : sp_groups >= 4 ? 4 : sp_groups >= 2 ? 2 : 1;
-        std::vector<int> order;
-        for (int s = 0; s < ME_NSHAPES; s++)
-            if (p->base[s] >= 0) order.push_back(s);
-        std::stable_sort(order.begin(), order.end(), [](int a, int b) {
-            return ME_SHAPES_HOST[a][0] * ME_SHAPES_HOST[a][1] > ME_SHAPES_HOST[b][0] * ME_SHAPES_HOST[b][1];
-        });
-        for (int s : order) {
-            SubShape q;
-            memset(&q, 0, sizeof(q));
-            q.pw = ME_SHAPES_HOST[s][0];
-            q.ph = ME_SHAPES_HOST[s][1];
-            had_tiling(q.pw, q.ph, &q.tt, &q.tw, &q.th);
-            const int T = (q.pw / q.tw) * (q.ph / q.th);
-            if (T > 64 && T != 256) {
-                me_set_error("me_search_init: no sub-pel tiling for a shape", hipErrorInvalidValue, __FILE__, __LINE__);
-                return -1;
-            }
-            q.tpr = q.pw / q.tw;
-            q.base = p->base[s];
-            q.per_ctu = (128 / q.pw) * (128 / q.ph);
-            q.lpu = q.tt == 3 || q.tt == 5 ? 2 * T : T;   // 1..128, or 256 for 128x128
-            // up to sp_groups search positions of a PU at a time, at most
-            // 256 lanes per PU; lanes per (CTU, reference): a multiple of 256
-            q.G = 256 / q.lpu < G ? 256 / q.lpu : G;
-            q.blk0 = (int)blk_shape.size();
-            const int blocks = q.per_ctu * q.lpu * q.G / 256;
-            for (int b = 0; b < blocks; b++) blk_shape.push_back((u8)shp.size());
-            shp.push_back(q);
-        }
-        p->sub_blocks = (int)blk_shape.size();
-    }
+    std::vector<u8> blk_shape(p->sub_blocks);
+    blk_shape_fill(shp, n_shp, p->sub_blocks, blk_shape.data());
     hipError_t e = hipMalloc((void **)&p->d_rate, p->span * sizeof(u32));
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tab_base, tab.size());
-    if (e == hipSuccess) e = hipMalloc((void **)&p->d_shp, shp.size() * sizeof(SubShape));
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_shp, n_shp * sizeof(SubShape));
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_blk_shape, blk_shape.size());
     if (e == hipSuccess) e = hipMemcpy(p->d_rate, rate.data(), p->span * sizeof(u32), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_tab_base, tab.data(), tab.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->d_shp, shp.data(), shp.size() * sizeof(SubShape), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_shp, shp, n_shp * sizeof(SubShape), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_blk_shape, blk_shape.data(), blk_shape.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         me_set_error("me_search_init", e, __FILE__, __LINE__);
@@ -776,4 +799,170 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     (void)hipFree(p->d_res);
     (void)hipFree(p->d_best);
     free(p);
+}
+
+// ---------------------------------------------------------------------------
+// Frame search without allocation
+// ---------------------------------------------------------------------------
+
+// Everything me_frame_search_noalloc keeps on the device -- the picture
+// descriptor, the integer-pel rate row, the MV-bits table, the sub-pel shape
+// tables and the key of every (CTU, PU, reference) -- lives in this array.
+// Each GPU holds its own copy until the process ends; host code reaches it
+// only through the per-device address from hipGetSymbolAddress.
+#define ME_NOALLOC_SCRATCH (8u << 20)
+static __device__ __align__(256) unsigned char scratch[ME_NOALLOC_SCRATCH];
+
+#define ME_NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_dev[ME_NOALLOC_MAX_DEVICES];
+
+// The scratch array's address on the current device, looked up once per
+// device (with that device current).
+static unsigned char *scratch_address(void)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= ME_NOALLOC_MAX_DEVICES) {
+        me_set_error("me_frame_search_noalloc: no current device", hipErrorInvalidDevice, __FILE__, __LINE__);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        unsigned char *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            me_set_error("me_frame_search_noalloc: hipGetSymbolAddress", e, __FILE__, __LINE__);
+            return nullptr;
+        }
+        g_scratch_dev[dev] = ptr;
+    }
+    return g_scratch_dev[dev];
+}
+
+// Device tables of a search, filled in place (the values me_search_init
+// uploads): the MV-bits table in its closed form, the rate row, the sub-pel
+// shape table, workgroup -> shape, and the picture descriptor.
+struct TableArgs {
+    SubShape shp[ME_NSHAPES];
+    PicDesc pic;
+    int n_shp, sub_blocks, span, R, tab_off;
+    u32 lambda;
+    u32 *rate;
+    u8 *tab_base;
+    SubShape *d_shp;
+    u8 *blk_shape;
+    PicDesc *d_pic;
+};
+
+__device__ __forceinline__ u32 mvbits_at(int v, int off)
+{
+    const int a = v < 0 ? -v : v;
+    if (a == 0) return 1;
+    if (a >= off) return 0;
+    return 2u * (u32)(31 - __clz(a + 1) + 1);   // 2 * (floor(log2(|v| + 1)) + 1)
+}
+
+__global__ void k_search_tables(TableArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= 2 * a.tab_off) a.tab_base[i] = (u8)mvbits_at(i - a.tab_off, a.tab_off);
+    if (i < a.span) a.rate[i] = mvbits_at(4 * (i - a.R), a.tab_off) * a.lambda;
+    if (i < a.n_shp) a.d_shp[i] = a.shp[i];
+    if (i < a.sub_blocks) {
+        int k = 0;
+        while (k + 1 < a.n_shp && a.shp[k + 1].blk0 <= i) k++;
+        a.blk_shape[i] = (u8)k;
+    }
+    if (i == 0) *a.d_pic = a.pic;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+ME_EXPORT int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *cur, me_ref_t const *refs,
+                                      me_pu_result_t *res, int8_t *best_ref, void *stream)
+{
+    if (!cfg || !cur || !refs || !res || !best_ref || cfg->pic_w <= 0 || cfg->pic_h <= 0 ||
+        cfg->pic_w > 128 * 0xFFFF) {
+        me_set_error("me_frame_search_noalloc: bad argument", hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    if (me_hip_init(-1)) return -1;
+    const int T = cfg->pic_w > cfg->pic_h ? cfg->pic_w : cfg->pic_h;
+    if (cfg->search_range > T - 4) {
+        me_set_error("me_frame_search_noalloc: search range exceeds max(pic_w, pic_h) - 4", hipErrorInvalidValue,
+                     __FILE__, __LINE__);
+        return -1;
+    }
+    MeSearch s;
+    SubShape shp[ME_NSHAPES];
+    int n_shp = 0;
+    if (search_layout(&s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, cfg->num_refs_l1, T,
+                      cfg->band_rows, cfg->waves, cfg->dx_cols, 1, shp, &n_shp))
+        return -1;
+    for (int r = 0; r < s.nref; r++)
+        if (!refs[r] || refs[r]->w != cfg->pic_w || refs[r]->h != cfg->pic_h) {
+            me_set_error("me_frame_search_noalloc: reference missing or of another size", hipErrorInvalidValue,
+                         __FILE__, __LINE__);
+            return -1;
+        }
+    const int ctu_cols = (cfg->pic_w + 127) / 128;
+    const int n_ctu = ctu_cols * ((cfg->pic_h + 127) / 128);
+    // scratch layout: descriptor, rate row, MV-bits table, shape table,
+    // workgroup -> shape, keys
+    const size_t o_rate = align256(sizeof(PicDesc));
+    const size_t o_tab = o_rate + align256((size_t)s.span * sizeof(u32));
+    const size_t o_shp = o_tab + align256((size_t)2 * s.tab_off + 1);
+    const size_t o_blk = o_shp + align256(sizeof(SubShape) * ME_NSHAPES);
+    const size_t o_keys = o_blk + align256((size_t)s.sub_blocks);
+    const size_t need = o_keys + (size_t)n_ctu * s.npu * s.nref * sizeof(u64);
+    if (need > ME_NOALLOC_SCRATCH) {
+        char msg[160];
+        snprintf(msg, sizeof(msg), "me_frame_search_noalloc: the frame needs %zu bytes of scratch, the maximum is %u",
+                 need, ME_NOALLOC_SCRATCH);
+        me_set_error(msg, hipErrorInvalidValue, __FILE__, __LINE__);
+        return -1;
+    }
+    unsigned char *base = scratch_address();
+    if (!base) return -1;
+    s.d_rate = (u32 *)(base + o_rate);
+    s.d_tab_base = base + o_tab;
+    s.d_shp = (SubShape *)(base + o_shp);
+    s.d_blk_shape = base + o_blk;
+    u64 *keys = (u64 *)(base + o_keys);
+    hipStream_t st = (hipStream_t)stream;
+
+    TableArgs a;
+    memset(&a, 0, sizeof(a));
+    memcpy(a.shp, shp, sizeof(shp));
+    a.pic.cur = cur;
+    for (int r = 0; r < s.nref; r++) {
+        a.pic.planes[r] = refs[r]->planes;
+        a.pic.chroma[r] = refs[r]->chroma;
+    }
+    a.pic.plane_elems = refs[0]->plane_elems;
+    a.pic.ref_stride = refs[0]->stride;
+    a.pic.pic_w = cfg->pic_w;
+    a.pic.pic_h = cfg->pic_h;
+    a.pic.ctu_cols = ctu_cols;
+    a.n_shp = n_shp;
+    a.sub_blocks = s.sub_blocks;
+    a.span = s.span;
+    a.R = s.R;
+    a.tab_off = s.tab_off;
+    a.lambda = s.lambda;
+    a.rate = s.d_rate;
+    a.tab_base = s.d_tab_base;
+    a.d_shp = s.d_shp;
+    a.blk_shape = s.d_blk_shape;
+    a.d_pic = (PicDesc *)base;
+    int n = 2 * s.tab_off + 1;
+    if (s.span > n) n = s.span;
+    if (s.sub_blocks > n) n = s.sub_blocks;
+    hipLaunchKernelGGL(k_search_tables, dim3((n + 255) / 256), dim3(256), 0, st, a);
+    ME_CHECK(hipGetLastError());
+    JobSet js;
+    js.pics = (const PicDesc *)base;
+    js.jobs = nullptr;
+    js.n_jobs = n_ctu;
+    return me_search_enqueue(&s, js, keys, res, best_ref, st, nullptr);
 }
