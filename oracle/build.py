@@ -3,11 +3,12 @@
 * ``build_oracle()`` compiles the C restatement ``oracle/me_oracle.c`` once per
   bit depth into ``oracle/lib/liboracle{8,10}.so``.
 * ``build_ref()`` compiles the reference uavs3e from its own sources where they
-  lie under ``/root/reference`` (all of ``src/*.c`` except ``uavs3e.c`` and
-  ``me.c``, plus ``src/sse`` and ``src/avx2``, with the reference's own flags
+  lie under ``/root/reference`` (all of ``src/*.c`` except ``uavs3e.c``,
+  ``me.c`` and ``inter.c``, plus ``src/sse`` and ``src/avx2``, with the reference's own flags
   from src/CMakeLists.txt:13-15) together with ``oracle/ref_harness.c`` into
   ``oracle/_ref/libuavs3e_ref{8,10}.so``.  ``uavs3e.c`` is left out because it
-  needs the generated ``version.h``; ``me.c`` is compiled inside the harness.
+  needs the generated ``version.h``; ``me.c`` and ``inter.c`` are compiled
+  inside the harness.
   The reference selects its sample type with a hard-coded ``#define BIT_DEPTH 8``
   (inc/com_api.h:46) and documents editing that line to switch (README.md:15-16),
   so the 10-bit library is compiled from a scratch copy of ``src/`` and ``inc/``
@@ -46,7 +47,7 @@ def build_oracle():
 def _ref_sources(root):
     src = os.path.join(root, "src")
     core = sorted(os.path.join(src, f) for f in os.listdir(src)
-                  if f.endswith(".c") and f not in ("uavs3e.c", "me.c"))
+                  if f.endswith(".c") and f not in ("uavs3e.c", "me.c", "inter.c"))
     sse = sorted(os.path.join(src, "sse", f) for f in os.listdir(os.path.join(src, "sse")) if f.endswith(".c"))
     avx2 = sorted(os.path.join(src, "avx2", f) for f in os.listdir(os.path.join(src, "avx2")) if f.endswith(".c"))
     return core, sse, avx2

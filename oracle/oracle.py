@@ -484,6 +484,13 @@ class RefLib(_Lib):
         L.refh_me_search_tz_bi.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32,
                                            C.c_void_p, C.c_void_p]
+        L.refh_analyze_bi.restype = None
+        L.refh_analyze_bi.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_uint32]
+        L.refh_smvd.restype = C.c_uint32
+        L.refh_smvd.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint32, C.c_void_p]
         L.refh_loka_cost.restype = C.c_double
         L.refh_loka_cost.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_longlong, C.c_void_p, C.c_int, C.c_void_p,
                                      C.c_int, C.c_int, C.c_int, C.c_int]
@@ -627,6 +634,39 @@ class RefLib(_Lib):
         c = self.lib.refh_me_search_tz_bi(pic, ptr(org_bi), w, x, y, w, h, refi, num_refp, lidx, mot_bits_other,
                                           mvr, cost_type, lambda_mv, ptr(mvp_a), ptr(mv_a))
         return c, (int(mv_a[0]), int(mv_a[1]))
+
+    def analyze_bi(self, cur, pics_l0, pics_l1, x, y, w, h, mvp, mv_scale, refi, mv, lidx_ref, mot_bits, mvr,
+                   cost_type, lambda_mv):
+        """The compiled reference's own analyze_bi (inter.c:972-1070) up to
+        the end of its search, through a minimal core_t: mvp / mv_scale
+        [2][4][2] (per list and reference index), refi [2], mv [2][2],
+        mot_bits [2].  Returns (refi, mv, mot_bits, mv_scale) after the
+        search, as cur_info->refi / mv, pi->mot_bits and pi->mv_scale."""
+        cur = np.ascontiguousarray(cur, self.dt)
+        ph, pw = cur.shape
+        a0 = (C.c_void_p * max(1, len(pics_l0)))(*pics_l0)
+        a1 = (C.c_void_p * max(1, len(pics_l1)))(*pics_l1)
+        mvp_a = np.ascontiguousarray(mvp, np.int16).reshape(2, 4, 2)
+        ms = np.ascontiguousarray(mv_scale, np.int16).reshape(2, 4, 2).copy()
+        ri = np.array(refi, np.int8)
+        mv_a = np.array(mv, np.int16).reshape(2, 2).copy()
+        mb = np.array(mot_bits, np.int32)
+        self.lib.refh_analyze_bi(ptr(cur), pw, ph, a0, len(pics_l0), a1, len(pics_l1), x, y, w, h, ptr(mvp_a), ptr(ms),
+                                 ptr(ri), ptr(mv_a), lidx_ref, ptr(mb), mvr, cost_type, C.c_uint32(lambda_mv))
+        return ri, mv_a, mb, ms
+
+    def smvd(self, cur, p0, p1, x, y, w, h, mvp, mv_scale_l0, mvr, cost_type, lambda_mv):
+        """analyze_smvd's search (inter.c:1194-1255) with the reference's own
+        smvd_refine: mvp [2][2] (reference 0 of each list), mv_scale_l0
+        [2].  Returns (mv [2][2], mecost)."""
+        cur = np.ascontiguousarray(cur, self.dt)
+        ph, pw = cur.shape
+        mvp_a = np.ascontiguousarray(mvp, np.int16).reshape(2, 2)
+        ms = np.array(mv_scale_l0, np.int16)
+        out = np.zeros((2, 2), np.int16)
+        c = self.lib.refh_smvd(ptr(cur), pw, ph, C.c_void_p(p0), C.c_void_p(p1), x, y, w, h, ptr(mvp_a), ptr(ms), mvr,
+                               cost_type, C.c_uint32(lambda_mv), ptr(out))
+        return out, int(c)
 
     def loka_cost(self, cur, pics_l0, pics_l1, cur_poc, gop, msr, is_padding):
         """loka_estimate_coding_cost (lookahead.c:82-290) of the compiled

@@ -7,13 +7,16 @@
  * baseline ("reference AVX2 path") of bench.py.  Nothing in the product
  * links it.
  *
- * me.c is compiled as part of this translation unit (it is #included from the
- * reference tree, not copied) so that its static functions
- * get_max_search_range and me_sub_pel_search (me.c:366-527) can be called.
+ * me.c and inter.c are compiled as part of this translation unit (they are
+ * #included from the reference tree, not copied) so that their static
+ * functions can be called: get_max_search_range and me_sub_pel_search
+ * (me.c:366-527), analyze_bi (inter.c:972-1070), smvd_refine
+ * (inter.c:1083-1191) and the helpers analyze_smvd uses.
  */
 #define _GNU_SOURCE
 #include "define.h"
 #include "me.c"
+#include "inter.c"
 #include <pthread.h>
 #include <time.h>
 
@@ -554,4 +557,181 @@ RH_EXPORT void refh_affine_cu(void *ref, int x, int y, int cu_w, int cu_h, const
     memcpy(pred, buf[Y_C], sizeof(pel) * (size_t)cu_w * cu_h);
     if (pred_u) memcpy(pred_u, buf[U_C], sizeof(pel) * (size_t)(cu_w / 2) * (cu_h / 2));
     if (pred_v) memcpy(pred_v, buf[V_C], sizeof(pel) * (size_t)(cu_w / 2) * (cu_h / 2));
+}
+
+/* ---------------------------------------------------------------------- */
+/* analyze_bi's search and analyze_smvd's search with a minimal core_t      */
+/* ---------------------------------------------------------------------- */
+
+/* The CU-level encoder state both searches read: the current picture, the
+ * reference lists, the CU geometry and an inter_search_t set up as
+ * analyze_inter_cu leaves it for one CU (inter_search_init, is_padding 1). */
+typedef struct {
+    core_t *core;
+    com_info_t info;
+    com_pic_t *org;
+    com_ref_pic_t refp[MAX_REFS][REFP_NUM];
+    int num_refp[REFP_NUM];
+    u8 *tab;
+} refh_cu_t;
+
+static void cu_setup(refh_cu_t *c, const pel *cur, int pic_w, int pic_h, void **l0, int n0, void **l1, int n1,
+                     int x, int y, int w, int h, int mvr, int cost_type, u32 lambda_mv)
+{
+    int err = 0;
+    memset(c, 0, sizeof(*c));
+    c->info.pic_width = pic_w;
+    c->info.pic_height = pic_h;
+    c->info.bit_depth_internal = BIT_DEPTH;
+    c->info.gop_size = 16;
+    c->info.i_scu = (pic_w + MIN_CU_SIZE - 1) >> MIN_CU_LOG2;
+    /* analyze_bi returns right after its search when the search cost
+     * exceeds inter_satd * satd_threshold (inter.c:1036-1038): with
+     * inter_satd 0 that is always, so inter_rdcost is never reached */
+    c->info.rmv_inter_candi_by_satd = 1;
+    c->org = com_pic_create(pic_w, pic_h, PIC_PAD_SIZE_L, PIC_PAD_SIZE_C, &err);
+    for (int i = 0; i < pic_h; i++) memcpy(c->org->y + i * c->org->stride_luma, cur + i * pic_w, sizeof(pel) * pic_w);
+    for (int i = 0; i < n0; i++) {
+        c->refp[i][REFP_0].pic = ((refh_pic_t *)l0[i])->pic;
+        c->refp[i][REFP_0].ptr = c->refp[i][REFP_0].pic->ptr;
+    }
+    for (int i = 0; i < n1; i++) {
+        c->refp[i][REFP_1].pic = ((refh_pic_t *)l1[i])->pic;
+        c->refp[i][REFP_1].ptr = c->refp[i][REFP_1].pic->ptr;
+    }
+    c->num_refp[REFP_0] = n0;
+    c->num_refp[REFP_1] = n1;
+    /* core_t holds 32-byte aligned buffers (the prediction of com_mode_t)
+     * that the SIMD kernels store to with aligned stores */
+    void *mem = NULL;
+    if (posix_memalign(&mem, 64, sizeof(core_t))) abort();
+    memset(mem, 0, sizeof(core_t));
+    core_t *core = c->core = (core_t *)mem;
+    core->info = &c->info;
+    core->pic_org = c->org;
+    core->refp = c->refp;
+    core->num_refp = c->num_refp;
+    core->cu_pix_x = x;
+    core->cu_pix_y = y;
+    core->cu_width = w;
+    core->cu_height = h;
+    core->cu_width_log2 = CONV_LOG2(w);
+    core->cu_height_log2 = CONV_LOG2(h);
+    core->cu_scup_in_pic = (x >> MIN_CU_LOG2) + (y >> MIN_CU_LOG2) * c->info.i_scu;
+    core->inter_satd = 0;
+    core->satd_threshold = 1.0;
+    core->mod_info_curr.pb_part = SIZE_2Nx2N;
+    core->mod_info_curr.mvr_idx = (u8)mvr;
+    inter_search_t *pi = &core->pinter;
+    int off = inter_search_create(&c->tab, &c->info);
+    c->info.tab_mvbits = c->tab;
+    inter_search_init(pi, &c->info, 1);
+    pi->tab_mvbits_offset = off;
+    pi->lambda_mv = lambda_mv;
+    pi->subpel_cost_type = cost_type;
+    pi->curr_mvr = (u8)mvr;
+}
+
+static void cu_free(refh_cu_t *c)
+{
+    inter_search_free(c->tab, c->core->pinter.tab_mvbits_offset);
+    free(c->core);
+    com_pic_destroy(c->org);
+}
+
+/* analyze_bi (inter.c:972-1070) of the reference itself, up to the end of
+ * its search: the CU at (x, y), w x h, of `cur` with the list-0 pictures
+ * l0[0..n0) and list-1 pictures l1[0..n1) (refh_pic_create).  In:
+ * mvp[l][r][2] and mv_scale[l][r][2] (pi->mvp_scale / mv_scale, quarter
+ * pel, r < 4), refi[2] and mv[2][2] (analyze_bi's refi_L0L1 / mv_L0L1),
+ * lidx_ref, mot_bits[2] (pi->mot_bits).  Out: refi, mv (cur_info->refi /
+ * mv), mot_bits (pi->mot_bits) and mv_scale as the search leaves them. */
+RH_EXPORT void refh_analyze_bi(const pel *cur, int pic_w, int pic_h, void **l0, int n0, void **l1, int n1, int x,
+                               int y, int w, int h, const s16 *mvp, s16 *mv_scale, s8 *refi, s16 *mv, int lidx_ref,
+                               int *mot_bits, int mvr, int cost_type, u32 lambda_mv)
+{
+    refh_cu_t c;
+    cu_setup(&c, cur, pic_w, pic_h, l0, n0, l1, n1, x, y, w, h, mvr, cost_type, lambda_mv);
+    inter_search_t *pi = &c.core->pinter;
+    for (int l = 0; l < REFP_NUM; l++)
+        for (int r = 0; r < 4; r++)
+            for (int k = 0; k < MV_D; k++) {
+                pi->mvp_scale[l][r][k] = mvp[(l * 4 + r) * 2 + k];
+                pi->mv_scale[l][r][k] = mv_scale[(l * 4 + r) * 2 + k];
+            }
+    pi->mot_bits[0] = mot_bits[0];
+    pi->mot_bits[1] = mot_bits[1];
+    s16 mv_l0l1[REFP_NUM][MV_D] = {{mv[0], mv[1]}, {mv[2], mv[3]}};
+    s8 refi_l0l1[REFP_NUM] = {refi[0], refi[1]};
+    analyze_bi(c.core, NULL, mv_l0l1, refi_l0l1, (s8)lidx_ref);
+    com_mode_t *ci = &c.core->mod_info_curr;
+    refi[0] = ci->refi[REFP_0];
+    refi[1] = ci->refi[REFP_1];
+    for (int k = 0; k < 4; k++) mv[k] = ci->mv[k >> 1][k & 1];
+    mot_bits[0] = pi->mot_bits[0];
+    mot_bits[1] = pi->mot_bits[1];
+    for (int l = 0; l < REFP_NUM; l++)
+        for (int r = 0; r < 4; r++)
+            for (int k = 0; k < MV_D; k++) mv_scale[(l * 4 + r) * 2 + k] = pi->mv_scale[l][r][k];
+    cu_free(&c);
+}
+
+/* analyze_smvd's search (inter.c:1194-1255) for the CU at (x, y), w x h:
+ * reference 0 of each list; the first two candidates (the MVP pair, and
+ * list 0's uni MV with list 1 mirrored about its MVP), then the reference's
+ * own smvd_refine twice (diamond, 8 rounds; cross, 1 round; offsets shifted
+ * by the AMVR index) and, for cost type 1, the com_had of the winner, all
+ * through the reference's com_mc_cu, get_smvd_cost and MV-bits functions in
+ * analyze_smvd's order.  In: mvp[l][2], mv_scale_l0[2] (pi->mvp_scale[l][0],
+ * pi->mv_scale[0][0]).  Out: mv[2][2]; returns mecost. */
+RH_EXPORT u32 refh_smvd(const pel *cur, int pic_w, int pic_h, void *p0, void *p1, int x, int y, int w, int h,
+                        const s16 *mvp, const s16 *mv_scale_l0, int mvr, int cost_type, u32 lambda_mv, s16 *mv_out)
+{
+    void *l0[1] = {p0}, *l1[1] = {p1};
+    refh_cu_t c;
+    cu_setup(&c, cur, pic_w, pic_h, l0, 1, l1, 1, x, y, w, h, mvr, cost_type, lambda_mv);
+    core_t *core = c.core;
+    inter_search_t *pi = &core->pinter;
+    com_mode_t *ci = &core->mod_info_curr;
+    const int log2w = core->cu_width_log2, log2h = core->cu_height_log2, widx = log2w - MIN_CU_LOG2;
+    pel *org = c.org->y + x + y * c.org->stride_luma;
+    init_inter_data(core);
+    ci->smvd_flag = 1;
+    ci->refi[REFP_0] = 0;
+    ci->refi[REFP_1] = 0;
+    s16 m[REFP_NUM][MV_D], p[REFP_NUM][MV_D], cand[REFP_NUM][MV_D], d[MV_D];
+    for (int l = 0; l < REFP_NUM; l++) {
+        p[l][MV_X] = m[l][MV_X] = mvp[2 * l];
+        p[l][MV_Y] = m[l][MV_Y] = mvp[2 * l + 1];
+    }
+    /* candidate 1: both MVPs, MVD 0 */
+    com_mc_cu(x, y, pic_w, pic_h, w, h, ci->refi, m, core->refp, ci->pred, w, CHANNEL_L, BIT_DEPTH);
+    u32 best = get_smvd_cost(cost_type, w, h, widx, org, c.org->stride_luma, ci->pred[0], w, BIT_DEPTH) +
+               MV_COST(get_mv_bits_with_mvr(pi, 0, 0, (u8)mvr));
+    /* candidate 2: list 0's uni MV, list 1 mirrored about its MVP */
+    cand[REFP_0][MV_X] = mv_scale_l0[0];
+    cand[REFP_0][MV_Y] = mv_scale_l0[1];
+    d[MV_X] = cand[REFP_0][MV_X] - p[REFP_0][MV_X];
+    d[MV_Y] = cand[REFP_0][MV_Y] - p[REFP_0][MV_Y];
+    cand[REFP_1][MV_X] = p[REFP_1][MV_X] - d[MV_X];
+    cand[REFP_1][MV_Y] = p[REFP_1][MV_Y] - d[MV_Y];
+    com_mc_cu(x, y, pic_w, pic_h, w, h, ci->refi, cand, core->refp, ci->pred, w, CHANNEL_L, BIT_DEPTH);
+    u32 c2 = get_smvd_cost(cost_type, w, h, widx, org, c.org->stride_luma, ci->pred[0], w, BIT_DEPTH) +
+             MV_COST(get_mv_bits_with_mvr(pi, d[MV_X], d[MV_Y], (u8)mvr));
+    if (c2 < best) {
+        best = c2;
+        memcpy(m, cand, sizeof(m));
+    }
+    best = smvd_refine(core, x, y, log2w, log2h, m, p, ci->refi, 0, 1, best, 2, 8, ci->mvr_idx);
+    best = smvd_refine(core, x, y, log2w, log2h, m, p, ci->refi, 0, 1, best, 0, 1, ci->mvr_idx);
+    if (cost_type) {
+        com_mc_cu(x, y, pic_w, pic_h, w, h, ci->refi, m, core->refp, ci->pred, w, CHANNEL_L, BIT_DEPTH);
+        d[MV_X] = m[REFP_0][MV_X] - p[REFP_0][MV_X];
+        d[MV_Y] = m[REFP_0][MV_Y] - p[REFP_0][MV_Y];
+        best = com_had(w, h, org, c.org->stride_luma, ci->pred[0], w, BIT_DEPTH) +
+               MV_COST(get_mv_bits_with_mvr(pi, d[MV_X], d[MV_Y], (u8)mvr));
+    }
+    for (int k = 0; k < 4; k++) mv_out[k] = m[k >> 1][k & 1];
+    cu_free(&c);
+    return best;
 }

@@ -6,9 +6,12 @@ The oracle's analyze_bi composes pinned parts -- me_search_tz(bi = 1)
 (golden vectors here), com_mc_cu luma (tests/golden/mc_bd*.npz), the MV-bits
 table and tbl_refi_bits (tests/test_oracle.py) -- with the reference's loop
 (two rounds, strict '<' against the running best, the searched list becomes
-the fixed one, a round without a change ends it); analyze_bi itself needs
-the encoder's core state and is not called from the compiled reference, so
-the loop is checked against properties only (test_analyze_bi_properties)."""
+the fixed one, a round without a change ends it).  The loop is pinned by
+tests/golden/bi_loop_bd*.npz: the compiled reference's own analyze_bi,
+called through a minimal core_t and returning right after its search, for
+every PU of a frame-plan layout; the oracle and the GPU are both compared
+with it.  best_mecost is not an output of analyze_bi; it is the
+me_search_tz(bi = 1) cost of the winner, pinned by bi_bd*.npz."""
 import ctypes
 import os
 
@@ -218,3 +221,161 @@ def test_bi_and_smvd_reject_bad_plans():
     hip.plan_destroy(plan)
     for r in refs:
         hip.ref_destroy(r)
+
+
+def _loop_golden(bd):
+    return np.load(os.path.join(GOLD, "bi_loop_bd%d.npz" % bd), allow_pickle=False)
+
+
+def _loop_pus(g):
+    """(ctu, pu, x, y, w, h) of every PU of the fixture's plan layout."""
+    w, h, mask = int(g["w"]), int(g["h"]), int(g["mask"])
+    cols = (w + 127) // 128
+    n_ctu = cols * ((h + 127) // 128)
+    out = []
+    for ctu in range(n_ctu):
+        cx, cy = (ctu % cols) * 128, (ctu // cols) * 128
+        pu = 0
+        for s in range(len(SHAPES)):
+            if mask >> s & 1:
+                pw, ph = SHAPES[s]
+                for i in range((128 // pw) * (128 // ph)):
+                    out.append((ctu, pu, cx + (i % (128 // pw)) * pw, cy + (i // (128 // pw)) * ph, pw, ph))
+                    pu += 1
+    return n_ctu, pu, out
+
+
+def _loop_refs(g, n0, n1):
+    refs = list(g["refs"])
+    return refs[:n0] + refs[2:2 + n1]
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_analyze_bi_loop_golden(bd):
+    """The oracle's analyze_bi against the compiled reference's own
+    analyze_bi (tests/golden/bi_loop_bd*.npz, gen_golden_bi_loop.py): the
+    references, MVs and mot_bits after the search and every updated
+    mv_scale, on a spread sample of each configuration's PUs."""
+    g, o = _loop_golden(bd), Oracle(bd)
+    cur = g["cur"]
+    h, w = cur.shape
+    _, _, pus = _loop_pus(g)
+    checked = 0
+    for k in range(3):
+        n0, n1, mvr, ct = (int(v) for v in g["cfg_%d" % k])
+        planes = [o.make_planes(r) for r in _loop_refs(g, n0, n1)]
+        act = g["active_%d" % k]
+        cand = [p for p in pus if act[p[0], p[1]] and p[2] + p[4] <= w and p[3] + p[5] <= h]
+        for ctu, pu, x, y, pw, ph in cand[::7]:
+            refi = [int(v) for v in g["refi_%d" % k][ctu, pu]]
+            mv = g["mv_%d" % k][ctu, pu].astype(int).tolist()
+            mot = [int(v) for v in g["mot_%d" % k][ctu, pu]]
+            ms = g["mv_scale_%d" % k][ctu, pu].astype(int).tolist()
+            o.analyze_bi(cur, planes, n0, x, y, pw, ph, g["mvp_%d" % k][ctu, pu].astype(int).tolist(), ms, refi, mv,
+                         int(g["lidx_%d" % k][ctu, pu]), mot, mvr, ct, LAMBDA)
+            where = (k, ctu, pu, x, y, pw, ph)
+            assert refi == g["out_refi_%d" % k][ctu, pu].astype(int).tolist(), where
+            assert [list(v) for v in mv] == g["out_mv_%d" % k][ctu, pu].astype(int).tolist(), where
+            assert mot == g["out_mot_%d" % k][ctu, pu].astype(int).tolist(), where
+            assert [list(v) for v in ms] == g["out_mv_scale_%d" % k][ctu, pu].astype(int).tolist(), where
+            checked += 1
+    assert checked > 250
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bd", [8, 10])
+def test_bi_gpu_golden(bd):
+    """me_plan_run_bi on the fixture's inputs against the compiled
+    reference's analyze_bi directly: every active PU inside the picture."""
+    g = _loop_golden(bd)
+    cur = g["cur"]
+    h, w = cur.shape
+    n_ctu, npu, pus_l = _loop_pus(g)
+    hip = ua.MeHip(bd)
+    checked = 0
+    for k in range(3):
+        n0, n1, mvr, ct = (int(v) for v in g["cfg_%d" % k])
+        dref = [hip.ref_create(r) for r in _loop_refs(g, n0, n1)]
+        plan = hip.plan_create(w, h, 8, LAMBDA, int(g["mask"]), n0 + n1, num_refs_l1=n1)
+        hip.plan_set_cur(plan, cur)
+        assert (hip.lib.me_plan_num_ctus(plan), hip.lib.me_plan_num_pus(plan)) == (n_ctu, npu)
+        pus = np.zeros((n_ctu, npu), ua.BI_PU)
+        pus["refi"] = g["refi_%d" % k]
+        pus["lidx_ref"] = g["lidx_%d" % k]
+        pus["active"] = g["active_%d" % k]
+        pus["mv"] = g["mv_%d" % k]
+        pus["mot_bits"] = g["mot_%d" % k]
+        ms = np.ascontiguousarray(g["mv_scale_%d" % k]).copy()
+        tz = ua.TzCfg(16, 64, 0, mvr, ct, 8, (ctypes.c_int64 * 4)(0, 0, 0, 0))
+        hip.plan_run_bi(plan, dref, g["mvp_%d" % k], ms, pus, tz)
+        hip.plan_destroy(plan)
+        for r in dref:
+            hip.ref_destroy(r)
+        for ctu, pu, x, y, pw, ph in pus_l:
+            if not g["active_%d" % k][ctu, pu] or x + pw > w or y + ph > h:
+                continue
+            where = (k, ctu, pu, x, y, pw, ph)
+            assert pus[ctu, pu]["refi"].tolist() == g["out_refi_%d" % k][ctu, pu].tolist(), where
+            assert pus[ctu, pu]["mv"].tolist() == g["out_mv_%d" % k][ctu, pu].tolist(), where
+            assert pus[ctu, pu]["mot_bits"].tolist() == g["out_mot_%d" % k][ctu, pu].tolist(), where
+            assert ms[ctu, pu].tolist() == g["out_mv_scale_%d" % k][ctu, pu].tolist(), where
+            checked += 1
+    assert checked > 1500
+    assert hip.error_count() == 0
+
+
+@pytest.mark.skipif(not RefLib.available(8), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("bd", [8, 10])
+def test_analyze_bi_and_smvd_live(bd):
+    """Random single PUs of every shape, lists of 1-2 references, every AMVR
+    precision and cost type: the oracle's analyze_bi and analyze_smvd equal
+    the compiled reference (C+SSE+AVX2 table) on seeds the fixtures do not
+    hold."""
+    o, ref = Oracle(bd), RefLib(bd, 2)
+    rng = np.random.default_rng(40 + bd)
+    top = (1 << bd) - 1
+    H, W = 96, 160
+    yy, xx = np.mgrid[0:H, 0:W]
+
+    def pic(dx, dy):
+        b = 0.5 + 0.25 * np.sin((xx - dx) / 6.0) * np.cos((yy - dy) / 4.0) + 0.1 * np.sin((xx + yy) / 2.5)
+        return np.clip(np.rint(b * top + rng.normal(0, top / 120, (H, W))), 0, top).astype(
+            np.uint8 if bd == 8 else np.uint16)
+
+    cur, refs = pic(0, 0), [pic(2 * k + 1, -k) for k in range(4)]
+    planes = [o.make_planes(r) for r in refs]
+    pics = [ref.pic_create(r, p) for r, p in zip(refs, (-1, -2, 1, 2))]
+    done = 0
+    while done < 40:
+        w, h = SHAPES[int(rng.integers(len(SHAPES)))]
+        if w > W or h > H:
+            continue
+        x, y = int(rng.integers(0, W - w + 1)), int(rng.integers(0, H - h + 1))
+        n0, n1 = int(rng.integers(1, 3)), int(rng.integers(1, 3))
+        mvr, ct, lr = int(rng.integers(0, 5)), int(rng.integers(0, 2)), int(rng.integers(0, 2))
+        gq = 1 << mvr
+        refi = [int(rng.integers(0, n0)), int(rng.integers(0, n1))]
+        mv = (rng.integers(-16, 17, (2, 2)) * gq).astype(int)
+        mot = [int(v) for v in rng.integers(2, 30, 2)]
+        mvp4, ms4 = np.zeros((2, 4, 2), np.int16), np.zeros((2, 4, 2), np.int16)
+        mvp4[0, :n0], mvp4[1, :n1] = rng.integers(-24, 25, (n0, 2)) * gq, rng.integers(-24, 25, (n1, 2)) * gq
+        ms4[0, :n0], ms4[1, :n1] = rng.integers(-16, 17, (n0, 2)) * gq, rng.integers(-16, 17, (n1, 2)) * gq
+        r_, m_, b_, s_ = ref.analyze_bi(cur, pics[:n0], pics[2:2 + n1], x, y, w, h, mvp4, ms4, refi, mv, lr, mot, mvr,
+                                        ct, LAMBDA)
+        ms = np.concatenate([ms4[0, :n0], ms4[1, :n1]]).astype(int).tolist()
+        orefi, omv, omot = list(refi), mv.tolist(), list(mot)
+        o.analyze_bi(cur, planes[:n0] + planes[2:2 + n1], n0, x, y, w, h,
+                     np.concatenate([mvp4[0, :n0], mvp4[1, :n1]]).astype(int).tolist(), ms, orefi, omv, lr, omot,
+                     mvr, ct, LAMBDA)
+        where = (x, y, w, h, n0, n1, mvr, ct, lr)
+        assert orefi == r_.tolist() and [list(v) for v in omv] == m_.tolist() and omot == b_.tolist(), where
+        assert [list(v) for v in ms] == np.concatenate([s_[0, :n0], s_[1, :n1]]).tolist(), where
+        mvp2 = (rng.integers(-12, 13, (2, 2)) * gq).astype(np.int16)
+        ms0 = (rng.integers(-12, 13, 2) * gq).astype(np.int16)
+        m_, c_ = ref.smvd(cur, pics[0], pics[2], x, y, w, h, mvp2, ms0, mvr, ct, LAMBDA)
+        mv2, c2 = o.analyze_smvd(cur, [planes[0], planes[2]], 1, x, y, w, h, mvp2.astype(int),
+                                 np.array([ms0, [0, 0]]).astype(int), mvr, ct, LAMBDA)
+        assert (mv2, c2) == (m_.tolist(), c_), where
+        done += 1
+    for p in pics:
+        ref.pic_free(p)

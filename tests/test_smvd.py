@@ -4,10 +4,13 @@ PU, and the oracle against what symmetric motion fixes.
 
 The oracle composes pinned parts -- com_mc_cu's bi-average
 (tests/golden/mc_bd*.npz), com_had and SAD (tests/golden/ref_bd*.npz), the
-MV-bits table -- with the reference's candidate loop; analyze_smvd needs the
-encoder's core state and is not called from the compiled reference, so the
-loop is checked against properties only."""
+MV-bits table -- with the reference's candidate loop.  The loop is pinned by
+tests/golden/smvd_bd*.npz: the compiled reference's SMVD search (its own
+smvd_refine after analyze_smvd's first two candidates, through a minimal
+core_t) for every PU of a frame-plan layout, which the oracle and the GPU
+are both compared with."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -16,6 +19,7 @@ from oracle.oracle import Oracle, SHAPES
 import uavs3e_amd as ua
 
 LAMBDA = 314169
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _pic(rng, h, w, bd, dx=0, dy=0, noise=1.0):
@@ -103,4 +107,88 @@ def test_smvd_gpu_vs_oracle(bd, n0, n1, mvr, ct):
                     assert pus[ctu, pu].tobytes() == pus0[ctu, pu].tobytes()
                 pu += 1
     assert checked > 300
+    assert hip.error_count() == 0
+
+
+def _smvd_golden(bd):
+    return np.load(os.path.join(GOLD, "smvd_bd%d.npz" % bd), allow_pickle=False)
+
+
+def _pus(g):
+    w, h, mask = int(g["w"]), int(g["h"]), int(g["mask"])
+    cols = (w + 127) // 128
+    n_ctu = cols * ((h + 127) // 128)
+    out = []
+    for ctu in range(n_ctu):
+        cx, cy = (ctu % cols) * 128, (ctu // cols) * 128
+        pu = 0
+        for s in range(len(SHAPES)):
+            if mask >> s & 1:
+                pw, ph = SHAPES[s]
+                for i in range((128 // pw) * (128 // ph)):
+                    out.append((ctu, pu, cx + (i % (128 // pw)) * pw, cy + (i // (128 // pw)) * ph, pw, ph))
+                    pu += 1
+    return n_ctu, pu, out
+
+
+@pytest.mark.parametrize("bd", [8, 10])
+def test_smvd_oracle_golden(bd):
+    """The oracle's analyze_smvd against the compiled reference's SMVD search
+    (its own smvd_refine; tests/golden/smvd_bd*.npz, gen_golden_bi_loop.py)
+    on a spread sample of each configuration's PUs: both MVs and mecost."""
+    g, o = _smvd_golden(bd), Oracle(bd)
+    cur = g["cur"]
+    h, w = cur.shape
+    refs = list(g["refs"])
+    _, _, pus = _pus(g)
+    checked = 0
+    for k in range(3):
+        n0, n1, mvr, ct = (int(v) for v in g["cfg_%d" % k])
+        lrefs = refs[:n0] + refs[2:2 + n1]
+        planes = [o.make_planes(r) for r in lrefs]
+        act = g["active_%d" % k]
+        cand = [p for p in pus if act[p[0], p[1]] and p[2] + p[4] <= w and p[3] + p[5] <= h]
+        for ctu, pu, x, y, pw, ph in cand[::7]:
+            mv, cost = o.analyze_smvd(cur, planes, n0, x, y, pw, ph, g["mvp_%d" % k][ctu, pu].astype(int),
+                                      g["mv_scale_%d" % k][ctu, pu].astype(int), mvr, ct, LAMBDA)
+            where = (k, ctu, pu, x, y, pw, ph)
+            assert mv == g["out_mv_%d" % k][ctu, pu].astype(int).tolist(), where
+            assert cost == int(g["out_cost_%d" % k][ctu, pu]), where
+            checked += 1
+    assert checked > 250
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bd", [8, 10])
+def test_smvd_gpu_golden(bd):
+    """me_plan_run_smvd on the fixture's inputs against the compiled
+    reference's SMVD search directly: every active PU inside the picture."""
+    g = _smvd_golden(bd)
+    cur = g["cur"]
+    h, w = cur.shape
+    refs = list(g["refs"])
+    n_ctu, npu, pus_l = _pus(g)
+    hip = ua.MeHip(bd)
+    checked = 0
+    for k in range(3):
+        n0, n1, mvr, ct = (int(v) for v in g["cfg_%d" % k])
+        dref = [hip.ref_create(r) for r in refs[:n0] + refs[2:2 + n1]]
+        plan = hip.plan_create(w, h, 8, LAMBDA, int(g["mask"]), n0 + n1, num_refs_l1=n1)
+        hip.plan_set_cur(plan, cur)
+        assert (hip.lib.me_plan_num_ctus(plan), hip.lib.me_plan_num_pus(plan)) == (n_ctu, npu)
+        pus = np.zeros((n_ctu, npu), ua.SMVD_PU)
+        pus["active"] = g["active_%d" % k]
+        tz = ua.TzCfg(16, 64, 0, mvr, ct, 8, (ctypes.c_int64 * 4)(0, 0, 0, 0))
+        hip.plan_run_smvd(plan, dref, g["mvp_%d" % k], g["mv_scale_%d" % k], pus, tz)
+        hip.plan_destroy(plan)
+        for r in dref:
+            hip.ref_destroy(r)
+        for ctu, pu, x, y, pw, ph in pus_l:
+            if not g["active_%d" % k][ctu, pu] or x + pw > w or y + ph > h:
+                continue
+            where = (k, ctu, pu, x, y, pw, ph)
+            assert pus[ctu, pu]["mv"].tolist() == g["out_mv_%d" % k][ctu, pu].tolist(), where
+            assert int(pus[ctu, pu]["cost"]) == int(g["out_cost_%d" % k][ctu, pu]), where
+            checked += 1
+    assert checked > 1500
     assert hip.error_count() == 0
