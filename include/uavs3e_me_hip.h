@@ -289,6 +289,40 @@ int me_affine_mc_luma(me_ref_t ref, const me_affine_cu_t *cus, int n, me_pel *pr
 int me_affine_mc_chroma(me_ref_t ref, const me_affine_cu_t *cus, int n, me_pel *pred_u, me_pel *pred_v,
                         uint64_t chroma_elems, void *stream);
 
+/* Affine gradient motion search, affine_me_gradient (inter.c:1339-1471),
+ * for a batch of CUs against one reference: from each CU's start control
+ * points (two, the 4-parameter model), up to 7 iterations (5 with bi) of --
+ * the affine luma prediction (as me_affine_mc_luma), the Sobel gradients of
+ * the prediction and the 4-parameter least-squares system with the residual
+ * (affine_sobel_flt_hor/ver, affine_coef_computer, com_util.c:2966-3049),
+ * solved in IEEE double as solve_equal does (inter.c:1280-1315), the
+ * control-point deltas rounded to the AMVR precision; zero deltas end the
+ * search; moved control points (clipped to 18 bits) are kept when
+ * MV_COST(affine_mv_bits (+ the other list's mot_bits with bi)) + (com_had
+ * >> bi) is strictly lower.  Per CU: mv receives the best control points
+ * and cost the best cost without its MV bits (the function's return value).
+ * org (host memory, org_elems samples) holds each CU's w x h original at
+ * org_offset with org_stride samples per row: the current picture at the
+ * CU for the uni-prediction search, the bi-prediction target of
+ * analyze_affine_bi (inter.c:1760-1790) with bi = 1.  CU sizes 16..128
+ * (powers of two), sub-blocks 4 or 8, mvr 0..2 (Tab_Affine_AMVR: 1/4, 1,
+ * 1/16 pel), num_refp 1..4.  Blocking. */
+typedef struct me_affine_me_cu_t {
+    int32_t x, y, w, h;
+    int32_t sub_w, sub_h;
+    int32_t refi, num_refp;     /* tbl_refi_bits of the reference */
+    int32_t mvr;                /* affine AMVR index 0..2 */
+    int32_t bi;                 /* 1: the bi-prediction form */
+    int32_t mot_bits_other;     /* bi: the other list's mot_bits */
+    int32_t org_stride;
+    int32_t mvp[2][2];          /* control-point predictors, 1/16 pel */
+    int32_t mv[2][2];           /* in: start control points; out: the best */
+    uint64_t org_offset;
+    uint64_t cost;              /* out */
+} me_affine_me_cu_t;
+int me_affine_me_gradient(me_ref_t ref, const me_pel *org, uint64_t org_elems, me_affine_me_cu_t *cus, int n,
+                          uint32_t lambda_mv, void *stream);
+
 /* Bi-prediction ME of analyze_bi (inter.c:972-1033), the search part, for
  * every PU of the plan that has references in both lists: starting from the
  * uni-prediction results, up to two rounds of -- the prediction of the fixed

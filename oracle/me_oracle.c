@@ -1058,3 +1058,155 @@ EXPORT void orc_affine_chroma(int x, int y, int pic_w, int pic_h, int cu_w, int 
             }
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Affine gradient ME (affine_me_gradient, inter.c:1339-1471)                */
+/* ------------------------------------------------------------------------ */
+
+static int ilog2(int v) { int l = 0; while ((2 << l) <= v) l++; return l; }
+
+/* Control-point MV bits (affine_mv_bits, inter.c:1317-1337): the reference
+ * index bits, then per control point 2 * (floor(log2(|dx|+1)) +
+ * floor(log2(|dy|+1))) + 2 + (dx != 0) + (dy != 0) with d = MVD >> shift. */
+static int affine_bits(const int mv[2][2], const int mvp[2][2], int num_refp, int refi, int shift)
+{
+    int bits = orc_refi_bits(num_refp, refi);
+    for (int v = 0; v < 2; v++) {
+        const int dx = abs((mv[v][0] - mvp[v][0]) >> shift), dy = abs((mv[v][1] - mvp[v][1]) >> shift);
+        bits += 2 * (ilog2(dx + 1) + ilog2(dy + 1)) + 2 + (dx != 0) + (dy != 0);
+    }
+    return bits;
+}
+
+/* MV_COST (inter.c:41): u32 arithmetic. */
+static u32 mv_cost32(u32 lambda_mv, int bits) { return (lambda_mv * (u32)bits + 32768u) >> 16; }
+
+/* Sobel gradients of the prediction (affine_sobel_flt_hor/ver,
+ * com_util.c:2966-3018): the 3x3 kernels at every interior sample; a
+ * border sample takes the value of the nearest interior one. */
+static int sobel(const pel *p, int w, int h, int j, int k, int vertical)
+{
+    j = CLIP3(1, h - 2, j);
+    k = CLIP3(1, w - 2, k);
+    const pel *c = p + j * w + k;
+    if (!vertical) return (c[-w + 1] - c[-w - 1]) + 2 * (c[1] - c[-1]) + (c[w + 1] - c[w - 1]);
+    return (c[w - 1] - c[-w - 1]) + 2 * (c[w] - c[-w]) + (c[w + 1] - c[-w + 1]);
+}
+
+/* The 4-parameter least-squares system of one iteration
+ * (affine_coef_computer, com_util.c:3020-3049): for every sample, with the
+ * gradients (gx, gy), its coordinates (k, j) and the residual e,
+ * g = (gx, k gx + j gy, gy, j gx - k gy); A[c][r] += g_c g_r and
+ * b[c] += 8 g_c e, summed exactly in 64 bits. */
+static void affine_system(const pel *org, int s_org, const pel *pred, int w, int h, s64 A[4][4], s64 b[4])
+{
+    memset(A, 0, sizeof(s64) * 16);
+    memset(b, 0, sizeof(s64) * 4);
+    for (int j = 0; j < h; j++)
+        for (int k = 0; k < w; k++) {
+            const int gx = sobel(pred, w, h, j, k, 0), gy = sobel(pred, w, h, j, k, 1);
+            const int g[4] = {gx, k * gx + j * gy, gy, j * gx - k * gy};
+            const s64 e = (s64)((int)org[j * s_org + k] - (int)pred[j * w + k]);
+            for (int c = 0; c < 4; c++) {
+                for (int r = 0; r < 4; r++) A[c][r] += (s64)g[c] * g[r];
+                b[c] += (s64)g[c] * e * 8;
+            }
+        }
+}
+
+/* solve_equal (inter.c:1280-1315) of the 4x4 system [A | b]: Gaussian
+ * elimination with partial pivoting in IEEE double, in the reference's
+ * exact operation order (the result is defined by its roundings): rows 1..4
+ * of m hold the equations, row 0 is the swap buffer; for each column c
+ * the first row of largest |m[.][c]| at or below row c + 1 becomes the
+ * pivot, and every element right of the column is reduced as
+ * m[j][k] - m[i][k] * m[j][c] / m[i][c]; back substitution sums the known
+ * terms from left to right. */
+static void solve4(const s64 A[4][4], const s64 b[4], double x[4])
+{
+    double m[5][5];
+    memset(m, 0, sizeof(m));
+    for (int r = 0; r < 4; r++) {
+        for (int c = 0; c < 4; c++) m[r + 1][c] = (double)A[r][c];
+        m[r + 1][4] = (double)b[r];
+    }
+    const int n = 4;
+    for (int i = 1; i < n; i++) {
+        int piv = i;
+        double big = fabs(m[i][i - 1]);
+        for (int j = i + 1; j <= n; j++)
+            if (fabs(m[j][i - 1]) > big) { big = fabs(m[j][i - 1]); piv = j; }
+        if (piv != i)
+            for (int c = 0; c <= n; c++) { m[0][c] = m[i][c]; m[i][c] = m[piv][c]; m[piv][c] = m[0][c]; }
+        for (int j = i + 1; j <= n; j++)
+            for (int k = i; k <= n; k++) m[j][k] = m[j][k] - m[i][k] * m[j][i - 1] / m[i][i - 1];
+    }
+    x[n - 1] = m[n][n] / m[n][n - 1];
+    for (int i = n - 2; i >= 0; i--) {
+        double t = 0;
+        for (int j = i + 1; j < n; j++) t += m[i + 1][j] * x[j];
+        x[i] = (m[i + 1][n] - t) / m[i + 1][i];
+    }
+}
+
+/* (s16)(v * scale + (v >= 0 ? 0.5 : -0.5)) as the reference computes it
+ * (inter.c:1413-1422). */
+static s16 round_delta(double v, double scale) { return (s16)(v * scale + (v >= 0 ? 0.5 : -0.5)); }
+
+/* affine_me_gradient (inter.c:1339-1471) with two control points for the
+ * cu_w x cu_h CU at (x, y): start from mv; per iteration (7, or 5 with bi)
+ * solve the gradient system of the current prediction for the change of
+ * the 4-parameter model, turn it into control-point deltas at the AMVR
+ * precision (Tab_Affine_AMVR: mvr 0 -> 1/4, 1 -> 1, 2 -> 1/16 pel), stop
+ * when both deltas are zero, else move and clip the control points
+ * (18-bit CPMVs, a clipped maximum rounded down to the precision) and keep
+ * them when their cost MV_COST(bits) + (com_had >> bi) is strictly lower.
+ * mv receives the best control points; returns that cost without its MV
+ * bits.  plane0: the padded integer plane of the reference at (0, 0);
+ * org / s_org: the CU's original (or bi target). */
+EXPORT u64 orc_affine_me_gradient(int x, int y, int pic_w, int pic_h, int cu_w, int cu_h, int sub_w, int sub_h,
+                                  const pel *plane0, int i_ref, const pel *org, int s_org, int refi, int num_refp,
+                                  int mvr, int bi, int mot_bits_other, u32 lambda_mv, const int mvp[2][2],
+                                  int mv[2][2], int bd)
+{
+    static pel pred[128 * 128];
+    const int shift = mvr == 0 ? 2 : (mvr == 1 ? 4 : 0);
+    const int iters = bi ? 5 : 7;
+    const int CPMV_LO = -131072, CPMV_HI = 131071;
+    int cp[3][2] = {{mv[0][0], mv[0][1]}, {mv[1][0], mv[1][1]}, {0, 0}};
+    orc_affine_luma(x, y, pic_w, pic_h, cu_w, cu_h, cp, 2, plane0, i_ref, pred, sub_w, sub_h, bd);
+    int best_bits = affine_bits((const int(*)[2])cp, mvp, num_refp, refi, shift) + (bi ? mot_bits_other : 0);
+    u64 best = (u64)mv_cost32(lambda_mv, best_bits) + (orc_had(cu_w, cu_h, org, s_org, pred, cu_w, bd) >> bi);
+    for (int it = 0; it < iters; it++) {
+        s64 A[4][4], b[4];
+        double p[4];
+        affine_system(org, s_org, pred, cu_w, cu_h, A, b);
+        solve4((const s64(*)[4])A, b, p);
+        const double d[4] = {p[0], p[2], p[1] * cu_w + p[0], -p[3] * cu_w + p[2]};   /* v0x, v0y, v1x, v1y */
+        s16 dv[4];
+        for (int i = 0; i < 4; i++) {
+            if (shift == 0) {
+                dv[i] = round_delta(d[i], 16);
+            } else {
+                dv[i] = (s16)(round_delta(d[i], 4) * 4);                   /* quarter pel -> 1/16 pel */
+                dv[i] = (s16)(((dv[i] + (1 << (shift - 1))) >> shift) << shift);
+            }
+        }
+        if (!dv[0] && !dv[1] && !dv[2] && !dv[3]) break;
+        for (int v = 0; v < 2; v++)
+            for (int c = 0; c < 2; c++) {
+                int t = CLIP3(CPMV_LO, CPMV_HI, cp[v][c] + dv[2 * v + c]);
+                if (t == CPMV_HI) t = (t >> shift) << shift;
+                cp[v][c] = t;
+            }
+        orc_affine_luma(x, y, pic_w, pic_h, cu_w, cu_h, cp, 2, plane0, i_ref, pred, sub_w, sub_h, bd);
+        const int bits = affine_bits((const int(*)[2])cp, mvp, num_refp, refi, shift) + (bi ? mot_bits_other : 0);
+        const u64 cost = (u64)mv_cost32(lambda_mv, bits) + (orc_had(cu_w, cu_h, org, s_org, pred, cu_w, bd) >> bi);
+        if (cost < best) {
+            best = cost;
+            best_bits = bits;
+            for (int v = 0; v < 2; v++) { mv[v][0] = cp[v][0]; mv[v][1] = cp[v][1]; }
+        }
+    }
+    return best - mv_cost32(lambda_mv, best_bits);
+}

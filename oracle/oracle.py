@@ -205,6 +205,22 @@ class Oracle(_Lib):
                                    cs, ptr(u), ptr(v), self.bd)
         return u, v
 
+    def affine_me_gradient(self, planes, pic_w, pic_h, org, x, y, cu_w, cu_h, sub_w, sub_h, refi, num_refp, mvr, bi,
+                           mot_bits_other, lambda_mv, mvp, mv, pad=PAD):
+        """affine_me_gradient (inter.c:1339-1471) from padded planes: org the
+        CU's (cu_h, cu_w) original (or bi target); mvp, mv [[x0, y0], [x1,
+        y1]] in 1/16 pel.  Returns (cost without the MV bits, best mv)."""
+        stride = pic_w + 2 * pad
+        org = np.ascontiguousarray(org, self.dt)
+        mvp_a = (C.c_int * 4)(*[int(v) for v in np.asarray(mvp).reshape(-1)])
+        mv_a = (C.c_int * 4)(*[int(v) for v in np.asarray(mv).reshape(-1)])
+        f = self.lib.orc_affine_me_gradient
+        f.restype = C.c_uint64
+        c = f(x, y, pic_w, pic_h, cu_w, cu_h, sub_w, sub_h,
+              C.c_void_p(planes[0].ctypes.data + (pad * stride + pad) * planes.itemsize), stride, ptr(org),
+              org.shape[1], refi, num_refp, mvr, bi, mot_bits_other, C.c_uint32(lambda_mv), mvp_a, mv_a, self.bd)
+        return int(c), [[mv_a[0], mv_a[1]], [mv_a[2], mv_a[3]]]
+
     def luma_taps_hp(self, frac):
         return np.ctypeslib.as_array(C.cast(self.lib.orc_luma_taps_hp(frac), C.POINTER(C.c_int8)), (8,)).copy()
 
@@ -667,6 +683,20 @@ class RefLib(_Lib):
         c = self.lib.refh_smvd(ptr(cur), pw, ph, C.c_void_p(p0), C.c_void_p(p1), x, y, w, h, ptr(mvp_a), ptr(ms), mvr,
                                cost_type, C.c_uint32(lambda_mv), ptr(out))
         return out, int(c)
+
+    def affine_me_gradient(self, pic, org, x, y, cu_w, cu_h, sub_w, sub_h, refi, num_refp, mvr, bi, mot_bits_other,
+                           lambda_mv, mvp, mv):
+        """The compiled reference's affine_me_gradient (inter.c:1339-1471);
+        returns (cost without the MV bits, best mv [[x0, y0], [x1, y1]])."""
+        org = np.ascontiguousarray(org, self.dt)
+        mvp_a = (C.c_int * 4)(*[int(v) for v in np.asarray(mvp).reshape(-1)])
+        mv_a = (C.c_int * 4)(*[int(v) for v in np.asarray(mv).reshape(-1)])
+        f = self.lib.refh_affine_me_gradient
+        f.restype = C.c_uint64
+        f.argtypes = [C.c_void_p, C.c_void_p] + [C.c_int] * 12 + [C.c_uint32, C.c_void_p, C.c_void_p]
+        c = f(pic, ptr(org), org.shape[1], x, y, cu_w, cu_h, sub_w, sub_h, refi, num_refp, mvr, bi, mot_bits_other,
+              lambda_mv, mvp_a, mv_a)
+        return int(c), [[mv_a[0], mv_a[1]], [mv_a[2], mv_a[3]]]
 
     def loka_cost(self, cur, pics_l0, pics_l1, cur_poc, gop, msr, is_padding):
         """loka_estimate_coding_cost (lookahead.c:82-290) of the compiled

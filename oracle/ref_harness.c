@@ -735,3 +735,44 @@ RH_EXPORT u32 refh_smvd(const pel *cur, int pic_w, int pic_h, void *p0, void *p1
     cu_free(&c);
     return best;
 }
+
+/* affine_me_gradient (inter.c:1339-1471) of the reference itself for the
+ * cu_w x cu_h CU at (x, y) against `ref`: org (stride i_org) is the CU's
+ * original (the current picture at the CU, or analyze_affine_bi's target
+ * for bi), mvp / mv the two control-point MVs (1/16 pel) as CPMV[2][2];
+ * mv receives the best control points.  refi / num_refp index
+ * tbl_refi_bits, mvr is the affine AMVR index (0..2), bi selects the
+ * bi-prediction form (5 iterations, distortion halved, mot_bits_other
+ * added to the bits).  Returns the cost without its MV bits, as the
+ * function does. */
+RH_EXPORT u64 refh_affine_me_gradient(void *ref, const pel *org, int i_org, int x, int y, int cu_w, int cu_h,
+                                      int sub_w, int sub_h, int refi, int num_refp, int mvr, int bi,
+                                      int mot_bits_other, u32 lambda_mv, const int *mvp, int *mv)
+{
+    refh_pic_t *r = (refh_pic_t *)ref;
+    refh_search_t s;
+    search_setup(&s, r->w, r->h, lambda_mv, 0, 1);
+    inter_search_t *pi = &s.pi;
+    pi->ref_pic = r->pic;
+    pi->org = (pel *)org;
+    pi->i_org = i_org;
+    pi->num_refp = (u8)num_refp;
+    pi->curr_mvr = (u8)mvr;
+    const int lidx = 0;
+    pi->mot_bits[1 - lidx] = mot_bits_other;
+    CPMV p[VER_NUM][MV_D], m[VER_NUM][MV_D];
+    memset(p, 0, sizeof(p));
+    memset(m, 0, sizeof(m));
+    for (int v = 0; v < 2; v++) {
+        p[v][MV_X] = mvp[2 * v]; p[v][MV_Y] = mvp[2 * v + 1];
+        m[v][MV_X] = mv[2 * v];  m[v][MV_Y] = mv[2 * v + 1];
+    }
+    s8 ri = (s8)refi;
+    u64 c = affine_me_gradient(pi, x, y, CONV_LOG2(cu_w), CONV_LOG2(cu_h), &ri, lidx, p, m, bi, sub_w, sub_h);
+    for (int v = 0; v < 2; v++) {
+        mv[2 * v] = m[v][MV_X];
+        mv[2 * v + 1] = m[v][MV_Y];
+    }
+    search_free(&s);
+    return c;
+}

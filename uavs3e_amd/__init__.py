@@ -123,6 +123,13 @@ AFFINE_CU = np.dtype([("x", np.int32), ("y", np.int32), ("w", np.int32), ("h", n
                       ("cpmv", np.int32, (3, 2)), ("pred_offset", np.uint64), ("chroma_offset", np.uint64)],
                      align=True)
 
+# me_affine_me_cu_t
+AFFINE_ME_CU = np.dtype([("x", np.int32), ("y", np.int32), ("w", np.int32), ("h", np.int32), ("sub_w", np.int32),
+                         ("sub_h", np.int32), ("refi", np.int32), ("num_refp", np.int32), ("mvr", np.int32),
+                         ("bi", np.int32), ("mot_bits_other", np.int32), ("org_stride", np.int32),
+                         ("mvp", np.int32, (2, 2)), ("mv", np.int32, (2, 2)), ("org_offset", np.uint64),
+                         ("cost", np.uint64)], align=True)
+
 # me_smvd_pu_t
 SMVD_PU = np.dtype([("active", np.int8), ("pad", np.int8, 3), ("mv", np.int16, (2, 2)), ("cost", np.uint32)],
                    align=True)
@@ -366,6 +373,15 @@ class MeHip:
         cus = np.ascontiguousarray(cus)
         self.check(self.lib.me_affine_mc_chroma(ref, _p(cus), len(cus), _p(pred_u), _p(pred_v), pred_u.size, None),
                    "me_affine_mc_chroma")
+
+    def affine_me_gradient(self, ref, org, cus, lambda_mv):
+        """me_affine_me_gradient: cus an AFFINE_ME_CU array (mv and cost
+        updated in place), org a flat array of the CUs' originals."""
+        assert cus.dtype == AFFINE_ME_CU and AFFINE_ME_CU.itemsize == 96 and cus.flags.c_contiguous
+        org = np.ascontiguousarray(org, self.dt)
+        f = self.lib.me_affine_me_gradient
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int, C.c_uint32, C.c_void_p]
+        self.check(f(ref, _p(org), org.size, _p(cus), len(cus), lambda_mv, None), "me_affine_me_gradient")
 
     def plan_run_smvd(self, plan, refs, mvp, mv_scale, pus, tz, stream=None):
         """me_plan_run_smvd: mvp, mv_scale int16 [ctu][pu][ref][2]; pus an

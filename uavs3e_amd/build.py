@@ -42,7 +42,7 @@ def host_lib(bd):
 # Translation units of the HIP library: (source, extra defines).  The
 # integer-search and sub-pel kernels are split by instance so that the
 # expensive device compiles run in parallel.
-HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []), ("server.hip", []), ("mc.hip", []), ("tz.hip", ["-DME_TZ_PART=0"]), ("tz.hip", ["-DME_TZ_PART=1"]), ("tz.hip", ["-DME_TZ_PART=2"]),
+HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []), ("server.hip", []), ("mc.hip", []), ("affine_me.hip", []), ("tz.hip", ["-DME_TZ_PART=0"]), ("tz.hip", ["-DME_TZ_PART=1"]), ("tz.hip", ["-DME_TZ_PART=2"]),
              ("intsearch.hip", ["-DME_IS_WAVES=1"]), ("intsearch.hip", ["-DME_IS_WAVES=2"]),
              ("intsearch.hip", ["-DME_IS_WAVES=4"]),
              ("subpel.hip", [])]

@@ -17,6 +17,7 @@
 // origin is >= -66, filter taps reach 1 sample before and 2 after, and a
 // 64-sample block starting at (pic+4)/2 ends 68 samples past the edge).
 #include "frame.h"
+#include "affine.h"
 
 #include <string.h>
 #include <algorithm>
@@ -220,18 +221,6 @@ int me_mc_enqueue(const MeSearch *s, const JobSet &js, const me_pu_result_t *res
 // Affine motion compensation, luma (com_mc_blk_affine_luma, com_mc.c:202-250)
 // ---------------------------------------------------------------------------
 
-// The 1/16-pel luma interpolation filter (com_tbl_mc_l_coeff_hp,
-// com_tables.c:130-147): the standard's table.
-__constant__ static const s8 AFF_TAPS[16][8] = {
-    {0, 0, 0, 64, 0, 0, 0, 0},        {0, 1, -3, 63, 4, -2, 1, 0},      {-1, 2, -5, 62, 8, -3, 1, 0},
-    {-1, 3, -8, 60, 13, -4, 1, 0},    {-1, 4, -10, 58, 17, -5, 1, 0},   {-1, 4, -11, 52, 26, -8, 3, -1},
-    {-1, 3, -9, 47, 31, -10, 4, -1},  {-1, 4, -11, 45, 34, -10, 4, -1}, {-1, 4, -11, 40, 40, -11, 4, -1},
-    {-1, 4, -10, 34, 45, -11, 4, -1}, {-1, 4, -10, 31, 47, -9, 3, -1},  {-1, 3, -8, 26, 52, -11, 4, -1},
-    {0, 1, -5, 17, 58, -10, 4, -1},   {0, 1, -4, 13, 60, -8, 3, -1},    {0, 1, -3, 8, 62, -5, 2, -1},
-    {0, 1, -2, 4, 63, -3, 1, 0}};
-
-__device__ __forceinline__ int aff_round7(int v) { return v >= 0 ? (v + 64) >> 7 : -((-v + 64) >> 7); }
-
 // The 1/32-pel chroma interpolation filter (com_tbl_mc_c_coeff_hp,
 // com_tables.c:156-189): the standard's table.
 __constant__ static const s8 AFF_CTAPS[32][4] = {
@@ -312,56 +301,15 @@ __global__ __launch_bounds__(256) void k_affine_chroma(const me_affine_cu_t *cus
 }
 
 // One output sample per thread; grid (blocks for the batch's largest CU, n):
-// block b of CU c covers its samples [256 b, 256 b + 256).  The sub-block MV (com_mc.c:224-241) is
-// recomputed per sample; the 1/16 phase picks a copy, a one-direction 8-tap
-// filter, or the two-pass filter (as ipflt / ipflt_ext, com_mc.c:461-657).
+// block b of CU c covers its samples [256 b, 256 b + 256).
 __global__ __launch_bounds__(256) void k_affine_luma(const me_affine_cu_t *cus, const pel *plane0, int stride,
                                                      int pic_w, int pic_h, pel *pred)
 {
     const me_affine_cu_t c = cus[blockIdx.y];
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= c.w * c.h) return;
-    const int sx = k % c.w, sy = k / c.w;
-    const int w0 = sx - sx % c.sub_w, h0 = sy - sy % c.sub_h;
-    const int lw = __builtin_ctz((unsigned)c.w), lh = __builtin_ctz((unsigned)c.h);
-    const int dhx = ((c.cpmv[1][0] - c.cpmv[0][0]) * 128) >> lw, dhy = ((c.cpmv[1][1] - c.cpmv[0][1]) * 128) >> lw;
-    const bool three = c.vertex_num == 3;
-    const int dvx = three ? ((c.cpmv[2][0] - c.cpmv[0][0]) * 128) >> lh : -dhy;
-    const int dvy = three ? ((c.cpmv[2][1] - c.cpmv[0][1]) * 128) >> lh : dhx;
-    int px = w0 + c.sub_w / 2, py = h0 + c.sub_h / 2;
-    if (w0 == 0 && h0 == 0) { px = 0; py = 0; }
-    else if (w0 + c.sub_w == c.w && h0 == 0) { px = c.w; py = 0; }
-    else if (w0 == 0 && h0 + c.sub_h == c.h && three) { px = 0; py = c.h; }
-    int mx = aff_round7(c.cpmv[0][0] * 128 + dhx * px + dvx * py);
-    int my = aff_round7(c.cpmv[0][1] * 128 + dhy * px + dvy * py);
-    mx = me_clip3(-131072, 131071, mx);
-    my = me_clip3(-131072, 131071, my);
-    const int gx = ((c.x + w0) << 4) + mx, gy = ((c.y + h0) << 4) + my;
-    const int fx = gx & 15, fy = gy & 15;
-    const int ix = me_clip3(-ME_MAX_CU - 4, pic_w + 4, gx >> 4), iy = me_clip3(-ME_MAX_CU - 4, pic_h + 4, gy >> 4);
-    const pel *src = plane0 + (ptrdiff_t)(iy + sy - h0) * stride + (ix + sx - w0);
-    const int maxv = (1 << BIT_DEPTH) - 1;
-    int v;
-    if (fx == 0 && fy == 0) {
-        v = src[0];
-    } else if (fy == 0 || fx == 0) {
-        const int step = fy == 0 ? 1 : stride, f = fy == 0 ? fx : fy;
-        int t = 0;
-        for (int j = 0; j < 8; j++) t += (int)src[(j - 3) * step] * AFF_TAPS[f][j];
-        v = me_clip3(0, maxv, (t + 32) >> 6);
-    } else {
-        const int shift1 = BIT_DEPTH == 8 ? 0 : 2, shift2 = BIT_DEPTH == 8 ? 12 : 10;
-        const int add1 = (1 << shift1) >> 1, add2 = 1 << (shift2 - 1);
-        int t = 0;
-        for (int j = 0; j < 8; j++) {
-            const pel *r = src + (ptrdiff_t)(j - 3) * stride;
-            int u = 0;
-            for (int i = 0; i < 8; i++) u += (int)r[i - 3] * AFF_TAPS[fx][i];
-            t += (int)(s16)((u + add1) >> shift1) * AFF_TAPS[fy][j];
-        }
-        v = me_clip3(0, maxv, (t + add2) >> shift2);
-    }
-    pred[c.pred_offset + k] = (pel)v;
+    pred[c.pred_offset + k] = (pel)aff_luma_sample(c.cpmv, c.vertex_num, c.x, c.y, c.w, c.h, c.sub_w, c.sub_h,
+                                                   k % c.w, k / c.w, plane0, stride, pic_w, pic_h);
 }
 
 ME_EXPORT int me_affine_mc_luma(me_ref_t r, const me_affine_cu_t *cus, int n, me_pel *pred, uint64_t pred_elems,
