@@ -9,8 +9,12 @@ against both references, as one hipGraph launch.  Inputs are resident in HBM
 before timing starts.
 
 Beside the batched number the line carries:
-  per_call     the per-call drop-in path (host driver, every SAD/SATD/filter a
-               synchronous HIP call) on a bounded sample, as CTUs/s
+  per_call     the 1080p 16x16 workload (+-32, 1 reference) on the same frames:
+               the per-call drop-in path (host driver, every SAD/SATD/filter a
+               synchronous HIP call) on whole CTUs, the batched plan and the
+               reference CPU path, each in CTUs/s
+  end_to_end   upload of the current picture + the frame + compact results
+               into pinned host memory, per frame
   cpu_baseline the reference uavs3e kernels (C+SSE+AVX2 table, compiled from
                the reference sources into oracle/_ref) on the host's cores, on
                a bounded sample of CTUs, else the oracle restatement
@@ -36,6 +40,30 @@ def pct(a, q):
     return float(np.percentile(np.asarray(a, dtype=np.float64), q))
 
 
+def host_cpu():
+    """(model name, logical CPUs, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, allowed
+
+
+def cpu_threads():
+    """Worker threads of the CPU baselines: one per CPU this process may use,
+    at most 16 (the CPU share of one GPU on the GPU box)."""
+    return max(1, min(host_cpu()[2], 16))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,7 +75,8 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="after the timed region, run a plain byte copy of this many bytes each way "
                          "(three launches, torch bitwise_not) to scale HBM counters against")
-    ap.add_argument("--no-percall", action="store_true", help="skip the per-call leg")
+    ap.add_argument("--no-percall", action="store_true", help="skip the 1080p-16x16 leg (batched vs per-call vs CPU)")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end leg (upload, frame, results)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-tz", action="store_true", help="skip the me_search_tz leg")
     ap.add_argument("--serve-frames", type=int, default=32, help="frames per stream in the serving leg")
@@ -131,6 +160,52 @@ def main():
 
     # candidate evaluations per frame (integer full search, all valid PUs and refs)
     res, best = hip.plan_results(plan, work.num_refs)
+
+    # ---- end to end: current picture up, frame, compact results down --------
+    e2e = None
+    if not args.no_e2e:
+        nl = 2 if bi else 1
+        shape_b = (n_ctu, npu, nl) if bi else (n_ctu, npu)
+        pin_cur = ua.Pinned(hip, cur.shape, cur.dtype)
+        pin_cur.array[...] = cur
+        pin_out = ua.Pinned(hip, shape_b, ua.PU_BEST)
+        pin_best = ua.Pinned(hip, shape_b, np.int8)
+
+        def e2e_step():
+            hip.plan_set_cur_raw(plan, pin_cur.ptr.value, work.w)
+            step()
+            hip.plan_best(plan, pin_out.array, pin_best.array)
+
+        for _ in range(2):
+            e2e_step()
+        e_lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s0 = time.perf_counter()
+            e2e_step()
+            e_lat.append((time.perf_counter() - s0) * 1e3)
+        e_el = time.perf_counter() - t0
+        # the compact results are the full results' best references
+        bsel = np.where(best >= 0, best, 0).astype(np.int64)
+        if bi:
+            full = np.take_along_axis(res, bsel, axis=2)
+        else:
+            full = np.take_along_axis(res, bsel[..., None], axis=2)[..., 0]
+        ok = bool(np.array_equal(pin_best.array, best) and
+                  np.array_equal(np.where(best >= 0, full["cost_final"], 0xFFFFFFFF), pin_out.array["cost"]) and
+                  np.array_equal(np.where((best >= 0)[..., None], full["mv_sub"], 0), pin_out.array["mv"]))
+        e2e = {"ms_per_frame": e_el * 1e3 / args.steps, "ctus_per_s": n_ctu * args.steps / e_el,
+               "p50_ms": pct(e_lat, 50), "p99_ms": pct(e_lat, 99),
+               "upload_bytes": int(cur.nbytes), "download_bytes": int(pin_out.array.nbytes + pin_best.array.nbytes),
+               "compact_equals_full_results": ok,
+               "covers": "per frame, serially: the current picture from pinned host memory to HBM (me_plan_set_cur), "
+                         "the frame's graph (reference rebuild + search), then per (PU, list) the best reference's "
+                         "MV and final cost (8 B) and index (1 B) into pinned host memory (me_plan_best); "
+                         "the headline value keeps the inputs resident and the results on the device"}
+        for b_ in (pin_cur, pin_out, pin_best):
+            b_.free()
+        print("[bench] end to end: %.3f ms/frame (%.0f CTUs/s)" % (e2e["ms_per_frame"], e2e["ctus_per_s"]),
+              file=sys.stderr, flush=True)
     valid = res["valid"] == 1
     geo = []
     for s in range(29):
@@ -188,7 +263,32 @@ def main():
             hip.plan_run_tz(plan, dref, mvp, tz)
         tz_ms = (time.perf_counter() - t0) * 1e3 / 3
         tres, _ = hip.plan_results(plan, work.num_refs)
-        search_tz = {"ms_per_frame": tz_ms, "ctus_per_s": n_ctu / (tz_ms * 1e-3),
+        tz_cpu = None
+        from oracle.oracle import RefLib
+        if RefLib.available(bd) and not args.no_cpu:
+            rl = RefLib(bd, level=2)
+            rl.set_pinning(1)
+            tpocs = [7, 6, 4, 0][:work.num_refs]
+            tpics = [rl.pic_create(r, p_) for r, p_ in zip(refs, tpocs)]
+            nthr = cpu_threads()
+            _, _, t1 = rl.tz_frame(cur, tpics, 8, work.R, 8, work.lam, work.mask, 0, 1, threads=1)
+            n_s = int(min(n_ctu, max(nthr, 10e3 * nthr / max(t1, 1e-3))))
+            cres_t, cbest_t, cms_t = rl.tz_frame(cur, tpics, 8, work.R, 8, work.lam, work.mask, 0, n_s, threads=nthr)
+            for q in tpics:
+                rl.pic_free(q)
+            g_ = tres[:n_s]
+            c_ = cres_t[:n_s]
+            v_ = c_["valid"] == 1
+            same = bool(np.array_equal(g_["valid"], c_["valid"]) and
+                        np.array_equal(g_["mv_sub"][v_], c_["mv_sub"][v_]) and
+                        np.array_equal(g_["mv_int"][v_], c_["mv_int"][v_]) and
+                        np.array_equal(g_["cost_final"][v_], c_["cost_final"][v_]))
+            tz_cpu = {"value": n_s / (cms_t * 1e-3), "unit": "CTUs/s", "cores": nthr, "kind": "reference",
+                      "sample": "CTUs 0..%d of %s: the reference's me_search_tz (C+SSE4.2+AVX2 table), same "
+                                "MVP / GOP / POCs / range, %d pinned threads, %.2f s; bit-exact with the GPU: %s"
+                                % (n_s - 1, work.name, nthr, cms_t * 1e-3, "yes" if same else "NO")}
+            print("[bench] me_search_tz CPU: %.1f CTUs/s" % tz_cpu["value"], file=sys.stderr, flush=True)
+        search_tz = {"ms_per_frame": tz_ms, "ctus_per_s": n_ctu / (tz_ms * 1e-3), "cpu_reference": tz_cpu,
                      "note": "me_search_tz (diamond, raster, refinement, sub-pel) for every PU and reference, "
                              "MVP (0,0), GOP 8, max search range %d, AMVR 0, SATD sub-pel; one wave per "
                              "(PU, reference); includes the MVP upload" % work.R,
@@ -312,51 +412,107 @@ def main():
                 rl = RefLib(bd, level=2)
                 p0 = [rl.pic_create(refs[i], pocs[i]) for i in range(n0)]
                 p1 = [rl.pic_create(refs[i], pocs[i]) for i in range(n0, work.num_refs)]
-                t0 = time.perf_counter()
-                rl.loka_cost(cur, p0, p1, 8, 16, 384, 1)
-                c_ms = (time.perf_counter() - t0) * 1e3
+                t_all, t_intra = [], []
+                for _ in range(3):
+                    t0 = time.perf_counter()
+                    rl.loka_cost(cur, p0, p1, 8, 16, 384, 1)
+                    t_all.append((time.perf_counter() - t0) * 1e3)
+                    t0 = time.perf_counter()
+                    rl.loka_cost(cur, [], [], 8, 16, 384, 1)
+                    t_intra.append((time.perf_counter() - t0) * 1e3)
                 for q in p0 + p1:
                     rl.pic_free(q)
+                c_ms = float(np.median(t_all))
+                i_ms = float(np.median(t_intra))
                 lookahead["cpu_reference_ms"] = c_ms
+                lookahead["cpu_reference_inter_ms"] = c_ms - i_ms
+                lookahead["speedup_inter_vs_cpu"] = (c_ms - i_ms) / l_ms
                 lookahead["cpu_note"] = ("loka_estimate_coding_cost of the compiled reference (C+SSE4.2+AVX2), "
-                                         "one thread as the encoder runs it; includes its intra part "
-                                         "(11 modes per unit)")
+                                         "one thread as the encoder runs it, median of 3; its inter part alone is "
+                                         "the call with the references minus the same call with none (intra only, "
+                                         "%.1f ms)" % i_ms)
             print("[bench] lookahead: %.3f ms/frame (%d units)%s" % (
-                l_ms, lcost.size, "; reference CPU %.1f ms" % lookahead["cpu_reference_ms"]
-                if "cpu_reference_ms" in lookahead else ""), file=sys.stderr, flush=True)
+                l_ms, lcost.size, "; reference CPU inter part %.1f ms" % lookahead["cpu_reference_inter_ms"]
+                if "cpu_reference_inter_ms" in lookahead else ""), file=sys.stderr, flush=True)
 
-    # ---- per-call drop-in leg -----------------------------------------------
+    # ---- the 1080p 16x16 workload: batched, per-call and CPU on the same frames
+    # (1920x1080 8-bit, 16x16 PUs, full search +-32, 1 reference, sub-pel)
     per_call = None
-    if not args.no_percall:
-        host = ua.MeHost(bd)
-        table = hip.dispatch_table()
-        sample_w = work.scaled(256, 256, 8)   # picture of 4 CTUs; window +-8
-        scur, srefs = sample_w.frames(host)
-        splanes = [host.planes_via_table(table, r) for r in srefs]
-        pmask = ua.MASK_16x16
+    if not args.no_percall and bd == 8:
+        from oracle.oracle import RefLib
+        w1 = wl.hd_16x16()
+        cur1, refs1 = w1.frames()
+        d1 = [hip.ref_create(r) for r in refs1]
+        plan1 = hip.plan_create(w1.w, w1.h, w1.R, w1.lam, w1.mask, 1)
+        hip.plan_set_cur(plan1, cur1)
+        n1 = hip.lib.me_plan_num_ctus(plan1)
+        for _ in range(args.warmup):
+            hip.plan_run(plan1, d1, 1)
+        hip.lib.me_plan_results(plan1, None, None)
         t0 = time.perf_counter()
-        pres, pbest, pms = host.me_frame(table, scur, splanes, 8, work.lam, pmask, 0, 1, threads=1)
-        p_el = time.perf_counter() - t0
-        pvalid = pres["valid"][0] == 1
-        pc = int(pvalid.sum()) * 17 * 17   # candidates of the sample (one CTU, 16x16 PUs, +-8, 2 refs)
-        p_cand_s = pc / p_el
-        cands_per_ctu = cands / max(1, int(valid[:, :, 0].any(axis=1).sum()))
-        per_call = {"value": p_cand_s / cands_per_ctu, "unit": "CTUs/s", "candidates_per_s": p_cand_s,
+        for _ in range(args.steps):
+            hip.plan_run(plan1, d1, 1)
+            hip.lib.me_plan_results(plan1, None, None)
+        b1 = n1 * args.steps / (time.perf_counter() - t0)
+        r1, bst1 = hip.plan_results(plan1, 1)
+        hip.plan_destroy(plan1)
+        for r in d1:
+            hip.ref_destroy(r)
+        # per-call: the host driver calling every SAD / SATD / filter through
+        # the HIP table, on whole CTUs of the same frame (the middle row)
+        host = ua.MeHost(8)
+        table = hip.dispatch_table()
+        planes1 = [host.planes_via_table(table, r) for r in refs1]
+        cols1 = (w1.w + 127) // 128
+        mid1 = ((w1.h + 127) // 128 // 2) * cols1 + cols1 // 2
+        pc = {}
+        for nthr in (1, 4):
+            t0 = time.perf_counter()
+            pres, pbest, _ = host.me_frame(table, cur1, planes1, w1.R, w1.lam, w1.mask, mid1, mid1 + nthr,
+                                           threads=nthr)
+            el = time.perf_counter() - t0
+            same = ua.compare_results(pres[mid1:mid1 + nthr], pbest[mid1:mid1 + nthr], r1[mid1:mid1 + nthr],
+                                      bst1[mid1:mid1 + nthr]) is None
+            pc[nthr] = {"ctus_per_s": nthr / el, "seconds": el, "bit_exact_with_batched": same}
+            print("[bench] per-call, %d thread(s): %.3f CTUs/s" % (nthr, nthr / el), file=sys.stderr, flush=True)
+        c1 = None
+        if RefLib.available(8) and not args.no_cpu:
+            rl = RefLib(8, level=2)
+            rl.set_pinning(1)
+            nthr = cpu_threads()
+            pics1 = [rl.pic_create(r) for r in refs1]
+            runs = []
+            for _ in range(3):
+                cres1, cbest1, cms1 = rl.me_frame(cur1, pics1, w1.R, w1.lam, w1.mask, 0, n1, threads=nthr)
+                runs.append(cms1)
+            for q in pics1:
+                rl.pic_free(q)
+            c1 = {"ctus_per_s": n1 / (float(np.median(runs)) * 1e-3), "cores": nthr, "kind": "reference",
+                  "bit_exact_with_batched": ua.compare_results(r1, bst1, cres1, cbest1) is None}
+            print("[bench] 16x16 workload CPU: %.0f CTUs/s" % c1["ctus_per_s"], file=sys.stderr, flush=True)
+        best_pc = max(v["ctus_per_s"] for v in pc.values())
+        per_call = {"value": pc[1]["ctus_per_s"], "unit": "CTUs/s", "workload": w1.name,
+                    "threads_1": pc[1], "threads_4": pc[4],
+                    "batched_ctus_per_s": b1,
+                    "speedup_batched_vs_per_call": b1 / best_pc,
+                    "cpu_reference": c1,
+                    "speedup_batched_vs_cpu": (b1 / c1["ctus_per_s"]) if c1 else None,
                     "hip_errors": int(hip.error_count()),
-                    "sample": "1 CTU of %s with 16x16 PUs, +-8, %d refs through the per-call table (%.1f s); "
-                              "CTUs/s = candidates/s / %.0f candidates per %s CTU"
-                              % (sample_w.name, work.num_refs, p_el, cands_per_ctu, work.name)}
-        print("[bench] per-call: %.0f candidates/s" % p_cand_s, file=sys.stderr, flush=True)
+                    "sample": "whole CTUs of the 1080p 16x16 workload (+-32, 1 reference): the per-call table "
+                              "(every SAD / SATD a synchronous HIP call from the host driver) on the middle "
+                              "CTU(s), with 1 and 4 host threads; batched = the frame plan on the same frame, "
+                              "same run; speedup against the faster per-call run"}
 
     # ---- CPU baseline leg -----------------------------------------------------
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = cpu_threads()
         from oracle.oracle import RefLib, Oracle
         if RefLib.available(bd):
             # the reference kernels on a bounded sample: the whole frame when a
             # pass fits in about 10 s on the host's threads, else interior CTUs
             rl = RefLib(bd, level=2)
+            rl.set_pinning(1)
             pics = [rl.pic_create(r) for r in refs]
             rows = (work.h + 127) // 128
             mid = (rows // 2) * cols
@@ -393,9 +549,13 @@ def main():
             diff = ua.compare_results(res[c0:c1], best[c0:c1], cres[c0:c1], best[c0:c1])
         else:
             diff = ua.compare_results(res[c0:c1], best[c0:c1], cres[c0:c1], cbest[c0:c1])
+        model, logical, allowed = host_cpu()
         cpu = {"value": (c1 - c0) / (cms * 1e-3), "unit": "CTUs/s", "cores": threads, "kind": kind,
+               "cpu_model": model, "host_logical_cpus": logical, "process_cpus": allowed,
+               "pinned": kind == "reference",
                "sample": "CTUs %d..%d of %s, %s, %d threads, %.2f s per pass; results bit-exact with the GPU: %s"
-                         % (c0, c1 - 1, work.name, what, threads, cms * 1e-3,
+                         % (c0, c1 - 1, work.name, what + (", one thread pinned per CPU" if kind == "reference"
+                                                            else ""), threads, cms * 1e-3,
                             "yes" if diff is None else "NO (%s)" % diff)}
         print("[bench] cpu baseline: %.2f CTUs/s (%s)" % (cpu["value"], kind), file=sys.stderr, flush=True)
 
@@ -475,7 +635,7 @@ def main():
                      "kernel": "k_int_search", "algorithmic_bytes": alg_bytes, "kernel_ms": k_ms,
                      "valu": {"ops": valu_ops, "frac": valu_frac, "peak_lane_ops_per_s": VALU_PEAK_OPS}},
         "per_call": per_call,
-        "speedup_vs_per_call": (ctus_per_s / per_call["value"]) if per_call else None,
+        "end_to_end": e2e,
         "serving": serving,
         "search_tz": search_tz,
         "lookahead": lookahead,

@@ -420,6 +420,21 @@ int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *cur, me_ref
  * (single list) or 2 (num_refs_l1 > 0): the reference index (into refs) of
  * list l's best reference, -1 = PU skipped. */
 int me_plan_results(me_plan_t plan, me_pu_result_t *res, int8_t *best_ref);
+/* The results as an encoder consumes them, 8 bytes per (PU, list) instead
+ * of 32 per (PU, reference): the best reference's quarter-pel MV and u32
+ * final cost (the value me_search_tz returns, me.c:560-567), cost
+ * 0xFFFFFFFF for a skipped PU; best_ref (optional) as in me_plan_results.
+ * out[(ctu*npu + pu)*L + l] in host memory (pinned memory from
+ * me_pinned_alloc makes the copy a single DMA).  Enqueued on `stream` (NULL:
+ * the plan's) after the last run; returns when the results are on the host. */
+typedef struct me_pu_best_t {
+    int16_t mv[2];
+    uint32_t cost;
+} me_pu_best_t;
+int me_plan_best(me_plan_t plan, me_pu_best_t *out, int8_t *best_ref, void *stream);
+/* Page-locked host memory for uploads and result downloads. */
+int me_pinned_alloc(void **out, uint64_t bytes);
+void me_pinned_free(void *p);
 /* Motion compensation of every PU from the last run's results: com_mc_cu
  * (com_mc.c:140-200) with the best reference of each list and its
  * quarter-pel MV -- luma copied from the quarter-pel planes, 4:2:0 chroma

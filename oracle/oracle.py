@@ -521,6 +521,11 @@ class RefLib(_Lib):
         L.refh_me_frame.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                     C.c_uint32, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int,
                                     C.c_void_p, C.c_void_p, C.c_void_p]
+        L.refh_tz_frame.restype = C.c_double
+        L.refh_tz_frame.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                    C.c_longlong, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_void_p]
+        L.refh_set_pinning.argtypes = [C.c_int]
         self.set_level(level)
 
     def set_level(self, level):
@@ -707,6 +712,28 @@ class RefLib(_Lib):
         a1 = (C.c_void_p * 4)(*pics_l1)
         return self.lib.refh_loka_cost(ptr(cur), w, h, cur_poc, a0, len(pics_l0), a1, len(pics_l1), gop, msr,
                                        is_padding)
+
+    def set_pinning(self, on):
+        """Pin the frame drivers' worker threads one per CPU of the process's
+        affinity set."""
+        self.lib.refh_set_pinning(1 if on else 0)
+
+    def tz_frame(self, cur, pics, gop_size, max_search_range, cur_poc, lambda_mv, mask, ctu0=0, ctu1=None, threads=1):
+        """me_search_tz from MVP (0,0) for every PU of CTUs [ctu0, ctu1) of
+        the compiled reference: (res, best, ms); res carries mv_int, mv_sub,
+        cost_final (the u32 me_search_tz returns) and valid."""
+        h, w = cur.shape
+        n_ctu = ((w + 127) // 128) * ((h + 127) // 128)
+        ctu1 = n_ctu if ctu1 is None else ctu1
+        npu = sum((128 // sw) * (128 // sh) for s, (sw, sh) in enumerate(SHAPES) if mask >> s & 1)
+        nref = len(pics)
+        arr = (C.c_void_p * nref)(*pics)
+        res = np.zeros((n_ctu, npu, nref), PU_RES_DTYPE)
+        best = np.full((n_ctu, npu), -1, np.int8)
+        cur = np.ascontiguousarray(cur, self.dt)
+        ms = self.lib.refh_tz_frame(w, h, ptr(cur), w, nref, arr, gop_size, max_search_range, cur_poc, lambda_mv, mask,
+                                    ctu0, ctu1, threads, ptr(res), ptr(best))
+        return res, best, ms
 
     def me_frame(self, cur, pics, R, lambda_mv, mask, ctu0=0, ctu1=None, cost_type=0, threads=1, ctu_ms=None):
         h, w = cur.shape

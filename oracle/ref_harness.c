@@ -18,6 +18,7 @@
 #include "me.c"
 #include "inter.c"
 #include <pthread.h>
+#include <sched.h>
 #include <time.h>
 
 #define RH_EXPORT __attribute__((visibility("default")))
@@ -375,8 +376,33 @@ typedef struct {
     refh_pu_res_t *res; s8 *best_ref;
     double *ctu_ms;
     int next;                 /* shared work counter */
+    int next_worker;          /* worker index for CPU pinning */
+    int tz;                   /* 0: full search + sub-pel; 1: me_search_tz from MVP (0,0) */
+    int gop_size, max_search_range;
+    long long cur_poc;
     pthread_mutex_t lock;
 } frame_job_t;
+
+/* Workers are pinned one per CPU of the process's affinity set, in order,
+ * when g_pin is set (refh_set_pinning). */
+static int g_pin = 0;
+RH_EXPORT void refh_set_pinning(int on) { g_pin = on; }
+
+static void pin_worker(int idx)
+{
+    cpu_set_t allowed, one;
+    if (sched_getaffinity(0, sizeof(allowed), &allowed)) return;
+    int n = CPU_COUNT(&allowed);
+    if (n <= 0) return;
+    int k = idx % n, seen = 0;
+    for (int c = 0; c < CPU_SETSIZE; c++)
+        if (CPU_ISSET(c, &allowed) && seen++ == k) {
+            CPU_ZERO(&one);
+            CPU_SET(c, &one);
+            pthread_setaffinity_np(pthread_self(), sizeof(one), &one);
+            return;
+        }
+}
 
 static void run_ctu(frame_job_t *j, refh_search_t *s, int ctu)
 {
@@ -408,6 +434,24 @@ static void run_ctu(frame_job_t *j, refh_search_t *s, int ctu)
                     memset(pi->hpel_start_mv, 0, sizeof(pi->hpel_start_mv));
                     memset(pi->qpel_start_mv, 0, sizeof(pi->qpel_start_mv));
                     s16 mv[2], mvp[2] = {0, 0};
+                    if (j->tz) {
+                        /* me_search_tz as the encoder calls it for uni-prediction (inter.c:896) */
+                        pi->gop_size = j->gop_size;
+                        pi->max_search_range = j->max_search_range;
+                        pi->adaptive_raster_range = 0;
+                        pi->ptr = j->cur_poc;
+                        pi->num_refp = (u8)j->num_refs;
+                        pi->curr_mvr = 0;
+                        u64 c = me_search_tz(pi, x, y, w, h, j->pic_w, j->pic_h, (s8)r, 0, mvp, mv, 0);
+                        if (c == COM_UINT64_MAX) continue;
+                        o->mv_int[0] = pi->mv_ipel[0][r][0];
+                        o->mv_int[1] = pi->mv_ipel[0][r][1];
+                        o->mv_sub[0] = mv[0]; o->mv_sub[1] = mv[1];
+                        o->cost_final = (u32)c;
+                        o->valid = 1;
+                        if (o->cost_final < bc) { bc = o->cost_final; br = (s8)r; }
+                        continue;
+                    }
                     u64 ci = full_search(pi, x, y, w, h, j->R, mv);
                     o->mv_int[0] = mv[0]; o->mv_int[1] = mv[1];
                     o->cost_int = ci;
@@ -437,6 +481,10 @@ static double now_ms(void)
 static void *frame_worker(void *arg)
 {
     frame_job_t *j = (frame_job_t *)arg;
+    pthread_mutex_lock(&j->lock);
+    const int widx = j->next_worker++;
+    pthread_mutex_unlock(&j->lock);
+    if (g_pin) pin_worker(widx);
     refh_search_t s;
     search_setup(&s, j->pic_w, j->pic_h, j->lambda_mv, j->cost_type, 1);
     for (;;) {
@@ -452,6 +500,8 @@ static void *frame_worker(void *arg)
     return NULL;
 }
 
+static double run_frame_job(frame_job_t *jp, int threads);
+
 /* Full search + me_sub_pel_search for CTUs [ctu0, ctu1) over `threads`
  * pthreads (dynamic CTU distribution).  Results use the oracle's layout.
  * ctu_ms (optional) receives each CTU's wall time.  Returns total ms. */
@@ -465,9 +515,38 @@ RH_EXPORT double refh_me_frame(int pic_w, int pic_h, const pel *cur, int i_cur, 
     j.num_refs = num_refs; j.refs = refs; j.R = R; j.lambda_mv = lambda_mv;
     j.cost_type = cost_type; j.mask = mask; j.ctu0 = ctu0; j.ctu1 = ctu1;
     j.res = res; j.best_ref = best_ref; j.ctu_ms = ctu_ms;
+    return run_frame_job(&j, threads);
+}
+
+/* me_search_tz (me.c:529-568) from MVP (0,0), AMVR 0, SATD sub-pel, for
+ * every PU of CTUs [ctu0, ctu1) and every reference (their POCs from
+ * refh_pic_create; the current picture's is cur_poc), over `threads`
+ * pthreads.  res receives mv_int (pi->mv_ipel), mv_sub and cost_final (the
+ * u32 me_search_tz returns); best_ref the first cheapest reference.
+ * Returns total ms. */
+RH_EXPORT double refh_tz_frame(int pic_w, int pic_h, const pel *cur, int i_cur, int num_refs, void **refs,
+                               int gop_size, int max_search_range, long long cur_poc, u32 lambda_mv, u32 mask,
+                               int ctu0, int ctu1, int threads, refh_pu_res_t *res, s8 *best_ref)
+{
+    frame_job_t j;
+    memset(&j, 0, sizeof(j));
+    j.pic_w = pic_w; j.pic_h = pic_h; j.cur = cur; j.i_cur = i_cur;
+    j.num_refs = num_refs; j.refs = refs; j.lambda_mv = lambda_mv;
+    j.mask = mask; j.ctu0 = ctu0; j.ctu1 = ctu1;
+    j.res = res; j.best_ref = best_ref;
+    j.tz = 1;
+    j.gop_size = gop_size;
+    j.max_search_range = max_search_range;
+    j.cur_poc = cur_poc;
+    return run_frame_job(&j, threads);
+}
+
+static double run_frame_job(frame_job_t *jp, int threads)
+{
+    frame_job_t j = *jp;
     int n = 0;
     for (int sh = 0; sh < 29; sh++)
-        if (mask >> sh & 1) n += (128 / SHAPES[sh][0]) * (128 / SHAPES[sh][1]);
+        if (j.mask >> sh & 1) n += (128 / SHAPES[sh][0]) * (128 / SHAPES[sh][1]);
     j.npu = n;
     pthread_mutex_init(&j.lock, NULL);
     if (threads < 1) threads = 1;

@@ -337,6 +337,19 @@ class MeHip:
         self.check(self.lib.me_plan_results(plan, _p(res), _p(best)), "me_plan_results")
         return res, best
 
+    def plan_best(self, plan, out, best=None, stream=None):
+        """me_plan_best into out (PU_BEST [ctu][pu] or [ctu][pu][list]) and,
+        optionally, best (int8, the same shape)."""
+        assert out.dtype == PU_BEST and out.flags.c_contiguous
+        f = self.lib.me_plan_best
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        self.check(f(plan, _p(out), _p(best) if best is not None else None, stream), "me_plan_best")
+
+    def plan_set_cur_raw(self, plan, ptr, stride, stream=None):
+        """me_plan_set_cur from a host address (e.g. pinned memory)."""
+        f = self.lib.me_plan_set_cur
+        self.check(f(plan, C.c_void_p(ptr), C.c_int(stride), C.c_int(0), stream), "me_plan_set_cur")
+
     def plan_run_tz(self, plan, refs, mvp, tz, mvi_prev=None, stream=None):
         """me_plan_run_tz: mvp int16 [ctu][pu][ref][2] quarter-pel MVPs; tz a
         TzCfg.  Results via plan_results."""
@@ -420,6 +433,31 @@ class MeHip:
     def plan_destroy(self, plan):
         getattr(self, "_nl", {}).pop(plan.value, None)
         self.lib.me_plan_destroy(plan)
+
+
+PU_BEST = np.dtype([("mv", "<i2", 2), ("cost", "<u4")])
+
+
+class Pinned:
+    """A numpy array over page-locked host memory from the library's own HIP
+    runtime (me_pinned_alloc), for uploads and result downloads."""
+
+    def __init__(self, hip, shape, dtype):
+        self.hip = hip
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        self.ptr = C.c_void_p()
+        hip.lib.me_pinned_alloc.argtypes = [C.c_void_p, C.c_uint64]
+        hip.check(hip.lib.me_pinned_alloc(C.byref(self.ptr), max(1, n)), "me_pinned_alloc")
+        buf = (C.c_ubyte * max(1, n)).from_address(self.ptr.value)
+        self.array = np.frombuffer(buf, np.uint8, n).view(dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.hip.lib.me_pinned_free.argtypes = [C.c_void_p]
+            self.hip.lib.me_pinned_free(self.ptr)
+            self.ptr = None
+            self.array = None
 
 
 def server_picture_bytes(w, h, bd):

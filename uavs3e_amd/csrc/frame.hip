@@ -311,6 +311,7 @@ struct me_plan_s {
     void *d_bist;         // analyze_bi per-PU state
     pel *d_orgbi;         // analyze_bi targets, [ctu][selected shape][16384]
     me_smvd_pu_t *d_smvd; // analyze_smvd per-PU results (allocated on first use)
+    me_pu_best_t *d_bestc; // me_plan_best output (allocated on first use)
     hipStream_t own;
     hipGraphExec_t exec;
     u64 graph_refs[ME_MAX_REFS];   // me_ref_s::id of the references the graph was captured with
@@ -497,6 +498,54 @@ ME_EXPORT int me_plan_results(me_plan_t p, me_pu_result_t *res, int8_t *best_ref
     if (res) ME_CHECK(hipMemcpy(res, p->d_res, n * p->s.nref * sizeof(me_pu_result_t), hipMemcpyDeviceToHost));
     if (best_ref) ME_CHECK(hipMemcpy(best_ref, p->d_best, n * p->s.nl, hipMemcpyDeviceToHost));
     return 0;
+}
+
+// The best reference of every (PU, list) as an encoder consumes it: its
+// quarter-pel MV and u32 final cost (cost 0xFFFFFFFF for a skipped PU).
+__global__ void k_best(const me_pu_result_t *res, const s8 *best, me_pu_best_t *out, int n, int nref, int nl)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * nl) return;
+    const int r = best[i];
+    me_pu_best_t o;
+    if (r >= 0) {
+        const me_pu_result_t &q = res[(size_t)(i / nl) * nref + r];
+        o.mv[0] = q.mv_sub[0];
+        o.mv[1] = q.mv_sub[1];
+        o.cost = q.cost_final;
+    } else {
+        o.mv[0] = o.mv[1] = 0;
+        o.cost = 0xFFFFFFFFu;
+    }
+    out[i] = o;
+}
+
+ME_EXPORT int me_plan_best(me_plan_t p, me_pu_best_t *out, int8_t *best_ref, void *stream)
+{
+    if (!p || !out) { me_set_error("me_plan_best: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    hipStream_t st = stream ? (hipStream_t)stream : (p->graph_stream ? p->graph_stream : p->own);
+    const int n = p->n_ctu * p->s.npu;
+    if (!p->d_bestc) ME_CHECK(hipMalloc((void **)&p->d_bestc, (size_t)n * p->s.nl * sizeof(me_pu_best_t)));
+    hipLaunchKernelGGL(k_best, dim3((n * p->s.nl + 255) / 256), dim3(256), 0, st, p->d_res, p->d_best, p->d_bestc, n,
+                       p->s.nref, p->s.nl);
+    ME_CHECK(hipGetLastError());
+    ME_CHECK(hipMemcpyAsync(out, p->d_bestc, (size_t)n * p->s.nl * sizeof(me_pu_best_t), hipMemcpyDeviceToHost, st));
+    if (best_ref) ME_CHECK(hipMemcpyAsync(best_ref, p->d_best, (size_t)n * p->s.nl, hipMemcpyDeviceToHost, st));
+    ME_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+ME_EXPORT int me_pinned_alloc(void **out, uint64_t bytes)
+{
+    if (!out) return -1;
+    if (me_hip_init(-1)) return -1;
+    ME_CHECK(hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+    return 0;
+}
+
+ME_EXPORT void me_pinned_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
 }
 
 ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const me_pel *cur, int stride, int iters,
@@ -791,6 +840,7 @@ ME_EXPORT void me_plan_destroy(me_plan_t p)
     (void)hipFree(p->d_bist);
     (void)hipFree(p->d_orgbi);
     (void)hipFree(p->d_smvd);
+    (void)hipFree(p->d_bestc);
     (void)hipFree(p->d_cur);
     (void)hipFree(p->d_pic);
     (void)hipFree(p->d_pred);

@@ -29,6 +29,12 @@
 #define ME_TZ_PART 0
 #endif
 
+// One shape of a grouped search launch: its geometry, lanes per (PU,
+// reference) and the first workgroup (grid x) of its range.
+struct TzShape {
+    int pw, ph, base, per_ctu, lpu, orgbi_off, blk0;
+};
+
 struct TzArgs {
     JobSet js;
     const s16 *mvp;          // [job][npu][nref][2], quarter pel
@@ -52,6 +58,10 @@ struct TzArgs {
     s16 *mv_scale;           // [job][npu][nref][2], quarter pel
     const struct BiState *bst;   // [job][npu]
     const me_bi_pu_t *bpu;       // [job][npu]
+    // k_tz: the shapes of the launch (one com_had tile type), largest PUs
+    // first; workgroup x belongs to the last shape whose blk0 <= x
+    TzShape shp[ME_NSHAPES];
+    int nshp;
 };
 
 // analyze_bi's running state of one PU (inter.c:986-1033)
@@ -109,21 +119,24 @@ __device__ __forceinline__ int satd_tile_fn(const pel *org, int so, const pel *c
 // tile sum is shifted by bd - 8.  One out-of-line copy (it is called from
 // four places of the sub-pel search).
 template <int TT>
-__device__ __noinline__ u32 wave_had(const TzArgs &a, const pel *org, int so, const pel *pred, int sp)
+__device__ __noinline__ u32 wave_had(int lpu, int pw, int ph, const pel *org, int so, const pel *pred, int sp)
 {
-    const int li = threadIdx.x & (a.lpu - 1), tpr = a.pw / a.tw, T = tpr * (a.ph / a.th);
+    // tile size of the type (had_tiling)
+    constexpr int tw = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 8 : (TT == 3 ? 16 : (TT == 4 ? 4 : 8))));
+    constexpr int th = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 4 : (TT == 3 ? 8 : (TT == 4 ? 8 : 16))));
+    const int li = threadIdx.x & (lpu - 1), tpr = pw / tw, T = tpr * (ph / th);
     int s = 0;
-    for (int t = li; t < T; t += a.lpu) {
-        const int tx = (t % tpr) * a.tw, ty = (t / tpr) * a.th;
+    for (int t = li; t < T; t += lpu) {
+        const int tx = (t % tpr) * tw, ty = (t / tpr) * th;
         s += satd_tile_fn<TT>(org + ty * so + tx, so, pred + ty * sp + tx, sp);
     }
-    return (u32)(group_sum(s, a.lpu) >> (BIT_DEPTH - 8));
+    return (u32)(group_sum(s, lpu) >> (BIT_DEPTH - 8));
 }
 
 struct TzUnit {
     const TzArgs *a;
     const PicDesc *P;
-    int r, x, y, w, h;
+    int r, x, y, w, h, lpu;
     const pel *org;
     int so;                  // org stride
     int mvpx, mvpy;
@@ -146,7 +159,7 @@ struct TzUnit {
     __device__ __forceinline__ bool try_ipel(int mx, int my)
     {
         const pel *pred = plane_ptr(*P, r, 0, x + mx, y + my);
-        const u32 sad = wave_sad(org, so, pred, P->ref_stride, w, h, a->lpu);
+        const u32 sad = wave_sad(org, so, pred, P->ref_stride, w, h, lpu);
         const u64 cost = ((u64)sad << shift) + (u64)(ipel_bits(mx, my) * a->lambda);
         if (cost < best) {
             best = cost;
@@ -254,8 +267,8 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
     const TzArgs &a = *u.a;
     const int shift = 16 - a.bi - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
     auto dist = [&](const pel *p) -> u32 {
-        return a.cost_type == 0 ? wave_had<TT>(a, u.org, u.so, p, u.P->ref_stride)
-                                : wave_sad(u.org, u.so, p, u.P->ref_stride, u.w, u.h, a.lpu);
+        return a.cost_type == 0 ? wave_had<TT>(u.lpu, u.w, u.h, u.org, u.so, p, u.P->ref_stride)
+                                : wave_sad(u.org, u.so, p, u.P->ref_stride, u.w, u.h, u.lpu);
     };
     const int cx = u.mv[0], cy = u.mv[1];
     u32 d = dist(plane_ptr(*u.P, u.r, 0, u.x + (cx >> 2), u.y + (cy >> 2)));
@@ -276,27 +289,41 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
         }
     }
     if (a.cost_type) {
-        d = wave_had<TT>(a, u.org, u.so, qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)),
-                     u.P->ref_stride);
+        d = wave_had<TT>(u.lpu, u.w, u.h, u.org, u.so,
+                         qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)), u.P->ref_stride);
         best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << (16 - a.bi));
     }
     return best;
 }
 
-// grid (ceil(per_ctu / (4 * 64 / lpu)), jobs, refs), 4 waves of 64 / lpu
-// (PU, reference) groups each.  TT: the shape's com_had tile type.
+// grid (the shapes' workgroups, jobs, refs): each shape of the launch (one
+// com_had tile type TT) has ceil(per_ctu / (4 * 64 / lpu)) workgroups of 4
+// waves of 64 / lpu (PU, reference) groups.  All shapes of a tile type run
+// in one launch, the largest PUs first, so that their long sequential
+// searches overlap with the many short ones of the small PUs.
+// The argument block is read where the launch put it, in the kernarg
+// segment (the first explicit argument sits at offset 0): a search unit keeps
+// a pointer to it, and taking the address of the parameter itself would make
+// every lane copy the whole block into private memory first.
+#define TZ_ARGS(T) (*(const T *)__builtin_amdgcn_kernarg_segment_ptr())
+
 template <int TT>
-__global__ __launch_bounds__(256) void k_tz(TzArgs a)
+__global__ __launch_bounds__(256) void k_tz(TzArgs)
 {
-    const int lg = __builtin_ctz((unsigned)a.lpu), upw = 64 >> lg;
-    const int li = threadIdx.x & (a.lpu - 1), wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = (blockIdx.x * 4 + wv) * upw + ((threadIdx.x & 63) >> lg);
-    if (i >= a.per_ctu) return;
+    const TzArgs &a = TZ_ARGS(TzArgs);
+    int k = 0;
+    while (k + 1 < a.nshp && a.shp[k + 1].blk0 <= (int)blockIdx.x) k++;
+    const TzShape &S = a.shp[k];
+    const int lpu = S.lpu, pw = S.pw, ph = S.ph, base = S.base;
+    const int lg = __builtin_ctz((unsigned)lpu), upw = 64 >> lg;
+    const int li = threadIdx.x & (lpu - 1), wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = (((int)blockIdx.x - S.blk0) * 4 + wv) * upw + ((threadIdx.x & 63) >> lg);
+    if (i >= S.per_ctu) return;
     const int job = blockIdx.y;
     const int n0 = a.n0, n1 = a.nref - a.n0;
     int r = blockIdx.z, blist = 0;
     if (a.bi) {   // the searched list's reference blockIdx.z
-        const BiState &bs = a.bst[(size_t)job * a.npu + a.base + i];
+        const BiState &bs = a.bst[(size_t)job * a.npu + base + i];
         if (bs.done) return;
         blist = 1 - bs.fixed;
         if (r >= (blist ? n1 : n0)) return;
@@ -304,16 +331,17 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     }
     int ctu;
     const PicDesc &P = *job_decode(a.js, job, ctu);
-    const int cols = 128 / a.pw;
+    const int cols = 128 / pw;
     TzUnit u;
     u.a = &a;
     u.P = &P;
     u.r = r;
-    u.w = a.pw;
-    u.h = a.ph;
-    u.x = (ctu % P.ctu_cols) * 128 + (i % cols) * a.pw;
-    u.y = (ctu / P.ctu_cols) * 128 + (i / cols) * a.ph;
-    const size_t slot = ((size_t)job * a.npu + a.base + i) * a.nref + r;
+    u.w = pw;
+    u.h = ph;
+    u.lpu = lpu;
+    u.x = (ctu % P.ctu_cols) * 128 + (i % cols) * pw;
+    u.y = (ctu / P.ctu_cols) * 128 + (i / cols) * ph;
+    const size_t slot = ((size_t)job * a.npu + base + i) * a.nref + r;
     me_pu_result_t out;
     out.mv_int[0] = out.mv_int[1] = out.mv_sub[0] = out.mv_sub[1] = 0;
     out.cost_int = out.cost_sub = ~0ull;
@@ -347,8 +375,8 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
         return;
     }
     if (a.bi) {
-        u.org = a.orgbi + ((size_t)job * a.nsel * 16384 + a.orgbi_off) + (size_t)i * a.pw * a.ph;
-        u.so = a.pw;
+        u.org = a.orgbi + ((size_t)job * a.nsel * 16384 + S.orgbi_off) + (size_t)i * pw * ph;
+        u.so = pw;
     } else {
         u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
         u.so = P.pic_w;
@@ -379,7 +407,7 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs a)
     out.cost_sub = best;
     const int rb = nlist <= 1 ? 0 : (refi < nlist - 1 ? refi + 1 : nlist - 1);   // tbl_refi_bits
     // bi: the fixed list's mot_bits + 1 (me.c:560-561)
-    const int last = a.bi ? a.bpu[(size_t)job * a.npu + a.base + i].mot_bits[1 - blist] + 1 : 2;
+    const int last = a.bi ? a.bpu[(size_t)job * a.npu + base + i].mot_bits[1 - blist] + 1 : 2;
     const u64 cf = best + (u64)(a.lambda * (u32)(rb + mvr + (mvr < 4) + last));
     out.cost_final = (u32)((cf + (1u << 15)) >> 16);
     out.valid = 1;
@@ -502,8 +530,9 @@ __device__ u32 smvd_refine(const SmvdUnit &u, int mv[2][2], u32 cost, int patter
 // grid (ceil(per_ctu / (2 * 64 / lpu)), jobs), 2 waves of 64 / lpu PU groups.
 // out[job][npu]: .active selects the PU; mv and cost are written.
 template <int TT>
-__global__ __launch_bounds__(128) void k_smvd(TzArgs a, me_smvd_pu_t *out)
+__global__ __launch_bounds__(128) void k_smvd(TzArgs, me_smvd_pu_t *out)
 {
+    const TzArgs &a = TZ_ARGS(TzArgs);
     __shared__ __align__(16) pel tiles[128 * SMVD_TILE];
     const int lg = __builtin_ctz((unsigned)a.lpu), upw = 64 >> lg;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -814,26 +843,52 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
         a.bpu = bi->pus;
         for (int k = 0; k < ME_NSHAPES; k++) a.nsel += s->base[k] >= 0;
     }
-    int rank = 0;
-    for (int k = 0; k < ME_NSHAPES; k++) {
-        if (s->base[k] < 0 || (only_shape >= 0 && k != only_shape)) continue;
-        a.orgbi_off = 16384 * rank++;
-        a.pw = ME_SHAPES_HOST[k][0];
-        a.ph = ME_SHAPES_HOST[k][1];
-        a.base = s->base[k];
-        a.per_ctu = (128 / a.pw) * (128 / a.ph);
-        had_tiling(a.pw, a.ph, &a.tt, &a.tw, &a.th);
-        const int area = a.pw * a.ph;
-        a.lpu = area / 16 < 4 ? 4 : (area / 16 > 64 ? 64 : area / 16);
-        const int upb = 4 * (64 / a.lpu);
-        tz_fn fn = a.tt == 3 ? me_tz_kernel_16x8(a.tt) : (a.tt == 5 ? me_tz_kernel_8x16(a.tt) : me_tz_kernel_small(a.tt));
-        if (!fn) { me_set_error("no search kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
-        if (bi) {
-            const int per_block = 256 / std::min(256, area);
-            hipLaunchKernelGGL(k_bi_org, dim3((a.per_ctu + per_block - 1) / per_block, js.n_jobs), dim3(256), 0, st, a);
+    // the selected shapes' ranks (bi targets are laid out in shape order)
+    int rank_of[ME_NSHAPES], rank = 0;
+    for (int k = 0; k < ME_NSHAPES; k++) rank_of[k] = s->base[k] >= 0 ? rank++ : -1;
+    const int nz = bi ? std::max(a.n0, a.nref - a.n0) : s->nref;
+    // one launch per com_had tile type, its shapes largest PUs first
+    for (int tt = 0; tt < 6; tt++) {
+        int order[ME_NSHAPES], no = 0;
+        for (int k = 0; k < ME_NSHAPES; k++) {
+            if (s->base[k] < 0 || (only_shape >= 0 && k != only_shape)) continue;
+            int t, tw, th;
+            had_tiling(ME_SHAPES_HOST[k][0], ME_SHAPES_HOST[k][1], &t, &tw, &th);
+            if (t == tt) order[no++] = k;
         }
-        const int nz = bi ? std::max(a.n0, a.nref - a.n0) : s->nref;
-        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs, nz), dim3(256), 0, st, a);
+        if (!no) continue;
+        std::stable_sort(order, order + no, [](int p, int q) {
+            return ME_SHAPES_HOST[p][0] * ME_SHAPES_HOST[p][1] > ME_SHAPES_HOST[q][0] * ME_SHAPES_HOST[q][1];
+        });
+        tz_fn fn = tt == 3 ? me_tz_kernel_16x8(tt) : (tt == 5 ? me_tz_kernel_8x16(tt) : me_tz_kernel_small(tt));
+        if (!fn) { me_set_error("no search kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+        a.nshp = no;
+        int blocks = 0;
+        for (int j = 0; j < no; j++) {
+            const int k = order[j];
+            TzShape &q = a.shp[j];
+            q.pw = ME_SHAPES_HOST[k][0];
+            q.ph = ME_SHAPES_HOST[k][1];
+            q.base = s->base[k];
+            q.per_ctu = (128 / q.pw) * (128 / q.ph);
+            const int area = q.pw * q.ph;
+            q.lpu = area / 16 < 4 ? 4 : (area / 16 > 64 ? 64 : area / 16);
+            q.orgbi_off = 16384 * rank_of[k];
+            q.blk0 = blocks;
+            const int upb = 4 * (64 / q.lpu);
+            blocks += (q.per_ctu + upb - 1) / upb;
+            if (bi) {   // the shape's bi-prediction targets (k_bi_org reads the per-shape fields)
+                a.pw = q.pw;
+                a.ph = q.ph;
+                a.base = q.base;
+                a.per_ctu = q.per_ctu;
+                a.orgbi_off = q.orgbi_off;
+                const int per_block = 256 / std::min(256, area);
+                hipLaunchKernelGGL(k_bi_org, dim3((a.per_ctu + per_block - 1) / per_block, js.n_jobs), dim3(256), 0,
+                                   st, a);
+            }
+        }
+        hipLaunchKernelGGL(fn, dim3(blocks, js.n_jobs, nz), dim3(256), 0, st, a);
         ME_CHECK(hipGetLastError());
     }
     if (bi) {
