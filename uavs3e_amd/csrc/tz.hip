@@ -104,6 +104,51 @@ __device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred,
     return (u32)group_sum((int)s, lpu);
 }
 
+// SADs of NB candidate blocks pred[0..NB) against the same original, by the
+// lpu lanes of a group: one walk over the block's samples that accumulates
+// every candidate at each step, so that the candidates' loads are in flight
+// together (a search evaluated one candidate at a time waits for each
+// candidate's loads in turn).  out[c] is in every lane of the group.
+template <int NB>
+__device__ __forceinline__ void wave_sad_n(const pel *org, int so, const pel *const *pred, int sp, int w, int h,
+                                           int lpu, u32 *out)
+{
+    const int li = threadIdx.x & (lpu - 1), q = w >> 2, lq = __builtin_ctz((unsigned)q);
+    u32 s[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) s[c] = 0;
+    for (int i = li; i < q * h; i += lpu) {
+        const int xx = (i & (q - 1)) << 2, yy = i >> lq;
+        const int oo = yy * so + xx, po = yy * sp + xx;
+#if BIT_DEPTH == 8
+        const u32 ow = *(const u32 *)(org + oo);
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const uintptr_t ad = (uintptr_t)(pred[c] + po);
+            const u32 *pq = (const u32 *)(ad & ~(uintptr_t)3);
+            s[c] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(pq[1], pq[0], (u32)(ad & 3)), ow, s[c]);
+        }
+#else
+        const uint2 ow = *(const uint2 *)(org + oo);
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const uintptr_t ad = (uintptr_t)(pred[c] + po);
+            const u32 *pq = (const u32 *)(ad & ~(uintptr_t)3);
+            const u32 sh = (u32)(ad & 3);
+            const u32 p0 = __builtin_amdgcn_alignbyte(pq[1], pq[0], sh), p1 = __builtin_amdgcn_alignbyte(pq[2], pq[1], sh);
+            s[c] = __builtin_amdgcn_sad_u16(p0, ow.x, __builtin_amdgcn_sad_u16(p1, ow.y, s[c]));
+        }
+#endif
+    }
+#pragma unroll
+    for (int c = 0; c < NB; c++) out[c] = (u32)group_sum((int)s[c], lpu);
+}
+
+// Candidates per batch of the integer searches: 4 keeps every instance of
+// k_tz at 3 waves per SIMD (at most 168 VGPRs, amdgpu_waves_per_eu(3));
+// each candidate in flight costs registers.
+template <int TT> struct TzBatch { static constexpr int NB = 4; };
+
 // One normalised com_had tile of type TT (com_cost.c:259, 340, 467, 586,
 // 649, 708), done by one lane.
 template <int TT>
@@ -126,9 +171,23 @@ __device__ __noinline__ u32 wave_had(int lpu, int pw, int ph, const pel *org, in
     constexpr int th = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 4 : (TT == 3 ? 8 : (TT == 4 ? 8 : 16))));
     const int li = threadIdx.x & (lpu - 1), tpr = pw / tw, T = tpr * (ph / th);
     int s = 0;
-    for (int t = li; t < T; t += lpu) {
-        const int tx = (t % tpr) * tw, ty = (t / tpr) * th;
-        s += satd_tile_fn<TT>(org + ty * so + tx, so, pred + ty * sp + tx, sp);
+    if (TT == 3 || TT == 5) {
+        // a 16x8 / 8x16 tile on a lane pair: the 16-point transform is the
+        // 8-point transform of the half sums (even lane) and of the half
+        // differences (odd lane); the pair's sum is normalised before the
+        // tiles are summed (com_cost.c:467, 586).  lpu is even, so both
+        // lanes of a pair run the same iterations.
+        for (int t2 = li; t2 < 2 * T; t2 += lpu) {
+            const int t = t2 >> 1, tx = (t % tpr) * tw, ty = (t / tpr) * th;
+            const int hs = had_half<TT>(org + ty * so + tx, so, pred + ty * sp + tx, sp, t2 & 1);
+            const int v = hs + __builtin_amdgcn_mov_dpp(hs, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+            s += (t2 & 1) ? 0 : me_satd_norm128(v);
+        }
+    } else {
+        for (int t = li; t < T; t += lpu) {
+            const int tx = (t % tpr) * tw, ty = (t / tpr) * th;
+            s += satd_tile_fn<TT>(org + ty * so + tx, so, pred + ty * sp + tx, sp);
+        }
     }
     return (u32)(group_sum(s, lpu) >> (BIT_DEPTH - 8));
 }
@@ -155,6 +214,26 @@ struct TzUnit {
         const int m = a->mvr;
         return (u32)a->tab[(mx >> m) - (mvpx >> m)] + (u32)a->tab[(my >> m) - (mvpy >> m)];
     }
+    // The costs of up to NB integer candidates (mx[c], my[c]), c < n,
+    // computed together; a candidate's cost is that of me.c:158-189.
+    // Entries c >= n read the first candidate's block and are not used.
+    template <int NB>
+    __device__ __forceinline__ void eval_ipel(const int *mx, const int *my, int n, u64 *cost) const
+    {
+        const pel *pred[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const int k = c < n ? c : 0;
+            pred[c] = plane_ptr(*P, r, 0, x + mx[k], y + my[k]);
+        }
+        u32 sad[NB];
+        wave_sad_n<NB>(org, so, pred, P->ref_stride, w, h, lpu, sad);
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const int k = c < n ? c : 0;
+            cost[c] = ((u64)sad[c] << shift) + (u64)(ipel_bits(mx[k], my[k]) * a->lambda);
+        }
+    }
     // try_ipel: one integer candidate of a running search (me.c:158-189)
     __device__ __forceinline__ bool try_ipel(int mx, int my)
     {
@@ -171,14 +250,41 @@ struct TzUnit {
     }
 };
 
+// Scans a batch of evaluated candidates in the reference's order: a
+// strictly cheaper valid one becomes the best (me.c:158-189).  Returns the
+// tag of the batch's last improvement, or `prev` without one.
+template <int NB>
+__device__ __forceinline__ int scan_batch(TzUnit &u, const int *mx, const int *my, const u64 *cost, const bool *ok,
+                                          const int *tag, int prev)
+{
+    int t = prev;
+#pragma unroll
+    for (int c = 0; c < NB; c++)
+        if (ok[c] && cost[c] < u.best) {
+            u.best = cost[c];
+            u.mv[0] = mx[c];
+            u.mv[1] = my[c];
+            t = tag[c];
+        }
+    return t;
+}
+
 // search_diamond (me.c:199-355); returns the step of the last improvement
 // (0: centre, 2: core square, else the diamond step).  With bi the core
 // square spans the distance to the rounded MVP (2..5, 6 above integer AMVR)
 // and the multilayer diamond is skipped (me.c:221-231, 317-319).
-__device__ __noinline__ int tz_diamond(TzUnit &u, int max_step)
+// Every candidate of the call lies around the entry MV, fixed before any is
+// evaluated: the centre, the core square in raster order, then each
+// multilayer round's points in table order (those outside the window are
+// not candidates).  They are evaluated NB at a time and scanned in that
+// order, which keeps the reference's first strict minimum.
+template <int NB>
+__device__ __noinline__ int tz_diamond(TzUnit &ur, int max_step)
 {
+    // work on a copy that stays in registers (the caller's unit is in
+    // private memory, as it is passed by reference); the result goes back
+    TzUnit u = ur;
     const int mvr = u.a->mvr;
-    int best_step = 0;
     int core = 2, cstep = 1;
     const int cx = u.mv[0], cy = u.mv[1];
     if (u.a->bi) {
@@ -191,31 +297,65 @@ __device__ __noinline__ int tz_diamond(TzUnit &u, int max_step)
     while (y0 <= u.lo[1]) y0 += cstep;
     while (x1 >= u.hi[0]) x1 -= cstep;
     while (y1 >= u.hi[1]) y1 -= cstep;
-    if (u.try_ipel(cx, cy)) best_step = 0;
-    if (max_step)
-        for (int my = y0; my <= y1; my += cstep)
-            for (int mx = x0; mx <= x1; mx += cstep) u.try_ipel(mx, my);
-    if (u.mv[0] != cx || u.mv[1] != cy) best_step = 2;
-    if (u.a->bi) return best_step;
+    const int nx = max_step && x1 >= x0 ? (x1 - x0) / cstep + 1 : 0;
+    const int ny = max_step && y1 >= y0 ? (y1 - y0) / cstep + 1 : 0;
+    const int ncore = nx * ny;
     // multilayer diamonds: 4 points at step 4, 8 at step 8, 16 beyond
     // (me.c:330-333), scaled by the AMVR precision
+    int nml = 0;
+    if (!u.a->bi)
+        for (int step = 4, round = 0; step < max_step; step <<= 1, round++) nml += round == 0 ? 4 : (round == 1 ? 8 : 16);
     const int scale = mvr - 2 > 0 ? mvr - 2 : 0;
-    for (int step = 4, round = 0; step < max_step; step <<= 1, round++) {
-        const int ss = (step << scale) >> 1;
-        const int num = round < 2 ? 8 : 16, add = round < 1 ? 2 : 1, big = round < 2 ? 0 : 1;
-        for (int i = 0; i < num; i += add) {
+    const int total = 1 + ncore + nml;
+    int best_step = 0;
+    for (int b0 = 0; b0 < total; b0 += NB) {
+        int mx[NB], my[NB], tag[NB];
+        bool ok[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            int k = b0 + c;
+            mx[c] = cx;
+            my[c] = cy;
+            tag[c] = 0;
+            ok[c] = k < total;
+            if (k == 0 || k >= total) continue;
+            k -= 1;
+            if (k < ncore) {
+                const int qy = k / nx;
+                mx[c] = x0 + (k - qy * nx) * cstep;
+                my[c] = y0 + qy * cstep;
+                tag[c] = 2;
+                continue;
+            }
+            k -= ncore;
+            const int round = k < 4 ? 0 : (k < 12 ? 1 : 2 + (k - 12) / 16);
+            const int i = round == 0 ? 2 * k : (round == 1 ? k - 4 : (k - 12) % 16);
+            const int step = 4 << round, big = round < 2 ? 0 : 1;
+            const int ss = (step << scale) >> 1;
             const int dx = big ? TZ_DIA16[i][0] : TZ_DIA8[i][0], dy = big ? TZ_DIA16[i][1] : TZ_DIA8[i][1];
-            const int mx = cx + (s16)(ss >> big) * dx, my = cy + (s16)(ss >> big) * dy;
-            if (mx <= u.hi[0] && mx >= u.lo[0] && my <= u.hi[1] && my >= u.lo[1])
-                if (u.try_ipel(mx, my)) best_step = step;
+            const int px = cx + (s16)(ss >> big) * dx, py = cy + (s16)(ss >> big) * dy;
+            tag[c] = step;
+            ok[c] = px <= u.hi[0] && px >= u.lo[0] && py <= u.hi[1] && py >= u.lo[1];
+            if (ok[c]) { mx[c] = px; my[c] = py; }
         }
+        u64 cost[NB];
+        u.eval_ipel<NB>(mx, my, NB, cost);
+        best_step = scan_batch<NB>(u, mx, my, cost, ok, tag, best_step);
     }
+    ur.best = u.best;
+    ur.mv[0] = u.mv[0];
+    ur.mv[1] = u.mv[1];
     return best_step;
 }
 
-// search_raster (me.c:103-197) with get_raster_range (me.c:65-100).
-__device__ __noinline__ void tz_raster(TzUnit &u, int refi, int mvi_prev_x, int mvi_prev_y)
+// search_raster (me.c:103-197) with get_raster_range (me.c:65-100).  Each
+// round's grid is fixed by its start, so it is evaluated NB points at a
+// time and scanned in raster order; a row is skipped while no candidate has
+// a cost yet (the reference's s64 tmp_best never improves then, me.c:158).
+template <int NB>
+__device__ __noinline__ void tz_raster(TzUnit &ur, int refi, int mvi_prev_x, int mvi_prev_y)
 {
+    TzUnit u = ur;   // a register copy, as in tz_diamond
     const TzArgs &a = *u.a;
     const int mvr = a.mvr;
     const int step_base = mvr == 3 ? 6 : (mvr == 4 ? 8 : 5), step_min = mvr == 3 ? 2 : (mvr == 4 ? 4 : 1);
@@ -250,20 +390,48 @@ __device__ __noinline__ void tz_raster(TzUnit &u, int refi, int mvi_prev_x, int 
             x1 = u.mv[0] + step < u.hi[0] ? u.mv[0] + step : u.hi[0];
             y1 = u.mv[1] + step < u.hi[1] ? u.mv[1] + step : u.hi[1];
         }
-        for (int my = y0; my <= y1; my += step) {
-            if (u.best == ~0ull) continue;   // the reference's s64 tmp_best (me.c:158) never improves then
-            for (int mx = x0; mx <= x1; mx += step) u.try_ipel(mx, my);
+        const int nx = x1 >= x0 ? (x1 - x0) / step + 1 : 0, ny = y1 >= y0 ? (y1 - y0) / step + 1 : 0;
+        const int total = nx * ny;
+        bool skip_row = false;
+        for (int b0 = 0; b0 < total; b0 += NB) {
+            int mx[NB], my[NB];
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                const int k = b0 + c < total ? b0 + c : 0, qy = k / nx;
+                mx[c] = x0 + (k - qy * nx) * step;
+                my[c] = y0 + qy * step;
+            }
+            u64 cost[NB];
+            u.eval_ipel<NB>(mx, my, NB, cost);
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                const int k = b0 + c;
+                if (k >= total) continue;
+                if (k % nx == 0) skip_row = u.best == ~0ull;
+                if (!skip_row && cost[c] < u.best) {
+                    u.best = cost[c];
+                    u.mv[0] = mx[c];
+                    u.mv[1] = my[c];
+                }
+            }
         }
         step >>= 1;
         if (mvr > 2) { const int sh = mvr - 2; step = (step >> sh) << sh; }
     }
+    ur.best = u.best;
+    ur.mv[0] = u.mv[0];
+    ur.mv[1] = u.mv[1];
 }
 
 // me_sub_pel_search (me.c:366-500); u.mv holds the quarter-pel
-// integer result on entry and the refined MV on return.
+// integer result on entry and the refined MV on return.  (Evaluating the
+// SATD candidates several at a time, as the integer searches do, measured
+// slower: the extra tiles in flight cost the small-tile kernels a wave per
+// SIMD.)
 template <int TT>
-__device__ __noinline__ u64 tz_subpel(TzUnit &u)
+__device__ __noinline__ u64 tz_subpel(TzUnit &ur)
 {
+    TzUnit u = ur;   // a register copy, as in tz_diamond
     const TzArgs &a = *u.a;
     const int shift = 16 - a.bi - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
     auto dist = [&](const pel *p) -> u32 {
@@ -293,6 +461,8 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
                          qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)), u.P->ref_stride);
         best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << (16 - a.bi));
     }
+    ur.mv[0] = u.mv[0];
+    ur.mv[1] = u.mv[1];
     return best;
 }
 
@@ -308,7 +478,7 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &u)
 #define TZ_ARGS(T) (*(const T *)__builtin_amdgcn_kernarg_segment_ptr())
 
 template <int TT>
-__global__ __launch_bounds__(256) void k_tz(TzArgs)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tz(TzArgs)
 {
     const TzArgs &a = TZ_ARGS(TzArgs);
     int k = 0;
@@ -387,13 +557,14 @@ __global__ __launch_bounds__(256) void k_tz(TzArgs)
     u.mv[0] = me_clip3(u.lo[0], u.hi[0], (sx + 2) >> 2);
     u.mv[1] = me_clip3(u.lo[1], u.hi[1], (sy + 2) >> 2);
     const int refi = r < a.n0 ? r : r - a.n0, nlist = r < a.n0 ? a.n0 : a.nref - a.n0;
-    const int bs = tz_diamond(u, 32);
+    constexpr int NB = TzBatch<TT>::NB;
+    const int bs = tz_diamond<NB>(u, 32);
     if (!a.bi && (abs(u.mvpx - (u.mv[0] << 2)) > 2 || abs(u.mvpy - (u.mv[1] << 2)) > 2)) {
         if (bs > 5) {
             const int pvx = a.mvi_prev ? a.mvi_prev[slot * 2] : 0, pvy = a.mvi_prev ? a.mvi_prev[slot * 2 + 1] : 0;
-            tz_raster(u, refi, pvx, pvy);
+            tz_raster<NB>(u, refi, pvx, pvy);
         }
-        tz_diamond(u, 8);
+        tz_diamond<NB>(u, 8);
     }
     out.mv_int[0] = (s16)u.mv[0];
     out.mv_int[1] = (s16)u.mv[1];
