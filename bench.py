@@ -77,6 +77,8 @@ def main():
                          "(three launches, torch bitwise_not) to scale HBM counters against")
     ap.add_argument("--no-percall", action="store_true", help="skip the 1080p-16x16 leg (batched vs per-call vs CPU)")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end leg (upload, frame, results)")
+    ap.add_argument("--valu-peak", action="store_true",
+                    help="also measure the issue rate of the search's VALU instructions (microbenchmarks)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-tz", action="store_true", help="skip the me_search_tz leg")
     ap.add_argument("--serve-frames", type=int, default=32, help="frames per stream in the serving leg")
@@ -601,6 +603,25 @@ def main():
     for r in dref:
         hip.ref_destroy(r)
 
+    valu_peak = None
+    if args.valu_peak:
+        import ctypes
+        from uavs3e_amd import build as ub
+        pk = ctypes.CDLL(ub.peak_lib())
+        pk.me_peak_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        names = ["v_add_u32", "v_sad_u8", "v_qsad_pk_u16_u8", "v_sad_u16", "v_min_f64", "v_add_u32_dpp"]
+        valu_peak = {"note": "lane-instructions/s of one instruction kind: 8 independent chains per lane, 2048 "
+                             "workgroups of 256 lanes (8 waves per SIMD), HIP events; per CU and clock = "
+                             "value / (256 CUs x clock)"}
+        for k, nm in enumerate(names):
+            ms_ = ctypes.c_double()
+            li_ = ctypes.c_double()
+            if pk.me_peak_run(k, 2048, 2048, ctypes.byref(ms_), ctypes.byref(li_)) != 0:
+                continue
+            valu_peak[nm] = li_.value / (ms_.value * 1e-3)
+        print("[bench] VALU peak: %s" % ", ".join("%s %.1f T/s" % (k, v / 1e12) for k, v in valu_peak.items()
+                                                  if k != "note"), file=sys.stderr, flush=True)
+
     line = {
         "metric": "1080p ME CTUs/s batched vs per-call vs AVX2, bit-exact MVs; p50/p99 CTU latency",
         "value": ctus_per_s,
@@ -642,6 +663,7 @@ def main():
         "bi_search": bi_search,
         "affine_mc": affine,
         "cpu_baseline": cpu,
+        "valu_peak": valu_peak,
         "speedup_vs_cpu": (ctus_per_s / cpu["value"]) if cpu else None,
     }
     print(json.dumps(line))

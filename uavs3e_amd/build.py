@@ -82,6 +82,20 @@ def _build_host(bd):
     return out
 
 
+def peak_lib():
+    return os.path.join(LIB, "libme_peak.so")
+
+
+def _build_peak():
+    """The VALU issue-rate microbenchmarks bench.py --valu-peak runs (a
+    library of its own: not part of the ME C ABI)."""
+    out = peak_lib()
+    tmp = out + ".tmp%d" % os.getpid()
+    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, "peak.hip"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build(jobs=None, bit_depths=(8, 10)):
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BUILD_DIR, exist_ok=True)
@@ -91,9 +105,11 @@ def build(jobs=None, bit_depths=(8, 10)):
         links = [ex.submit(_link_hip, bd, [f.result() for f in units[bd]]) for bd in bit_depths]
         for f in links:
             f.result()
+        peak = ex.submit(_build_peak)
         hosts = [ex.submit(_build_host, bd) for bd in bit_depths]   # links the HIP library
         for f in hosts:
             f.result()
+        peak.result()
 
 
 if __name__ == "__main__":
