@@ -781,15 +781,22 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
 
     const int lane = tid & 63;
     const int rx = lane & 7, ry = lane >> 3;
-    // A node's window (me.c:508-512) binds only on the right and bottom for
-    // R <= 127: dx <= W-1-px, dy <= H-1-py.  Blocks whose displacement tile
-    // keeps every node of the CTU (px <= cx+124, py <= cy+124) inside skip
-    // the per-node test.
-    const int dy_last = b.dy0 + a.band_rows - 1 < a.hi ? b.dy0 + a.band_rows - 1 : a.hi;
-    const bool edge = !(b.cx + 124 + b.dxe <= P.pic_w - 1 && b.cy + 124 + dy_last <= P.pic_h - 1 && a.R <= 127);
+    // A node's window (me.c:508-512) is -127-px <= dx <= W-1-px and the same
+    // for y.  The upper bounds never need a test: for a node inside the
+    // picture (the only ones whose keys are used) a displacement past
+    // W-1-px puts the whole block into the edge-replicated padding, so its
+    // SAD equals the SAD at dx = W-1-px (whose left column is the last
+    // picture column and the rest padding) while its MV bits are no smaller
+    // (bits grow with |dx|, dx > W-1-px >= 0) and its raster index is
+    // larger: its key can never be the minimum.  The same holds below the
+    // picture and in the corner.  The lower bounds do need it, where they
+    // bind: only for R >= 128, in blocks at the left or top of the picture
+    // whose tile reaches past -127.
+    const bool edge = b.cx + b.dxs < -127 || b.cy + b.dy0 < -127;
     u64 *kb = a.keys + (size_t)job * a.npu * a.nref + b.ref;
-    // Interior blocks with sh == 0 (the common case) take the fast path;
-    // the others test each node's window and shift the SAD.
+    // Blocks without a lower-bound test and with sh == 0 (every 10-bit
+    // configuration and 8-bit with R <= 127) take the fast path; the others
+    // test each node's window and shift the SAD.
     const bool fast = !edge && a.sh == 0;
     if (MODE == 1) {
         double st[1];
