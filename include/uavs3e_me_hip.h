@@ -185,7 +185,7 @@ typedef struct me_frame_cfg_t {
     uint32_t shape_mask;   /* bit s selects shape s of the 29-shape list below */
     int num_refs;          /* references searched per PU (tbl_refi_bits row) */
     int band_rows;         /* tuning: displacement rows per workgroup, 0 = default */
-    int waves;             /* tuning: waves per workgroup (1..4), 0 = default */
+    int waves;             /* tuning: waves per workgroup (1, 2, 4 or 8), 0 = default */
     int dx_cols;           /* tuning: displacement columns per workgroup (multiple of 4), 0 = all */
     int num_refs_l1;       /* 0: every reference is in list 0.  n > 0: refs[num_refs-n ..] form list 1
                               (bi-directional frames); the final cost of a reference then uses
@@ -215,7 +215,7 @@ int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, voi
 /* Autotuner of the integer-search schedule (no counterpart in the
  * reference).  Times the integer search of cfg's frame against refs, with
  * the current picture `cur` (host memory, `stride` samples per row), for
- * every candidate workgroup size (1, 2, 4 waves) x displacement rows per
+ * every candidate workgroup size (1, 2, 4, 8 waves) x displacement rows per
  * workgroup (4..32) x displacement columns per workgroup (all, 64, 32) whose
  * LDS window fits, `iters` runs each (HIP events on the plan's stream), and
  * writes the fastest into cfg->band_rows / waves / dx_cols.  The schedule

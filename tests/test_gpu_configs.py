@@ -70,7 +70,7 @@ def uhd10_r128():
 
 # (band_rows, waves, dx_cols): 0 = the library default, which is the
 # schedule the autotuner picks at 2160p (16 rows, 4 waves, all columns).
-@pytest.mark.parametrize("sched", [(0, 0, 0), (8, 2, 64), (4, 1, 32), (32, 4, 64), (5, 4, 128)],
+@pytest.mark.parametrize("sched", [(0, 0, 0), (8, 2, 64), (4, 1, 32), (32, 4, 64), (5, 4, 128), (16, 8, 0)],
                          ids=lambda s: "b%d_w%d_c%d" % s)
 def test_10bit_4ref_r128(uhd10_r128, sched):
     cur, refs, ores, obest = uhd10_r128

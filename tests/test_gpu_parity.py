@@ -266,7 +266,7 @@ def test_plan_tuning_variants_agree():
     hip = ua.MeHip(8)
     lam = wl.lambda_mv(32)
     base = _run_plan(hip, cur, refs, 12, lam, MASK_ALL)
-    for band, waves, cols in ((1, 1, 0), (5, 2, 8), (25, 4, 12), (3, 4, 4)):
+    for band, waves, cols in ((1, 1, 0), (5, 2, 8), (25, 4, 12), (3, 4, 4), (11, 8, 0), (16, 8, 8)):
         res, best = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, band_rows=band, waves=waves, dx_cols=cols)
         assert np.array_equal(res, base[0]) and np.array_equal(best, base[1]), (band, waves, cols)
 

@@ -44,7 +44,7 @@ def host_lib(bd):
 # expensive device compiles run in parallel.
 HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []), ("server.hip", []), ("mc.hip", []), ("affine_me.hip", []), ("tz.hip", ["-DME_TZ_PART=0"]), ("tz.hip", ["-DME_TZ_PART=1"]), ("tz.hip", ["-DME_TZ_PART=2"]),
              ("intsearch.hip", ["-DME_IS_WAVES=1"]), ("intsearch.hip", ["-DME_IS_WAVES=2"]),
-             ("intsearch.hip", ["-DME_IS_WAVES=4"]),
+             ("intsearch.hip", ["-DME_IS_WAVES=4"]), ("intsearch.hip", ["-DME_IS_WAVES=8"]),
              ("subpel.hip", [])]
 BUILD_DIR = os.path.join(ROOT, "_build")
 

@@ -104,12 +104,13 @@ struct IntArgs {
     int base[ME_NSHAPES];    // first PU index of each shape inside a CTU (-1: not selected)
 };
 
-// Launches k_int_search<mode, waves> (waves 1, 2 or 4; mode 1 = 16x16 PUs
+// Launches k_int_search<mode, waves> (waves 1, 2, 4 or 8; mode 1 = 16x16 PUs
 // only) on grid (bands * nbx, jobs, refs) with `lds` bytes of LDS.
 int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
 int me_int_search_launch_w1(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
 int me_int_search_launch_w2(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
 int me_int_search_launch_w4(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
+int me_int_search_launch_w8(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Sub-pel refinement (me_sub_pel_search, me.c:366-500, cost_type 0, bi 0)
@@ -149,6 +150,7 @@ int me_subpel_preload(hipStream_t st);
 int me_int_search_preload_w1(int mode, size_t lds, hipStream_t st);
 int me_int_search_preload_w2(int mode, size_t lds, hipStream_t st);
 int me_int_search_preload_w4(int mode, size_t lds, hipStream_t st);
+int me_int_search_preload_w8(int mode, size_t lds, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // One search configuration: window geometry, key layout, schedule and the

@@ -52,6 +52,7 @@ int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_
     switch (waves) {
     case 1: return me_int_search_launch_w1(mode, a, grid, lds, st);
     case 2: return me_int_search_launch_w2(mode, a, grid, lds, st);
+    case 8: return me_int_search_launch_w8(mode, a, grid, lds, st);
     default: return me_int_search_launch_w4(mode, a, grid, lds, st);
     }
 }
@@ -109,7 +110,7 @@ static int search_layout(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nre
         else p->base[s] = -1;
     }
     p->npu = n;
-    p->waves = waves > 0 ? (waves >= 4 ? 4 : (waves >= 2 ? 2 : 1)) : 4;
+    p->waves = waves > 0 ? (waves >= 8 ? 8 : waves >= 4 ? 4 : (waves >= 2 ? 2 : 1)) : 4;
     p->band_rows = band_rows > 0 ? band_rows : 16;
     if (p->band_rows > p->span) p->band_rows = p->span;
     p->nbands = (p->span + p->band_rows - 1) / p->band_rows;
@@ -223,9 +224,10 @@ void me_search_free(MeSearch *p)
 
 int me_search_warmup(const MeSearch *p, hipStream_t st)
 {
-    int rc = p->waves == 1 ? me_int_search_preload_w1(p->mode, p->lds, st)
-                           : (p->waves == 2 ? me_int_search_preload_w2(p->mode, p->lds, st)
-                                            : me_int_search_preload_w4(p->mode, p->lds, st));
+    int rc = p->waves == 1   ? me_int_search_preload_w1(p->mode, p->lds, st)
+             : p->waves == 2 ? me_int_search_preload_w2(p->mode, p->lds, st)
+             : p->waves == 8 ? me_int_search_preload_w8(p->mode, p->lds, st)
+                             : me_int_search_preload_w4(p->mode, p->lds, st);
     if (rc || me_subpel_preload(st)) return -1;
     // n = 0: no access
     hipLaunchKernelGGL(k_fill_keys, dim3(1), dim3(256), 0, st, (u64 *)nullptr, (size_t)0);
@@ -557,7 +559,7 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
     }
     const int span = 2 * cfg->search_range + 1;
     const int full = (span + 3) & ~3;
-    const int bands[] = {4, 8, 16, 32}, waves[] = {1, 2, 4}, cols[] = {0, 64, 32};
+    const int bands[] = {4, 8, 16, 32}, waves[] = {1, 2, 4, 8}, cols[] = {0, 64, 32};
     double best = 1e30;
     me_frame_cfg_t c = *cfg, pick = *cfg;
     int n = 0;

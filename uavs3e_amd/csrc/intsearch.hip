@@ -4,7 +4,7 @@
 //   window        get_max_search_range (is_padding = 1)           me.c:502-527
 //   full search   raster y-major, first strict minimum wins        me.c:150-190
 //
-// This file is compiled once per workgroup size (ME_IS_WAVES = 1, 2, 4) so
+// This file is compiled once per workgroup size (ME_IS_WAVES = 1, 2, 4, 8) so
 // that the instances build in parallel.
 #include "frame.h"
 
@@ -708,8 +708,9 @@ __device__ __forceinline__ bool merge_waves(double *st, unsigned char *smem)
 // 58 classes per displacement (ME_IS_FUSED, the default) or in two passes
 // (small PUs, then large PUs) with half the live node states.
 template <int MODE, int WAVES>
-// The LDS window allows at most 2 workgroups per CU (8 waves: 2 per SIMD),
-// so the register budget of 2 waves per SIMD costs no occupancy.
+// The register budget is 2 waves per SIMD: 2 workgroups of 4 waves, or one
+// of 8 (the 10-bit windows at R = 128 leave room for only one workgroup's
+// LDS per CU).
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_int_search(IntArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
