@@ -13,18 +13,16 @@
 #ifndef ME_IS_WAVES
 #define ME_IS_WAVES 4
 #endif
-// 1: one pass over the displacements updates all 58 node classes of a lane;
-// 0: two passes (33 small, then 25 large classes), recomputing the leaves.
-// One pass is faster: the leaves' SAD instructions (v_qsad_pk_u16_u8 issues
-// at a quarter of the rate of a plain 32-bit op) then run once.
-#ifndef ME_IS_FUSED
-#define ME_IS_FUSED 1
-#endif
+// One pass over the displacements updates all 58 node classes of a lane.
+// (Two passes, 33 small and then 25 large classes with half the live node
+// states, measured slower: the leaves' SAD instructions then run twice.)
 
-// Values of NK displacements of one node: at 8 bits four displacements are
+// A step of the search covers four displacements dxb .. dxb+3 of one row
+// (NKEY).  Values of NK displacements of one node: at 8 bits all four are
 // packed as u16 lanes (lo = d0,d1; hi = d2,d3) -- a sum of at most 256
 // samples stays below 2^16, so plain 32-bit adds never carry between lanes;
-// at 10 bits one displacement per u32.
+// at 10 bits one displacement per u32 (a step then holds four of them).
+#define NKEY 4
 #if BIT_DEPTH == 8
 #define NK 4
 struct Val {
@@ -47,14 +45,14 @@ __device__ __forceinline__ Val operator+(Val a, Val b) { return {a.v + b.v}; }
 __device__ __forceinline__ Val vsel(bool c, Val a, Val b) { return c ? a : b; }
 #endif
 
-// Uniform key parts of the NK displacements being processed.  A displacement
+// Uniform key parts of the step's four displacements.  A displacement
 // outside the block's range gets a high word that can never win.
 // The key parts are vector values rather than arrays: an array member let
 // the compiler keep them in private memory and reload them as pairs.
-typedef u32 u32xk __attribute__((ext_vector_type(NK == 1 ? 2 : NK)));
+typedef u32 u32xk __attribute__((ext_vector_type(NKEY)));
 struct KeyCtx {
     u32xk khi, klo;
-    int dx[NK];
+    int dx[NKEY];
     int dy, sh;
 };
 
@@ -148,13 +146,15 @@ struct Region {
 //   st[0..7] 4x8 (c, r2)   st[8..15] 8x4 (c2, r)  st[16..19] 4x16 (c)
 //   st[20..23] 16x4 (r)    st[24..27] 8x8 (c2,r2) st[28..29] 8x16 (c2)
 //   st[30..31] 16x8 (r2)   st[32] 16x16
+// The region's values hold displacements kb .. kb+NK-1 of the step.
 #define N_SMALL 33
 template <bool EDGE, bool SH0>
-__device__ __forceinline__ void pass_small(const Region &g, double *st, const KeyCtx &c, int px, int py, int wlim, int hlim)
+__device__ __forceinline__ void pass_small(const Region &g, double *st, const KeyCtx &c, int kb, int px, int py, int wlim,
+                                           int hlim)
 {
 #pragma unroll
     for (int k = 0; k < NK; k++) {
-#define U(I, V, X_, Y_) upd1<EDGE, SH0>(st[I], (V).get(k), c, k, X_, Y_, wlim, hlim)
+#define U(I, V, X_, Y_) upd1<EDGE, SH0>(st[I], (V).get(k), c, kb + k, X_, Y_, wlim, hlim)
 #pragma unroll
         for (int cc = 0; cc < 4; cc++)
 #pragma unroll
@@ -180,7 +180,6 @@ __device__ __forceinline__ void pass_small(const Region &g, double *st, const Ke
     }
 }
 
-#if BIT_DEPTH == 8
 // Key parts of one displacement per lane (k chosen by lane bits).
 struct KeyV {
     u32 hi, lo;
@@ -206,9 +205,27 @@ __device__ __forceinline__ KeyV key_sel(bool b, const KeyV &one, const KeyV &zer
 
 __device__ __forceinline__ KeyV key_k(const KeyCtx &c, int k) { return {c.khi[k], c.klo[k], c.dx[k]}; }
 
-// 16-bit lane sums without carries between lanes: lo + lo, hi + hi.
-__device__ __forceinline__ u32 lo16(u32 v) { return v & 0xFFFFu; }
-__device__ __forceinline__ u32 hi16(u32 v) { return v >> 16; }
+// The values of one node for the step's four displacements (Quad), and two
+// of them, displacements 2p and 2p+1 (Pair), as the sparse classes take
+// them.  At 8 bits a Quad is a Val and a Pair one of its words (u16 lanes);
+// at 10 bits a Quad is four u32 and a Pair two.
+#if BIT_DEPTH == 8
+typedef Val Quad;
+typedef u32 Pair;
+__device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return p ? q.hi : q.lo; }
+__device__ __forceinline__ u32 pair_elem(Pair v, int j) { return j ? (v >> 16) : (v & 0xFFFFu); }
+template <typename F> __device__ __forceinline__ Pair pair_map(Pair v, F f) { return f(v); }
+#else
+struct Quad {
+    u32 v[4];
+};
+struct Pair {
+    u32 a, b;
+};
+__device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return {p ? q.v[2] : q.v[0], p ? q.v[3] : q.v[1]}; }
+__device__ __forceinline__ u32 pair_elem(Pair v, int j) { return j ? v.b : v.a; }
+template <typename F> __device__ __forceinline__ Pair pair_map(Pair v, F f) { return {f(v.a), f(v.b)}; }
+#endif
 
 // The 19 sparse classes (nodes of 32x16 and larger that are not dense pair
 // slots) with the four displacements spread over the lanes that share a
@@ -225,7 +242,7 @@ __device__ __forceinline__ u32 hi16(u32 v) { return v >> 16; }
 // The lanes holding different displacements of a node are merged before
 // publishing (merge_sparse).
 template <bool EDGE, bool SH0>
-__device__ __forceinline__ void sparse_scatter(const Region &g, const Val *t32x8, const Val *t8x32, double *st,
+__device__ __forceinline__ void sparse_scatter(const Quad &s, const Quad *t32x8, const Quad *t8x32, double *st,
                                                const KeyCtx &c, int rx, int ry, int cx, int cy, int wlim, int hlim)
 {
     const int X = rx * 16, Y = ry * 16, px = cx + X, py = cy + Y;
@@ -234,12 +251,16 @@ __device__ __forceinline__ void sparse_scatter(const Region &g, const Val *t32x8
     const KeyV kxa = key_sel(x0, key_k(c, 2), key_k(c, 0)), kxb = key_sel(x0, key_k(c, 3), key_k(c, 1));
     const KeyV kya = key_sel(y0, key_k(c, 2), key_k(c, 0)), kyb = key_sel(y0, key_k(c, 3), key_k(c, 1));
     const KeyV kxy = key_sel(y0, kxb, kxa), kxx = key_sel(x1, kxb, kxa), kyy = key_sel(y1, kyb, kya);
-    const Val s = g.s16;
-    // x pairs: lane x0 keeps displacements 2x0, 2x0+1 (the lo word holds 0, 1)
-    const u32 sx = px1(x0 ? s.lo : s.hi), kx = x0 ? s.hi : s.lo;
-    const u32 t32x16a = lo16(kx) + lo16(sx), t32x16b = hi16(kx) + hi16(sx);
-    const u32 sy = py1(y0 ? s.lo : s.hi), ky = y0 ? s.hi : s.lo;
-    const u32 t16x32a = lo16(ky) + lo16(sy), t16x32b = hi16(ky) + hi16(sy);
+    const auto fx1 = [](u32 v) { return px1(v); };
+    const auto fy1 = [](u32 v) { return py1(v); };
+    const auto fx2 = [](u32 v) { return px2(v); };
+    const auto fy2 = [](u32 v) { return py2(v); };
+    // x pairs: lane x0 keeps displacements 2x0, 2x0+1 and receives them from
+    // its partner, which selects the pair by its own (opposite) x0
+    const Pair sx = pair_map(quad_pair(s, !x0), fx1), kx = quad_pair(s, x0);
+    const u32 t32x16a = pair_elem(kx, 0) + pair_elem(sx, 0), t32x16b = pair_elem(kx, 1) + pair_elem(sx, 1);
+    const Pair sy = pair_map(quad_pair(s, !y0), fy1), ky = quad_pair(s, y0);
+    const u32 t16x32a = pair_elem(ky, 0) + pair_elem(sy, 0), t16x32b = pair_elem(ky, 1) + pair_elem(sy, 1);
     updv<EDGE, SH0>(st[6], t32x16a, kxa, c, x2, py, wlim, hlim);
     updv<EDGE, SH0>(st[6], t32x16b, kxb, c, x2, py, wlim, hlim);
     updv<EDGE, SH0>(st[7], t16x32a, kya, c, px, y2, wlim, hlim);
@@ -273,15 +294,15 @@ __device__ __forceinline__ void sparse_scatter(const Region &g, const Val *t32x8
     // 64x8 (from 32x8, equal on both lanes of an x pair) and 8x64
 #pragma unroll
     for (int r2 = 0; r2 < 2; r2++) {
-        const u32 v = x0 ? t32x8[r2].hi : t32x8[r2].lo, w = px2(v);
-        updv<EDGE, SH0>(st[9 + r2], lo16(v) + lo16(w), kxa, c, x4, py + 8 * r2, wlim, hlim);
-        updv<EDGE, SH0>(st[9 + r2], hi16(v) + hi16(w), kxb, c, x4, py + 8 * r2, wlim, hlim);
+        const Pair v = quad_pair(t32x8[r2], x0), w = pair_map(v, fx2);
+        updv<EDGE, SH0>(st[9 + r2], pair_elem(v, 0) + pair_elem(w, 0), kxa, c, x4, py + 8 * r2, wlim, hlim);
+        updv<EDGE, SH0>(st[9 + r2], pair_elem(v, 1) + pair_elem(w, 1), kxb, c, x4, py + 8 * r2, wlim, hlim);
     }
 #pragma unroll
     for (int c2 = 0; c2 < 2; c2++) {
-        const u32 v = y0 ? t8x32[c2].hi : t8x32[c2].lo, w = py2(v);
-        updv<EDGE, SH0>(st[12 + c2], lo16(v) + lo16(w), kya, c, px + 8 * c2, y4, wlim, hlim);
-        updv<EDGE, SH0>(st[12 + c2], hi16(v) + hi16(w), kyb, c, px + 8 * c2, y4, wlim, hlim);
+        const Pair v = quad_pair(t8x32[c2], y0), w = pair_map(v, fy2);
+        updv<EDGE, SH0>(st[12 + c2], pair_elem(v, 0) + pair_elem(w, 0), kya, c, px + 8 * c2, y4, wlim, hlim);
+        updv<EDGE, SH0>(st[12 + c2], pair_elem(v, 1) + pair_elem(w, 1), kyb, c, px + 8 * c2, y4, wlim, hlim);
     }
 }
 
@@ -332,13 +353,12 @@ __device__ __forceinline__ void merge_sparse(double *st)
     st[19] = kmin(st[19], dpp64_y1(st[19]));
     st[19] = kmin(st[19], dpp64_y2(st[19]));
 }
-#endif
 
 // Pass L: the 609 PUs larger than a 16x16 region.
 // Pairs of regions (x pairs, y pairs) hold 14 nodes between two lanes; each
 // lane keeps half of them (3 "dense" slots per direction, chosen by the lane's
-// rank in the pair).  The 19 sparser classes are summed on every lane of their
-// group and kept by all of them; one lane per group publishes.
+// rank in the pair).  The 19 sparser classes are reduce-scattered over the
+// lanes of their group (sparse_scatter); one lane per group publishes.
 //   st[0..2] x-pair slots: 32x4 rows {0,2}, {1,3} by rank; 32x8 row = rank
 //   st[3..5] y-pair slots: 4x32 cols {0,2}, {1,3}; 8x32 col = rank
 //   st[6] 32x16  st[7] 16x32  st[8] 32x32  st[9..10] 64x8 (r2)  st[11] 64x16
@@ -346,15 +366,16 @@ __device__ __forceinline__ void merge_sparse(double *st)
 //   st[18] 128x16  st[19] 16x128  st[20] 128x32  st[21] 32x128  st[22] 128x64
 //   st[23] 64x128  st[24] 128x128
 #define N_LARGE 25
+// The dense slots for the region's displacements kb .. kb+NK-1; returns the
+// 32x8 and 8x32 sums that the sparse classes build on.
 template <bool EDGE, bool SH0>
-__device__ __forceinline__ void pass_large(const Region &g, double *st, const KeyCtx &c, int rx, int ry, int cx, int cy,
-                                           int wlim, int hlim)
+__device__ __forceinline__ void pass_dense(const Region &g, double *st, const KeyCtx &c, int kb, int rx, int ry, int cx,
+                                           int cy, int wlim, int hlim, Val *t32x8, Val *t8x32)
 {
     const int X = rx * 16, Y = ry * 16, px = cx + X, py = cy + Y;
     const bool xr = rx & 1, yr = ry & 1;
-    const int x2 = cx + (X & ~31), x4 = cx + (X & ~63), y2 = cy + (Y & ~31), y4 = cy + (Y & ~63);
-    // dense x pairs
-    Val t32x4[4], t32x8[2], t4x32[4], t8x32[2];
+    const int x2 = cx + (X & ~31), y2 = cy + (Y & ~31);
+    Val t32x4[4], t4x32[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) t32x4[r] = g.h16[r] + vpx1(g.h16[r]);
 #pragma unroll
@@ -367,60 +388,13 @@ __device__ __forceinline__ void pass_large(const Region &g, double *st, const Ke
     const Val d3 = vsel(yr, t4x32[2], t4x32[0]), d4 = vsel(yr, t4x32[3], t4x32[1]), d5 = vsel(yr, t8x32[1], t8x32[0]);
 #pragma unroll
     for (int k = 0; k < NK; k++) {
-        upd1<EDGE, SH0>(st[0], d0.get(k), c, k, x2, py + (xr ? 8 : 0), wlim, hlim);
-        upd1<EDGE, SH0>(st[1], d1.get(k), c, k, x2, py + (xr ? 12 : 4), wlim, hlim);
-        upd1<EDGE, SH0>(st[2], d2.get(k), c, k, x2, py + (xr ? 8 : 0), wlim, hlim);
-        upd1<EDGE, SH0>(st[3], d3.get(k), c, k, px + (yr ? 8 : 0), y2, wlim, hlim);
-        upd1<EDGE, SH0>(st[4], d4.get(k), c, k, px + (yr ? 12 : 4), y2, wlim, hlim);
-        upd1<EDGE, SH0>(st[5], d5.get(k), c, k, px + (yr ? 8 : 0), y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[0], d0.get(k), c, kb + k, x2, py + (xr ? 8 : 0), wlim, hlim);
+        upd1<EDGE, SH0>(st[1], d1.get(k), c, kb + k, x2, py + (xr ? 12 : 4), wlim, hlim);
+        upd1<EDGE, SH0>(st[2], d2.get(k), c, kb + k, x2, py + (xr ? 8 : 0), wlim, hlim);
+        upd1<EDGE, SH0>(st[3], d3.get(k), c, kb + k, px + (yr ? 8 : 0), y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[4], d4.get(k), c, kb + k, px + (yr ? 12 : 4), y2, wlim, hlim);
+        upd1<EDGE, SH0>(st[5], d5.get(k), c, kb + k, px + (yr ? 8 : 0), y2, wlim, hlim);
     }
-#if BIT_DEPTH == 8
-    sparse_scatter<EDGE, SH0>(g, t32x8, t8x32, st, c, rx, ry, cx, cy, wlim, hlim);
-#else
-    // sparse classes, one displacement at a time
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const u32 s = g.s16.get(k);
-        const u32 t32x16 = s + px1(s), t16x32 = s + py1(s);
-        const u32 t32x32 = t32x16 + py1(t32x16);
-        const u32 a8 = t32x8[0].get(k), b8 = t32x8[1].get(k), c8 = t8x32[0].get(k), d8 = t8x32[1].get(k);
-        const u32 t64x8a = a8 + px2(a8), t64x8b = b8 + px2(b8);
-        const u32 t8x64a = c8 + py2(c8), t8x64b = d8 + py2(d8);
-        const u32 t64x16 = t32x16 + px2(t32x16), t16x64 = t16x32 + py2(t16x32);
-        const u32 t64x32 = t32x32 + px2(t32x32), t32x64 = t32x32 + py2(t32x32);
-        const u32 t64x64 = t64x32 + py2(t64x32);
-        const u32 t128x16 = t64x16 + px4(t64x16), t16x128 = t16x64 + py4(t16x64);
-        const u32 t128x32 = t64x32 + px4(t64x32), t32x128 = t32x64 + py4(t32x64);
-        const u32 t128x64 = t64x64 + px4(t64x64), t64x128 = t64x64 + py4(t64x64);
-        const u32 t128x128 = t128x64 + py4(t128x64);
-        upd1<EDGE, SH0>(st[6], t32x16, c, k, x2, py, wlim, hlim);
-        upd1<EDGE, SH0>(st[7], t16x32, c, k, px, y2, wlim, hlim);
-        upd1<EDGE, SH0>(st[8], t32x32, c, k, x2, y2, wlim, hlim);
-        upd1<EDGE, SH0>(st[9], t64x8a, c, k, x4, py, wlim, hlim);
-        upd1<EDGE, SH0>(st[10], t64x8b, c, k, x4, py + 8, wlim, hlim);
-        upd1<EDGE, SH0>(st[11], t64x16, c, k, x4, py, wlim, hlim);
-        upd1<EDGE, SH0>(st[12], t8x64a, c, k, px, y4, wlim, hlim);
-        upd1<EDGE, SH0>(st[13], t8x64b, c, k, px + 8, y4, wlim, hlim);
-        upd1<EDGE, SH0>(st[14], t16x64, c, k, px, y4, wlim, hlim);
-        upd1<EDGE, SH0>(st[15], t64x32, c, k, x4, y2, wlim, hlim);
-        upd1<EDGE, SH0>(st[16], t32x64, c, k, x2, y4, wlim, hlim);
-        upd1<EDGE, SH0>(st[17], t64x64, c, k, x4, y4, wlim, hlim);
-        upd1<EDGE, SH0>(st[18], t128x16, c, k, cx, py, wlim, hlim);
-        upd1<EDGE, SH0>(st[19], t16x128, c, k, px, cy, wlim, hlim);
-        upd1<EDGE, SH0>(st[20], t128x32, c, k, cx, y2, wlim, hlim);
-        upd1<EDGE, SH0>(st[21], t32x128, c, k, x2, cy, wlim, hlim);
-        upd1<EDGE, SH0>(st[22], t128x64, c, k, cx, y4, wlim, hlim);
-        upd1<EDGE, SH0>(st[23], t64x128, c, k, x4, cy, wlim, hlim);
-        upd1<EDGE, SH0>(st[24], t128x128, c, k, cx, cy, wlim, hlim);
-    }
-#endif
-}
-
-template <bool EDGE, bool SH0>
-__device__ __forceinline__ void pass_16(const Region &g, double *st, const KeyCtx &c, int px, int py, int wlim, int hlim)
-{
-#pragma unroll
-    for (int k = 0; k < NK; k++) upd1<EDGE, SH0>(st[0], g.s16.get(k), c, k, px, py, wlim, hlim);
 }
 
 __device__ __forceinline__ void atomic_key_min(u64 *p, double st)
@@ -474,40 +448,50 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
         for (int c = 0; c < 4; c++) L[r][c] = {(u32)acc[r][c], (u32)(acc[r][c] >> 32)};
 }
 #else
-// v_sad_u16 on sample pairs.  Two displacements dxb + k0 (k0 even: its pairs
-// are dword aligned) and dxb + k0 + 1 (pairs realigned with v_alignbyte)
-// share each window row read: La and Lb receive their 16 leaves.
-__device__ __forceinline__ void leaves2(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0,
-                                        int ccol0, int k0, Val La[4][4], Val Lb[4][4])
+// v_sad_u16 on sample pairs.  The step's four displacements dxb + k share
+// each window row read (10 dwords, samples 0..19 from the step's column):
+// k = 0 and 2 take dword-aligned pairs, k = 1 and 3 pairs realigned with
+// v_alignbyte (nine realigned dwords serve both).  L[k] receives the 16
+// leaves of displacement k.
+__device__ __forceinline__ void leaves4(const pel *win, int wcols, int wrow0, int c0, const pel *curl, int crow0,
+                                        int ccol0, Val L[NKEY][4][4])
 {
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int k = 0; k < NKEY; k++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) La[r][c].v = Lb[r][c].v = 0;
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) L[k][r][c].v = 0;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0 + k0);
-        u32 d[9];
+        const u32 *rp = (const u32 *)(win + (wrow0 + r) * wcols + wskew(wrow0 + r) + c0);
+        u32 d[10], o[9];
 #pragma unroll
-        for (int j = 0; j < 9; j++) d[j] = rp[j];
+        for (int j = 0; j < 10; j++) d[j] = rp[j];
+#pragma unroll
+        for (int m = 0; m < 9; m++) o[m] = __builtin_amdgcn_alignbyte(d[m + 1], d[m], 2);   // samples 2m+1, 2m+2
         const uint4 c01 = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0);
         const uint4 c23 = *(const uint4 *)(curl + (crow0 + r) * 128 + ccol0 + 8);
         const u32 cc[8] = {c01.x, c01.y, c01.z, c01.w, c23.x, c23.y, c23.z, c23.w};
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            La[r >> 2][c].v = __builtin_amdgcn_sad_u16(d[2 * c], cc[2 * c],
-                                                       __builtin_amdgcn_sad_u16(d[2 * c + 1], cc[2 * c + 1], La[r >> 2][c].v));
-            const u32 p0 = __builtin_amdgcn_alignbyte(d[2 * c + 1], d[2 * c], 2);
-            const u32 p1 = __builtin_amdgcn_alignbyte(d[2 * c + 2], d[2 * c + 1], 2);
-            Lb[r >> 2][c].v = __builtin_amdgcn_sad_u16(p0, cc[2 * c], __builtin_amdgcn_sad_u16(p1, cc[2 * c + 1], Lb[r >> 2][c].v));
+            // leaf column c: current samples 4c .. 4c+3, window samples 4c+k ..
+            u32 *l = &L[0][r >> 2][c].v;
+            l[0] = __builtin_amdgcn_sad_u16(d[2 * c], cc[2 * c], __builtin_amdgcn_sad_u16(d[2 * c + 1], cc[2 * c + 1], l[0]));
+            u32 &l1 = L[1][r >> 2][c].v, &l2 = L[2][r >> 2][c].v, &l3 = L[3][r >> 2][c].v;
+            l1 = __builtin_amdgcn_sad_u16(o[2 * c], cc[2 * c], __builtin_amdgcn_sad_u16(o[2 * c + 1], cc[2 * c + 1], l1));
+            l2 = __builtin_amdgcn_sad_u16(d[2 * c + 1], cc[2 * c], __builtin_amdgcn_sad_u16(d[2 * c + 2], cc[2 * c + 1], l2));
+            l3 = __builtin_amdgcn_sad_u16(o[2 * c + 1], cc[2 * c], __builtin_amdgcn_sad_u16(o[2 * c + 2], cc[2 * c + 1], l3));
         }
-        // keep the compiler from hoisting the LDS reads of all 16 rows (17
-        // dwords each) ahead of their use, which alone exceeds the registers:
-        // a block of 4 rows is summed before the next block is read
+        // keep the compiler from hoisting the LDS reads of all 16 rows ahead
+        // of their use, which alone exceeds the registers: a block of 4 rows
+        // is summed before the next block is read
         if ((r & 3) == 3) {
-            Val *a = La[r >> 2], *b = Lb[r >> 2];
-            asm volatile("" : "+v"(a[0].v), "+v"(a[1].v), "+v"(a[2].v), "+v"(a[3].v), "+v"(b[0].v), "+v"(b[1].v),
-                         "+v"(b[2].v), "+v"(b[3].v)::"memory");
+#pragma unroll
+            for (int k = 0; k < NKEY; k++) {
+                Val *q = L[k][r >> 2];
+                asm volatile("" : "+v"(q[0].v), "+v"(q[1].v), "+v"(q[2].v), "+v"(q[3].v)::"memory");
+            }
         }
     }
 }
@@ -517,8 +501,8 @@ struct Block {
     int cx, cy, dy0, dxs, dxe, ref;
 };
 
-// One pass over the block's displacements: PASS 0 = small PUs, 1 = large
-// PUs, 2 = 16x16 only.  Returns the node states in st.
+// One pass over the block's displacements: PASS 3 = every PU class, 2 =
+// 16x16 only.  Returns the node states in st.
 template <int PASS, int WAVES, bool EDGE, bool SH0, int NS>
 __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, const Block &b, const pel *win, const pel *curl,
                                          double *st)
@@ -538,67 +522,62 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
         const u32 ry_rate = a.rate[dyi];
         for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
             if (dxb + 3 < a.lo) continue;
-#if BIT_DEPTH == 8
-            {
-                KeyCtx kc;
-                kc.dy = dy;
-                kc.sh = a.sh;
+            KeyCtx kc;
+            kc.dy = dy;
+            kc.sh = a.sh;
 #pragma unroll
-                for (int k = 0; k < NK; k++) {
-                    const int dx = dxb + k;
-                    kc.dx[k] = dx;
-                    if (dx < a.lo || dx > b.dxe) {
-                        kc.khi[k] = KEY_OUT_HI;   // above every valid key, never NaN
-                        kc.klo[k] = 0;
-                    } else {
-                        const int dxi = dx + a.R;
-                        const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
-                        kc.khi[k] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
-                        kc.klo[k] = __builtin_amdgcn_readfirstlane((u32)K);
-                    }
+            for (int k = 0; k < NKEY; k++) {
+                const int dx = dxb + k;
+                kc.dx[k] = dx;
+                if (dx < a.lo || dx > b.dxe) {
+                    kc.khi[k] = KEY_OUT_HI;   // above every valid key, never NaN
+                    kc.klo[k] = 0;
+                } else {
+                    const int dxi = dx + a.R;
+                    const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
+                    kc.khi[k] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
+                    kc.klo[k] = __builtin_amdgcn_readfirstlane((u32)K);
                 }
-                Val L[4][4];
-                leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
-                Region g;
-                g.build(L);
-                if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
-                else if (PASS == 1) pass_large<EDGE, SH0>(g, st, kc, rx, ry, b.cx, b.cy, wlim, hlim);
-                else if (PASS == 3) {
-                    pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
-                    pass_large<EDGE, SH0>(g, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
-                } else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
+            }
+#if BIT_DEPTH == 8
+            Val L[4][4];
+            leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
+            Region g;
+            g.build(L);
+            if (PASS == 3) {
+                pass_small<EDGE, SH0>(g, st, kc, 0, b.cx + X, b.cy + Y, wlim, hlim);
+                Val t32x8[2], t8x32[2];
+                pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, 0, rx, ry, b.cx, b.cy, wlim, hlim, t32x8, t8x32);
+                sparse_scatter<EDGE, SH0>(g.s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NK; k++) upd1<EDGE, SH0>(st[0], g.s16.get(k), kc, k, b.cx + X, b.cy + Y, wlim, hlim);
             }
 #else
-            // two displacements per window read; each then composed and
-            // compared on its own (one displacement per u32 at 10 bits)
-            auto process = [&](int k0, const Val(&L)[4][4]) {
-                KeyCtx kc;
-                kc.dy = dy;
-                kc.sh = a.sh;
-                const int dx = dxb + k0, dxi = dx + a.R;
-                kc.dx[0] = dx;
-                const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
-                kc.khi[0] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
-                kc.klo[0] = __builtin_amdgcn_readfirstlane((u32)K);
-                Region g;
-                g.build(L);
-                if (PASS == 0) pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
-                else if (PASS == 1) pass_large<EDGE, SH0>(g, st, kc, rx, ry, b.cx, b.cy, wlim, hlim);
-                else if (PASS == 3) {
-                    pass_small<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
-                    pass_large<EDGE, SH0>(g, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
-                } else pass_16<EDGE, SH0>(g, st, kc, b.cx + X, b.cy + Y, wlim, hlim);
-            };
+            // the four displacements' leaves from one pass over the window
+            // rows; then each displacement's region is composed and compared
+            // on its own, and the sparse classes take all four together
+            Val L[NKEY][4][4];
+            leaves4(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
+            Quad s16, t32x8[2], t8x32[2];
 #pragma unroll
-            for (int k0 = 0; k0 < 4; k0 += 2) {
-                const bool v0 = !(dxb + k0 < a.lo || dxb + k0 > b.dxe);
-                const bool v1 = !(dxb + k0 + 1 < a.lo || dxb + k0 + 1 > b.dxe);
-                if (!v0 && !v1) continue;
-                Val La[4][4], Lb[4][4];
-                leaves2(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, k0, La, Lb);
-                if (v0) process(k0, La);
-                if (v1) process(k0 + 1, Lb);
+            for (int k = 0; k < NKEY; k++) {
+                Region g;
+                g.build(L[k]);
+                if (PASS == 3) {
+                    pass_small<EDGE, SH0>(g, st, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
+                    Val a8[2], b8[2];
+                    pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, k, rx, ry, b.cx, b.cy, wlim, hlim, a8, b8);
+                    s16.v[k] = g.s16.v;
+                    t32x8[0].v[k] = a8[0].v;
+                    t32x8[1].v[k] = a8[1].v;
+                    t8x32[0].v[k] = b8[0].v;
+                    t8x32[1].v[k] = b8[1].v;
+                } else {
+                    upd1<EDGE, SH0>(st[0], g.s16.v, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
+                }
             }
+            if (PASS == 3) sparse_scatter<EDGE, SH0>(s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
 #endif
         }
     }
@@ -705,8 +684,7 @@ __device__ __forceinline__ bool merge_waves(double *st, unsigned char *smem)
 // reference window of the tile and the current CTU are staged in LDS once;
 // then each wave covers the whole CTU (lane = 16x16 region) for every
 // WAVES-th displacement row of the tile, updating the node states of all
-// 58 classes per displacement (ME_IS_FUSED, the default) or in two passes
-// (small PUs, then large PUs) with half the live node states.
+// 58 classes per displacement.
 template <int MODE, int WAVES>
 // The register budget is 2 waves per SIMD: 2 workgroups of 4 waves, or one
 // of 8 (the 10-bit windows at R = 128 leave room for only one workgroup's
@@ -806,36 +784,16 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
         if (merge_waves<WAVES, 1>(st, smem)) PUB(S16x16, 16, 16, rx * 16, ry * 16, st[0]);
         return;
     }
-#if ME_IS_FUSED
     {
         double st[N_SMALL + N_LARGE];
         if (fast) run_pass<3, WAVES, false, true, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
         else run_pass<3, WAVES, true, false, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
         if (merge_waves<WAVES, N_SMALL + N_LARGE>(st, smem)) {
-#if BIT_DEPTH == 8
             merge_sparse(st + N_SMALL);
-#endif
             publish_small(a, kb, st, rx * 16, ry * 16);
             publish_large(a, kb, st + N_SMALL, rx, ry);
         }
     }
-#else
-    {
-        double st[N_SMALL];
-        if (fast) run_pass<0, WAVES, false, true, N_SMALL>(a, P, b, win, curl, st);
-        else run_pass<0, WAVES, true, false, N_SMALL>(a, P, b, win, curl, st);
-        publish_small(a, kb, st, rx * 16, ry * 16);
-    }
-    {
-        double st[N_LARGE];
-        if (fast) run_pass<1, WAVES, false, true, N_LARGE>(a, P, b, win, curl, st);
-        else run_pass<1, WAVES, true, false, N_LARGE>(a, P, b, win, curl, st);
-#if BIT_DEPTH == 8
-        merge_sparse(st);
-#endif
-        publish_large(a, kb, st, rx, ry);
-    }
-#endif
 }
 #undef PUB
 
