@@ -89,6 +89,7 @@ def main():
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--dx-cols", type=int, default=0)
+    ap.add_argument("--sp-groups", type=int, default=0, help="sub-pel position groups (0: 1, or the autotuner's)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -119,11 +120,17 @@ def main():
         t0 = time.perf_counter()
         pick, table = hip.autotune(work.w, work.h, work.R, work.lam, work.mask, dref, cur, work.num_refs_l1,
                                    iters=args.tune_iters)
+        # the integer-search candidates come first (sp_groups 0: the
+        # default), then the sub-pel position groups with the picked schedule
+        it = [t for t in table if t["sp_groups"] == 0]
+        sp = [t for t in table if t["sp_groups"] > 0]
         tune = {"pick": pick, "candidates": len(table), "seconds": time.perf_counter() - t0,
-                "fastest_ms": min(t["int_search_ms"] for t in table),
-                "default_ms": next((t["int_search_ms"] for t in table
-                                    if t["band_rows"] == 16 and t["waves"] == 4 and t["dx_cols"] == 0), None)}
+                "fastest_ms": min(t["int_search_ms"] for t in it),
+                "default_ms": next((t["int_search_ms"] for t in it
+                                    if t["band_rows"] == 16 and t["waves"] == 4 and t["dx_cols"] == 0), None),
+                "subpel_ms_by_groups": {str(t["sp_groups"]): t["subpel_ms"] for t in sp}}
         args.band_rows, args.waves, args.dx_cols = pick["band_rows"], pick["waves"], pick["dx_cols"]
+        args.sp_groups = pick["sp_groups"]
         print("[bench] autotune: %s (%.3f ms; default %s ms) over %d schedules in %.1fs" % (
             pick, tune["fastest_ms"], tune["default_ms"], len(table), tune["seconds"]), file=sys.stderr, flush=True)
         if args.tune_log:
@@ -131,7 +138,7 @@ def main():
                 json.dump({"workload": work.name, "pick": pick, "table": table}, f, indent=1)
     plan = hip.plan_create(work.w, work.h, work.R, work.lam, work.mask, work.num_refs,
                            band_rows=args.band_rows, waves=args.waves, dx_cols=args.dx_cols,
-                           num_refs_l1=work.num_refs_l1)
+                           num_refs_l1=work.num_refs_l1, sp_groups=args.sp_groups)
     hip.plan_set_cur(plan, cur)
     n_ctu = hip.lib.me_plan_num_ctus(plan)
     npu = hip.lib.me_plan_num_pus(plan)
@@ -227,7 +234,8 @@ def main():
     cand_per_s = cands * args.steps / elapsed
 
     # ---- stage timing with HIP events on the plan's stream -------------------
-    stage = hip.plan_profile(plan, dref, prep, iters=5)
+    hip.plan_profile(plan, dref, prep, iters=2)   # warm-up of the eager (non-graph) launches
+    stage = hip.plan_profile(plan, dref, prep, iters=10)
     mc_ms = None
     if bi:
         t0 = time.perf_counter()

@@ -191,6 +191,8 @@ typedef struct me_frame_cfg_t {
                               (bi-directional frames); the final cost of a reference then uses
                               tbl_refi_bits of its own list (tables.c:52-70) and the best reference is
                               chosen per list */
+    int sp_groups;         /* tuning: sub-pel search positions of a PU evaluated side by side by one
+                              workgroup (1, 2, 4 or 8), 0 = 1 */
 } me_frame_cfg_t;
 
 /* The 29 PU shapes, in mask-bit order: 4x8 8x4 4x16 16x4 4x32 32x4 8x8 8x16
@@ -212,17 +214,21 @@ int me_plan_set_cur(me_plan_t plan, const me_pel *cur, int stride, int on_device
  * reference r from its device luma inside the same graph (a reference that
  * was just reconstructed).  Results stay on the device until me_plan_results. */
 int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, void *stream);
-/* Autotuner of the integer-search schedule (no counterpart in the
- * reference).  Times the integer search of cfg's frame against refs, with
- * the current picture `cur` (host memory, `stride` samples per row), for
- * every candidate workgroup size (1, 2, 4, 8 waves) x displacement rows per
- * workgroup (4..32) x displacement columns per workgroup (all, 64, 32) whose
- * LDS window fits, `iters` runs each (HIP events on the plan's stream), and
- * writes the fastest into cfg->band_rows / waves / dx_cols.  The schedule
- * never changes results.  log (optional) receives up to log_cap timings. */
+/* Autotuner of the search schedule (no counterpart in the reference).
+ * Times cfg's frame against refs, with the current picture `cur` (host
+ * memory, `stride` samples per row), `iters` runs per candidate (HIP events
+ * on the plan's stream): first the integer search for every workgroup size
+ * (1, 2, 4, 8 waves) x displacement rows per workgroup (4..32) x
+ * displacement columns per workgroup (all, 64, 32) whose LDS window fits,
+ * then, with the fastest of those, the sub-pel search for every position
+ * group count (1, 2, 4, 8).  Writes the fastest into cfg->band_rows / waves
+ * / dx_cols / sp_groups.  The schedule never changes results.  log
+ * (optional) receives up to log_cap timings, one per candidate. */
 typedef struct me_tune_result_t {
     int band_rows, waves, dx_cols;
     double int_search_ms;
+    int sp_groups;
+    double subpel_ms;
 } me_tune_result_t;
 int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const me_pel *cur, int stride, int iters,
                      me_tune_result_t *log, int log_cap, int *n_log);

@@ -68,15 +68,18 @@ def uhd10_r128():
     return cur, refs, ores, obest
 
 
-# (band_rows, waves, dx_cols): 0 = the library default, which is the
-# schedule the autotuner picks at 2160p (16 rows, 4 waves, all columns).
-@pytest.mark.parametrize("sched", [(0, 0, 0), (8, 2, 64), (4, 1, 32), (32, 4, 64), (5, 4, 128), (16, 8, 0)],
-                         ids=lambda s: "b%d_w%d_c%d" % s)
+# (band_rows, waves, dx_cols, sp_groups): 0 = the library default (16 rows,
+# 4 waves, all columns, one sub-pel position group); the autotuner picks 16
+# rows of 8 waves at 2160p.
+@pytest.mark.parametrize("sched", [(0, 0, 0, 0), (8, 2, 64, 2), (4, 1, 32, 0), (32, 4, 64, 8), (5, 4, 128, 0),
+                                   (16, 8, 0, 4)],
+                         ids=lambda s: "b%d_w%d_c%d_g%d" % s)
 def test_10bit_4ref_r128(uhd10_r128, sched):
     cur, refs, ores, obest = uhd10_r128
     hip = ua.MeHip(10)
-    band, waves, cols = sched
-    res, best, _ = _run_plan(hip, cur, refs, 128, CLASS_MASK, band_rows=band, waves=waves, dx_cols=cols)
+    band, waves, cols, g = sched
+    res, best, _ = _run_plan(hip, cur, refs, 128, CLASS_MASK, band_rows=band, waves=waves, dx_cols=cols,
+                             sp_groups=g)
     _compare(res, best, ores, obest)
     # the window reaches the range's edge: some PUs end on a displacement of 128
     assert res["valid"].sum() > 100

@@ -260,15 +260,18 @@ def test_frame_plan_high_contrast(bd, nref):
 
 
 def test_plan_tuning_variants_agree():
-    """Band heights and waves per block change only the schedule."""
+    """Band heights, waves per block and sub-pel position groups change only
+    the schedule."""
     work = wl.hd_allpu().scaled(256, 200, 12)
     cur, refs = work.frames()
     hip = ua.MeHip(8)
     lam = wl.lambda_mv(32)
     base = _run_plan(hip, cur, refs, 12, lam, MASK_ALL)
-    for band, waves, cols in ((1, 1, 0), (5, 2, 8), (25, 4, 12), (3, 4, 4), (11, 8, 0), (16, 8, 8)):
-        res, best = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, band_rows=band, waves=waves, dx_cols=cols)
-        assert np.array_equal(res, base[0]) and np.array_equal(best, base[1]), (band, waves, cols)
+    for band, waves, cols, g in ((1, 1, 0, 0), (5, 2, 8, 2), (25, 4, 12, 4), (3, 4, 4, 8), (11, 8, 0, 1),
+                                 (16, 8, 8, 8)):
+        res, best = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, band_rows=band, waves=waves, dx_cols=cols,
+                              sp_groups=g)
+        assert np.array_equal(res, base[0]) and np.array_equal(best, base[1]), (band, waves, cols, g)
 
 
 def test_host_driver_percall_vs_oracle():
@@ -331,9 +334,15 @@ def test_autotune_picks_fastest_and_keeps_results():
     lam = wl.lambda_mv(32)
     dref = [hip.ref_create(r) for r in refs]
     pick, table = hip.autotune(work.w, work.h, work.R, lam, MASK_ALL, dref, cur, iters=2)
-    assert len(table) >= 6 and all(t["int_search_ms"] > 0 for t in table)
-    fastest = min(table, key=lambda t: t["int_search_ms"])
-    assert pick == {k: fastest[k] for k in ("band_rows", "waves", "dx_cols")}
+    assert all(t["int_search_ms"] > 0 and t["subpel_ms"] > 0 for t in table)
+    it = [t for t in table if t["sp_groups"] == 0]   # integer-search schedules, default groups
+    sp = [t for t in table if t["sp_groups"] > 0]    # then the sub-pel position groups
+    assert len(it) >= 6 and sorted(t["sp_groups"] for t in sp) == [1, 2, 4, 8]
+    fastest = min(it, key=lambda t: t["int_search_ms"])
+    assert {k: pick[k] for k in ("band_rows", "waves", "dx_cols")} == {k: fastest[k] for k in ("band_rows", "waves", "dx_cols")}
+    assert all({k: t[k] for k in ("band_rows", "waves", "dx_cols")} == {k: fastest[k] for k in ("band_rows", "waves", "dx_cols")}
+               for t in sp)
+    assert pick["sp_groups"] == min(sp, key=lambda t: t["subpel_ms"])["sp_groups"]
     for r in dref:
         hip.ref_destroy(r)
     res, best = _run_plan(hip, cur, refs, work.R, lam, MASK_ALL, **pick)

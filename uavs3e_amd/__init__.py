@@ -71,7 +71,7 @@ def compare_results(res, best, ores, obest):
 class FrameCfg(C.Structure):
     _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("search_range", C.c_int), ("lambda_mv", C.c_uint32),
                 ("shape_mask", C.c_uint32), ("num_refs", C.c_int), ("band_rows", C.c_int), ("waves", C.c_int),
-                ("dx_cols", C.c_int), ("num_refs_l1", C.c_int)]
+                ("dx_cols", C.c_int), ("num_refs_l1", C.c_int), ("sp_groups", C.c_int)]
 
 
 def mc_layout(mask):
@@ -91,7 +91,8 @@ def mc_layout(mask):
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("band_rows", C.c_int), ("waves", C.c_int), ("dx_cols", C.c_int), ("int_search_ms", C.c_double)]
+    _fields_ = [("band_rows", C.c_int), ("waves", C.c_int), ("dx_cols", C.c_int), ("int_search_ms", C.c_double),
+                ("sp_groups", C.c_int), ("subpel_ms", C.c_double)]
 
 
 class TzCfg(C.Structure):
@@ -290,8 +291,9 @@ class MeHip:
         self.lib.me_ref_destroy(ref)
 
     # -- plans ---------------------------------------------------------------
-    def plan_create(self, w, h, R, lambda_mv, mask, num_refs, band_rows=0, waves=0, dx_cols=0, num_refs_l1=0):
-        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols, num_refs_l1)
+    def plan_create(self, w, h, R, lambda_mv, mask, num_refs, band_rows=0, waves=0, dx_cols=0, num_refs_l1=0,
+                    sp_groups=0):
+        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols, num_refs_l1, sp_groups)
         p = C.c_void_p()
         self.check(self.lib.me_plan_create(C.byref(p), C.byref(cfg)), "me_plan_create")
         self._nl = getattr(self, "_nl", {})
@@ -299,18 +301,19 @@ class MeHip:
         return p
 
     def autotune(self, w, h, R, lambda_mv, mask, refs, cur, num_refs_l1=0, iters=3):
-        """me_plan_autotune: ({band_rows, waves, dx_cols} of the fastest
-        integer-search schedule, [timings of every candidate])."""
-        cfg = FrameCfg(w, h, R, lambda_mv, mask, len(refs), 0, 0, 0, num_refs_l1)
+        """me_plan_autotune: ({band_rows, waves, dx_cols, sp_groups} of the
+        fastest schedule, [timings of every candidate])."""
+        cfg = FrameCfg(w, h, R, lambda_mv, mask, len(refs), 0, 0, 0, num_refs_l1, 0)
         arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
         cur = np.ascontiguousarray(cur, self.dt)
         log = (TuneResult * 64)()
         n = C.c_int()
         self.check(self.lib.me_plan_autotune(C.byref(cfg), arr, _p(cur), cur.shape[1], iters, log, 64, C.byref(n)),
                    "me_plan_autotune")
-        table = [{"band_rows": t.band_rows, "waves": t.waves, "dx_cols": t.dx_cols, "int_search_ms": t.int_search_ms}
-                 for t in log[:n.value]]
-        return {"band_rows": cfg.band_rows, "waves": cfg.waves, "dx_cols": cfg.dx_cols}, table
+        table = [{"band_rows": t.band_rows, "waves": t.waves, "dx_cols": t.dx_cols, "int_search_ms": t.int_search_ms,
+                  "sp_groups": t.sp_groups, "subpel_ms": t.subpel_ms} for t in log[:n.value]]
+        return {"band_rows": cfg.band_rows, "waves": cfg.waves, "dx_cols": cfg.dx_cols,
+                "sp_groups": cfg.sp_groups}, table
 
     def plan_set_cur(self, plan, cur):
         cur = np.ascontiguousarray(cur, self.dt)

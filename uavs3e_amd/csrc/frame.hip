@@ -341,7 +341,7 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
         return -1;
     }
     if (me_search_init(&p->s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, cfg->num_refs_l1, T,
-                       cfg->band_rows, cfg->waves, cfg->dx_cols, 1)) {
+                       cfg->band_rows, cfg->waves, cfg->dx_cols, cfg->sp_groups)) {
         free(p);
         return -1;
     }
@@ -579,18 +579,37 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
                 if (!rc) rc = me_plan_profile(p, refs, 0, iters, ms);
                 me_plan_destroy(p);
                 if (rc) return -1;
-                if (log && n < log_cap) log[n] = {b, w, dc, ms[2]};
+                if (log && n < log_cap) log[n] = {b, w, dc, ms[2], c.sp_groups, ms[3]};
                 n++;
                 if (ms[2] < best) { best = ms[2]; pick = c; }
             }
-    if (n_log) *n_log = n < log_cap ? n : log_cap;
     if (best >= 1e30) {
+        if (n_log) *n_log = n < log_cap ? n : log_cap;
         me_set_error("me_plan_autotune: no schedule fits", hipErrorInvalidValue, __FILE__, __LINE__);
         return -1;
     }
+    // sub-pel position groups, with the integer schedule just picked
+    c = pick;
+    double best_sp = 1e30;
+    for (int g : {1, 2, 4, 8}) {
+        c.sp_groups = g;
+        me_plan_t p;
+        if (me_plan_create(&p, &c)) continue;
+        double ms[4] = {0, 0, 0, 0};
+        int rc = me_plan_set_cur(p, cur, stride, 0, nullptr);
+        if (!rc) rc = me_plan_profile(p, refs, 0, 1, ms);          // warm-up
+        if (!rc) rc = me_plan_profile(p, refs, 0, iters, ms);
+        me_plan_destroy(p);
+        if (rc) return -1;
+        if (log && n < log_cap) log[n] = {c.band_rows, c.waves, c.dx_cols, ms[2], g, ms[3]};
+        n++;
+        if (ms[3] < best_sp) { best_sp = ms[3]; pick.sp_groups = g; }
+    }
+    if (n_log) *n_log = n < log_cap ? n : log_cap;
     cfg->band_rows = pick.band_rows;
     cfg->waves = pick.waves;
     cfg->dx_cols = pick.dx_cols;
+    cfg->sp_groups = pick.sp_groups;
     return 0;
 }
 
@@ -949,7 +968,7 @@ ME_EXPORT int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *c
     SubShape shp[ME_NSHAPES];
     int n_shp = 0;
     if (search_layout(&s, cfg->search_range, cfg->lambda_mv, cfg->shape_mask, cfg->num_refs, cfg->num_refs_l1, T,
-                      cfg->band_rows, cfg->waves, cfg->dx_cols, 1, shp, &n_shp))
+                      cfg->band_rows, cfg->waves, cfg->dx_cols, cfg->sp_groups, shp, &n_shp))
         return -1;
     for (int r = 0; r < s.nref; r++)
         if (!refs[r] || refs[r]->w != cfg->pic_w || refs[r]->h != cfg->pic_h) {
