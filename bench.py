@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--dx-cols", type=int, default=0)
     ap.add_argument("--sp-groups", type=int, default=0, help="sub-pel position groups (0: 1, or the autotuner's)")
+    ap.add_argument("--int-grid", type=int, default=0,
+                    help="integer-search grid: 0 persistent where the frame fills the GPU, 1 one workgroup per band")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -127,10 +129,12 @@ def main():
         tune = {"pick": pick, "candidates": len(table), "seconds": time.perf_counter() - t0,
                 "fastest_ms": min(t["int_search_ms"] for t in it),
                 "default_ms": next((t["int_search_ms"] for t in it
-                                    if t["band_rows"] == 16 and t["waves"] == 4 and t["dx_cols"] == 0), None),
+                                    if t["band_rows"] == 16 and t["waves"] == 4 and t["dx_cols"] == 0
+                                    and t["int_grid"] == 0), None),
+                "per_band_grid_ms": next((t["int_search_ms"] for t in it if t["int_grid"] == 1), None),
                 "subpel_ms_by_groups": {str(t["sp_groups"]): t["subpel_ms"] for t in sp}}
         args.band_rows, args.waves, args.dx_cols = pick["band_rows"], pick["waves"], pick["dx_cols"]
-        args.sp_groups = pick["sp_groups"]
+        args.sp_groups, args.int_grid = pick["sp_groups"], pick["int_grid"]
         print("[bench] autotune: %s (%.3f ms; default %s ms) over %d schedules in %.1fs" % (
             pick, tune["fastest_ms"], tune["default_ms"], len(table), tune["seconds"]), file=sys.stderr, flush=True)
         if args.tune_log:
@@ -138,7 +142,7 @@ def main():
                 json.dump({"workload": work.name, "pick": pick, "table": table}, f, indent=1)
     plan = hip.plan_create(work.w, work.h, work.R, work.lam, work.mask, work.num_refs,
                            band_rows=args.band_rows, waves=args.waves, dx_cols=args.dx_cols,
-                           num_refs_l1=work.num_refs_l1, sp_groups=args.sp_groups)
+                           num_refs_l1=work.num_refs_l1, sp_groups=args.sp_groups, int_grid=args.int_grid)
     hip.plan_set_cur(plan, cur)
     n_ctu = hip.lib.me_plan_num_ctus(plan)
     npu = hip.lib.me_plan_num_pus(plan)
@@ -650,7 +654,8 @@ def main():
                            (" + MC of every PU (luma, 4:2:0 chroma, bi-average)" if bi else ""),
                    "lists": "2 L0 + 2 L1" if bi else "L0"},
         "autotune": tune,
-        "schedule": {"band_rows": args.band_rows, "waves": args.waves, "dx_cols": args.dx_cols},
+        "schedule": {"band_rows": args.band_rows, "waves": args.waves, "dx_cols": args.dx_cols,
+                     "int_grid": args.int_grid, "sp_groups": args.sp_groups},
         "candidate_evals_per_s": cand_per_s,
         "ctu_latency_ms": ({"p50": serving["latency_ms"]["p50"], "p99": serving["latency_ms"]["p99"],
                             "note": "serving leg: per-CTU request arrival to result on the host"}

@@ -193,6 +193,11 @@ typedef struct me_frame_cfg_t {
                               chosen per list */
     int sp_groups;         /* tuning: sub-pel search positions of a PU evaluated side by side by one
                               workgroup (1, 2, 4 or 8), 0 = 1 */
+    int int_grid;          /* tuning: integer-search grid.  0: persistent workgroups (as many as the
+                              GPU holds at once, each sweeping a contiguous run of displacement bands
+                              and writing each key once) where the frame has the work to fill the GPU
+                              twice over; 1: one workgroup per band, keys merged with atomics; n >= 2:
+                              n persistent workgroups */
 } me_frame_cfg_t;
 
 /* The 29 PU shapes, in mask-bit order: 4x8 8x4 4x16 16x4 4x32 32x4 8x8 8x16
@@ -219,16 +224,23 @@ int me_plan_run(me_plan_t plan, me_ref_t const *refs, uint32_t prepare_mask, voi
  * memory, `stride` samples per row), `iters` runs per candidate (HIP events
  * on the plan's stream): first the integer search for every workgroup size
  * (1, 2, 4, 8 waves) x displacement rows per workgroup (4..32) x
- * displacement columns per workgroup (all, 64, 32) whose LDS window fits,
- * then, with the fastest of those, the sub-pel search for every position
- * group count (1, 2, 4, 8).  Writes the fastest into cfg->band_rows / waves
- * / dx_cols / sp_groups.  The schedule never changes results.  log
- * (optional) receives up to log_cap timings, one per candidate. */
+ * displacement columns per workgroup (all, 64, 32) whose LDS window fits
+ * (persistent grid), then the fastest of those with one workgroup per band
+ * (int_grid 1), then, with the faster, the sub-pel search for every
+ * position group count (1, 2, 4, 8).  Writes the fastest into
+ * cfg->band_rows / waves / dx_cols / int_grid / sp_groups.  The schedule
+ * never changes results.  log (optional) receives up to log_cap timings, one
+ * per candidate. */
+/* The integer search's grid of a plan (no counterpart in the reference):
+ * *groups workgroups and *slots key slots per displacement-column tile (0:
+ * one workgroup per band, keys merged with atomics). */
+int me_plan_int_grid(me_plan_t plan, int *groups, int *slots);
 typedef struct me_tune_result_t {
     int band_rows, waves, dx_cols;
     double int_search_ms;
     int sp_groups;
     double subpel_ms;
+    int int_grid;
 } me_tune_result_t;
 int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const me_pel *cur, int stride, int iters,
                      me_tune_result_t *log, int log_cap, int *n_log);

@@ -68,18 +68,23 @@ def uhd10_r128():
     return cur, refs, ores, obest
 
 
-# (band_rows, waves, dx_cols, sp_groups): 0 = the library default (16 rows,
-# 4 waves, all columns, one sub-pel position group); the autotuner picks 16
-# rows of 8 waves at 2160p.
-@pytest.mark.parametrize("sched", [(0, 0, 0, 0), (8, 2, 64, 2), (4, 1, 32, 0), (32, 4, 64, 8), (5, 4, 128, 0),
-                                   (16, 8, 0, 4)],
-                         ids=lambda s: "b%d_w%d_c%d_g%d" % s)
+# (band_rows, waves, dx_cols, sp_groups, int_grid): 0 = the library default
+# (16 rows, 4 waves, all columns, one sub-pel position group, persistent grid
+# where the frame fills the GPU); the autotuner picks 32 rows of 8 waves over
+# 64-column tiles on the persistent grid at 2160p.  Grids of a few persistent
+# workgroups split the segments of the top CTU row, whose first band tests
+# the window (R = 128 reaches past -127) and whose others do not, so one
+# workgroup runs both kinds and merges their keys.
+@pytest.mark.parametrize("sched", [(0, 0, 0, 0, 1), (8, 2, 64, 2, 1), (4, 1, 32, 0, 1), (32, 4, 64, 8, 1),
+                                   (5, 4, 128, 0, 1), (16, 8, 0, 4, 1), (16, 8, 0, 0, 5), (32, 8, 64, 0, 7),
+                                   (8, 4, 0, 1, 3), (5, 2, 128, 0, 11)],
+                         ids=lambda s: "b%d_w%d_c%d_g%d_grid%d" % s)
 def test_10bit_4ref_r128(uhd10_r128, sched):
     cur, refs, ores, obest = uhd10_r128
     hip = ua.MeHip(10)
-    band, waves, cols, g = sched
+    band, waves, cols, g, grid = sched
     res, best, _ = _run_plan(hip, cur, refs, 128, CLASS_MASK, band_rows=band, waves=waves, dx_cols=cols,
-                             sp_groups=g)
+                             sp_groups=g, int_grid=grid)
     _compare(res, best, ores, obest)
     # the window reaches the range's edge: some PUs end on a displacement of 128
     assert res["valid"].sum() > 100

@@ -103,10 +103,12 @@ def test_planes_device_and_table(libs, size):
     assert hip.error_count() == 0
 
 
-def _run_plan(hip, cur, refs, R, lam, mask, **kw):
+def _run_plan(hip, cur, refs, R, lam, mask, grid_out=None, **kw):
     h, w = cur.shape
     dref = [hip.ref_create(r) for r in refs]
     plan = hip.plan_create(w, h, R, lam, mask, len(refs), **kw)
+    if grid_out is not None:
+        grid_out.append(hip.plan_int_grid(plan))
     hip.plan_set_cur(plan, cur)
     hip.plan_run(plan, dref)
     res, best = hip.plan_results(plan, len(refs))
@@ -260,18 +262,27 @@ def test_frame_plan_high_contrast(bd, nref):
 
 
 def test_plan_tuning_variants_agree():
-    """Band heights, waves per block and sub-pel position groups change only
-    the schedule."""
+    """Band heights, waves per block, sub-pel position groups and the
+    integer-search grid change only the schedule.  Persistent grids of a few
+    workgroups split segments (CTU, reference, column tile) into 1-4 pieces
+    at every band position, each piece storing its own key slot."""
     work = wl.hd_allpu().scaled(256, 200, 12)
     cur, refs = work.frames()
     hip = ua.MeHip(8)
     lam = wl.lambda_mv(32)
-    base = _run_plan(hip, cur, refs, 12, lam, MASK_ALL)
-    for band, waves, cols, g in ((1, 1, 0, 0), (5, 2, 8, 2), (25, 4, 12, 4), (3, 4, 4, 8), (11, 8, 0, 1),
-                                 (16, 8, 8, 8)):
-        res, best = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, band_rows=band, waves=waves, dx_cols=cols,
-                              sp_groups=g)
-        assert np.array_equal(res, base[0]) and np.array_equal(best, base[1]), (band, waves, cols, g)
+    base = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, int_grid=1)
+    for band, waves, cols, g, grid in ((1, 1, 0, 0, 1), (5, 2, 8, 2, 1), (25, 4, 12, 4, 0), (3, 4, 4, 8, 1),
+                                       (11, 8, 0, 1, 0), (16, 8, 8, 8, 1), (1, 4, 0, 0, 7), (5, 2, 8, 2, 5),
+                                       (3, 8, 4, 1, 13), (2, 4, 0, 4, 2), (4, 1, 12, 0, 9), (25, 8, 0, 0, 3)):
+        got = []
+        res, best = _run_plan(hip, cur, refs, 12, lam, MASK_ALL, grid_out=got, band_rows=band, waves=waves,
+                              dx_cols=cols, sp_groups=g, int_grid=grid)
+        groups, slots = got[0]
+        if grid >= 2:   # persistent on `grid` workgroups
+            assert groups == grid and 1 <= slots <= 4, (grid, groups, slots)
+        elif grid == 1:
+            assert slots == 0
+        assert np.array_equal(res, base[0]) and np.array_equal(best, base[1]), (band, waves, cols, g, grid)
 
 
 def test_host_driver_percall_vs_oracle():

@@ -71,7 +71,7 @@ def compare_results(res, best, ores, obest):
 class FrameCfg(C.Structure):
     _fields_ = [("pic_w", C.c_int), ("pic_h", C.c_int), ("search_range", C.c_int), ("lambda_mv", C.c_uint32),
                 ("shape_mask", C.c_uint32), ("num_refs", C.c_int), ("band_rows", C.c_int), ("waves", C.c_int),
-                ("dx_cols", C.c_int), ("num_refs_l1", C.c_int), ("sp_groups", C.c_int)]
+                ("dx_cols", C.c_int), ("num_refs_l1", C.c_int), ("sp_groups", C.c_int), ("int_grid", C.c_int)]
 
 
 def mc_layout(mask):
@@ -92,7 +92,7 @@ def mc_layout(mask):
 
 class TuneResult(C.Structure):
     _fields_ = [("band_rows", C.c_int), ("waves", C.c_int), ("dx_cols", C.c_int), ("int_search_ms", C.c_double),
-                ("sp_groups", C.c_int), ("subpel_ms", C.c_double)]
+                ("sp_groups", C.c_int), ("subpel_ms", C.c_double), ("int_grid", C.c_int)]
 
 
 class TzCfg(C.Structure):
@@ -292,8 +292,9 @@ class MeHip:
 
     # -- plans ---------------------------------------------------------------
     def plan_create(self, w, h, R, lambda_mv, mask, num_refs, band_rows=0, waves=0, dx_cols=0, num_refs_l1=0,
-                    sp_groups=0):
-        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols, num_refs_l1, sp_groups)
+                    sp_groups=0, int_grid=0):
+        cfg = FrameCfg(w, h, R, lambda_mv, mask, num_refs, band_rows, waves, dx_cols, num_refs_l1, sp_groups,
+                       int_grid)
         p = C.c_void_p()
         self.check(self.lib.me_plan_create(C.byref(p), C.byref(cfg)), "me_plan_create")
         self._nl = getattr(self, "_nl", {})
@@ -301,8 +302,8 @@ class MeHip:
         return p
 
     def autotune(self, w, h, R, lambda_mv, mask, refs, cur, num_refs_l1=0, iters=3):
-        """me_plan_autotune: ({band_rows, waves, dx_cols, sp_groups} of the
-        fastest schedule, [timings of every candidate])."""
+        """me_plan_autotune: ({band_rows, waves, dx_cols, int_grid, sp_groups}
+        of the fastest schedule, [timings of every candidate])."""
         cfg = FrameCfg(w, h, R, lambda_mv, mask, len(refs), 0, 0, 0, num_refs_l1, 0)
         arr = (C.c_void_p * len(refs))(*[r.value for r in refs])
         cur = np.ascontiguousarray(cur, self.dt)
@@ -310,9 +311,10 @@ class MeHip:
         n = C.c_int()
         self.check(self.lib.me_plan_autotune(C.byref(cfg), arr, _p(cur), cur.shape[1], iters, log, 64, C.byref(n)),
                    "me_plan_autotune")
-        table = [{"band_rows": t.band_rows, "waves": t.waves, "dx_cols": t.dx_cols, "int_search_ms": t.int_search_ms,
-                  "sp_groups": t.sp_groups, "subpel_ms": t.subpel_ms} for t in log[:n.value]]
-        return {"band_rows": cfg.band_rows, "waves": cfg.waves, "dx_cols": cfg.dx_cols,
+        table = [{"band_rows": t.band_rows, "waves": t.waves, "dx_cols": t.dx_cols, "int_grid": t.int_grid,
+                  "int_search_ms": t.int_search_ms, "sp_groups": t.sp_groups, "subpel_ms": t.subpel_ms}
+                 for t in log[:n.value]]
+        return {"band_rows": cfg.band_rows, "waves": cfg.waves, "dx_cols": cfg.dx_cols, "int_grid": cfg.int_grid,
                 "sp_groups": cfg.sp_groups}, table
 
     def plan_set_cur(self, plan, cur):
@@ -329,6 +331,13 @@ class MeHip:
         ms = (C.c_double * 4)()
         self.check(self.lib.me_plan_profile(plan, arr, prepare_mask, iters, ms), "me_plan_profile")
         return list(ms)
+
+    def plan_int_grid(self, plan):
+        """(workgroups, key slots per column tile) of the plan's integer
+        search; 0 slots: one workgroup per band with atomic keys."""
+        g, k = C.c_int(), C.c_int()
+        self.check(self.lib.me_plan_int_grid(plan, C.byref(g), C.byref(k)), "me_plan_int_grid")
+        return g.value, k.value
 
     def plan_results(self, plan, num_refs):
         """(res [ctu][pu][ref], best [ctu][pu] or [ctu][pu][list] for two lists)."""

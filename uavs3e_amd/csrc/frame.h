@@ -89,28 +89,82 @@ __device__ __forceinline__ const PicDesc *job_decode(const JobSet &js, int job, 
 #define KEY_OUT_HI 0x60000000u
 #define KEY_INIT 0x7FF0000000000000ull
 
+// How the integer search's work is spread over workgroups and where each
+// workgroup leaves its keys.  A unit is (job, reference, column tile bx,
+// band of band_rows displacement rows), numbered band fastest:
+//   u = ((job * nref + ref) * nbx + bx) * nbands + band.
+// Workgroup g of `groups` takes the contiguous units
+// [units * g / groups, units * (g + 1) / groups).  The units of one
+// (job, reference, bx) -- a segment -- are therefore split into pieces over
+// consecutive workgroups, and a workgroup keeps a segment's node states in
+// registers across its bands.
+//   nslots == 0: every unit is a workgroup of its own (groups == units) and
+//     publishes with a 64-bit atomicMin into one key array (filled with ones
+//     first).
+//   nslots > 0 (persistent): piece p of a segment stores its keys, with
+//     plain stores, into slot bx * nslots + p, each slot a [job][npu][nref]
+//     array of slot_elems keys; the piece count of a segment follows from
+//     the partition (key_pieces), so a reader takes the minimum over exactly
+//     the slots written and nothing needs filling.
+// A segment with no candidate in a piece stores KEY_INIT there; a key
+// >= KEY_INIT after the merge means "no candidate".
+struct KeyLayout {
+    u32 units, groups;
+    int nbands, nbx, nslots;
+    size_t slot_elems;
+};
+
+// The workgroup whose range holds unit u.
+__host__ __device__ __forceinline__ u32 key_owner(const KeyLayout &k, u64 u)
+{
+    return (u32)(((u + 1) * k.groups - 1) / k.units);
+}
+
+// Pieces of segment x (x = (job * nref + ref) * nbx + bx).
+__host__ __device__ __forceinline__ int key_pieces(const KeyLayout &k, u64 x)
+{
+    const u64 u0 = x * (u64)k.nbands;
+    return (int)(key_owner(k, u0 + k.nbands - 1) - key_owner(k, u0)) + 1;
+}
+
+// The merged integer-search key of (job, PU, reference); idx is its index in
+// a [job][npu][nref] array.
+__device__ __forceinline__ u64 key_merged(const KeyLayout &k, const u64 *keys, size_t idx, int job, int ref, int nref)
+{
+    if (k.nslots == 0) return keys[idx];
+    u64 m = ~0ull;
+    for (int bx = 0; bx < k.nbx; bx++) {
+        const int np = key_pieces(k, ((u64)job * nref + ref) * k.nbx + bx);
+        for (int p = 0; p < np; p++) {
+            const u64 v = keys[(size_t)(bx * k.nslots + p) * k.slot_elems + idx];
+            m = v < m ? v : m;
+        }
+    }
+    return m;
+}
+
 struct IntArgs {
     JobSet js;
-    int ref0;                // reference of blockIdx.z == 0
     int R;                   // window half size
     int lo, hi;              // uniform displacement clip: [max(-R,-T+4), min(R,T-4)]
     int IB, sh;              // index bits; sad shift into the high word
     const u32 *rate;         // [2R+1]: bits(4*(i-R)) * lambda
-    u64 *keys;               // [job][npu][nref]
+    u64 *keys;               // see KeyLayout
+    KeyLayout kl;
     int npu, nref;
-    int band_rows, wcols;    // displacement rows per block; LDS window row length
-    int dx_cols, nbx;        // displacement columns per block; blocks along x
+    int band_rows, wcols;    // displacement rows per band; LDS window row length
+    int dx_cols;             // displacement columns per column tile
     int wpitch;              // LDS row pitch of the window (wcols + skew room)
     int base[ME_NSHAPES];    // first PU index of each shape inside a CTU (-1: not selected)
 };
 
 // Launches k_int_search<mode, waves> (waves 1, 2, 4 or 8; mode 1 = 16x16 PUs
-// only) on grid (bands * nbx, jobs, refs) with `lds` bytes of LDS.
-int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
-int me_int_search_launch_w1(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
-int me_int_search_launch_w2(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
-int me_int_search_launch_w4(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
-int me_int_search_launch_w8(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st);
+// only) on a.kl.groups workgroups with `lds` bytes of LDS.
+int me_int_search_launch(int mode, int waves, const IntArgs &a, size_t lds, hipStream_t st);
+int me_int_search_launch_w1(int mode, const IntArgs &a, size_t lds, hipStream_t st);
+int me_int_search_launch_w2(int mode, const IntArgs &a, size_t lds, hipStream_t st);
+int me_int_search_launch_w4(int mode, const IntArgs &a, size_t lds, hipStream_t st);
+int me_int_search_launch_w8(int mode, const IntArgs &a, size_t lds, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Sub-pel refinement (me_sub_pel_search, me.c:366-500, cost_type 0, bi 0)
@@ -134,7 +188,8 @@ struct SubArgs {
     int R, IB;
     u32 lambda;
     const u8 *tab;              // MV-bits table, centred (tab[0] is mv 0)
-    const u64 *keys;
+    const u64 *keys;            // integer-search keys (KeyLayout kl)
+    KeyLayout kl;
     me_pu_result_t *res;
     int npu, nref;
     const SubShape *shp;        // device table of the searched shapes
@@ -185,17 +240,27 @@ void me_search_free(MeSearch *s);
 // kernels' code and creates the stream's device queue (first launches on a
 // new stream otherwise pay for both), then waits for st.
 int me_search_warmup(const MeSearch *s, hipStream_t st);
+// The integer search's partition and key slots for n_jobs jobs.  grid 0:
+// with enough units to fill the device twice over, as many workgroups as the
+// device holds at once (persistent); grid n >= 2: n persistent workgroups
+// when that leaves some with several units; otherwise, and for grid 1, one
+// workgroup per unit (atomic keys).  A persistent partition whose segments
+// would take more than 4 pieces also falls back to atomic keys.  Returns the
+// keys the buffer must hold.
+size_t me_search_keys(const MeSearch *s, int n_jobs, int grid, KeyLayout *kl);
 // Enqueues key initialisation, integer search, sub-pel refinement and the
-// reference choice for the jobs of js.  keys/res are [job][npu][nref],
-// best is [job][npu][nl].  ev, if given, receives 4 events recorded before the
-// key fill, before the integer search, before sub-pel and at the end.
-// The same in three parts, so that the integer search of references [r0, r1)
-// can run on its own stream: key fill, integer search, sub-pel + choice.
-int me_search_fill(const MeSearch *s, const JobSet &js, u64 *keys, hipStream_t st);
-int me_search_int(const MeSearch *s, const JobSet &js, u64 *keys, int r0, int r1, hipStream_t st);
-int me_search_finish(const MeSearch *s, const JobSet &js, const u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st);
-int me_search_enqueue(const MeSearch *s, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
-                      hipEvent_t *ev);
+// reference choice for the jobs of js.  keys hold kl (me_search_keys for
+// js.n_jobs); res is [job][npu][nref], best [job][npu][nl].  ev, if given,
+// receives 4 events recorded before the key fill, before the integer search,
+// before sub-pel and at the end.
+// The same in three parts: key fill (atomic layouts only), integer search,
+// sub-pel + choice.
+int me_search_fill(const MeSearch *s, const JobSet &js, const KeyLayout &kl, u64 *keys, hipStream_t st);
+int me_search_int(const MeSearch *s, const JobSet &js, const KeyLayout &kl, u64 *keys, hipStream_t st);
+int me_search_finish(const MeSearch *s, const JobSet &js, const KeyLayout &kl, const u64 *keys, me_pu_result_t *res,
+                     s8 *best, hipStream_t st);
+int me_search_enqueue(const MeSearch *s, const JobSet &js, const KeyLayout &kl, u64 *keys, me_pu_result_t *res,
+                      s8 *best, hipStream_t st, hipEvent_t *ev);
 // The MV-bits table of inter_search_create (inter.c:2029-2058) in closed
 // form: tab[off + v] for |v| < off, 0 at +-off.
 void me_mvbits_fill(u8 *base, int off);

@@ -47,13 +47,13 @@ __global__ void k_select(const me_pu_result_t *res, s8 *best, int n, int nref, i
     }
 }
 
-int me_int_search_launch(int mode, int waves, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st)
+int me_int_search_launch(int mode, int waves, const IntArgs &a, size_t lds, hipStream_t st)
 {
     switch (waves) {
-    case 1: return me_int_search_launch_w1(mode, a, grid, lds, st);
-    case 2: return me_int_search_launch_w2(mode, a, grid, lds, st);
-    case 8: return me_int_search_launch_w8(mode, a, grid, lds, st);
-    default: return me_int_search_launch_w4(mode, a, grid, lds, st);
+    case 1: return me_int_search_launch_w1(mode, a, lds, st);
+    case 2: return me_int_search_launch_w2(mode, a, lds, st);
+    case 8: return me_int_search_launch_w8(mode, a, lds, st);
+    default: return me_int_search_launch_w4(mode, a, lds, st);
     }
 }
 
@@ -237,28 +237,82 @@ int me_search_warmup(const MeSearch *p, hipStream_t st)
     return 0;
 }
 
-int me_search_fill(const MeSearch *p, const JobSet &js, u64 *keys, hipStream_t st)
+// Workgroups of the integer search the current device holds at once: the
+// kernels are built for 2 waves per SIMD (4 SIMDs per CU), and a CU's LDS
+// holds floor(LDS per CU / lds) workgroups.
+static int resident_groups(const MeSearch *p)
 {
-    const size_t nkeys = (size_t)js.n_jobs * p->npu * p->nref;
+    int dev = 0, cus = 0, lds_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        lds_cu <= 0)
+        lds_cu = 160 * 1024;
+    int per_cu = 8 / p->waves;
+    const int by_lds = (int)((size_t)lds_cu / p->lds);
+    if (by_lds < per_cu) per_cu = by_lds;
+    if (per_cu < 1) per_cu = 1;
+    return cus * per_cu;
+}
+
+size_t me_search_keys(const MeSearch *p, int n_jobs, int grid, KeyLayout *kl)
+{
+    memset(kl, 0, sizeof(*kl));
+    kl->nbands = p->nbands;
+    kl->nbx = p->nbx;
+    kl->slot_elems = (size_t)n_jobs * p->npu * p->nref;
+    const u64 units = (u64)n_jobs * p->nref * p->nbx * p->nbands;
+    kl->units = (u32)units;
+    kl->groups = (u32)units;
+    // grid 0: as many workgroups as the device holds, when there are units
+    // to fill it twice over; n >= 2: n workgroups whenever that leaves some
+    // with several units; 1: one workgroup per unit
+    const int cap = grid == 0 ? resident_groups(p) : grid >= 2 ? grid : 0;
+    const bool enough = grid == 0 ? units >= 2 * (u64)cap : units > (u64)cap;
+    if (cap > 0 && enough && units < 0x7FFFFFFFull) {
+        KeyLayout t = *kl;
+        t.groups = (u32)cap;
+        int most = 1;
+        const u64 nseg = (u64)n_jobs * p->nref * p->nbx;
+        for (u64 x = 0; x < nseg; x++) {
+            const int np = key_pieces(t, x);
+            if (np > most) most = np;
+        }
+        // a persistent launch writes most * nbx slots; beyond 4 pieces the
+        // atomic layout moves fewer bytes
+        if (most <= 4) {
+            t.nslots = most;
+            *kl = t;
+            return (size_t)p->nbx * most * kl->slot_elems;
+        }
+    }
+    return kl->slot_elems;
+}
+
+int me_search_fill(const MeSearch *p, const JobSet &js, const KeyLayout &kl, u64 *keys, hipStream_t st)
+{
+    (void)p;
+    if (kl.nslots) return 0;   // persistent: every slot read is written by its piece
+    const size_t nkeys = kl.slot_elems;
     hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, st, keys, nkeys);
     ME_CHECK(hipGetLastError());
     return 0;
 }
 
-int me_search_int(const MeSearch *p, const JobSet &js, u64 *keys, int r0, int r1, hipStream_t st)
+int me_search_int(const MeSearch *p, const JobSet &js, const KeyLayout &kl, u64 *keys, hipStream_t st)
 {
     IntArgs a;
     memset(&a, 0, sizeof(a));
     a.js = js;
-    a.ref0 = r0;
     a.R = p->R; a.lo = p->lo; a.hi = p->hi; a.IB = p->IB; a.sh = p->sh;
-    a.rate = p->d_rate; a.keys = keys; a.npu = p->npu; a.nref = p->nref;
-    a.band_rows = p->band_rows; a.wcols = p->wcols; a.dx_cols = p->dx_cols; a.nbx = p->nbx; a.wpitch = p->wpitch;
+    a.rate = p->d_rate; a.keys = keys; a.kl = kl; a.npu = p->npu; a.nref = p->nref;
+    a.band_rows = p->band_rows; a.wcols = p->wcols; a.dx_cols = p->dx_cols; a.wpitch = p->wpitch;
     memcpy(a.base, p->base, sizeof(a.base));
-    return me_int_search_launch(p->mode, p->waves, a, dim3(p->nbands * p->nbx, js.n_jobs, r1 - r0), p->lds, st);
+    return me_int_search_launch(p->mode, p->waves, a, p->lds, st);
 }
 
-int me_search_finish(const MeSearch *p, const JobSet &js, const u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st)
+int me_search_finish(const MeSearch *p, const JobSet &js, const KeyLayout &kl, const u64 *keys, me_pu_result_t *res,
+                     s8 *best, hipStream_t st)
 {
     SubArgs b;
     memset(&b, 0, sizeof(b));
@@ -266,7 +320,7 @@ int me_search_finish(const MeSearch *p, const JobSet &js, const u64 *keys, me_pu
     b.n0 = p->nref - p->nref_l1;
     b.R = p->R; b.IB = p->IB; b.lambda = p->lambda;
     b.tab = p->d_tab_base + p->tab_off;
-    b.keys = keys; b.res = res;
+    b.keys = keys; b.kl = kl; b.res = res;
     b.npu = p->npu; b.nref = p->nref;
     b.shp = p->d_shp;
     b.blk_shape = p->d_blk_shape;
@@ -278,16 +332,16 @@ int me_search_finish(const MeSearch *p, const JobSet &js, const u64 *keys, me_pu
     return 0;
 }
 
-int me_search_enqueue(const MeSearch *p, const JobSet &js, u64 *keys, me_pu_result_t *res, s8 *best, hipStream_t st,
-                      hipEvent_t *ev)
+int me_search_enqueue(const MeSearch *p, const JobSet &js, const KeyLayout &kl, u64 *keys, me_pu_result_t *res,
+                      s8 *best, hipStream_t st, hipEvent_t *ev)
 {
     if (js.n_jobs <= 0) return 0;
     if (ev) ME_CHECK(hipEventRecord(ev[0], st));
-    if (me_search_fill(p, js, keys, st)) return -1;
+    if (me_search_fill(p, js, kl, keys, st)) return -1;
     if (ev) ME_CHECK(hipEventRecord(ev[1], st));
-    if (me_search_int(p, js, keys, 0, p->nref, st)) return -1;
+    if (me_search_int(p, js, kl, keys, st)) return -1;
     if (ev) ME_CHECK(hipEventRecord(ev[2], st));
-    if (me_search_finish(p, js, keys, res, best, st)) return -1;
+    if (me_search_finish(p, js, kl, keys, res, best, st)) return -1;
     if (ev) ME_CHECK(hipEventRecord(ev[3], st));
     return 0;
 }
@@ -306,6 +360,7 @@ struct me_plan_s {
     pel *d_pred;          // plan-owned predictions of me_plan_mc (y, then u, then v), or null
     s16 *d_mvp;           // me_plan_run_tz inputs: MVPs, then previous integer MVs
     u64 *d_keys;
+    KeyLayout kl;         // the integer search's partition and key slots for the whole frame
     me_pu_result_t *d_res;
     s8 *d_best;
     u32 *d_loka;          // lookahead unit costs (allocated on first use)
@@ -346,11 +401,12 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
         return -1;
     }
     const size_t nkeys = (size_t)p->n_ctu * p->s.npu * p->s.nref;
+    const size_t nslot_keys = me_search_keys(&p->s, p->n_ctu, cfg->int_grid, &p->kl);
     hipError_t e = hipSuccess;
 #define ALLOC(ptr, bytes) if (e == hipSuccess) e = hipMalloc((void **)&(ptr), (bytes))
     ALLOC(p->d_cur, (size_t)cfg->pic_w * cfg->pic_h * sizeof(pel) + 64);   // slack for dword row reads
     ALLOC(p->d_pic, sizeof(PicDesc));
-    ALLOC(p->d_keys, nkeys * sizeof(u64));
+    ALLOC(p->d_keys, nslot_keys * sizeof(u64));
     ALLOC(p->d_res, nkeys * sizeof(me_pu_result_t));
     ALLOC(p->d_best, (size_t)p->n_ctu * p->s.npu * p->s.nl);
 #undef ALLOC
@@ -361,6 +417,14 @@ ME_EXPORT int me_plan_create(me_plan_t *out, const me_frame_cfg_t *cfg)
         return -1;
     }
     *out = p;
+    return 0;
+}
+
+ME_EXPORT int me_plan_int_grid(me_plan_t p, int *groups, int *slots)
+{
+    if (!p) { me_set_error("me_plan_int_grid: bad argument", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
+    if (groups) *groups = (int)p->kl.groups;
+    if (slots) *slots = p->kl.nslots;
     return 0;
 }
 
@@ -413,7 +477,7 @@ static int plan_enqueue_stages(me_plan_s *p, me_ref_t const *refs, uint32_t prep
     js.pics = p->d_pic;
     js.jobs = nullptr;
     js.n_jobs = p->n_ctu;
-    return me_search_enqueue(&p->s, js, p->d_keys, p->d_res, p->d_best, st, ev ? ev + 1 : nullptr);
+    return me_search_enqueue(&p->s, js, p->kl, p->d_keys, p->d_res, p->d_best, st, ev ? ev + 1 : nullptr);
 }
 
 static int plan_enqueue(me_plan_t p, me_ref_t const *refs, uint32_t prepare_mask, hipStream_t st)
@@ -563,6 +627,17 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
     double best = 1e30;
     me_frame_cfg_t c = *cfg, pick = *cfg;
     int n = 0;
+    // the integer search's time: fill (atomic layouts only) + search
+    auto time_int = [&](const me_frame_cfg_t &cc, double *ms) -> int {
+        me_plan_t p;
+        if (me_plan_create(&p, &cc)) return 1;   // LDS window too large
+        int rc = me_plan_set_cur(p, cur, stride, 0, nullptr);
+        if (!rc) rc = me_plan_profile(p, refs, 0, 1, ms);          // warm-up
+        if (!rc) rc = me_plan_profile(p, refs, 0, iters, ms);
+        me_plan_destroy(p);
+        return rc ? -1 : 0;
+    };
+    c.int_grid = 0;
     for (int w : waves)
         for (int b : bands)
             for (int dc : cols) {
@@ -571,18 +646,28 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
                 c.band_rows = b;
                 c.waves = w;
                 c.dx_cols = dc;
-                me_plan_t p;
-                if (me_plan_create(&p, &c)) continue;   // LDS window too large
                 double ms[4] = {0, 0, 0, 0};
-                int rc = me_plan_set_cur(p, cur, stride, 0, nullptr);
-                if (!rc) rc = me_plan_profile(p, refs, 0, 1, ms);          // warm-up
-                if (!rc) rc = me_plan_profile(p, refs, 0, iters, ms);
-                me_plan_destroy(p);
-                if (rc) return -1;
-                if (log && n < log_cap) log[n] = {b, w, dc, ms[2], c.sp_groups, ms[3]};
+                const int rc = time_int(c, ms);
+                if (rc > 0) continue;
+                if (rc < 0) return -1;
+                const double t = ms[1] + ms[2];
+                if (log && n < log_cap) log[n] = {b, w, dc, t, c.sp_groups, ms[3], c.int_grid};
                 n++;
-                if (ms[2] < best) { best = ms[2]; pick = c; }
+                if (t < best) { best = t; pick = c; }
             }
+    if (best < 1e30) {   // the fastest schedule with one workgroup per band
+        c = pick;
+        c.int_grid = 1;
+        double ms[4] = {0, 0, 0, 0};
+        const int rc = time_int(c, ms);
+        if (rc < 0) return -1;
+        if (rc == 0) {
+            const double t = ms[1] + ms[2];
+            if (log && n < log_cap) log[n] = {c.band_rows, c.waves, c.dx_cols, t, c.sp_groups, ms[3], 1};
+            n++;
+            if (t < best) { best = t; pick = c; }
+        }
+    }
     if (best >= 1e30) {
         if (n_log) *n_log = n < log_cap ? n : log_cap;
         me_set_error("me_plan_autotune: no schedule fits", hipErrorInvalidValue, __FILE__, __LINE__);
@@ -601,7 +686,7 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
         if (!rc) rc = me_plan_profile(p, refs, 0, iters, ms);
         me_plan_destroy(p);
         if (rc) return -1;
-        if (log && n < log_cap) log[n] = {c.band_rows, c.waves, c.dx_cols, ms[2], g, ms[3]};
+        if (log && n < log_cap) log[n] = {c.band_rows, c.waves, c.dx_cols, ms[1] + ms[2], g, ms[3], c.int_grid};
         n++;
         if (ms[3] < best_sp) { best_sp = ms[3]; pick.sp_groups = g; }
     }
@@ -609,6 +694,7 @@ ME_EXPORT int me_plan_autotune(me_frame_cfg_t *cfg, me_ref_t const *refs, const 
     cfg->band_rows = pick.band_rows;
     cfg->waves = pick.waves;
     cfg->dx_cols = pick.dx_cols;
+    cfg->int_grid = pick.int_grid;
     cfg->sp_groups = pick.sp_groups;
     return 0;
 }
@@ -1035,5 +1121,7 @@ ME_EXPORT int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *c
     js.pics = (const PicDesc *)base;
     js.jobs = nullptr;
     js.n_jobs = n_ctu;
-    return me_search_enqueue(&s, js, keys, res, best_ref, st, nullptr);
+    KeyLayout kl;
+    me_search_keys(&s, n_ctu, 1, &kl);   // one key per (CTU, PU, reference), merged with atomics
+    return me_search_enqueue(&s, js, kl, keys, res, best_ref, st, nullptr);
 }

@@ -501,25 +501,29 @@ struct Block {
     int cx, cy, dy0, dxs, dxe, ref;
 };
 
-// One pass over the block's displacements: PASS 3 = every PU class, 2 =
-// 16x16 only.  Returns the node states in st.
+// The rate row is read-only for the whole launch: reading it through the
+// constant address space keeps its loads scalar even where the workgroup's
+// key stores of an earlier segment precede them in the loop.
+typedef const __attribute__((address_space(4))) u32 *const_u32p;
+__device__ __forceinline__ u32 rate_at(const IntArgs &a, int i) { return ((const_u32p)a.rate)[i]; }
+
+// One pass over the band's displacements: PASS 3 = every PU class, 2 =
+// 16x16 only.  Updates the node states in st.
 template <int PASS, int WAVES, bool EDGE, bool SH0, int NS>
 __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, const Block &b, const pel *win, const pel *curl,
-                                         double *st)
+                                         double *st, int tid)
 {
     // the wave index is uniform across the wave: keep it (and everything the
     // displacement loop derives from it) in scalar registers
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int rx = lane & 7, ry = lane >> 3, X = rx * 16, Y = ry * 16;
     const int wlim = P.pic_w + 126, hlim = P.pic_h + 126;
     const int span = 2 * a.R + 1;
-#pragma unroll
-    for (int i = 0; i < NS; i++) st[i] = __builtin_bit_cast(double, KEY_INIT);
     for (int row = wave; row < a.band_rows; row += WAVES) {
         const int dy = b.dy0 + row;
         if (dy < a.lo || dy > a.hi) continue;
         const int dyi = dy + a.R;
-        const u32 ry_rate = a.rate[dyi];
+        const u32 ry_rate = rate_at(a, dyi);
         for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
             if (dxb + 3 < a.lo) continue;
             KeyCtx kc;
@@ -534,7 +538,7 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                     kc.klo[k] = 0;
                 } else {
                     const int dxi = dx + a.R;
-                    const u64 K = ((u64)(a.rate[dxi] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
+                    const u64 K = ((u64)(rate_at(a, dxi) + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
                     kc.khi[k] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
                     kc.klo[k] = __builtin_amdgcn_readfirstlane((u32)K);
                 }
@@ -583,12 +587,22 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
     }
 }
 
-#define PUB(S, W, H, XX, YY, ST)                                                                    \
-    do {                                                                                            \
-        if (a.base[S] >= 0) atomic_key_min(kb + (size_t)pu_index(a, S, W, H, XX, YY) * a.nref, ST); \
+// A node's key into the slot of this piece (kb), as the minimum with the key
+// there when `merge` is set, or merged into the one key array with atomicMin
+// (atomic layouts, kb there).
+#define PUB(S, W, H, XX, YY, ST)                                                    \
+    do {                                                                            \
+        if (a.base[S] >= 0) {                                                       \
+            u64 *kp_ = kb + (size_t)pu_index(a, S, W, H, XX, YY) * a.nref;          \
+            const u64 k_ = __builtin_bit_cast(u64, ST);                             \
+            if (atomic) atomic_key_min(kp_, ST);                                    \
+            else if (merge) { const u64 o_ = *kp_; *kp_ = k_ < o_ ? k_ : o_; }      \
+            else *kp_ = k_;                                                         \
+        }                                                                           \
     } while (0)
 
-__device__ __forceinline__ void publish_small(const IntArgs &a, u64 *kb, const double *st, int X, int Y)
+__device__ __forceinline__ void publish_small(const IntArgs &a, u64 *kb, bool atomic, bool merge, const double *st, int X,
+                                              int Y)
 {
 #pragma unroll
     for (int c = 0; c < 4; c++)
@@ -613,7 +627,8 @@ __device__ __forceinline__ void publish_small(const IntArgs &a, u64 *kb, const d
     PUB(S16x16, 16, 16, X, Y, st[32]);
 }
 
-__device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, const double *st, int rx, int ry)
+__device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, bool atomic, bool merge, const double *st, int rx,
+                                              int ry)
 {
     const int X = rx * 16, Y = ry * 16, X2 = X & ~31, Y2 = Y & ~31, X4 = X & ~63, Y4 = Y & ~63;
     const bool xr = rx & 1, yr = ry & 1;
@@ -680,119 +695,242 @@ __device__ __forceinline__ bool merge_waves(double *st, unsigned char *smem)
     return true;
 }
 
-// One block = (displacement tile, CTU, reference); 64*WAVES threads.  The
-// reference window of the tile and the current CTU are staged in LDS once;
-// then each wave covers the whole CTU (lane = 16x16 region) for every
-// WAVES-th displacement row of the tile, updating the node states of all
-// 58 classes per displacement.
+// Stages the reference window of band b in LDS: rows cy + dy0 ..
+// cy + dy0 + 128 + band_rows - 2, columns cx + dxs .. cx + dxs + wcols - 1,
+// a dword (4 / sizeof(pel) samples) per thread and step.  Samples are
+// clamped to the padded plane (it is edge-replicated, so clamping is exact);
+// away from the plane's edges a dword is realigned from two aligned loads.
+__device__ __forceinline__ void stage_window(const IntArgs &a, const PicDesc &P, const Block &b, pel *win, int nthr,
+                                             int tid)
+{
+    constexpr int SPD = 4 / (int)sizeof(pel);
+    const int wrows = 128 + a.band_rows - 1;
+    const pel *ref0 = P.planes[b.ref] + (size_t)ME_PAD * P.ref_stride + ME_PAD;
+    const int xmin = -ME_PAD, xmax = P.pic_w + ME_PAD - 1, ymin = -ME_PAD, ymax = P.pic_h + ME_PAD - 1;
+    const int wq = a.wcols / SPD, dq = nthr % wq, dr = nthr / wq;
+    int r = tid / wq, q = tid - r * wq;
+    for (; r < wrows; r += dr, q += dq) {
+        if (q >= wq) { q -= wq; r++; if (r >= wrows) break; }
+        const int x0 = b.cx + b.dxs + q * SPD, y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
+        const pel *src = ref0 + (ptrdiff_t)y * P.ref_stride;
+        u32 v;
+        if (x0 >= xmin && x0 + SPD - 1 <= xmax) {
+            const uintptr_t ad = (uintptr_t)(src + x0);
+            const u32 *qa = (const u32 *)(ad & ~(uintptr_t)3);
+            v = __builtin_amdgcn_alignbyte(qa[1], qa[0], (u32)(ad & 3));
+        } else {
+            v = 0;
+#pragma unroll
+            for (int k = 0; k < SPD; k++)
+                v |= (u32)src[me_clip3(xmin, xmax, x0 + k)] << (8 * (int)sizeof(pel) * k);
+        }
+        *(u32 *)(win + r * a.wpitch + wskew(r) + q * SPD) = v;
+    }
+}
+
+// Stages the current CTU (clamped to the picture) in LDS, 128 x 128 samples.
+__device__ __forceinline__ void stage_cur(const PicDesc &P, const Block &b, pel *curl, int nthr, int tid)
+{
+    constexpr int SPD = 4 / (int)sizeof(pel);
+    constexpr int CQ = 128 / SPD;   // dwords per CTU row
+    const int q = tid % CQ;
+    const bool inner = b.cx + 127 <= P.pic_w - 1;
+    for (int r = tid / CQ; r < 128; r += nthr / CQ) {
+        const int y = me_clip3(0, P.pic_h - 1, b.cy + r), x0 = b.cx + q * SPD;
+        const pel *src = P.cur + (size_t)y * P.pic_w;
+        u32 v;
+        if (inner) {
+            v = *(const u32 *)(src + x0);
+        } else {
+            v = 0;
+#pragma unroll
+            for (int k = 0; k < SPD; k++)
+                v |= (u32)src[me_clip3(0, P.pic_w - 1, x0 + k)] << (8 * (int)sizeof(pel) * k);
+        }
+        *(u32 *)(curl + r * 128 + q * SPD) = v;
+    }
+}
+
+// The bands [band0, band1) of one segment (job, reference, column tile):
+// per band the window is staged (with the current CTU at the first band) and
+// every wave covers the whole CTU (lane = 16x16 region) for every WAVES-th
+// displacement row, updating the node states st of all classes.
+template <int PASS, int WAVES, bool EDGE, bool SH0, int NS>
+__device__ __forceinline__ void segment_bands(const IntArgs &a, const PicDesc &P, const Block seg, int band0, int band1,
+                                              unsigned char *smem, double *st)
+{
+    const int wrows = 128 + a.band_rows - 1, nthr = 64 * WAVES;
+    pel *win = (pel *)smem;
+    pel *curl = win + (size_t)wrows * a.wpitch;
+    for (int band = band0; band < band1; band++) {
+        Block b = seg;
+        b.dy0 = -a.R + band * a.band_rows;
+        __syncthreads();   // every wave is done with the previous window (or merge area)
+        // The staging addresses and the lane's window offsets repeat from
+        // band to band; an opaque thread index keeps them from being hoisted
+        // out of the band and segment loops into registers held through the
+        // whole kernel: each band computes them anew.
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        stage_window(a, P, b, win, nthr, tid);
+        if (band == band0) stage_cur(P, b, curl, nthr, tid);
+        __syncthreads();
+        run_pass<PASS, WAVES, EDGE, SH0, NS>(a, P, b, win, curl, st, tid);
+    }
+}
+
+// The bands [band0, band1) of one segment in this workgroup, then one
+// publication of the node states.  merge: this workgroup has already stored
+// keys of the segment into its slot (non-atomic layouts), so each key is
+// stored as the minimum with the one there (the same lane wrote it).
+template <int MODE, int WAVES, int NS, bool EDGE, bool SH0>
+__device__ __forceinline__ void search_segment(const IntArgs &a, const PicDesc &P, const Block b, int band0, int band1,
+                                               unsigned char *smem, u64 *kb, bool atomic, bool merge)
+{
+    const int lane = threadIdx.x & 63, rx = lane & 7, ry = lane >> 3;
+    constexpr int PASS = MODE == 1 ? 2 : 3;
+    double st[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) st[i] = __builtin_bit_cast(double, KEY_INIT);
+    segment_bands<PASS, WAVES, EDGE, SH0, NS>(a, P, b, band0, band1, smem, st);
+    if (merge_waves<WAVES, NS>(st, smem)) {
+        // The publishing addresses do not change from segment to segment; kept
+        // opaque here, they are computed when used rather than hoisted out of
+        // the workgroup's segment loop into registers held for the whole kernel.
+        int px = rx, py = ry;
+        asm volatile("" : "+v"(px), "+v"(py));
+        if (MODE == 1) {
+            PUB(S16x16, 16, 16, px * 16, py * 16, st[0]);
+        } else {
+            merge_sparse(st + N_SMALL);
+            publish_small(a, kb, atomic, merge, st, px * 16, py * 16);
+            publish_large(a, kb, atomic, merge, st + N_SMALL, px, py);
+        }
+    }
+}
+
+// Values every lane of a wave holds, marked uniform again inside a function
+// that is not inlined (its arguments arrive in vector registers).
+__device__ __forceinline__ u32 uni(u32 v) { return (u32)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ u64 uni64(u64 v) { return ((u64)uni((u32)(v >> 32)) << 32) | uni((u32)v); }
+
+typedef const __attribute__((address_space(4))) IntArgs *IntArgsC;
+
+// One segment's bands [band0, band1) in this workgroup, with one
+// publication.  Kept out of line: the search needs the whole register
+// budget, and the workgroup's segment loop around it would otherwise share
+// that budget (its invariants hoisted and held through the search).  The
+// launch arguments are read from the kernel's argument segment through ap.
+// A node's window (me.c:508-512) is -127-px <= dx <= W-1-px and the same for
+// y.  The upper bounds never need a test: for a node inside the picture (the
+// only ones whose keys are used) a displacement past W-1-px puts the whole
+// block into the edge-replicated padding, so its SAD equals the SAD at
+// dx = W-1-px (whose left column is the last picture column and the rest
+// padding) while its MV bits are no smaller (bits grow with |dx|,
+// dx > W-1-px >= 0) and its raster index is larger: its key can never be the
+// minimum.  The same holds below the picture and in the corner.  The lower
+// bounds do need it, where they bind: only for R >= 128, in every band of a
+// segment at the left of the picture whose column tile reaches past -127,
+// and at the top of the picture in the bands that reach past -127 (a
+// segment's first bands: dy grows with the band).  Bands without the test and
+// with sh == 0 (every 10-bit configuration and 8-bit with R <= 127) take the
+// fast path (EDGE false, SH0 true); the others test each node's window and
+// shift the SAD.  merge: see search_segment.
+template <int MODE, int WAVES, int NS, bool EDGE, bool SH0>
+__device__ __attribute__((noinline)) void search_piece(IntArgsC ap, u32 job, u32 ref, u32 bx, u32 band0, u32 band1,
+                                                       u64 kb_addr, u32 flags)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const IntArgs &a = *(const IntArgs *)(IntArgsC)(uintptr_t)uni64((u64)(uintptr_t)ap);
+    job = uni(job);
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, (int)job, ctu);
+    Block b;
+    b.ref = (int)uni(ref);
+    b.cx = (ctu % P.ctu_cols) * 128;
+    b.cy = (ctu / P.ctu_cols) * 128;
+    b.dy0 = 0;
+    b.dxs = -a.R + (int)uni(bx) * a.dx_cols;
+    b.dxe = b.dxs + a.dx_cols - 1 < a.hi ? b.dxs + a.dx_cols - 1 : a.hi;
+    flags = uni(flags);
+    search_segment<MODE, WAVES, NS, EDGE, SH0>(a, P, b, (int)uni(band0), (int)uni(band1), smem,
+                                              (u64 *)(uintptr_t)uni64(kb_addr), flags & 1, (flags >> 1) & 1);
+}
+
+// One workgroup per unit (atomic layouts, groups == units): the search of
+// one band inline, as a kernel of its own -- no segment loop around it and
+// no call.
 template <int MODE, int WAVES>
+__device__ __forceinline__ void search_unit(const IntArgs &a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NS = MODE == 1 ? 1 : N_SMALL + N_LARGE;
+    const u32 nb = (u32)a.kl.nbands, u = blockIdx.x;
+    const u32 x = u / nb;
+    const int band = (int)(u - x * nb);
+    const int bx = (int)(x % (u32)a.kl.nbx);
+    const u32 jr = x / (u32)a.kl.nbx;
+    const int job = (int)(jr / (u32)a.nref);
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    Block b;
+    b.ref = (int)(jr % (u32)a.nref);
+    b.cx = (ctu % P.ctu_cols) * 128;
+    b.cy = (ctu / P.ctu_cols) * 128;
+    b.dy0 = 0;
+    b.dxs = -a.R + bx * a.dx_cols;
+    b.dxe = b.dxs + a.dx_cols - 1 < a.hi ? b.dxs + a.dx_cols - 1 : a.hi;
+    u64 *kb = a.keys + (size_t)job * a.npu * a.nref + b.ref;
+    const bool edge = b.cx + b.dxs < -127 || b.cy - a.R + band * a.band_rows < -127;
+    if (!edge && a.sh == 0) search_segment<MODE, WAVES, NS, false, true>(a, P, b, band, band + 1, smem, kb, true, false);
+    else search_segment<MODE, WAVES, NS, true, false>(a, P, b, band, band + 1, smem, kb, true, false);
+}
+
+// Workgroup g of the partition (KeyLayout) with 64*WAVES threads works
+// through its units segment by segment (PERSIST), or takes one unit
+// (atomic layouts).  A segment with both window-testing and fast bands runs
+// them as two searches, the second merging its keys into the first's.
+template <int MODE, int WAVES, bool PERSIST>
 // The register budget is 2 waves per SIMD: 2 workgroups of 4 waves, or one
 // of 8 (the 10-bit windows at R = 128 leave room for only one workgroup's
 // LDS per CU).
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_int_search(IntArgs a)
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_int_search(IntArgs a)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     if (a.js.n_jobs == 0) return;   // a warm-up launch
-    Block b;
-    const int band = blockIdx.x / a.nbx, bandx = blockIdx.x - band * a.nbx, job = blockIdx.y;
-    int ctu;
-    const PicDesc &P = *job_decode(a.js, job, ctu);
-    b.ref = a.ref0 + blockIdx.z;
-    b.cx = (ctu % P.ctu_cols) * 128;
-    b.cy = (ctu / P.ctu_cols) * 128;
-    b.dy0 = -a.R + band * a.band_rows;
-    b.dxs = -a.R + bandx * a.dx_cols;
-    b.dxe = b.dxs + a.dx_cols - 1 < a.hi ? b.dxs + a.dx_cols - 1 : a.hi;
-    const int wrows = 128 + a.band_rows - 1;
-    pel *win = (pel *)smem;
-    pel *curl = win + (size_t)wrows * a.wpitch;
-    const int tid = threadIdx.x, nthr = 64 * WAVES;
-
-    // Stage the window and the current CTU in LDS, a dword (4 / sizeof(pel)
-    // samples) per thread and step.  Reference samples (cx + dxs + c,
-    // cy + dy0 + r) are clamped to the padded plane (it is edge-replicated, so
-    // clamping is exact); the current CTU to the picture.  Away from the
-    // plane's edges a dword is realigned from two aligned loads.
-    constexpr int SPD = 4 / (int)sizeof(pel);
-    const pel *ref0 = P.planes[b.ref] + (size_t)ME_PAD * P.ref_stride + ME_PAD;
-    const int xmin = -ME_PAD, xmax = P.pic_w + ME_PAD - 1, ymin = -ME_PAD, ymax = P.pic_h + ME_PAD - 1;
-    {
-        const int wq = a.wcols / SPD, dq = nthr % wq, dr = nthr / wq;
-        int r = tid / wq, q = tid - r * wq;
-        for (; r < wrows; r += dr, q += dq) {
-            if (q >= wq) { q -= wq; r++; if (r >= wrows) break; }
-            const int x0 = b.cx + b.dxs + q * SPD, y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
-            const pel *src = ref0 + (ptrdiff_t)y * P.ref_stride;
-            u32 v;
-            if (x0 >= xmin && x0 + SPD - 1 <= xmax) {
-                const uintptr_t ad = (uintptr_t)(src + x0);
-                const u32 *qa = (const u32 *)(ad & ~(uintptr_t)3);
-                v = __builtin_amdgcn_alignbyte(qa[1], qa[0], (u32)(ad & 3));
-            } else {
-                v = 0;
-#pragma unroll
-                for (int k = 0; k < SPD; k++)
-                    v |= (u32)src[me_clip3(xmin, xmax, x0 + k)] << (8 * (int)sizeof(pel) * k);
-            }
-            *(u32 *)(win + r * a.wpitch + wskew(r) + q * SPD) = v;
-        }
-    }
-    {
-        constexpr int CQ = 128 / SPD;   // dwords per CTU row
-        const int q = tid % CQ;
-        const bool inner = b.cx + 127 <= P.pic_w - 1;
-        for (int r = tid / CQ; r < 128; r += nthr / CQ) {
-            const int y = me_clip3(0, P.pic_h - 1, b.cy + r), x0 = b.cx + q * SPD;
-            const pel *src = P.cur + (size_t)y * P.pic_w;
-            u32 v;
-            if (inner) {
-                v = *(const u32 *)(src + x0);
-            } else {
-                v = 0;
-#pragma unroll
-                for (int k = 0; k < SPD; k++)
-                    v |= (u32)src[me_clip3(0, P.pic_w - 1, x0 + k)] << (8 * (int)sizeof(pel) * k);
-            }
-            *(u32 *)(curl + r * 128 + q * SPD) = v;
-        }
-    }
-    __syncthreads();
-
-    const int lane = tid & 63;
-    const int rx = lane & 7, ry = lane >> 3;
-    // A node's window (me.c:508-512) is -127-px <= dx <= W-1-px and the same
-    // for y.  The upper bounds never need a test: for a node inside the
-    // picture (the only ones whose keys are used) a displacement past
-    // W-1-px puts the whole block into the edge-replicated padding, so its
-    // SAD equals the SAD at dx = W-1-px (whose left column is the last
-    // picture column and the rest padding) while its MV bits are no smaller
-    // (bits grow with |dx|, dx > W-1-px >= 0) and its raster index is
-    // larger: its key can never be the minimum.  The same holds below the
-    // picture and in the corner.  The lower bounds do need it, where they
-    // bind: only for R >= 128, in blocks at the left or top of the picture
-    // whose tile reaches past -127.
-    const bool edge = b.cx + b.dxs < -127 || b.cy + b.dy0 < -127;
-    u64 *kb = a.keys + (size_t)job * a.npu * a.nref + b.ref;
-    // Blocks without a lower-bound test and with sh == 0 (every 10-bit
-    // configuration and 8-bit with R <= 127) take the fast path; the others
-    // test each node's window and shift the SAD.
-    const bool fast = !edge && a.sh == 0;
-    if (MODE == 1) {
-        double st[1];
-        if (fast) run_pass<2, WAVES, false, true, 1>(a, P, b, win, curl, st);
-        else run_pass<2, WAVES, true, false, 1>(a, P, b, win, curl, st);
-        if (merge_waves<WAVES, 1>(st, smem)) PUB(S16x16, 16, 16, rx * 16, ry * 16, st[0]);
+    if (!PERSIST) {
+        search_unit<MODE, WAVES>(a);
         return;
     }
-    {
-        double st[N_SMALL + N_LARGE];
-        if (fast) run_pass<3, WAVES, false, true, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
-        else run_pass<3, WAVES, true, false, N_SMALL + N_LARGE>(a, P, b, win, curl, st);
-        if (merge_waves<WAVES, N_SMALL + N_LARGE>(st, smem)) {
-            merge_sparse(st + N_SMALL);
-            publish_small(a, kb, st, rx * 16, ry * 16);
-            publish_large(a, kb, st + N_SMALL, rx, ry);
+    const IntArgsC ap = (IntArgsC)__builtin_amdgcn_kernarg_segment_ptr();   // a is the only argument
+    constexpr int NS = MODE == 1 ? 1 : N_SMALL + N_LARGE;
+    const KeyLayout &kl = a.kl;
+    const u32 g = blockIdx.x, nb = (u32)kl.nbands;
+    const u32 ub = (u32)((u64)kl.units * g / kl.groups), ue = (u32)((u64)kl.units * (g + 1) / kl.groups);
+    for (u32 u = ub; u < ue;) {
+        const u32 x = u / nb;
+        const u32 band0 = u - x * nb;
+        const u32 left = ue - u;
+        const u32 band1 = left < nb - band0 ? band0 + left : nb;
+        const u32 bx = x % (u32)kl.nbx;
+        const u32 jr = x / (u32)kl.nbx;
+        const u32 job = jr / (u32)a.nref, ref = jr % (u32)a.nref;
+        int ctu;
+        const PicDesc &P = *job_decode(a.js, (int)job, ctu);
+        const int cx = (ctu % P.ctu_cols) * 128, cy = (ctu / P.ctu_cols) * 128;
+        // this workgroup's piece of the segment
+        const u64 kba = (u64)(uintptr_t)(a.keys + (size_t)job * a.npu * a.nref + ref +
+                                         (size_t)(bx * kl.nslots + (int)(g - key_owner(kl, (u64)x * nb))) * kl.slot_elems);
+        if (cx - a.R + (int)bx * a.dx_cols < -127 || a.sh != 0) {
+            search_piece<MODE, WAVES, NS, true, false>(ap, job, ref, bx, band0, band1, kba, 0);
+        } else {
+            const int need = a.R - 127 - cy;   // window-testing bands: -R + band * band_rows < -127 - cy
+            u32 split = need <= 0 ? 0u : (u32)((need + a.band_rows - 1) / a.band_rows);
+            split = split < band0 ? band0 : (split > band1 ? band1 : split);
+            if (split > band0) search_piece<MODE, WAVES, NS, true, false>(ap, job, ref, bx, band0, split, kba, 0);
+            if (band1 > split)
+                search_piece<MODE, WAVES, NS, false, true>(ap, job, ref, bx, split, band1, kba, split > band0 ? 2u : 0u);
         }
+        u += band1 - band0;
     }
 }
 #undef PUB
@@ -800,12 +938,19 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(1, 2
 #define ME_IS_CAT2(a, b) a##b
 #define ME_IS_CAT(a, b) ME_IS_CAT2(a, b)
 
-int ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(int mode, const IntArgs &a, dim3 grid, size_t lds, hipStream_t st)
+int ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(int mode, const IntArgs &a, size_t lds, hipStream_t st)
 {
-    const void *fn = mode == 1 ? (const void *)k_int_search<1, ME_IS_WAVES> : (const void *)k_int_search<0, ME_IS_WAVES>;
+    const dim3 grid(a.kl.groups ? a.kl.groups : 1), block(64 * ME_IS_WAVES);
+    const bool persist = a.kl.nslots > 0;
+    const void *fn = mode == 1 ? (persist ? (const void *)k_int_search<1, ME_IS_WAVES, true>
+                                          : (const void *)k_int_search<1, ME_IS_WAVES, false>)
+                               : (persist ? (const void *)k_int_search<0, ME_IS_WAVES, true>
+                                          : (const void *)k_int_search<0, ME_IS_WAVES, false>);
     if (lds > 65536) ME_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (mode == 1) hipLaunchKernelGGL((k_int_search<1, ME_IS_WAVES>), grid, dim3(64 * ME_IS_WAVES), lds, st, a);
-    else hipLaunchKernelGGL((k_int_search<0, ME_IS_WAVES>), grid, dim3(64 * ME_IS_WAVES), lds, st, a);
+    if (mode == 1 && persist) hipLaunchKernelGGL((k_int_search<1, ME_IS_WAVES, true>), grid, block, lds, st, a);
+    else if (mode == 1) hipLaunchKernelGGL((k_int_search<1, ME_IS_WAVES, false>), grid, block, lds, st, a);
+    else if (persist) hipLaunchKernelGGL((k_int_search<0, ME_IS_WAVES, true>), grid, block, lds, st, a);
+    else hipLaunchKernelGGL((k_int_search<0, ME_IS_WAVES, false>), grid, block, lds, st, a);
     ME_CHECK(hipGetLastError());
     return 0;
 }
@@ -816,5 +961,5 @@ int ME_IS_CAT(me_int_search_preload_w, ME_IS_WAVES)(int mode, size_t lds, hipStr
     // costs milliseconds, which a request should not wait for
     IntArgs a;
     memset(&a, 0, sizeof(a));
-    return ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(mode, a, dim3(1), lds, st);
+    return ME_IS_CAT(me_int_search_launch_w, ME_IS_WAVES)(mode, a, lds, st);
 }

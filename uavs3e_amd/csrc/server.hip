@@ -208,7 +208,10 @@ static int launch_batch(me_server_s *S, Slot &sl)
     js.pics = sl.d_pics;
     js.jobs = sl.d_jobs;
     js.n_jobs = n;
-    if (me_search_enqueue(&S->search, js, sl.d_keys, sl.d_res, sl.d_best, sl.st, nullptr)) return -1;
+    // batches are a few CTUs: one workgroup per band, keys merged with atomics
+    KeyLayout kl;
+    me_search_keys(&S->search, n, 1, &kl);
+    if (me_search_enqueue(&S->search, js, kl, sl.d_keys, sl.d_res, sl.d_best, sl.st, nullptr)) return -1;
     const size_t nres = (size_t)n * S->search.npu * nref;
     copy_out(sl, nres, (size_t)n * S->search.npu);
     ME_CHECK(hipGetLastError());

@@ -37,7 +37,7 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
     const int x = (ctu % P.ctu_cols) * 128 + (i % cols) * S.pw, y = (ctu / P.ctu_cols) * 128 + (i / cols) * S.ph;
     const size_t slot = ((size_t)job * a.npu + S.base + i) * a.nref + r;
     const bool inside = x + S.pw <= P.pic_w && y + S.ph <= P.pic_h;
-    const u64 key = a.keys[slot];
+    const u64 key = key_merged(a.kl, a.keys, slot, job, r, a.nref);
     const bool lead = li == 0 && g == 0;
     me_pu_result_t out;
     out.mv_int[0] = out.mv_int[1] = out.mv_sub[0] = out.mv_sub[1] = 0;
@@ -46,7 +46,7 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
     out.valid = 0;
     // Units without a search result take no sample reads; they still pass
     // every barrier of the workgroup.
-    const bool active = inside && key != ~0ull;
+    const bool active = inside && key < KEY_INIT;
     const bool sync = G > 1 || LPU > 64;
     if (!active) {
         if (lead) a.res[slot] = out;
