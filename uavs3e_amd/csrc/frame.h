@@ -6,6 +6,8 @@
 #include "../../include/uavs3e_me_hip.h"
 
 #define ME_MAX_REFS 4
+// Entries past the end of the integer-pel rate row (see IntArgs::rate).
+#define ME_RATE_PAD 8
 
 // A reference picture resident in device memory: the edge-padded integer
 // plane + 15 quarter-pel planes (com_pic_t / com_subpel_t, com_util.c:62-91,
@@ -148,7 +150,7 @@ struct IntArgs {
     int R;                   // window half size
     int lo, hi;              // uniform displacement clip: [max(-R,-T+4), min(R,T-4)]
     int IB, sh;              // index bits; sad shift into the high word
-    const u32 *rate;         // [2R+1]: bits(4*(i-R)) * lambda
+    const u32 *rate;         // [2R+1 + ME_RATE_PAD]: bits(4*(i-R)) * lambda, then padding
     u64 *keys;               // see KeyLayout
     KeyLayout kl;
     int npu, nref;

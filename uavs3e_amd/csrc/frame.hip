@@ -190,15 +190,15 @@ int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int
     // rate[i] = bits(4*(i-R)) * lambda (GET_MVBITS_IPEL, util.h:43-44)
     std::vector<u8> tab(2 * p->tab_off + 1);
     me_mvbits_fill(tab.data(), p->tab_off);
-    std::vector<u32> rate(p->span);
+    std::vector<u32> rate(p->span + ME_RATE_PAD, 0u);
     for (int i = 0; i < p->span; i++) rate[i] = (u32)tab[p->tab_off + 4 * (i - R)] * lambda;
     std::vector<u8> blk_shape(p->sub_blocks);
     blk_shape_fill(shp, n_shp, p->sub_blocks, blk_shape.data());
-    hipError_t e = hipMalloc((void **)&p->d_rate, p->span * sizeof(u32));
+    hipError_t e = hipMalloc((void **)&p->d_rate, rate.size() * sizeof(u32));
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tab_base, tab.size());
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_shp, n_shp * sizeof(SubShape));
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_blk_shape, blk_shape.size());
-    if (e == hipSuccess) e = hipMemcpy(p->d_rate, rate.data(), p->span * sizeof(u32), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_rate, rate.data(), rate.size() * sizeof(u32), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_tab_base, tab.data(), tab.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_shp, shp, n_shp * sizeof(SubShape), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_blk_shape, blk_shape.data(), blk_shape.size(), hipMemcpyHostToDevice);
@@ -1024,6 +1024,7 @@ __global__ void k_search_tables(TableArgs a)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i <= 2 * a.tab_off) a.tab_base[i] = (u8)mvbits_at(i - a.tab_off, a.tab_off);
     if (i < a.span) a.rate[i] = mvbits_at(4 * (i - a.R), a.tab_off) * a.lambda;
+    else if (i < a.span + ME_RATE_PAD) a.rate[i] = 0;
     if (i < a.n_shp) a.d_shp[i] = a.shp[i];
     if (i < a.sub_blocks) {
         int k = 0;
@@ -1067,7 +1068,7 @@ ME_EXPORT int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *c
     // scratch layout: descriptor, rate row, MV-bits table, shape table,
     // workgroup -> shape, keys
     const size_t o_rate = align256(sizeof(PicDesc));
-    const size_t o_tab = o_rate + align256((size_t)s.span * sizeof(u32));
+    const size_t o_tab = o_rate + align256((size_t)(s.span + ME_RATE_PAD) * sizeof(u32));
     const size_t o_shp = o_tab + align256((size_t)2 * s.tab_off + 1);
     const size_t o_blk = o_shp + align256(sizeof(SubShape) * ME_NSHAPES);
     const size_t o_keys = o_blk + align256((size_t)s.sub_blocks);
@@ -1113,7 +1114,7 @@ ME_EXPORT int me_frame_search_noalloc(const me_frame_cfg_t *cfg, const me_pel *c
     a.blk_shape = s.d_blk_shape;
     a.d_pic = (PicDesc *)base;
     int n = 2 * s.tab_off + 1;
-    if (s.span > n) n = s.span;
+    if (s.span + ME_RATE_PAD > n) n = s.span + ME_RATE_PAD;
     if (s.sub_blocks > n) n = s.sub_blocks;
     hipLaunchKernelGGL(k_search_tables, dim3((n + 255) / 256), dim3(256), 0, st, a);
     ME_CHECK(hipGetLastError());

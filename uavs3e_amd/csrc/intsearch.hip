@@ -503,9 +503,17 @@ struct Block {
 
 // The rate row is read-only for the whole launch: reading it through the
 // constant address space keeps its loads scalar even where the workgroup's
-// key stores of an earlier segment precede them in the loop.
+// key stores of an earlier segment precede them in the loop.  The row holds
+// ME_RATE_PAD entries past its end, so that four consecutive entries can be
+// read from any step of four displacements, and the step after the last.
 typedef const __attribute__((address_space(4))) u32 *const_u32p;
+typedef u32 u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+typedef const __attribute__((address_space(4))) u32x4a *const_u32x4p;
 __device__ __forceinline__ u32 rate_at(const IntArgs &a, int i) { return ((const_u32p)a.rate)[i]; }
+__device__ __forceinline__ u32x4a rate4_at(const IntArgs &a, int i)
+{
+    return *(const_u32x4p)((const_u32p)a.rate + i);
+}
 
 // One pass over the band's displacements: PASS 3 = every PU class, 2 =
 // 16x16 only.  Updates the node states in st.
@@ -524,65 +532,73 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
         if (dy < a.lo || dy > a.hi) continue;
         const int dyi = dy + a.R;
         const u32 ry_rate = rate_at(a, dyi);
-        for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
-            if (dxb + 3 < a.lo) continue;
+        // The key parts of the step's four displacements.  A displacement
+        // outside the range gets a high word above every valid key (never
+        // NaN).  They are made at the end of the previous step, with one
+        // scalar load of the four rates: waiting for a scalar load also waits
+        // for every LDS read in flight, and at a step's end none is.
+        auto make_keys = [&](int dxb, const u32x4a &rx4) {
             KeyCtx kc;
             kc.dy = dy;
             kc.sh = a.sh;
 #pragma unroll
             for (int k = 0; k < NKEY; k++) {
-                const int dx = dxb + k;
+                const int dx = dxb + k, dxi = dx + a.R;
                 kc.dx[k] = dx;
-                if (dx < a.lo || dx > b.dxe) {
-                    kc.khi[k] = KEY_OUT_HI;   // above every valid key, never NaN
-                    kc.klo[k] = 0;
-                } else {
-                    const int dxi = dx + a.R;
-                    const u64 K = ((u64)(rate_at(a, dxi) + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
-                    kc.khi[k] = __builtin_amdgcn_readfirstlane((u32)(K >> 32) + KEY_BIAS_HI);
-                    kc.klo[k] = __builtin_amdgcn_readfirstlane((u32)K);
-                }
+                const u64 K = ((u64)(rx4[k] + ry_rate) << a.IB) | (u64)(dyi * span + dxi);
+                const bool out = dx < a.lo || dx > b.dxe;
+                kc.khi[k] = __builtin_amdgcn_readfirstlane(out ? KEY_OUT_HI : (u32)(K >> 32) + KEY_BIAS_HI);
+                kc.klo[k] = __builtin_amdgcn_readfirstlane(out ? 0u : (u32)K);
             }
+            return kc;
+        };
+        KeyCtx kc = make_keys(b.dxs, rate4_at(a, b.dxs + a.R));
+        for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
+            const u32x4a rnext = rate4_at(a, dxb + 4 + a.R);
+            if (dxb + 3 >= a.lo) {
 #if BIT_DEPTH == 8
-            Val L[4][4];
-            leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
-            Region g;
-            g.build(L);
-            if (PASS == 3) {
-                pass_small<EDGE, SH0>(g, st, kc, 0, b.cx + X, b.cy + Y, wlim, hlim);
-                Val t32x8[2], t8x32[2];
-                pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, 0, rx, ry, b.cx, b.cy, wlim, hlim, t32x8, t8x32);
-                sparse_scatter<EDGE, SH0>(g.s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
-            } else {
-#pragma unroll
-                for (int k = 0; k < NK; k++) upd1<EDGE, SH0>(st[0], g.s16.get(k), kc, k, b.cx + X, b.cy + Y, wlim, hlim);
-            }
-#else
-            // the four displacements' leaves from one pass over the window
-            // rows; then each displacement's region is composed and compared
-            // on its own, and the sparse classes take all four together
-            Val L[NKEY][4][4];
-            leaves4(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
-            Quad s16, t32x8[2], t8x32[2];
-#pragma unroll
-            for (int k = 0; k < NKEY; k++) {
+                Val L[4][4];
+                leaves(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
                 Region g;
-                g.build(L[k]);
+                g.build(L);
                 if (PASS == 3) {
-                    pass_small<EDGE, SH0>(g, st, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
-                    Val a8[2], b8[2];
-                    pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, k, rx, ry, b.cx, b.cy, wlim, hlim, a8, b8);
-                    s16.v[k] = g.s16.v;
-                    t32x8[0].v[k] = a8[0].v;
-                    t32x8[1].v[k] = a8[1].v;
-                    t8x32[0].v[k] = b8[0].v;
-                    t8x32[1].v[k] = b8[1].v;
+                    pass_small<EDGE, SH0>(g, st, kc, 0, b.cx + X, b.cy + Y, wlim, hlim);
+                    Val t32x8[2], t8x32[2];
+                    pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, 0, rx, ry, b.cx, b.cy, wlim, hlim, t32x8, t8x32);
+                    sparse_scatter<EDGE, SH0>(g.s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
                 } else {
-                    upd1<EDGE, SH0>(st[0], g.s16.v, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
+#pragma unroll
+                    for (int k = 0; k < NK; k++) upd1<EDGE, SH0>(st[0], g.s16.get(k), kc, k, b.cx + X, b.cy + Y, wlim, hlim);
                 }
-            }
-            if (PASS == 3) sparse_scatter<EDGE, SH0>(s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
+#else
+                // the four displacements' leaves from one pass over the window
+                // rows; then each displacement's region is composed and compared
+                // on its own, and the sparse classes take all four together
+                Val L[NKEY][4][4];
+                leaves4(win, a.wpitch, Y + row, X + (dxb - b.dxs), curl, Y, X, L);
+                Quad s16, t32x8[2], t8x32[2];
+#pragma unroll
+                for (int k = 0; k < NKEY; k++) {
+                    Region g;
+                    g.build(L[k]);
+                    if (PASS == 3) {
+                        pass_small<EDGE, SH0>(g, st, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
+                        Val a8[2], b8[2];
+                        pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, k, rx, ry, b.cx, b.cy, wlim, hlim, a8, b8);
+                        s16.v[k] = g.s16.v;
+                        t32x8[0].v[k] = a8[0].v;
+                        t32x8[1].v[k] = a8[1].v;
+                        t8x32[0].v[k] = b8[0].v;
+                        t8x32[1].v[k] = b8[1].v;
+                    } else {
+                        upd1<EDGE, SH0>(st[0], g.s16.v, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
+                    }
+                }
+                if (PASS == 3) sparse_scatter<EDGE, SH0>(s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
 #endif
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            kc = make_keys(dxb + 4, rnext);
         }
     }
 }
