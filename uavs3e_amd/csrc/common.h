@@ -74,6 +74,19 @@ __host__ __device__ __forceinline__ int me_satd_norm128(int s) { return (int)((d
 
 __device__ __forceinline__ int me_clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Workgroup order for L2 locality.  A launch's workgroups are dealt to the
+// GPU's 8 XCDs (each with its own L2) in turn by linear index, so
+// consecutive workgroups -- which here work on the same CTU and reference
+// and read the same planes -- would fill 8 L2s with the same lines.  This
+// bijection of [0, n) gives the workgroups of XCD x (linear index = x mod 8)
+// one contiguous run of work indices instead.  Results never depend on it.
+#define ME_XCDS 8
+__device__ __forceinline__ unsigned me_xcd_order(unsigned lin, unsigned n)
+{
+    const unsigned q = n / ME_XCDS, rem = n % ME_XCDS, x = lin % ME_XCDS, slot = lin / ME_XCDS;
+    return x < rem ? x * (q + 1) + slot : rem * (q + 1) + (x - rem) * q + slot;
+}
+
 // One wave64 min of two u64 keys held as the bit patterns of positive normal
 // doubles: for such values the IEEE order equals the unsigned integer order,
 // and v_min_f64 does it in one instruction (the compiler's fmin would first

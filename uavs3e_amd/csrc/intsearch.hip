@@ -881,7 +881,8 @@ __device__ __forceinline__ void search_unit(const IntArgs &a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NS = MODE == 1 ? 1 : N_SMALL + N_LARGE;
-    const u32 nb = (u32)a.kl.nbands, u = blockIdx.x;
+    // units in XCD order: the bands of a segment read overlapping windows
+    const u32 nb = (u32)a.kl.nbands, u = me_xcd_order(blockIdx.x, gridDim.x);
     const u32 x = u / nb;
     const int band = (int)(u - x * nb);
     const int bx = (int)(x % (u32)a.kl.nbx);
@@ -920,7 +921,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(2, 2
     const IntArgsC ap = (IntArgsC)__builtin_amdgcn_kernarg_segment_ptr();   // a is the only argument
     constexpr int NS = MODE == 1 ? 1 : N_SMALL + N_LARGE;
     const KeyLayout &kl = a.kl;
-    const u32 g = blockIdx.x, nb = (u32)kl.nbands;
+    // workgroups in XCD order: neighbouring workgroups share a segment
+    const u32 g = me_xcd_order(blockIdx.x, gridDim.x), nb = (u32)kl.nbands;
     const u32 ub = (u32)((u64)kl.units * g / kl.groups), ue = (u32)((u64)kl.units * (g + 1) / kl.groups);
     for (u32 u = ub; u < ue;) {
         const u32 x = u / nb;

@@ -19,7 +19,7 @@
 // makes the same decisions.  Taking the smallest (cost, index) pair is the
 // reference's strict '<' scan in its evaluation order.
 template <int TT>
-__device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S, int blk)
+__device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S, int blk, int job, int r)
 {
     __shared__ u64 gk[256];   // per (unit, group): best key of a phase
     __shared__ int red[4];    // per wave: SATD partial sums of groups > 64 lanes
@@ -30,7 +30,6 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
     const int tid = threadIdx.x;
     const int li = tid & (LPU - 1), g = (tid >> lg) & (G - 1), ul = tid >> (lg + gg);
     const int i = (int)((blk * 256 + tid) >> (lg + gg));
-    const int job = blockIdx.y, r = blockIdx.z;
     int ctu;
     const PicDesc &P = *job_decode(a.js, job, ctu);
     const int cols = 128 / S.pw;
@@ -175,17 +174,22 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_subpel(SubArgs a)
 {
     if (a.js.n_jobs == 0) return;   // a warm-up launch (me_subpel_preload)
+    // workgroup w = (block, job, reference), block fastest, in XCD order: the
+    // blocks of one (job, reference) read the same plane neighbourhoods
+    const unsigned nx = gridDim.x, ny = gridDim.y;
+    const unsigned w = me_xcd_order(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z), nx * ny * gridDim.z);
+    const int bxi = (int)(w % nx), job = (int)((w / nx) % ny), r = (int)(w / (nx * ny));
     // the shape and its parameters are uniform across the workgroup
-    const int si = __builtin_amdgcn_readfirstlane(a.blk_shape[blockIdx.x]);
+    const int si = __builtin_amdgcn_readfirstlane(a.blk_shape[bxi]);
     const SubShape S = a.shp[si];
-    const int blk = blockIdx.x - S.blk0;
+    const int blk = bxi - S.blk0;
     switch (S.tt) {
-    case 0: subpel_body<0>(a, S, blk); break;
-    case 1: subpel_body<1>(a, S, blk); break;
-    case 2: subpel_body<2>(a, S, blk); break;
-    case 3: subpel_body<3>(a, S, blk); break;
-    case 4: subpel_body<4>(a, S, blk); break;
-    default: subpel_body<5>(a, S, blk); break;
+    case 0: subpel_body<0>(a, S, blk, job, r); break;
+    case 1: subpel_body<1>(a, S, blk, job, r); break;
+    case 2: subpel_body<2>(a, S, blk, job, r); break;
+    case 3: subpel_body<3>(a, S, blk, job, r); break;
+    case 4: subpel_body<4>(a, S, blk, job, r); break;
+    default: subpel_body<5>(a, S, blk, job, r); break;
     }
 }
 
