@@ -252,17 +252,28 @@ def main():
     alg_bytes = work.w * work.h * pel * (1 + work.num_refs) + n_ctu * npu * work.num_refs * 8
     achieved_gbs = alg_bytes / (k_ms * 1e-3) / 1e9
     # algorithmic VALU work of the integer search: every candidate SAD built
-    # from 4x4-block SADs (16 abs-diffs per 4x4 block and displacement, 4 per
-    # v_sad-class lane op) plus one compare per PU candidate
+    # from 4x4-block SADs plus one compare per PU candidate
     blocks4 = (work.w // 4) * (work.h // 4)
     span = 2 * work.R + 1
-    valu_ops = blocks4 * span * span * work.num_refs * 16 / 4 + cands
+    # abs-diffs per lane-op of the widest SAD instruction at the bit depth:
+    # v_sad_u8 (and v_qsad_pk_u16_u8 at a quarter of its rate) 4, v_sad_u16 2
+    ad_per_op = 4 if bd == 8 else 2
+    valu_ops = blocks4 * span * span * work.num_refs * 16 / ad_per_op + cands
     valu_frac = valu_ops / (k_ms * 1e-3) / VALU_PEAK_OPS
-    traffic = None
+    valu_ops4 = blocks4 * span * span * work.num_refs * 16 / 4 + cands
+    # HBM bytes per launch of the integer search, from the rocprofv3 counter
+    # runs summarised in profiles/ (measured with the schedule they name)
+    # (the entry of the same grid, the same schedule if there is one)
+    traffic, traffic_sched = None, None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % work.name)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            entries = json.load(f).get("entries", [])
+        sched = {"band_rows": args.band_rows, "waves": args.waves, "dx_cols": args.dx_cols, "int_grid": args.int_grid}
+        same = [e for e in entries if e["schedule"] == sched] or \
+               [e for e in entries if e["schedule"]["int_grid"] == args.int_grid]
+        if same:
+            traffic, traffic_sched = same[0]["bytes_per_launch"], same[0]["schedule"]
 
     # ---- the reference's own search (me_search_tz) on the GPU ----------------
     search_tz = None
@@ -665,9 +676,15 @@ def main():
         "stage_ms": {"prepare_ref": stage[0], "fill": stage[1], "int_search": stage[2], "subpel": stage[3],
                      "mc": mc_ms},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_schedule": traffic_sched,
                      "kernel": "k_int_search", "algorithmic_bytes": alg_bytes, "kernel_ms": k_ms,
-                     "valu": {"ops": valu_ops, "frac": valu_frac, "peak_lane_ops_per_s": VALU_PEAK_OPS}},
+                     "valu": {"ops": valu_ops, "frac": valu_frac, "peak_lane_ops_per_s": VALU_PEAK_OPS,
+                              "abs_diffs_per_lane_op": ad_per_op,
+                              "frac_at_4_abs_diffs_per_op": valu_ops4 / (k_ms * 1e-3) / VALU_PEAK_OPS,
+                              "ops_note": "every candidate SAD composed from 4x4-block SADs (16 abs-diffs per "
+                                          "block and displacement, abs_diffs_per_lane_op per lane-op: 4 for 8-bit "
+                                          "samples with v_sad_u8, 2 for 10-bit with v_sad_u16) plus one compare "
+                                          "per PU candidate"}},
         "per_call": per_call,
         "end_to_end": e2e,
         "serving": serving,
