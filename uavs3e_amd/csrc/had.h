@@ -89,8 +89,9 @@ __device__ __forceinline__ void fwht(int *v, int step)
 // other intermediate of a transform of N samples is at most N/2 times the
 // largest residual: 8 bits, any tile and the 16x8 / 8x16 halves (residuals
 // within +-510): at most 16320; 10 bits, tiles up to 8x8 (residuals within
-// +-1023): at most 32736 -- both inside a signed 16-bit lane.  (The 10-bit
-// 16x8 / 8x16 halves, residuals to +-2046, stay on 32-bit lanes.)
+// +-1023): at most 32736 -- both inside a signed 16-bit lane.  The 10-bit
+// 16x8 / 8x16 halves (residuals to +-2046) take four stages on 16-bit lanes
+// (at most 32736) and their fifth on 32-bit lanes (had_pk_sum_half).
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 
 // NW dwords of the samples starting at p (any alignment).
@@ -199,6 +200,38 @@ __device__ __forceinline__ int had_pk_sum(s16x2 *v)
     return (int)(2 * s);
 }
 
+// had_pk_sum<8, 8> of the half of a 16x8 / 8x16 tile.  At 10 bits the half
+// residual reaches +-2046, so the column transform's last stage (rows y and
+// y + 4, at most 65472) and the pair stage run on 32-bit lanes: for the
+// pairs P, Q of rows y and y + 4, the four coefficients of their last two
+// stages sum to 2 max(|p.lo + q.lo|, |p.hi + q.hi|) +
+// 2 max(|p.lo - q.lo|, |p.hi - q.hi|).
+__device__ __forceinline__ int had_pk_sum_half(s16x2 *v)
+{
+#if BIT_DEPTH == 8
+    return had_pk_sum<8, 8>(v);
+#else
+#pragma unroll
+    for (int y = 0; y < 8; y++) fwht_pk<4>(v + y * 4, 1);
+    // the column stages within rows 0-3 and within rows 4-7
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        fwht_pk<4>(v + k, 4);
+        fwht_pk<4>(v + k + 16, 4);
+    }
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const s16x2 p = v[i], q = v[i + 16];   // rows y and y + 4 of column pair k
+        const int plo = p.x, phi = p.y, qlo = q.x, qhi = q.y;
+        const int a = plo + qlo, b = phi + qhi, c = plo - qlo, d = phi - qhi;
+        s += __builtin_elementwise_max(__builtin_elementwise_max(a, -a), __builtin_elementwise_max(b, -b)) +
+             __builtin_elementwise_max(__builtin_elementwise_max(c, -c), __builtin_elementwise_max(d, -d));
+    }
+    return 2 * s;
+#endif
+}
+
 // Sum of |coef| of the 2-D Walsh-Hadamard transform of a TW x TH residual.
 template <int TW, int TH>
 __device__ __forceinline__ int had_abs(const pel *org, int so, const pel *cur, int sc)
@@ -218,7 +251,6 @@ __device__ __forceinline__ int had_half(const pel *org, int so, const pel *cur, 
 {
     const int sg = half ? -1 : 1;
     const int o2 = TT == 3 ? 8 : 8 * so, c2 = TT == 3 ? 8 : 8 * sc;
-#if BIT_DEPTH == 8
     s16x2 v[32];
     const s16x2 sgv = (s16x2)(short)sg;
 #pragma unroll
@@ -229,27 +261,7 @@ __device__ __forceinline__ int had_half(const pel *org, int so, const pel *cur, 
 #pragma unroll
         for (int k = 0; k < 4; k++) v[y * 4 + k] += b[k] * sgv;
     }
-    return had_pk_sum<8, 8>(v);
-#else
-    int m[64];
-#pragma unroll
-    for (int y = 0; y < 8; y++) {
-        int oa[8], ca[8], ob[8], cb[8];
-        load_row<8>(org + y * so, oa);
-        load_row<8>(cur + y * sc, ca);
-        load_row<8>(org + y * so + o2, ob);
-        load_row<8>(cur + y * sc + c2, cb);
-#pragma unroll
-        for (int x = 0; x < 8; x++) m[y * 8 + x] = (oa[x] - ca[x]) + sg * (ob[x] - cb[x]);
-        fwht<8>(m + y * 8, 1);
-    }
-#pragma unroll
-    for (int x = 0; x < 8; x++) fwht<8>(m + x, 8);
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < 64; i++) s += abs(m[i]);
-    return s;
-#endif
+    return had_pk_sum_half(v);
 }
 
 // One normalised cost_satd tile (com_cost.c:259, 340, 467, 586, 649, 708)
@@ -265,13 +277,12 @@ __device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, in
 
 // The same costs with the current block's pairs made once (a sub-pel search
 // compares one current block with 17 predictions): ORG_PAIRS<TT> pairs per
-// lane, from org_pairs<TT>, then satd_pre<TT> / had_half_pre<TT> per
-// prediction.  The 10-bit 16x8 / 8x16 halves keep had_half.
+// lane, from org_pairs<TT>, then satd_pre<TT> / had_pre<TT> per prediction.
 template <int TT> struct tile_dims { static constexpr int W = TT == 0 || TT == 4 ? 4 : 8, H = TT == 0 || TT == 2 ? 4 : 8; };
 template <int TT> constexpr int org_pairs_n() { return TT == 3 || TT == 5 ? 32 : tile_dims<TT>::W * tile_dims<TT>::H / 2; }
 
-// For TT 3/5 (8 bits) the pairs of the tile's half sum (half 0) or
-// difference (half 1) of its two 8x8 halves.
+// For TT 3/5 the pairs of the tile's half sum (half 0) or difference (half 1)
+// of its two 8x8 halves (at most 2046 at 10 bits).
 template <int TT>
 __device__ __forceinline__ void org_pairs(const pel *org, int so, int half, s16x2 *o)
 {
@@ -314,7 +325,7 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 #pragma unroll
                 for (int k = 0; k < 4; k++) v[y * 4 + k] = o[y * 4 + k] - (a[k] + b[k] * sgv);
             }
-            return had_pk_sum<8, 8>(v);
+            return had_pk_sum_half(v);
         } else {
             constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
 #pragma unroll
@@ -338,7 +349,7 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 #pragma unroll
             for (int k = 0; k < 4; k++) v[y * 4 + k] = o[y * 4 + k] - (a[k] + b[k] * sgv);
         }
-        return had_pk_sum<8, 8>(v);
+        return had_pk_sum_half(v);
     } else {
         constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
 #pragma unroll

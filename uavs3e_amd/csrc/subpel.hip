@@ -65,11 +65,9 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
     const int tx = (t % S.tpr) * S.tw, ty = (t / S.tpr) * S.th;
     const int toff_o = ty * P.pic_w + tx, toff_p = ty * P.ref_stride + tx;
     const int red_w = LPU < 64 ? LPU : 64;
-    // the lane's current tile as 16-bit pairs, made once for all 17
-    // positions (the 10-bit 16x8 / 8x16 halves read it per position)
-    constexpr bool PRE = !SPLIT || BIT_DEPTH == 8;
-    s16x2 opairs[PRE ? org_pairs_n<TT>() : 1];
-    if (PRE && active) org_pairs<TT>(org + toff_o, P.pic_w, li & 1, opairs);
+    // the lane's current tile as 16-bit pairs, made once for all 17 positions
+    s16x2 opairs[org_pairs_n<TT>()];
+    if (active) org_pairs<TT>(org + toff_o, P.pic_w, li & 1, opairs);
     // com_had of the unit at pred, in every lane of the group; run = 0 skips
     // the sample reads (the sum is then 0) but not the barriers
     auto had = [&](const pel *pred, bool run) -> u32 {
@@ -79,8 +77,7 @@ __device__ __forceinline__ void subpel_body(const SubArgs &a, const SubShape &S,
                 // the two lanes of a tile hold its sum and difference halves;
                 // the tile is normalised (com_cost.c:467, 586) before tiles
                 // are summed
-                const int hs = PRE ? had_pre<TT>(opairs, pred + toff_p, P.ref_stride, li & 1)
-                                   : had_half<TT>(org + toff_o, P.pic_w, pred + toff_p, P.ref_stride, li & 1);
+                const int hs = had_pre<TT>(opairs, pred + toff_p, P.ref_stride, li & 1);
                 const int s = hs + __builtin_amdgcn_mov_dpp(hs, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
                 v = (li & 1) ? 0 : me_satd_norm128(s);
             } else {
