@@ -190,6 +190,20 @@ __device__ __forceinline__ int had_pk_sum(s16x2 *v)
     for (int y = 0; y < TH; y++) fwht_pk<K>(v + y * K, 1);
 #pragma unroll
     for (int k = 0; k < K; k++) fwht_pk<TH>(v + k, K);
+#if BIT_DEPTH == 8
+    // The pair stage formed (every |lo +- hi| <= 32640): t = (lo + hi, lo - hi)
+    // in one packed multiply-add, and |t.lo| + |t.hi| summed by v_sad_u16
+    // against the sign bit (x ^ 0x8000 = x + 32768 for a signed 16-bit x).
+    u32 s = 0;
+    const s16x2 pm = {1, -1};
+#pragma unroll
+    for (int i = 0; i < K * TH; i++) {
+        const s16x2 t = v[i].yy * pm + v[i].xx;
+        s = __builtin_amdgcn_sad_u16(__builtin_bit_cast(u32, t) ^ 0x80008000u, 0x80008000u, s);
+    }
+    return (int)s;
+#else
+    // 10 bits: the pair stage could leave 16 bits, so 2 max(|lo|, |hi|)
     u32 s = 0;
 #pragma unroll
     for (int i = 0; i < K * TH; i++) {
@@ -198,6 +212,7 @@ __device__ __forceinline__ int had_pk_sum(s16x2 *v)
         s += lo > hi ? lo : hi;
     }
     return (int)(2 * s);
+#endif
 }
 
 // had_pk_sum<8, 8> of the half of a 16x8 / 8x16 tile.  At 10 bits the half

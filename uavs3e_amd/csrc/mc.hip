@@ -54,49 +54,8 @@ __device__ __forceinline__ void load4(const pel *p, int o[4])
 #endif
 }
 
-// One chroma sample (cx, cy) of a PU's (w/2) x (h/2) block predicted from
-// plane `pl` (cw x chh, unpadded) at luma quarter-pel position (qx, qy).
-__device__ __forceinline__ int mc_chroma_sample(const pel *pl, int cw, int chh, int pic_w, int pic_h, int qx, int qy,
-                                                int cx, int cy)
-{
-    const int fx = qx & 7, fy = qy & 7;
-    const int ix = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_w + 4) >> 1, qx >> 3) + cx;
-    const int iy = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_h + 4) >> 1, qy >> 3) + cy;
-    const int maxv = (1 << BIT_DEPTH) - 1;
-    auto S = [&](int xx, int yy) -> int {
-        return (int)pl[(size_t)me_clip3(0, chh - 1, yy) * cw + me_clip3(0, cw - 1, xx)];
-    };
-    if (fx == 0 && fy == 0) return S(ix, iy);
-    if (fy == 0) {
-        int t = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fx][k] * S(ix - 1 + k, iy);
-        return me_clip3(0, maxv, (t + 32) >> 6);
-    }
-    if (fx == 0) {
-        int t = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fy][k] * S(ix, iy - 1 + k);
-        return me_clip3(0, maxv, (t + 32) >> 6);
-    }
-    // 2-D: horizontal pass stored as s16, then vertical (com_mc.c:587-643;
-    // 8-bit: shift1 0, shift2 12; 10-bit: shift1 2, shift2 10)
-    const int shift1 = BIT_DEPTH == 8 ? 0 : 2, shift2 = BIT_DEPTH == 8 ? 12 : 10;
-    const int add1 = (1 << shift1) >> 1, add2 = 1 << (shift2 - 1);
-    int v = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int t = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fx][k] * S(ix - 1 + k, iy - 1 + j);
-        v += ME_CHROMA_TAPS[fy][j] * (int)(s16)((t + add1) >> shift1);
-    }
-    return me_clip3(0, maxv, (v + add2) >> shift2);
-}
-
 // grid (16, jobs): 4096 threads per (CTU, shape); thread t writes luma
-// samples 4t..4t+3 of the shape's 16384 and, for chroma shapes, chroma
-// sample t of u and of v.
+// samples 4t..4t+3 of the shape's 16384 (chroma: k_mc_chroma).
 __global__ __launch_bounds__(256) void k_mc(McArgs a)
 {
     const int job = blockIdx.y;
@@ -141,32 +100,123 @@ __global__ __launch_bounds__(256) void k_mc(McArgs a)
 #endif
     }
 
-    // chroma
-    if (a.chroma) {
-        const int cw = a.pw / 2, chh = a.ph / 2, area = cw * chh;
-        const int i = t / area, q = t - i * area;
-        const int row = q / cw, col = q - row * cw;
-        const int x = cx0 + (i % cols) * a.pw, y = cy0 + (i / cols) * a.ph;
-        const s8 *bp = a.best + (pu0 + i) * a.nl;
-        const int pcw = P.pic_w / 2, pch = P.pic_h / 2;
-        int u = 0, v = 0;
-        bool ok = true;
-        for (int l = 0; l < a.nl; l++) ok &= bp[l] >= 0;
-        if (ok) {
-            for (int l = 0; l < a.nl; l++) {
-                const int r = bp[l];
-                const me_pu_result_t &o = a.res[(pu0 + i) * a.nref + r];
-                const int qx = (x << 2) + o.mv_sub[0], qy = (y << 2) + o.mv_sub[1];
-                const pel *pu = P.chroma[r], *pv = pu + (size_t)pcw * pch;
-                const int su = mc_chroma_sample(pu, pcw, pch, P.pic_w, P.pic_h, qx, qy, col, row);
-                const int sv = mc_chroma_sample(pv, pcw, pch, P.pic_w, P.pic_h, qx, qy, col, row);
-                u = l ? (u + su + 1) >> 1 : su;
-                v = l ? (v + sv + 1) >> 1 : sv;
+}
+
+// Four vertically adjacent chroma samples (rows row0 .. row0+3, column col)
+// of a PU's (w/2) x (h/2) block predicted from plane `pl` (cw x chh,
+// unpadded) at luma quarter-pel position (qx, qy), com_mc_blk_chroma: the
+// position read as 1/8-pel, its integer part clipped, then copy / H_4 / V_4
+// / 2-D, the 2-D filter's horizontal pass made once per input row for all
+// four outputs (each output is the reference's own sum).  The 7 x 4
+// input window (rows row0-1 .. row0+5, columns col-1 .. col+2) is read with
+// dword loads away from the plane's edges, sample by sample with clamped
+// coordinates near them.
+__device__ __forceinline__ void mc_chroma_col4(const pel *pl, int cw, int chh, int pic_w, int pic_h, int qx, int qy,
+                                               int col, int row0, int out[4])
+{
+    const int fx = qx & 7, fy = qy & 7;
+    const int ix = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_w + 4) >> 1, qx >> 3) + col;
+    const int iy = me_clip3(-(ME_MAX_CU >> 1) - 2, (pic_h + 4) >> 1, qy >> 3) + row0;
+    const int maxv = (1 << BIT_DEPTH) - 1;
+    const int x0 = ix - 1, y0 = iy - 1;
+    int W[7][4];
+    // load4 may read 2 samples past the fourth: keep that inside the row
+    if (x0 >= 0 && x0 + 5 <= cw - 1 && y0 >= 0 && y0 + 6 <= chh - 1) {
+#pragma unroll
+        for (int j = 0; j < 7; j++) load4(pl + (size_t)(y0 + j) * cw + x0, W[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            const pel *rp = pl + (size_t)me_clip3(0, chh - 1, y0 + j) * cw;
+#pragma unroll
+            for (int k = 0; k < 4; k++) W[j][k] = (int)rp[me_clip3(0, cw - 1, x0 + k)];
+        }
+    }
+    if (fx == 0 && fy == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) out[r] = W[r + 1][1];
+    } else if (fy == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int t = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fx][k] * W[r + 1][k];
+            out[r] = me_clip3(0, maxv, (t + 32) >> 6);
+        }
+    } else if (fx == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int t = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) t += ME_CHROMA_TAPS[fy][j] * W[r + j][1];
+            out[r] = me_clip3(0, maxv, (t + 32) >> 6);
+        }
+    } else {
+        // 2-D: horizontal pass stored as s16, then vertical (com_mc.c:587-643;
+        // 8-bit: shift1 0, shift2 12; 10-bit: shift1 2, shift2 10)
+        const int shift1 = BIT_DEPTH == 8 ? 0 : 2, shift2 = BIT_DEPTH == 8 ? 12 : 10;
+        const int add1 = (1 << shift1) >> 1, add2 = 1 << (shift2 - 1);
+        int h[7];
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            int t = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) t += ME_CHROMA_TAPS[fx][k] * W[j][k];
+            h[j] = (int)(s16)((t + add1) >> shift1);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int v = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) v += ME_CHROMA_TAPS[fy][j] * h[r + j];
+            out[r] = me_clip3(0, maxv, (v + add2) >> shift2);
+        }
+    }
+}
+
+// grid (4, jobs): 1024 threads per (CTU, shape with chroma); thread t
+// writes a column of 4 samples of u and of v: PU i = t / (cw * ch / 4), then
+// row group and column inside the PU's cw x ch chroma block.
+__global__ __launch_bounds__(256) void k_mc_chroma(McArgs a)
+{
+    const int job = blockIdx.y;
+    int ctu;
+    const PicDesc &P = *job_decode(a.js, job, ctu);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int cx0 = (ctu % P.ctu_cols) * 128, cy0 = (ctu / P.ctu_cols) * 128;
+    const int cols = 128 / a.pw;
+    const size_t pu0 = (size_t)job * a.npu + a.base;
+    const int cw = a.pw / 2, chh = a.ph / 2, area = cw * chh;
+    const int per_pu = area / 4;
+    const int i = t / per_pu, q = t - i * per_pu;
+    const int rg = q / cw, col = q - rg * cw, row0 = rg * 4;
+    const int x = cx0 + (i % cols) * a.pw, y = cy0 + (i / cols) * a.ph;
+    const s8 *bp = a.best + (pu0 + i) * a.nl;
+    const int pcw = P.pic_w / 2, pch = P.pic_h / 2;
+    int u[4] = {0, 0, 0, 0}, v[4] = {0, 0, 0, 0};
+    bool ok = true;
+    for (int l = 0; l < a.nl; l++) ok &= bp[l] >= 0;
+    if (ok) {
+        for (int l = 0; l < a.nl; l++) {
+            const int r = bp[l];
+            const me_pu_result_t &o = a.res[(pu0 + i) * a.nref + r];
+            const int qx = (x << 2) + o.mv_sub[0], qy = (y << 2) + o.mv_sub[1];
+            const pel *pu = P.chroma[r], *pv = pu + (size_t)pcw * pch;
+            int su[4], sv[4];
+            mc_chroma_col4(pu, pcw, pch, P.pic_w, P.pic_h, qx, qy, col, row0, su);
+            mc_chroma_col4(pv, pcw, pch, P.pic_w, P.pic_h, qx, qy, col, row0, sv);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                u[k] = l ? (u[k] + su[k] + 1) >> 1 : su[k];
+                v[k] = l ? (v[k] + sv[k] + 1) >> 1 : sv[k];
             }
         }
-        const size_t o = (size_t)job * a.chroma_ctu + a.chroma_off + (size_t)i * area + q;
-        a.pu[o] = (pel)u;
-        a.pv[o] = (pel)v;
+    }
+    const size_t o = (size_t)job * a.chroma_ctu + a.chroma_off + (size_t)i * area + (size_t)row0 * cw + col;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        a.pu[o + (size_t)k * cw] = (pel)u[k];
+        a.pv[o + (size_t)k * cw] = (pel)v[k];
     }
 }
 
@@ -211,6 +261,10 @@ int me_mc_enqueue(const MeSearch *s, const JobSet &js, const me_pu_result_t *res
         a.chroma = with_chroma && has_c;
         hipLaunchKernelGGL(k_mc, dim3(16, js.n_jobs), dim3(256), 0, st, a);
         ME_CHECK(hipGetLastError());
+        if (a.chroma) {
+            hipLaunchKernelGGL(k_mc_chroma, dim3(4, js.n_jobs), dim3(256), 0, st, a);
+            ME_CHECK(hipGetLastError());
+        }
         lo += 16384;
         co += has_c ? 4096 : 0;
     }
