@@ -566,6 +566,10 @@ class MeHost:
         L.me_synth_uniform.argtypes = [C.c_uint64, C.c_uint64]
         L.me_synth_sin.restype = C.c_double
         L.me_synth_sin.argtypes = [C.c_double]
+        L.me_synth_ln.restype = C.c_double
+        L.me_synth_ln.argtypes = [C.c_double]
+        L.me_synth_gauss.restype = C.c_double
+        L.me_synth_gauss.argtypes = [C.c_uint64, C.c_uint64]
         L.me_synth_region_offset.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                              C.c_void_p]
         L.me_host_frame.restype = C.c_double

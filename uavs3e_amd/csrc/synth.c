@@ -3,13 +3,15 @@
  *
  * Frames are a sum of six low-frequency 2-D sinusoids plus noise, sampled
  * analytically at motion-shifted coordinates (so motion needs no
- * interpolation).  Everything is computed with IEEE double +, -, *, / only
- * (no libm calls, no FMA contraction: build with -ffp-contract=off), so a
- * seed gives the same samples on any host.
+ * interpolation).  Everything is computed with IEEE double +, -, *, / and
+ * sqrt (all correctly rounded; no other libm call, no FMA contraction: build
+ * with -ffp-contract=off), so a seed gives the same samples on any host.
  *
  *   random stream  splitmix64, counter based, keyed by (seed, stream, frame,
  *                  purpose)
- *   noise          Irwin-Hall: sum of 12 U[0,1) minus 6 (mean 0, var 1)
+ *   noise          standard normal by the Box-Muller transform,
+ *                  sqrt(-2 ln u1) cos(2 pi u2), with this file's own ln and
+ *                  sin (series evaluated in IEEE double)
  *   sinusoid k     A_k in [20,60] (x4 at 10 bit), f_k in [1/256, 1/32]
  *                  cycles/px, orientation in [0,pi), phase in [0,2pi)
  *   sample         clip(floor(DC + sum_k A_k sin(2 pi f_k (x c_k + y s_k)
@@ -46,15 +48,10 @@ static double uni(u64 key, u64 i) { return (double)(mix(key + i * 0xD1B54A32D192
 
 EXPORT double me_synth_uniform(u64 key, u64 i) { return uni(key, i); }
 
-static double gauss(u64 key, u64 i)
-{
-    double s = 0;
-    for (int j = 0; j < 12; j++) s += uni(key, 12 * i + (u64)j);
-    return s - 6.0;
-}
-
 static const double PI = 3.141592653589793115997963468544185161590576171875;
 static const double TWO_PI = 6.28318530717958623199592693708837032318115234375;
+static const double LN2 = 0.69314718055994528622676398299518041312694549560546875;
+static const double SQRT2 = 1.4142135623730951454746218587388284504413604736328125;
 
 /* sin on [-pi/2, pi/2]: Taylor series to x^17 (truncation < 1e-11). */
 static double sin_poly(double x)
@@ -84,6 +81,35 @@ EXPORT double me_synth_sin(double x)
 }
 
 static double cos_(double x) { return me_synth_sin(x + PI / 2); }
+
+/* ln x for a normal double x > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2))
+ * (exponent and mantissa taken from the bits, exactly), then
+ * ln m = 2 atanh(t), t = (m - 1) / (m + 1), |t| < 0.172, as the odd series
+ * to t^29 (truncation below 1e-20). */
+EXPORT double me_synth_ln(double x)
+{
+    u64 b;
+    memcpy(&b, &x, sizeof(b));
+    int e = (int)((b >> 52) & 0x7FF) - 1023;
+    b = (b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    double m;
+    memcpy(&m, &b, sizeof(m));
+    if (m > SQRT2) {
+        m *= 0.5;
+        e++;
+    }
+    double t = (m - 1.0) / (m + 1.0), t2 = t * t;
+    double s = 1.0 / 29.0;
+    for (int k = 27; k >= 1; k -= 2) s = 1.0 / k + t2 * s;
+    return (double)e * LN2 + 2.0 * t * s;
+}
+
+/* Standard normal sample i of stream `key` (Box-Muller; u1 in (0, 1]). */
+EXPORT double me_synth_gauss(u64 key, u64 i)
+{
+    double u1 = 1.0 - uni(key, 2 * i), u2 = uni(key, 2 * i + 1);
+    return __builtin_sqrt(-2.0 * me_synth_ln(u1)) * cos_(TWO_PI * u2);
+}
 
 #define NS 6
 typedef struct {
@@ -144,7 +170,7 @@ EXPORT void me_synth_frame(u64 seed, int stream, int frame, int w, int h, int bd
                 ry = oy / 4.0;
             }
             double v = field_at(&f, x + sx + rx, y + sy + ry) + offset;
-            if (sigma != 0.0) v += sigma * gauss(kn, (u64)y * (u64)w + (u64)x);
+            if (sigma != 0.0) v += sigma * me_synth_gauss(kn, (u64)y * (u64)w + (u64)x);
             double fl = v + 0.5;
             long long iv = (long long)fl;
             if ((double)iv > fl) iv--;
