@@ -517,6 +517,14 @@ __device__ __forceinline__ u32x4a rate4_at(const IntArgs &a, int i)
 
 // One pass over the band's displacements: PASS 3 = every PU class, 2 =
 // 16x16 only.  Updates the node states in st.
+//
+// The band's (row, step) pairs -- rows inside [lo, hi], steps of four
+// displacements from the first that reaches lo -- are split in row-major
+// order into WAVES contiguous ranges of equal length, one per wave.  A full
+// band of band_rows = k * WAVES rows gives each wave k whole rows; a short
+// band (a segment's last, 2R+1 being odd) gives each wave a part of a row
+// instead of leaving all but one wave idle.  Keys carry their raster index,
+// so the order in which a node's candidates are met does not matter.
 template <int PASS, int WAVES, bool EDGE, bool SH0, int NS>
 __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, const Block &b, const pel *win, const pel *curl,
                                          double *st, int tid)
@@ -527,9 +535,20 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
     const int rx = lane & 7, ry = lane >> 3, X = rx * 16, Y = ry * 16;
     const int wlim = P.pic_w + 126, hlim = P.pic_h + 126;
     const int span = 2 * a.R + 1;
-    for (int row = wave; row < a.band_rows; row += WAVES) {
+    const int r0 = a.lo - b.dy0 > 0 ? a.lo - b.dy0 : 0;
+    const int r1 = a.hi - b.dy0 + 1 < a.band_rows ? a.hi - b.dy0 + 1 : a.band_rows;
+    const int s0 = a.lo - 3 - b.dxs > 0 ? (a.lo - 3 - b.dxs + 3) >> 2 : 0;   // first step with dxb + 3 >= lo
+    const int S = ((b.dxe - b.dxs) >> 2) + 1 - s0;                          // steps per row
+    const int total = r1 > r0 && S > 0 ? (r1 - r0) * S : 0;
+    int p = total * wave / WAVES;
+    const int pe = total * (wave + 1) / WAVES;
+    while (p < pe) {
+        const int rr = p / S, sb = p - rr * S;
+        const int n = S - sb < pe - p ? S - sb : pe - p;
+        p += n;
+        const int row = r0 + rr;
         const int dy = b.dy0 + row;
-        if (dy < a.lo || dy > a.hi) continue;
+        const int dxs = b.dxs + 4 * (s0 + sb), dxl = dxs + 4 * (n - 1);   // the chunk's first and last step
         const int dyi = dy + a.R;
         const u32 ry_rate = rate_at(a, dyi);
         // The key parts of the step's four displacements.  A displacement
@@ -552,9 +571,10 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
             }
             return kc;
         };
-        KeyCtx kc = make_keys(b.dxs, rate4_at(a, b.dxs + a.R));
-        for (int dxb = b.dxs; dxb <= b.dxe; dxb += 4) {
-            const u32x4a rnext = rate4_at(a, dxb + 4 + a.R);
+        KeyCtx kc = make_keys(dxs, rate4_at(a, dxs + a.R));
+        for (int dxb = dxs; dxb <= dxl; dxb += 4) {
+            // always true from s0 on; the branch keeps the step's search in a
+            // block of its own, which the scheduler orders better
             if (dxb + 3 >= a.lo) {
 #if BIT_DEPTH == 8
                 Val L[4][4];
@@ -597,8 +617,9 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                 if (PASS == 3) sparse_scatter<EDGE, SH0>(s16, t32x8, t8x32, st + N_SMALL, kc, rx, ry, b.cx, b.cy, wlim, hlim);
 #endif
             }
+            // the next step's rates are loaded only here (see make_keys)
             __builtin_amdgcn_sched_barrier(0);
-            kc = make_keys(dxb + 4, rnext);
+            kc = make_keys(dxb + 4, rate4_at(a, dxb + 4 + a.R));
         }
     }
 }
@@ -713,57 +734,104 @@ __device__ __forceinline__ bool merge_waves(double *st, unsigned char *smem)
 
 // Stages the reference window of band b in LDS: rows cy + dy0 ..
 // cy + dy0 + 128 + band_rows - 2, columns cx + dxs .. cx + dxs + wcols - 1,
-// a dword (4 / sizeof(pel) samples) per thread and step.  Samples are
+// a dword (4 / sizeof(pel) samples) per thread and item.  Samples are
 // clamped to the padded plane (it is edge-replicated, so clamping is exact);
 // away from the plane's edges a dword is realigned from two aligned loads.
+// A thread issues the loads of SB items before it stores any of them: the
+// planes are read through global (not generic) pointers, which cannot alias
+// the LDS, so the loads stay in flight together instead of one memory round
+// trip per item.
+typedef const __attribute__((address_space(1))) pel *gpel_p;
+typedef const __attribute__((address_space(1))) u32 *gu32_p;
+#define SB 8
 __device__ __forceinline__ void stage_window(const IntArgs &a, const PicDesc &P, const Block &b, pel *win, int nthr,
                                              int tid)
 {
     constexpr int SPD = 4 / (int)sizeof(pel);
     const int wrows = 128 + a.band_rows - 1;
-    const pel *ref0 = P.planes[b.ref] + (size_t)ME_PAD * P.ref_stride + ME_PAD;
+    const gpel_p ref0 = (gpel_p)P.planes[b.ref] + (size_t)ME_PAD * P.ref_stride + ME_PAD;
     const int xmin = -ME_PAD, xmax = P.pic_w + ME_PAD - 1, ymin = -ME_PAD, ymax = P.pic_h + ME_PAD - 1;
     const int wq = a.wcols / SPD, dq = nthr % wq, dr = nthr / wq;
     int r = tid / wq, q = tid - r * wq;
-    for (; r < wrows; r += dr, q += dq) {
-        if (q >= wq) { q -= wq; r++; if (r >= wrows) break; }
-        const int x0 = b.cx + b.dxs + q * SPD, y = me_clip3(ymin, ymax, b.cy + b.dy0 + r);
-        const pel *src = ref0 + (ptrdiff_t)y * P.ref_stride;
-        u32 v;
-        if (x0 >= xmin && x0 + SPD - 1 <= xmax) {
-            const uintptr_t ad = (uintptr_t)(src + x0);
-            const u32 *qa = (const u32 *)(ad & ~(uintptr_t)3);
-            v = __builtin_amdgcn_alignbyte(qa[1], qa[0], (u32)(ad & 3));
-        } else {
-            v = 0;
+    while (r < wrows) {
+        // the loads of the batch from straight-line code: every item reads
+        // an address inside the padded plane (items past the window repeat
+        // its last row; items off the plane's sides read a clamped position
+        // and are redone sample by sample below)
+        u64 w[SB];
+        u32 sh[SB];
+        int o[SB];
+        bool side = false;
 #pragma unroll
-            for (int k = 0; k < SPD; k++)
-                v |= (u32)src[me_clip3(xmin, xmax, x0 + k)] << (8 * (int)sizeof(pel) * k);
+        for (int j = 0; j < SB; j++) {
+            const int x0 = b.cx + b.dxs + q * SPD;
+            const int rc = r < wrows ? r : wrows - 1;
+            const int y = me_clip3(ymin, ymax, b.cy + b.dy0 + rc);
+            const int xc = me_clip3(xmin, xmax - SPD + 1, x0);
+            const uintptr_t ad = (uintptr_t)(ref0 + (ptrdiff_t)y * P.ref_stride + xc);
+            const gu32_p qa = (gu32_p)(ad & ~(uintptr_t)3);
+            w[j] = ((u64)qa[1] << 32) | qa[0];
+            sh[j] = (u32)(ad & 3);
+            o[j] = r < wrows ? r * a.wpitch + wskew(r) + q * SPD : -1;
+            side |= r < wrows && xc != x0;
+            r += dr;
+            q += dq;
+            if (q >= wq) { q -= wq; r++; }
         }
-        *(u32 *)(win + r * a.wpitch + wskew(r) + q * SPD) = v;
+#pragma unroll
+        for (int j = 0; j < SB; j++)
+            if (o[j] >= 0) *(u32 *)(win + o[j]) = __builtin_amdgcn_alignbyte((u32)(w[j] >> 32), (u32)w[j], sh[j]);
+        if (side) {
+            // the batch's items off the plane's sides (rare: windows reaching
+            // more than the padding past the picture), sample by sample
+#pragma unroll
+            for (int j = 0; j < SB; j++) {
+                if (o[j] < 0) continue;
+                const int rj = o[j] / a.wpitch, qj = (o[j] - rj * a.wpitch - wskew(rj)) / SPD;
+                const int x0 = b.cx + b.dxs + qj * SPD;
+                if (x0 >= xmin && x0 + SPD - 1 <= xmax) continue;
+                const gpel_p src = ref0 + (ptrdiff_t)me_clip3(ymin, ymax, b.cy + b.dy0 + rj) * P.ref_stride;
+                u32 v = 0;
+#pragma unroll
+                for (int k = 0; k < SPD; k++) v |= (u32)src[me_clip3(xmin, xmax, x0 + k)] << (8 * (int)sizeof(pel) * k);
+                *(u32 *)(win + o[j]) = v;
+            }
+        }
     }
 }
 
-// Stages the current CTU (clamped to the picture) in LDS, 128 x 128 samples.
+// Stages the current CTU (clamped to the picture) in LDS, 128 x 128 samples,
+// SB loads in flight per thread as above.
 __device__ __forceinline__ void stage_cur(const PicDesc &P, const Block &b, pel *curl, int nthr, int tid)
 {
     constexpr int SPD = 4 / (int)sizeof(pel);
     constexpr int CQ = 128 / SPD;   // dwords per CTU row
-    const int q = tid % CQ;
+    const int q = tid % CQ, rstep = nthr / CQ;
     const bool inner = b.cx + 127 <= P.pic_w - 1;
-    for (int r = tid / CQ; r < 128; r += nthr / CQ) {
-        const int y = me_clip3(0, P.pic_h - 1, b.cy + r), x0 = b.cx + q * SPD;
-        const pel *src = P.cur + (size_t)y * P.pic_w;
-        u32 v;
-        if (inner) {
-            v = *(const u32 *)(src + x0);
-        } else {
-            v = 0;
+    const gpel_p cur = (gpel_p)P.cur;
+    for (int r0 = tid / CQ; r0 < 128; r0 += SB * rstep) {
+        u32 v[SB];
 #pragma unroll
-            for (int k = 0; k < SPD; k++)
-                v |= (u32)src[me_clip3(0, P.pic_w - 1, x0 + k)] << (8 * (int)sizeof(pel) * k);
+        for (int j = 0; j < SB; j++) {
+            const int r = r0 + j * rstep;
+            v[j] = 0;
+            if (r < 128) {
+                const int y = me_clip3(0, P.pic_h - 1, b.cy + r), x0 = b.cx + q * SPD;
+                const gpel_p src = cur + (size_t)y * P.pic_w;
+                if (inner) {
+                    v[j] = *(gu32_p)(src + x0);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < SPD; k++)
+                        v[j] |= (u32)src[me_clip3(0, P.pic_w - 1, x0 + k)] << (8 * (int)sizeof(pel) * k);
+                }
+            }
         }
-        *(u32 *)(curl + r * 128 + q * SPD) = v;
+#pragma unroll
+        for (int j = 0; j < SB; j++) {
+            const int r = r0 + j * rstep;
+            if (r < 128) *(u32 *)(curl + r * 128 + q * SPD) = v[j];
+        }
     }
 }
 
