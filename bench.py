@@ -632,7 +632,8 @@ def main():
         from uavs3e_amd import build as ub
         pk = ctypes.CDLL(ub.peak_lib())
         pk.me_peak_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
-        names = ["v_add_u32", "v_sad_u8", "v_qsad_pk_u16_u8", "v_sad_u16", "v_min_f64", "v_add_u32_dpp"]
+        names = ["v_add_u32", "v_sad_u8", "v_qsad_pk_u16_u8", "v_sad_u16", "v_min_f64", "v_add_u32_dpp",
+                 "v_lshl_add_u64", "v_add_u32_sdwa"]
         valu_peak = {"note": "lane-instructions/s of one instruction kind: 8 independent chains per lane, 2048 "
                              "workgroups of 256 lanes (8 waves per SIMD), HIP events; per CU and clock = "
                              "value / (256 CUs x clock)"}

@@ -19,21 +19,25 @@
 
 // A step of the search covers four displacements dxb .. dxb+3 of one row
 // (NKEY).  Values of NK displacements of one node: at 8 bits all four are
-// packed as u16 lanes (lo = d0,d1; hi = d2,d3) -- a sum of at most 256
-// samples stays below 2^16, so plain 32-bit adds never carry between lanes;
-// at 10 bits one displacement per u32 (a step then holds four of them).
+// packed as the u16 lanes of one 64-bit value (bits 0-15 d0 ... 48-63 d3) --
+// a sum of at most 256 samples stays below 2^16, so a sum of two values is
+// one 64-bit add (v_lshl_add_u64) that never carries between lanes; at 10
+// bits one displacement per u32 (a step then holds four of them).
 #define NKEY 4
 #if BIT_DEPTH == 8
 #define NK 4
 struct Val {
-    u32 lo, hi;
+    u64 v;
+    __device__ __forceinline__ u32 lo() const { return (u32)v; }
+    __device__ __forceinline__ u32 hi() const { return (u32)(v >> 32); }
     __device__ __forceinline__ u32 get(int k) const
     {
-        const u32 w = k < 2 ? lo : hi;
+        const u32 w = k < 2 ? lo() : hi();
         return (k & 1) ? (w >> 16) : (w & 0xFFFFu);
     }
 };
-__device__ __forceinline__ Val operator+(Val a, Val b) { return {a.lo + b.lo, a.hi + b.hi}; }
+__device__ __forceinline__ Val mkval(u32 lo, u32 hi) { return {((u64)hi << 32) | lo}; }
+__device__ __forceinline__ Val operator+(Val a, Val b) { return {a.v + b.v}; }
 __device__ __forceinline__ Val vsel(bool c, Val a, Val b) { return c ? a : b; }
 #else
 #define NK 1
@@ -106,12 +110,14 @@ __device__ __forceinline__ u32 px4(u32 v) { return dppx<0x141>(v); }
 __device__ __forceinline__ u32 py1(u32 v) { return dppx<0x128>(v); }
 __device__ __forceinline__ u32 py2(u32 v) { return (u32)__builtin_amdgcn_ds_swizzle((int)v, 0x401F); }
 __device__ __forceinline__ u32 py4(u32 v) { return (u32)__shfl_xor((int)v, 32, 64); }
+// A value plus its x- or y-partner's: adds on 32-bit halves, which take
+// the partner's value as a DPP operand.
 #if BIT_DEPTH == 8
-__device__ __forceinline__ Val vpx1(Val a) { return {px1(a.lo), px1(a.hi)}; }
-__device__ __forceinline__ Val vpy1(Val a) { return {py1(a.lo), py1(a.hi)}; }
+__device__ __forceinline__ Val vaddx1(Val a) { return mkval(a.lo() + px1(a.lo()), a.hi() + px1(a.hi())); }
+__device__ __forceinline__ Val vaddy1(Val a) { return mkval(a.lo() + py1(a.lo()), a.hi() + py1(a.hi())); }
 #else
-__device__ __forceinline__ Val vpx1(Val a) { return {px1(a.v)}; }
-__device__ __forceinline__ Val vpy1(Val a) { return {py1(a.v)}; }
+__device__ __forceinline__ Val vaddx1(Val a) { return {a.v + px1(a.v)}; }
+__device__ __forceinline__ Val vaddy1(Val a) { return {a.v + py1(a.v)}; }
 #endif
 
 // Region-level composition from the 16 leaves L[r][c] (4x4 blocks, row r,
@@ -212,7 +218,7 @@ __device__ __forceinline__ KeyV key_k(const KeyCtx &c, int k) { return {c.khi[k]
 #if BIT_DEPTH == 8
 typedef Val Quad;
 typedef u32 Pair;
-__device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return p ? q.hi : q.lo; }
+__device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return p ? q.hi() : q.lo(); }
 __device__ __forceinline__ u32 pair_elem(Pair v, int j) { return j ? (v >> 16) : (v & 0xFFFFu); }
 template <typename F> __device__ __forceinline__ Pair pair_map(Pair v, F f) { return f(v); }
 #else
@@ -377,13 +383,13 @@ __device__ __forceinline__ void pass_dense(const Region &g, double *st, const Ke
     const int x2 = cx + (X & ~31), y2 = cy + (Y & ~31);
     Val t32x4[4], t4x32[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) t32x4[r] = g.h16[r] + vpx1(g.h16[r]);
+    for (int r = 0; r < 4; r++) t32x4[r] = vaddx1(g.h16[r]);
 #pragma unroll
-    for (int r2 = 0; r2 < 2; r2++) t32x8[r2] = g.h16x8[r2] + vpx1(g.h16x8[r2]);
+    for (int r2 = 0; r2 < 2; r2++) t32x8[r2] = vaddx1(g.h16x8[r2]);
 #pragma unroll
-    for (int cc = 0; cc < 4; cc++) t4x32[cc] = g.v16[cc] + vpy1(g.v16[cc]);
+    for (int cc = 0; cc < 4; cc++) t4x32[cc] = vaddy1(g.v16[cc]);
 #pragma unroll
-    for (int c2 = 0; c2 < 2; c2++) t8x32[c2] = g.v8x16[c2] + vpy1(g.v8x16[c2]);
+    for (int c2 = 0; c2 < 2; c2++) t8x32[c2] = vaddy1(g.v8x16[c2]);
     const Val d0 = vsel(xr, t32x4[2], t32x4[0]), d1 = vsel(xr, t32x4[3], t32x4[1]), d2 = vsel(xr, t32x8[1], t32x8[0]);
     const Val d3 = vsel(yr, t4x32[2], t4x32[0]), d4 = vsel(yr, t4x32[3], t4x32[1]), d5 = vsel(yr, t8x32[1], t8x32[0]);
 #pragma unroll
@@ -445,7 +451,7 @@ __device__ __forceinline__ void leaves(const pel *win, int wcols, int wrow0, int
 #pragma unroll
     for (int r = 0; r < 4; r++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) L[r][c] = {(u32)acc[r][c], (u32)(acc[r][c] >> 32)};
+        for (int c = 0; c < 4; c++) L[r][c] = {acc[r][c]};
 }
 #else
 // v_sad_u16 on sample pairs.  The step's four displacements dxb + k share

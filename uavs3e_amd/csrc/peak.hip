@@ -15,7 +15,7 @@
 typedef uint32_t u32;
 typedef uint64_t u64;
 
-enum { K_ADD_U32 = 0, K_SAD_U8, K_QSAD_PK, K_SAD_U16, K_MIN_F64, K_ADD_DPP, K_NKINDS };
+enum { K_ADD_U32 = 0, K_SAD_U8, K_QSAD_PK, K_SAD_U16, K_MIN_F64, K_ADD_DPP, K_LSHL_ADD_U64, K_ADD_SDWA, K_NKINDS };
 
 template <int KIND>
 __global__ __launch_bounds__(256) void k_peak(u32 seed, int iters, u32 *out)
@@ -45,6 +45,10 @@ __global__ __launch_bounds__(256) void k_peak(u32 seed, int iters, u32 *out)
             if (KIND == K_ADD_DPP)
                 asm volatile("v_add_u32_dpp %0, %1, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf bound_ctrl:1"
                              : "+v"(a[k]) : "v"(b));
+            if (KIND == K_LSHL_ADD_U64) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(q[k]) : "v"(qb));
+            if (KIND == K_ADD_SDWA)
+                asm volatile("v_add_u32_sdwa %0, %1, %0 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:DWORD"
+                             : "+v"(a[k]) : "v"(b));
         }
     }
     u32 r = 0;
@@ -54,7 +58,8 @@ __global__ __launch_bounds__(256) void k_peak(u32 seed, int iters, u32 *out)
 }
 
 // Runs kind `kind` (0 v_add_u32, 1 v_sad_u8, 2 v_qsad_pk_u16_u8, 3
-// v_sad_u16, 4 v_min_f64, 5 v_add_u32 with a DPP operand) on `blocks`
+// v_sad_u16, 4 v_min_f64, 5 v_add_u32 with a DPP operand, 6 v_lshl_add_u64,
+// 7 v_add_u32 of a 16-bit half (SDWA)) on `blocks`
 // workgroups of 256 lanes, iters x 16 instructions per lane, and returns the
 // kernel's time (HIP events) and the lane-instructions it issued.
 PEAK_EXPORT int me_peak_run(int kind, int iters, int blocks, double *ms, double *lane_instrs)
@@ -71,7 +76,9 @@ PEAK_EXPORT int me_peak_run(int kind, int iters, int blocks, double *ms, double 
     case K_QSAD_PK: fn = k_peak<K_QSAD_PK>; break;
     case K_SAD_U16: fn = k_peak<K_SAD_U16>; break;
     case K_MIN_F64: fn = k_peak<K_MIN_F64>; break;
-    default: fn = k_peak<K_ADD_DPP>; break;
+    case K_ADD_DPP: fn = k_peak<K_ADD_DPP>; break;
+    case K_LSHL_ADD_U64: fn = k_peak<K_LSHL_ADD_U64>; break;
+    default: fn = k_peak<K_ADD_SDWA>; break;
     }
     hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, 0, 7u, 4, out);   // warm-up
     (void)hipEventRecord(e0, 0);
