@@ -123,7 +123,7 @@ static int search_layout(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nre
     p->wcols = ((128 + p->dx_cols + 16 + 15) / 16) * 16;
     p->wpitch = p->wcols + 16;
     p->lds = (size_t)(128 + p->band_rows - 1) * p->wpitch * sizeof(pel) + 128 * 128 * sizeof(pel);
-    // the same LDS later merges the waves' node states: 58 keys per lane
+    // the same LDS later merges the waves' node states: at most 58 keys per lane
     if (p->lds < 58 * 64 * sizeof(u64)) p->lds = 58 * 64 * sizeof(u64);
     if (p->lds > 160 * 1024) {
         me_set_error("me_search_init: LDS window too large (reduce band_rows or dx_cols)", hipErrorInvalidValue,

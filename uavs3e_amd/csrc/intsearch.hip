@@ -96,7 +96,8 @@ __device__ __forceinline__ void upd1(double &st, u32 sad, const KeyCtx &c, int k
 //   x octet      : DPP row_half_mirror       (the other x quad of a row of 8)
 //   xor 8        : DPP row_ror:8             (y pairs)
 //   xor 16       : ds_swizzle xor mode       (y quads)
-//   xor 32       : ds_bpermute               (the other half of the CTU)
+//   xor 32       : v_permlane32_swap         (the other half of the CTU; only
+//                  pair sums over the halves are needed, see hsum2)
 // Every source lane of these four patterns exists, so bound_ctrl (zero for a
 // missing source) never applies; with it the compiler folds each move into
 // the add that consumes it (v_add_u32_dpp).
@@ -109,7 +110,16 @@ __device__ __forceinline__ u32 px2(u32 v) { return dppx<0x4E>(v); }
 __device__ __forceinline__ u32 px4(u32 v) { return dppx<0x141>(v); }
 __device__ __forceinline__ u32 py1(u32 v) { return dppx<0x128>(v); }
 __device__ __forceinline__ u32 py2(u32 v) { return (u32)__builtin_amdgcn_ds_swizzle((int)v, 0x401F); }
-__device__ __forceinline__ u32 py4(u32 v) { return (u32)__shfl_xor((int)v, 32, 64); }
+// Pair sums over the wave's halves for two values at once: lanes 0..31 get
+// a + a[lane + 32], lanes 32..63 get b + b[lane - 32].  v_permlane32_swap
+// exchanges lanes 32..63 of its first operand with lanes 0..31 of its
+// second, so afterwards the lower halves of the two results hold a's two
+// halves and the upper halves b's.
+__device__ __forceinline__ u32 hsum2(u32 a, u32 b)
+{
+    const auto w = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    return w[0] + w[1];
+}
 // A value plus its x- or y-partner's: adds on 32-bit halves, which take
 // the partner's value as a DPP operand.
 #if BIT_DEPTH == 8
@@ -246,7 +256,9 @@ template <typename F> __device__ __forceinline__ Pair pair_map(Pair v, F f) { re
 //   k = 2x0 + y0: 32x32 and every class built from it (64x32 ... 128x128)
 //   k = 2x0 + x1: 64x16, 128x16                       k = 2y0 + y1: 16x64, 16x128
 // The lanes holding different displacements of a node are merged before
-// publishing (merge_sparse).
+// publishing (merge_sparse).  A 128-tall node spans both halves of the wave,
+// so after its sum over the halves each half holds it whole: the four such
+// classes live in two node states, one class per half.
 template <bool EDGE, bool SH0>
 __device__ __forceinline__ void sparse_scatter(const Quad &s, const Quad *t32x8, const Quad *t8x32, double *st,
                                                const KeyCtx &c, int rx, int ry, int cx, int cy, int wlim, int hlim)
@@ -275,18 +287,19 @@ __device__ __forceinline__ void sparse_scatter(const Quad &s, const Quad *t32x8,
     const u32 t32x32 = (y0 ? t32x16b : t32x16a) + py1(y0 ? t32x16a : t32x16b);
     const u32 t64x32 = t32x32 + px2(t32x32), t32x64 = t32x32 + py2(t32x32);
     const u32 t64x64 = t64x32 + py2(t64x32);
-    const u32 t128x32 = t64x32 + px4(t64x32), t32x128 = t32x64 + py4(t32x64);
-    const u32 t128x64 = t64x64 + px4(t64x64), t64x128 = t64x64 + py4(t64x64);
-    const u32 t128x128 = t128x64 + py4(t128x64);
+    const u32 t128x32 = t64x32 + px4(t64x32), t128x64 = t64x64 + px4(t64x64);
+    // The 128-tall classes: one sum over the wave's halves serves two of
+    // them, one per half, which then share a node state (st[21]: 32x128 in
+    // the lower half, 64x128 in the upper; st[19]: 128x128 / 16x128).
+    const bool upper = ry >= 4;
+    const u32 t32x128_64x128 = hsum2(t32x64, t64x64);
     updv<EDGE, SH0>(st[8], t32x32, kxy, c, x2, y2, wlim, hlim);
     updv<EDGE, SH0>(st[15], t64x32, kxy, c, x4, y2, wlim, hlim);
     updv<EDGE, SH0>(st[16], t32x64, kxy, c, x2, y4, wlim, hlim);
     updv<EDGE, SH0>(st[17], t64x64, kxy, c, x4, y4, wlim, hlim);
     updv<EDGE, SH0>(st[20], t128x32, kxy, c, cx, y2, wlim, hlim);
-    updv<EDGE, SH0>(st[21], t32x128, kxy, c, x2, cy, wlim, hlim);
+    updv<EDGE, SH0>(st[21], t32x128_64x128, kxy, c, upper ? x4 : x2, cy, wlim, hlim);
     updv<EDGE, SH0>(st[22], t128x64, kxy, c, cx, y4, wlim, hlim);
-    updv<EDGE, SH0>(st[23], t64x128, kxy, c, x4, cy, wlim, hlim);
-    updv<EDGE, SH0>(st[24], t128x128, kxy, c, cx, cy, wlim, hlim);
     // 64x16, 128x16: lane x1 keeps displacement 2x0 + x1
     const u32 t64x16 = (x1 ? t32x16b : t32x16a) + px2(x1 ? t32x16a : t32x16b);
     const u32 t128x16 = t64x16 + px4(t64x16);
@@ -294,9 +307,11 @@ __device__ __forceinline__ void sparse_scatter(const Quad &s, const Quad *t32x8,
     updv<EDGE, SH0>(st[18], t128x16, kxx, c, cx, py, wlim, hlim);
     // 16x64, 16x128: lane y1 keeps displacement 2y0 + y1
     const u32 t16x64 = (y1 ? t16x32b : t16x32a) + py2(y1 ? t16x32a : t16x32b);
-    const u32 t16x128 = t16x64 + py4(t16x64);
     updv<EDGE, SH0>(st[14], t16x64, kyy, c, px, y4, wlim, hlim);
-    updv<EDGE, SH0>(st[19], t16x128, kyy, c, px, cy, wlim, hlim);
+    // 128x128 (lower half, displacement 2x0 + y0) and 16x128 (upper half,
+    // 2y0 + y1)
+    const u32 t128x128_16x128 = hsum2(t128x64, t16x64);
+    updv<EDGE, SH0>(st[19], t128x128_16x128, key_sel(upper, kyy, kxy), c, upper ? px : cx, cy, wlim, hlim);
     // 64x8 (from 32x8, equal on both lanes of an x pair) and 8x64
 #pragma unroll
     for (int r2 = 0; r2 < 2; r2++) {
@@ -344,9 +359,9 @@ __device__ __forceinline__ void merge_sparse(double *st)
     st[7] = kmin(st[7], dpp64_y1(st[7]));
     st[12] = kmin(st[12], dpp64_y1(st[12]));
     st[13] = kmin(st[13], dpp64_y1(st[13]));
-    const int xy[] = {8, 15, 16, 17, 20, 21, 22, 23, 24};
+    const int xy[] = {8, 15, 16, 17, 20, 21, 22};   // 21: 32x128 and 64x128
 #pragma unroll
-    for (int i = 0; i < 9; i++) {
+    for (int i = 0; i < 7; i++) {
         st[xy[i]] = kmin(st[xy[i]], dpp64_x1(st[xy[i]]));
         st[xy[i]] = kmin(st[xy[i]], dpp64_y1(st[xy[i]]));
     }
@@ -356,8 +371,11 @@ __device__ __forceinline__ void merge_sparse(double *st)
     st[18] = kmin(st[18], dpp64_x2(st[18]));
     st[14] = kmin(st[14], dpp64_y1(st[14]));
     st[14] = kmin(st[14], dpp64_y2(st[14]));
+    // st[19]: 128x128 in the lower half (partners x1, y1), 16x128 in the
+    // upper half (y1, y2)
     st[19] = kmin(st[19], dpp64_y1(st[19]));
-    st[19] = kmin(st[19], dpp64_y2(st[19]));
+    const double o19x = dpp64_x1(st[19]), o19y = dpp64_y2(st[19]);
+    st[19] = kmin(st[19], (threadIdx.x & 63) >= 32 ? o19y : o19x);
 }
 
 // Pass L: the 609 PUs larger than a 16x16 region.
@@ -369,9 +387,10 @@ __device__ __forceinline__ void merge_sparse(double *st)
 //   st[3..5] y-pair slots: 4x32 cols {0,2}, {1,3}; 8x32 col = rank
 //   st[6] 32x16  st[7] 16x32  st[8] 32x32  st[9..10] 64x8 (r2)  st[11] 64x16
 //   st[12..13] 8x64 (c2)  st[14] 16x64  st[15] 64x32  st[16] 32x64  st[17] 64x64
-//   st[18] 128x16  st[19] 16x128  st[20] 128x32  st[21] 32x128  st[22] 128x64
-//   st[23] 64x128  st[24] 128x128
-#define N_LARGE 25
+//   st[18] 128x16  st[19] 128x128 (lanes 0..31) / 16x128 (lanes 32..63)
+//   st[20] 128x32  st[21] 32x128 (lanes 0..31) / 64x128 (lanes 32..63)
+//   st[22] 128x64
+#define N_LARGE 23
 // The dense slots for the region's displacements kb .. kb+NK-1; returns the
 // 32x8 and 8x32 sums that the sparse classes build on.
 template <bool EDGE, bool SH0>
@@ -702,12 +721,15 @@ __device__ __forceinline__ void publish_large(const IntArgs &a, u64 *kb, bool at
     if (ex1 && ey2) PUB(S32x64, 32, 64, X, Y, st[16]);
     if (ex2 && ey2) PUB(S64x64, 64, 64, X, Y, st[17]);
     if (ex3) PUB(S128x16, 128, 16, X, Y, st[18]);
-    if (ey3) PUB(S16x128, 16, 128, X, Y, st[19]);
+    // the 128-tall classes kept in the upper half publish from its first row
+    // (ry == 4; pu_index rounds Y down to the node)
+    const bool eyh = ry == 4;
+    if (eyh) PUB(S16x128, 16, 128, X, Y, st[19]);
     if (ex3 && ey1) PUB(S128x32, 128, 32, X, Y, st[20]);
     if (ex1 && ey3) PUB(S32x128, 32, 128, X, Y, st[21]);
     if (ex3 && ey2) PUB(S128x64, 128, 64, X, Y, st[22]);
-    if (ex2 && ey3) PUB(S64x128, 64, 128, X, Y, st[23]);
-    if (ex3 && ey3) PUB(S128x128, 128, 128, X, Y, st[24]);
+    if (ex2 && eyh) PUB(S64x128, 64, 128, X, Y, st[21]);
+    if (ex3 && ey3) PUB(S128x128, 128, 128, X, Y, st[19]);
 }
 
 // The waves of a block hold node states for different displacement rows of
