@@ -227,6 +227,12 @@ struct MeSearch {
     SubShape *d_shp;         // sub-pel shape table of the searched shapes
     u8 *d_blk_shape;         // sub-pel workgroup -> shape
     int sub_blocks;          // sub-pel workgroups per (job, reference)
+    // me_search_tz runs one launch per com_had tile type; the launches write
+    // disjoint results, so they run side by side on these streams (default
+    // priority), forked from the caller's stream by tz_ev[6] and joined back
+    // by tz_ev[0..5]
+    hipStream_t tz_side[6];
+    hipEvent_t tz_ev[7];
 };
 
 // max_dim = max(pic_w, pic_h) of the pictures searched (the |mv| <= T - 4

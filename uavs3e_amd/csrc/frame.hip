@@ -202,6 +202,8 @@ int me_search_init(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nref, int
     if (e == hipSuccess) e = hipMemcpy(p->d_tab_base, tab.data(), tab.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_shp, shp, n_shp * sizeof(SubShape), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_blk_shape, blk_shape.data(), blk_shape.size(), hipMemcpyHostToDevice);
+    for (int i = 0; i < 6 && e == hipSuccess; i++) e = hipStreamCreateWithFlags(&p->tz_side[i], hipStreamNonBlocking);
+    for (int i = 0; i < 7 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&p->tz_ev[i], hipEventDisableTiming);
     if (e != hipSuccess) {
         me_set_error("me_search_init", e, __FILE__, __LINE__);
         me_search_free(p);
@@ -220,6 +222,14 @@ void me_search_free(MeSearch *p)
     p->d_tab_base = nullptr;
     p->d_shp = nullptr;
     p->d_blk_shape = nullptr;
+    for (int i = 0; i < 6; i++) {
+        if (p->tz_side[i]) (void)hipStreamDestroy(p->tz_side[i]);
+        p->tz_side[i] = nullptr;
+    }
+    for (int i = 0; i < 7; i++) {
+        if (p->tz_ev[i]) (void)hipEventDestroy(p->tz_ev[i]);
+        p->tz_ev[i] = nullptr;
+    }
 }
 
 int me_search_warmup(const MeSearch *p, hipStream_t st)

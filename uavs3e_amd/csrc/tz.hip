@@ -1018,16 +1018,29 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
     int rank_of[ME_NSHAPES], rank = 0;
     for (int k = 0; k < ME_NSHAPES; k++) rank_of[k] = s->base[k] >= 0 ? rank++ : -1;
     const int nz = bi ? std::max(a.n0, a.nref - a.n0) : s->nref;
-    // one launch per com_had tile type, its shapes largest PUs first
+    // one launch per com_had tile type, its shapes largest PUs first.  The
+    // launches write disjoint results: with more than one, each runs on a
+    // side stream of its own, forked from st and joined back into it, so that
+    // the long searches of one launch's largest PUs overlap the others.
+    int tt_of[ME_NSHAPES], nlaunch = 0;
+    bool has_tt[6] = {false, false, false, false, false, false};
+    for (int k = 0; k < ME_NSHAPES; k++) {
+        int tw, th;
+        had_tiling(ME_SHAPES_HOST[k][0], ME_SHAPES_HOST[k][1], &tt_of[k], &tw, &th);
+        if (s->base[k] >= 0 && (only_shape < 0 || k == only_shape)) has_tt[tt_of[k]] = true;
+    }
+    for (int tt = 0; tt < 6; tt++) nlaunch += has_tt[tt];
+    const bool side = nlaunch > 1 && s->tz_side[0];
+    if (side) ME_CHECK(hipEventRecord(s->tz_ev[6], st));
     for (int tt = 0; tt < 6; tt++) {
         int order[ME_NSHAPES], no = 0;
         for (int k = 0; k < ME_NSHAPES; k++) {
             if (s->base[k] < 0 || (only_shape >= 0 && k != only_shape)) continue;
-            int t, tw, th;
-            had_tiling(ME_SHAPES_HOST[k][0], ME_SHAPES_HOST[k][1], &t, &tw, &th);
-            if (t == tt) order[no++] = k;
+            if (tt_of[k] == tt) order[no++] = k;
         }
         if (!no) continue;
+        const hipStream_t ls = side ? s->tz_side[tt] : st;
+        if (side) ME_CHECK(hipStreamWaitEvent(ls, s->tz_ev[6], 0));
         std::stable_sort(order, order + no, [](int p, int q) {
             return ME_SHAPES_HOST[p][0] * ME_SHAPES_HOST[p][1] > ME_SHAPES_HOST[q][0] * ME_SHAPES_HOST[q][1];
         });
@@ -1056,11 +1069,15 @@ int me_tz_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, const
                 a.orgbi_off = q.orgbi_off;
                 const int per_block = 256 / std::min(256, area);
                 hipLaunchKernelGGL(k_bi_org, dim3((a.per_ctu + per_block - 1) / per_block, js.n_jobs), dim3(256), 0,
-                                   st, a);
+                                   ls, a);
             }
         }
-        hipLaunchKernelGGL(fn, dim3(blocks, js.n_jobs, nz), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(fn, dim3(blocks, js.n_jobs, nz), dim3(256), 0, ls, a);
         ME_CHECK(hipGetLastError());
+        if (side) {
+            ME_CHECK(hipEventRecord(s->tz_ev[tt], ls));
+            ME_CHECK(hipStreamWaitEvent(st, s->tz_ev[tt], 0));
+        }
     }
     if (bi) {
         const int n = js.n_jobs * s->npu;
