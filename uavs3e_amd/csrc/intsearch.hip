@@ -232,13 +232,23 @@ __device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return p ? q.
 __device__ __forceinline__ u32 pair_elem(Pair v, int j) { return j ? (v >> 16) : (v & 0xFFFFu); }
 template <typename F> __device__ __forceinline__ Pair pair_map(Pair v, F f) { return f(v); }
 #else
+// Four named members, not an array: a choice between two of them by a lane
+// bit (quad_pair) would otherwise be folded into a variable array index,
+// which puts the array in private memory.
 struct Quad {
-    u32 v[4];
+    u32 v0, v1, v2, v3;
+    __device__ __forceinline__ void set(int k, u32 x)   // k a compile-time constant
+    {
+        if (k == 0) v0 = x;
+        else if (k == 1) v1 = x;
+        else if (k == 2) v2 = x;
+        else v3 = x;
+    }
 };
 struct Pair {
     u32 a, b;
 };
-__device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return {p ? q.v[2] : q.v[0], p ? q.v[3] : q.v[1]}; }
+__device__ __forceinline__ Pair quad_pair(const Quad &q, bool p) { return {p ? q.v2 : q.v0, p ? q.v3 : q.v1}; }
 __device__ __forceinline__ u32 pair_elem(Pair v, int j) { return j ? v.b : v.a; }
 template <typename F> __device__ __forceinline__ Pair pair_map(Pair v, F f) { return {f(v.a), f(v.b)}; }
 #endif
@@ -630,11 +640,11 @@ __device__ __forceinline__ void run_pass(const IntArgs &a, const PicDesc &P, con
                         pass_small<EDGE, SH0>(g, st, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
                         Val a8[2], b8[2];
                         pass_dense<EDGE, SH0>(g, st + N_SMALL, kc, k, rx, ry, b.cx, b.cy, wlim, hlim, a8, b8);
-                        s16.v[k] = g.s16.v;
-                        t32x8[0].v[k] = a8[0].v;
-                        t32x8[1].v[k] = a8[1].v;
-                        t8x32[0].v[k] = b8[0].v;
-                        t8x32[1].v[k] = b8[1].v;
+                        s16.set(k, g.s16.v);
+                        t32x8[0].set(k, a8[0].v);
+                        t32x8[1].set(k, a8[1].v);
+                        t8x32[0].set(k, b8[0].v);
+                        t8x32[1].set(k, b8[1].v);
                     } else {
                         upd1<EDGE, SH0>(st[0], g.s16.v, kc, k, b.cx + X, b.cy + Y, wlim, hlim);
                     }
