@@ -81,6 +81,7 @@ def main():
                     help="also measure the issue rate of the search's VALU instructions (microbenchmarks)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-tz", action="store_true", help="skip the me_search_tz leg")
+    ap.add_argument("--no-lookahead", action="store_true", help="skip the lookahead leg (its me_search_tz launches too)")
     ap.add_argument("--serve-frames", type=int, default=32, help="frames per stream in the serving leg")
     ap.add_argument("--serve-rate", type=float, default=40000.0, help="offered load of the serving leg, CTUs/s")
     ap.add_argument("--no-tune", action="store_true", help="use the default schedule instead of the autotuner's")
@@ -415,7 +416,7 @@ def main():
 
     # ---- lookahead cost estimator (inter part) -------------------------------
     lookahead = None
-    if not args.no_tz and hip.lib.me_plan_num_pus(plan) > 0:
+    if not args.no_tz and not args.no_lookahead and hip.lib.me_plan_num_pus(plan) > 0:
         import ctypes
         n0 = work.num_refs - work.num_refs_l1
         pocs = [4, 0, 2, 6][:n0] + [12, 16][:work.num_refs_l1]
