@@ -78,6 +78,8 @@ __constant__ static const s8 TZ_DIA16[16][2] = {{-4, 0}, {-3, 1}, {-2, 2}, {-1, 
                                                 {4, 0}, {3, -1}, {2, -2}, {1, -3}, {0, -4}, {-1, -3}, {-2, -2}, {-3, -1}};
 
 
+typedef const __attribute__((address_space(1))) u32 gu32_t;
+
 // SAD of a w x h block by the lpu lanes of a group; every lane gets the sum.
 // A lane takes 4 horizontally adjacent samples at a time: the original's
 // are aligned (PU x and the picture width are multiples of 4), the
@@ -104,11 +106,71 @@ __device__ __forceinline__ u32 wave_sad(const pel *org, int so, const pel *pred,
     return (u32)group_sum((int)s, lpu);
 }
 
-// SADs of NB candidate blocks pred[0..NB) against the same original, by the
-// lpu lanes of a group: one walk over the block's samples that accumulates
-// every candidate at each step, so that the candidates' loads are in flight
+// A 64-bit address every lane of the wave holds, moved to scalar registers
+// (the search's picture and plane bases are the workgroup's own).
+template <typename T> __device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const u64 v = (u64)(uintptr_t)p;
+    return (T *)(uintptr_t)(((u64)(u32)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                            (u32)__builtin_amdgcn_readfirstlane((int)(u32)v));
+}
+typedef const __attribute__((address_space(1))) unsigned char gbyte_t;
+__device__ __forceinline__ u32 ld32(gbyte_t *base, u32 off) { return *(const gu32_t *)(base + off); }
+
+// SADs of NB candidate blocks against the same original, by the lpu lanes of
+// a group: one walk over the block's samples that accumulates every
+// candidate at each step, so that the candidates' loads are in flight
 // together (a search evaluated one candidate at a time waits for each
-// candidate's loads in turn).  out[c] is in every lane of the group.
+// candidate's loads in turn).  The original is at byte offset ooff from ob,
+// candidate c at poff[c] from pb: both bases are the workgroup's (scalar
+// registers), so each load takes a 32-bit lane offset instead of a 64-bit
+// address.  out[c] is in every lane of the group.
+template <int NB>
+__device__ __forceinline__ void wave_sad_n_off(const pel *ob_, u32 ooff, int so, const pel *pb_, const u32 *poff, int sp,
+                                           int w, int h, int lpu, u32 *out)
+{
+    gbyte_t *ob = (gbyte_t *)uniform_ptr(ob_), *pb = (gbyte_t *)uniform_ptr(pb_);
+    const int li = threadIdx.x & (lpu - 1), q = w >> 2, lq = __builtin_ctz((unsigned)q);
+    u32 s[NB];
+#pragma unroll
+    for (int c = 0; c < NB; c++) s[c] = 0;
+    for (int i = li; i < q * h; i += lpu) {
+        const int xx = (i & (q - 1)) << 2, yy = i >> lq;
+        const u32 oo = ooff + (u32)((yy * so + xx) * (int)sizeof(pel)), po = (u32)((yy * sp + xx) * (int)sizeof(pel));
+        // every load of the step is issued before the first is used (the
+        // scheduler would otherwise wait for each candidate's pair in turn)
+        constexpr int NW = BIT_DEPTH == 8 ? 2 : 3;
+        u32 q[NB][NW], sh[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const u32 ad = poff[c] + po, al = ad & ~3u;
+            sh[c] = ad & 3u;
+#pragma unroll
+            for (int j = 0; j < NW; j++) q[c][j] = ld32(pb, al + 4 * j);
+        }
+#if BIT_DEPTH == 8
+        const u32 ow = ld32(ob, oo);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < NB; c++)
+            s[c] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(q[c][1], q[c][0], sh[c]), ow, s[c]);
+#else
+        const u32 ow0 = ld32(ob, oo), ow1 = ld32(ob, oo + 4);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            const u32 p0 = __builtin_amdgcn_alignbyte(q[c][1], q[c][0], sh[c]);
+            const u32 p1 = __builtin_amdgcn_alignbyte(q[c][2], q[c][1], sh[c]);
+            s[c] = __builtin_amdgcn_sad_u16(p0, ow0, __builtin_amdgcn_sad_u16(p1, ow1, s[c]));
+        }
+#endif
+    }
+#pragma unroll
+    for (int c = 0; c < NB; c++) out[c] = (u32)group_sum((int)s[c], lpu);
+}
+
+// The same from 64-bit lane addresses (bi = 1: the lanes' PUs may search
+// different lists, so the plane base is not the workgroup's).
 template <int NB>
 __device__ __forceinline__ void wave_sad_n(const pel *org, int so, const pel *const *pred, int sp, int w, int h,
                                            int lpu, u32 *out)
@@ -198,6 +260,9 @@ struct TzUnit {
     int r, x, y, w, h, lpu;
     const pel *org;
     int so;                  // org stride
+    const pel *ob;           // the workgroup's original picture (or bi targets) and ...
+    u32 ooff;                // ... org's byte offset from it
+    const pel *pb;           // reference r's integer plane set
     int mvpx, mvpy;
     int lo[2], hi[2];
     u64 best;
@@ -220,33 +285,29 @@ struct TzUnit {
     template <int NB>
     __device__ __forceinline__ void eval_ipel(const int *mx, const int *my, int n, u64 *cost) const
     {
-        const pel *pred[NB];
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-            const int k = c < n ? c : 0;
-            pred[c] = plane_ptr(*P, r, 0, x + mx[k], y + my[k]);
-        }
         u32 sad[NB];
-        wave_sad_n<NB>(org, so, pred, P->ref_stride, w, h, lpu, sad);
+        if (!a->bi) {
+            u32 poff[NB];
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                const int k = c < n ? c : 0;
+                poff[c] = (u32)(((y + my[k] + ME_PAD) * P->ref_stride + (x + mx[k] + ME_PAD)) * (int)sizeof(pel));
+            }
+            wave_sad_n_off<NB>(ob, ooff, so, pb, poff, P->ref_stride, w, h, lpu, sad);
+        } else {
+            const pel *pred[NB];
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                const int k = c < n ? c : 0;
+                pred[c] = plane_ptr(*P, r, 0, x + mx[k], y + my[k]);
+            }
+            wave_sad_n<NB>(org, so, pred, P->ref_stride, w, h, lpu, sad);
+        }
 #pragma unroll
         for (int c = 0; c < NB; c++) {
             const int k = c < n ? c : 0;
             cost[c] = ((u64)sad[c] << shift) + (u64)(ipel_bits(mx[k], my[k]) * a->lambda);
         }
-    }
-    // try_ipel: one integer candidate of a running search (me.c:158-189)
-    __device__ __forceinline__ bool try_ipel(int mx, int my)
-    {
-        const pel *pred = plane_ptr(*P, r, 0, x + mx, y + my);
-        const u32 sad = wave_sad(org, so, pred, P->ref_stride, w, h, lpu);
-        const u64 cost = ((u64)sad << shift) + (u64)(ipel_bits(mx, my) * a->lambda);
-        if (cost < best) {
-            best = cost;
-            mv[0] = mx;
-            mv[1] = my;
-            return true;
-        }
-        return false;
     }
 };
 
@@ -547,10 +608,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (a.bi) {
         u.org = a.orgbi + ((size_t)job * a.nsel * 16384 + S.orgbi_off) + (size_t)i * pw * ph;
         u.so = pw;
+        u.ob = a.orgbi;
     } else {
         u.org = P.cur + (size_t)u.y * P.pic_w + u.x;
         u.so = P.pic_w;
+        u.ob = P.cur;
     }
+    u.ooff = (u32)((u.org - u.ob) * (int)sizeof(pel));
+    u.pb = P.planes[r];
     u.shift = 16 - a.bi - (BIT_DEPTH - 8);
     u.best = ~0ull;
     const int sx = a.bi ? a.mv_scale[slot * 2] : u.mvpx, sy = a.bi ? a.mv_scale[slot * 2 + 1] : u.mvpy;
