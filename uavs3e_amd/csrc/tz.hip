@@ -485,21 +485,72 @@ __device__ __noinline__ void tz_raster(TzUnit &ur, int refi, int mvi_prev_x, int
 }
 
 // me_sub_pel_search (me.c:366-500); u.mv holds the quarter-pel
-// integer result on entry and the refined MV on return.  (Evaluating the
-// SATD candidates several at a time, as the integer searches do, measured
-// slower: the extra tiles in flight cost the small-tile kernels a wave per
-// SIMD.)
+// integer result on entry and the refined MV on return.
+//
+// With SATD costs (cost_type 0) a PU's lanes are split into G position
+// groups of Lg lanes, Lg the lanes its com_had tiles take (one per tile,
+// two per 16x8 / 8x16 tile): a group evaluates one candidate position, so G
+// positions of a phase (the centre and the 8 half-pel points, then the 8
+// quarter-pel points) are evaluated at a time and their costs then scanned
+// in the reference's order with its strict '<' (with one lane per tile and
+// lpu = area/16 lanes, three of four lanes would otherwise idle through every
+// SATD).  SAD costs (cost_type 1) take every lane per candidate.
 template <int TT>
 __device__ __noinline__ u64 tz_subpel(TzUnit &ur)
 {
     TzUnit u = ur;   // a register copy, as in tz_diamond
     const TzArgs &a = *u.a;
     const int shift = 16 - a.bi - (a.cost_type == 0 ? 0 : (BIT_DEPTH - 8));
-    auto dist = [&](const pel *p) -> u32 {
-        return a.cost_type == 0 ? wave_had<TT>(u.lpu, u.w, u.h, u.org, u.so, p, u.P->ref_stride)
-                                : wave_sad(u.org, u.so, p, u.P->ref_stride, u.w, u.h, u.lpu);
-    };
     const int cx = u.mv[0], cy = u.mv[1];
+    if (a.cost_type == 0) {
+        constexpr int tw = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 8 : (TT == 3 ? 16 : (TT == 4 ? 4 : 8))));
+        constexpr int th = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 4 : (TT == 3 ? 8 : (TT == 4 ? 8 : 16))));
+        const int T = (u.w / tw) * (u.h / th), L = (TT == 3 || TT == 5) ? 2 * T : T;
+        int G = L < u.lpu ? u.lpu / L : 1;
+        if (G > 4) G = 4;
+        const int Lg = G > 1 ? L : u.lpu;                         // lanes per position group
+        const int lane = threadIdx.x & 63, base = lane & ~(u.lpu - 1);
+        int g = (lane - base) / Lg;
+        if (g >= G) g = 0;                                        // spare lanes repeat group 0's position
+        // position k of a phase: the half phase's k = 0 is the centre (the
+        // integer plane, no clip, me.c:400) and k = 1..8 its half-pel points;
+        // the quarter phase's k = 0..7 are the quarter-pel points around
+        // (ox, oy)
+        auto pos = [&](bool half, int ox, int oy, int k, int &mx, int &my) {
+            if (half) {
+                mx = ox + (k ? 2 * pat(ME_SUBPEL_HALF_X, k - 1) : 0);
+                my = oy + (k ? 2 * pat(ME_SUBPEL_HALF_Y, k - 1) : 0);
+            } else {
+                mx = ox + pat(ME_SUBPEL_QUARTER_X, k);
+                my = oy + pat(ME_SUBPEL_QUARTER_Y, k);
+            }
+        };
+        auto phase = [&](bool half, int ox, int oy, u64 &best, int &bx, int &by) {
+            const int n = half ? 9 : 8;
+            for (int b0 = 0; b0 < n; b0 += G) {
+                const int k = b0 + g < n ? b0 + g : b0;
+                int mx, my;
+                pos(half, ox, oy, k, mx, my);
+                const pel *ptr = half && k == 0 ? plane_ptr(*u.P, u.r, 0, u.x + (mx >> 2), u.y + (my >> 2))
+                                                : qpel_ptr(*u.P, u.r, mx + (u.x << 2), my + (u.y << 2));
+                const u32 d = wave_had<TT>(Lg, u.w, u.h, u.org, u.so, ptr, u.P->ref_stride);
+                for (int q = 0; q < G && b0 + q < n; q++) {
+                    const u32 dq = (u32)__shfl((int)d, base + q * Lg, 64);
+                    pos(half, ox, oy, b0 + q, mx, my);
+                    const u64 c = (u64)(a.lambda * u.qpel_bits(mx, my)) + ((u64)dq << shift);
+                    if (c < best) { best = c; bx = mx; by = my; }
+                }
+            }
+        };
+        u64 best = ~0ull;
+        int bx = cx, by = cy;
+        phase(true, cx, cy, best, bx, by);
+        if (a.mvr == 0) phase(false, bx, by, best, bx, by);
+        ur.mv[0] = bx;
+        ur.mv[1] = by;
+        return best;
+    }
+    auto dist = [&](const pel *p) -> u32 { return wave_sad(u.org, u.so, p, u.P->ref_stride, u.w, u.h, u.lpu); };
     u32 d = dist(plane_ptr(*u.P, u.r, 0, u.x + (cx >> 2), u.y + (cy >> 2)));
     u64 best = (u64)(a.lambda * u.qpel_bits(cx, cy)) + ((u64)d << shift);
     for (int p = 0; p < 8; p++) {
@@ -517,11 +568,10 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &ur)
             if (c < best) { best = c; u.mv[0] = mx; u.mv[1] = my; }
         }
     }
-    if (a.cost_type) {
-        d = wave_had<TT>(u.lpu, u.w, u.h, u.org, u.so,
-                         qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)), u.P->ref_stride);
-        best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << (16 - a.bi));
-    }
+    // cost_type 1: the final cost is com_had's at the chosen position
+    d = wave_had<TT>(u.lpu, u.w, u.h, u.org, u.so,
+                     qpel_ptr(*u.P, u.r, u.mv[0] + (u.x << 2), u.mv[1] + (u.y << 2)), u.P->ref_stride);
+    best = (u64)(a.lambda * u.qpel_bits(u.mv[0], u.mv[1])) + ((u64)d << (16 - a.bi));
     ur.mv[0] = u.mv[0];
     ur.mv[1] = u.mv[1];
     return best;
