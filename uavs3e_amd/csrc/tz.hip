@@ -23,6 +23,13 @@
 #include <string.h>
 #include <algorithm>
 
+#ifndef TZ_U
+#define TZ_U 2
+#endif
+#ifndef TZ_WPE
+#define TZ_WPE 3
+#endif
+
 // compiled once per group of com_had tile types, like subpel.hip (ME_TZ_PART
 // 0: 4x4, 8x8, 8x4, 4x8 tiles and the launcher; 1: 16x8; 2: 8x16)
 #ifndef ME_TZ_PART
@@ -134,36 +141,45 @@ __device__ __forceinline__ void wave_sad_n_off(const pel *ob_, u32 ooff, int so,
     u32 s[NB];
 #pragma unroll
     for (int c = 0; c < NB; c++) s[c] = 0;
-    for (int i = li; i < q * h; i += lpu) {
-        const int xx = (i & (q - 1)) << 2, yy = i >> lq;
-        const u32 oo = ooff + (u32)((yy * so + xx) * (int)sizeof(pel)), po = (u32)((yy * sp + xx) * (int)sizeof(pel));
-        // every load of the step is issued before the first is used (the
-        // scheduler would otherwise wait for each candidate's pair in turn)
-        constexpr int NW = BIT_DEPTH == 8 ? 2 : 3;
-        u32 q[NB][NW], sh[NB];
+    // TZ_U steps of the walk at a time: every load of the chunk is issued
+    // before the first is used (the scheduler would otherwise wait for each
+    // candidate's pair in turn).  A lane's step count, q * h / lpu, is the
+    // same for every lane; steps past it repeat the chunk's first reads and
+    // are not summed.
+    constexpr int NW = BIT_DEPTH == 8 ? 2 : 3, OW = BIT_DEPTH == 8 ? 1 : 2;
+    const int total = q * h;
+    for (int i0 = li; i0 < total; i0 += TZ_U * lpu) {
+        u32 qd[TZ_U][NB][NW], sh[TZ_U][NB], ow[TZ_U][OW];
 #pragma unroll
-        for (int c = 0; c < NB; c++) {
-            const u32 ad = poff[c] + po, al = ad & ~3u;
-            sh[c] = ad & 3u;
+        for (int t = 0; t < TZ_U; t++) {
+            const int i = i0 + t * lpu < total ? i0 + t * lpu : i0;
+            const int xx = (i & (q - 1)) << 2, yy = i >> lq;
+            const u32 oo = ooff + (u32)((yy * so + xx) * (int)sizeof(pel)), po = (u32)((yy * sp + xx) * (int)sizeof(pel));
 #pragma unroll
-            for (int j = 0; j < NW; j++) q[c][j] = ld32(pb, al + 4 * j);
+            for (int c = 0; c < NB; c++) {
+                const u32 ad = poff[c] + po, al = ad & ~3u;
+                sh[t][c] = ad & 3u;
+#pragma unroll
+                for (int j = 0; j < NW; j++) qd[t][c][j] = ld32(pb, al + 4 * j);
+            }
+#pragma unroll
+            for (int j = 0; j < OW; j++) ow[t][j] = ld32(ob, oo + 4 * j);
         }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < TZ_U; t++) {
+            if (t > 0 && i0 + t * lpu >= total) break;
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
 #if BIT_DEPTH == 8
-        const u32 ow = ld32(ob, oo);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int c = 0; c < NB; c++)
-            s[c] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(q[c][1], q[c][0], sh[c]), ow, s[c]);
+                s[c] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(qd[t][c][1], qd[t][c][0], sh[t][c]), ow[t][0], s[c]);
 #else
-        const u32 ow0 = ld32(ob, oo), ow1 = ld32(ob, oo + 4);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-            const u32 p0 = __builtin_amdgcn_alignbyte(q[c][1], q[c][0], sh[c]);
-            const u32 p1 = __builtin_amdgcn_alignbyte(q[c][2], q[c][1], sh[c]);
-            s[c] = __builtin_amdgcn_sad_u16(p0, ow0, __builtin_amdgcn_sad_u16(p1, ow1, s[c]));
-        }
+                const u32 p0 = __builtin_amdgcn_alignbyte(qd[t][c][1], qd[t][c][0], sh[t][c]);
+                const u32 p1 = __builtin_amdgcn_alignbyte(qd[t][c][2], qd[t][c][1], sh[t][c]);
+                s[c] = __builtin_amdgcn_sad_u16(p0, ow[t][0], __builtin_amdgcn_sad_u16(p1, ow[t][1], s[c]));
 #endif
+            }
+        }
     }
 #pragma unroll
     for (int c = 0; c < NB; c++) out[c] = (u32)group_sum((int)s[c], lpu);
@@ -589,7 +605,7 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &ur)
 #define TZ_ARGS(T) (*(const T *)__builtin_amdgcn_kernarg_segment_ptr())
 
 template <int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_tz(TzArgs)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TZ_WPE))) void k_tz(TzArgs)
 {
     const TzArgs &a = TZ_ARGS(TzArgs);
     int k = 0;
