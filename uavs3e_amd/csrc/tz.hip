@@ -23,6 +23,9 @@
 #include <string.h>
 #include <algorithm>
 
+// Steps of a block walk whose loads are issued together (TZ_U), and k_tz's
+// waves per SIMD (TZ_WPE): 2 and 3 measured fastest (4 steps at 2 waves per
+// SIMD: 6.8 against 5.6 ms per 1080p all-PU frame).
 #ifndef TZ_U
 #define TZ_U 2
 #endif
@@ -541,6 +544,33 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &ur)
                 my = oy + pat(ME_SUBPEL_QUARTER_Y, k);
             }
         };
+        // Where every lane of a group holds at most one tile (the PU's tiles
+        // fit its lanes), the lane's tile pairs of the original are made once
+        // for all 17 positions and each position reads only its prediction
+        // rows (had_pre / satd_pre); otherwise wave_had walks the tiles.
+        constexpr bool SPLIT = TT == 3 || TT == 5;
+        const bool pre = L <= Lg;
+        const int li = lane - base - g * Lg;                      // lane within its group
+        const bool tile = li < L;
+        const int t = SPLIT ? li >> 1 : li, tpr = u.w / tw;
+        const int tx = (t % tpr) * tw, ty = (t / tpr) * th;
+        s16x2 opairs[org_pairs_n<TT>()];
+        if (pre && tile) org_pairs<TT>(u.org + ty * u.so + tx, u.so, li & 1, opairs);
+        auto had = [&](const pel *ptr) -> u32 {
+            if (!pre) return wave_had<TT>(Lg, u.w, u.h, u.org, u.so, ptr, u.P->ref_stride);
+            int v = 0;
+            if (SPLIT) {
+                // the two lanes of a tile hold its sum and difference halves;
+                // the tile is normalised (com_cost.c:467, 586) before tiles
+                // are summed
+                const int hs = tile ? had_pre<TT>(opairs, ptr + ty * u.P->ref_stride + tx, u.P->ref_stride, li & 1) : 0;
+                const int sv = hs + __builtin_amdgcn_mov_dpp(hs, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+                v = tile && !(li & 1) ? me_satd_norm128(sv) : 0;
+            } else if (tile) {
+                v = satd_pre<TT>(opairs, ptr + ty * u.P->ref_stride + tx, u.P->ref_stride);
+            }
+            return (u32)(group_sum(v, Lg) >> (BIT_DEPTH - 8));
+        };
         auto phase = [&](bool half, int ox, int oy, u64 &best, int &bx, int &by) {
             const int n = half ? 9 : 8;
             for (int b0 = 0; b0 < n; b0 += G) {
@@ -549,7 +579,7 @@ __device__ __noinline__ u64 tz_subpel(TzUnit &ur)
                 pos(half, ox, oy, k, mx, my);
                 const pel *ptr = half && k == 0 ? plane_ptr(*u.P, u.r, 0, u.x + (mx >> 2), u.y + (my >> 2))
                                                 : qpel_ptr(*u.P, u.r, mx + (u.x << 2), my + (u.y << 2));
-                const u32 d = wave_had<TT>(Lg, u.w, u.h, u.org, u.so, ptr, u.P->ref_stride);
+                const u32 d = had(ptr);
                 for (int q = 0; q < G && b0 + q < n; q++) {
                     const u32 dq = (u32)__shfl((int)d, base + q * Lg, 64);
                     pos(half, ox, oy, b0 + q, mx, my);
