@@ -126,6 +126,24 @@ template <typename T> __device__ __forceinline__ T *uniform_ptr(T *p)
 }
 typedef const __attribute__((address_space(1))) unsigned char gbyte_t;
 __device__ __forceinline__ u32 ld32(gbyte_t *base, u32 off) { return *(const gu32_t *)(base + off); }
+// NW consecutive dwords in one load instruction (dword-aligned)
+typedef u32 u32x2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef u32 u32x3u __attribute__((ext_vector_type(3), aligned(4)));
+template <int NW> __device__ __forceinline__ void ldn(gbyte_t *base, u32 off, u32 *d)
+{
+    if (NW == 1) {
+        d[0] = ld32(base, off);
+    } else if (NW == 2) {
+        const u32x2u v = *(const __attribute__((address_space(1))) u32x2u *)(base + off);
+        d[0] = v.x;
+        d[1] = v.y;
+    } else {
+        const u32x3u v = *(const __attribute__((address_space(1))) u32x3u *)(base + off);
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+    }
+}
 
 // SADs of NB candidate blocks against the same original, by the lpu lanes of
 // a group: one walk over the block's samples that accumulates every
@@ -160,13 +178,11 @@ __device__ __forceinline__ void wave_sad_n_off(const pel *ob_, u32 ooff, int so,
             const u32 oo = ooff + (u32)((yy * so + xx) * (int)sizeof(pel)), po = (u32)((yy * sp + xx) * (int)sizeof(pel));
 #pragma unroll
             for (int c = 0; c < NB; c++) {
-                const u32 ad = poff[c] + po, al = ad & ~3u;
+                const u32 ad = poff[c] + po;
                 sh[t][c] = ad & 3u;
-#pragma unroll
-                for (int j = 0; j < NW; j++) qd[t][c][j] = ld32(pb, al + 4 * j);
+                ldn<NW>(pb, ad & ~3u, qd[t][c]);
             }
-#pragma unroll
-            for (int j = 0; j < OW; j++) ow[t][j] = ld32(ob, oo + 4 * j);
+            ldn<OW>(ob, oo, ow[t]);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
