@@ -797,19 +797,21 @@ struct SmvdUnit {
 
 // get_smvd_cost (inter.c:1071-1081) of the bi-average of list 0 ref 0 at m0
 // and list 1 ref 0 at m1 (com_mc_cu, com_mc.c:140-200: the rounding
-// average of two quarter-pel plane blocks); `had` forces com_had.
+// average of two quarter-pel plane blocks); `had` forces com_had.  The
+// aligned groups of `lanes` lanes (a.lpu, or fewer for position groups) sum
+// their own candidates.
 template <int TT>
-__device__ u32 smvd_cost(const SmvdUnit &u, int m0x, int m0y, int m1x, int m1y, bool had)
+__device__ u32 smvd_cost(const SmvdUnit &u, int m0x, int m0y, int m1x, int m1y, bool had, int lanes)
 {
     const TzArgs &a = *u.a;
     const PicDesc &P = *u.P;
     const pel *p0 = qpel_ptr(P, 0, (u.x << 2) + m0x, (u.y << 2) + m0y);
     const pel *p1 = qpel_ptr(P, a.n0, (u.x << 2) + m1x, (u.y << 2) + m1y);
-    const int li = threadIdx.x & (a.lpu - 1);
+    const int li = threadIdx.x & (lanes - 1);
     int s = 0;
     if (!had) {   // SAD >> (bd - 8)
         const int q = a.pw >> 2;
-        for (int i = li; i < q * a.ph; i += a.lpu) {
+        for (int i = li; i < q * a.ph; i += lanes) {
             const int xx = (i % q) * 4, yy = i / q;
             int v0[4], v1[4];
             load_row<4>(p0 + yy * P.ref_stride + xx, v0);
@@ -817,54 +819,84 @@ __device__ u32 smvd_cost(const SmvdUnit &u, int m0x, int m0y, int m1x, int m1y, 
             const pel *o = u.org + yy * P.pic_w + xx;
             for (int k = 0; k < 4; k++) s += abs((int)o[k] - ((v0[k] + v1[k] + 1) >> 1));
         }
-        return (u32)(group_sum(s, a.lpu) >> (BIT_DEPTH - 8));
+        return (u32)(group_sum(s, lanes) >> (BIT_DEPTH - 8));
     }
-    constexpr int PW = 4 / (int)sizeof(pel);
-    const int tpr = a.pw / a.tw, T = tpr * (a.ph / a.th);
-    for (int t = li; t < T; t += a.lpu) {
-        const int tx = (t % tpr) * a.tw, ty = (t / tpr) * a.th;
+    // the tile type's tile size (had_tiling)
+    constexpr int TW = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 8 : (TT == 3 ? 16 : (TT == 4 ? 4 : 8))));
+    constexpr int TH = TT == 0 ? 4 : (TT == 1 ? 8 : (TT == 2 ? 4 : (TT == 3 ? 8 : (TT == 4 ? 8 : 16))));
+    constexpr int NW = TW * (int)sizeof(pel) / 4;   // dwords per tile row
+    // (a + b + 1) >> 1 of every sample of two dwords: (a | b) - ((a ^ b) >> 1)
+    // per sample, the shifted-in bit of the next sample masked off
+    constexpr u32 HM = sizeof(pel) == 1 ? 0x7F7F7F7Fu : 0x7FFF7FFFu;
+    const int tpr = a.pw / TW, T = tpr * (a.ph / TH);
+    for (int t = li; t < T; t += lanes) {
+        const int tx = (t % tpr) * TW, ty = (t / tpr) * TH;
         // the tile's average, stored as whole dwords (com_had reads dwords)
-        for (int yy = 0; yy < a.th; yy++) {
-            u32 *d = (u32 *)(u.tile + yy * a.tw);
-            const pel *r0 = p0 + (ty + yy) * P.ref_stride + tx, *r1 = p1 + (ty + yy) * P.ref_stride + tx;
-            for (int k = 0; k < a.tw / PW; k++) {
-                u32 v = 0;
-                for (int j = 0; j < PW; j++)
-                    v |= (u32)((r0[k * PW + j] + r1[k * PW + j] + 1) >> 1) << (8 * (int)sizeof(pel) * j);
-                d[k] = v;
-            }
+#pragma unroll
+        for (int yy = 0; yy < TH; yy++) {
+            u32 *d = (u32 *)(u.tile + yy * TW);
+            u32 w0[NW], w1[NW];
+            load_words<NW>(p0 + (ty + yy) * P.ref_stride + tx, w0);
+            load_words<NW>(p1 + (ty + yy) * P.ref_stride + tx, w1);
+#pragma unroll
+            for (int k = 0; k < NW; k++) d[k] = (w0[k] | w1[k]) - (((w0[k] ^ w1[k]) >> 1) & HM);
         }
-        s += satd_tile_fn<TT>(u.org + ty * P.pic_w + tx, P.pic_w, u.tile, a.tw);
+        s += satd_tile_fn<TT>(u.org + ty * P.pic_w + tx, P.pic_w, u.tile, TW);
     }
-    return (u32)(group_sum(s, a.lpu) >> (BIT_DEPTH - 8));
+    return (u32)(group_sum(s, lanes) >> (BIT_DEPTH - 8));
 }
 
 // smvd_refine (inter.c:1083-1191) for the patterns analyze_smvd uses:
 // diamond (pattern 2) and cross (pattern 0), offsets scaled by the AMVR shift.
+// A round's points all lie around the round's centre, so with com_had costs
+// (one lane per tile, area/16 lanes per PU) the PU's lanes form up to four
+// position groups that evaluate several points at a time; the costs are then
+// scanned in the reference's order with its strict '<'.
 template <int TT>
 __device__ u32 smvd_refine(const SmvdUnit &u, int mv[2][2], u32 cost, int pattern, int rounds)
 {
     const TzArgs &a = *u.a;
     const int mvr = a.mvr;
     const u32 ref_mvr_bits = (u32)(mvr + (mvr < 4));   // tbl_refi_bits[0][0] = 0
+    const bool had = a.cost_type == 0;
+    const int T = (a.pw / a.tw) * (a.ph / a.th);
+    int G = had && T < a.lpu ? a.lpu / T : 1;
+    if (G > 4) G = 4;
+    const int Lg = G > 1 ? T : a.lpu;
+    const int lane = threadIdx.x & 63, base = lane & ~(a.lpu - 1);
+    int g = (lane - base) / Lg;
+    if (g >= G) g = 0;   // spare lanes repeat group 0's point
     int ds = 0, de = pattern == 0 ? 3 : 7;
     const int rnd = pattern == 0 ? 4 : 8, mask = pattern == 0 ? 3 : 7;
     for (int round = 0; round < rounds; round++) {
         const int cx = mv[0][0], cy = mv[0][1];
-        int best_dir = -1;
-        for (int idx = ds; idx <= de; idx++) {
-            const int dir = (idx + rnd) & mask;
+        // point idx of the round: its direction, both lists' MVs and MV bits
+        auto point = [&](int idx, int &dir, int &c0x, int &c0y, int &c1x, int &c1y, u32 &bits) {
+            dir = (idx + rnd) & mask;
             const int ox = pattern == 0 ? SMVD_CROSS[dir][0] : SMVD_DIAMOND[dir][0];
             const int oy = pattern == 0 ? SMVD_CROSS[dir][1] : SMVD_DIAMOND[dir][1];
-            const int c0x = (s16)(cx + (s16)(ox << mvr)), c0y = (s16)(cy + (s16)(oy << mvr));
+            c0x = (s16)(cx + (s16)(ox << mvr));
+            c0y = (s16)(cy + (s16)(oy << mvr));
             const int dx = c0x - u.mvp[0][0], dy = c0y - u.mvp[0][1];
-            const int c1x = (s16)(u.mvp[1][0] - dx), c1y = (s16)(u.mvp[1][1] - dy);
-            const u32 bits = ref_mvr_bits + u.mvbits(dx) + u.mvbits(dy);
-            const u32 t = smvd_cost<TT>(u, c0x, c0y, c1x, c1y, a.cost_type == 0) + u.mv_cost(bits);
-            if (t < cost) {
-                cost = t;
-                mv[0][0] = c0x; mv[0][1] = c0y; mv[1][0] = c1x; mv[1][1] = c1y;
-                best_dir = dir;
+            c1x = (s16)(u.mvp[1][0] - dx);
+            c1y = (s16)(u.mvp[1][1] - dy);
+            bits = ref_mvr_bits + u.mvbits(dx) + u.mvbits(dy);
+        };
+        int best_dir = -1;
+        for (int b0 = ds; b0 <= de; b0 += G) {
+            const int idx = b0 + g <= de ? b0 + g : b0;
+            int dir, c0x, c0y, c1x, c1y;
+            u32 bits;
+            point(idx, dir, c0x, c0y, c1x, c1y, bits);
+            const u32 d = smvd_cost<TT>(u, c0x, c0y, c1x, c1y, had, Lg);
+            for (int q = 0; q < G && b0 + q <= de; q++) {
+                point(b0 + q, dir, c0x, c0y, c1x, c1y, bits);
+                const u32 t = (u32)__shfl((int)d, base + q * Lg, 64) + u.mv_cost(bits);
+                if (t < cost) {
+                    cost = t;
+                    mv[0][0] = c0x; mv[0][1] = c0y; mv[1][0] = c1x; mv[1][1] = c1y;
+                    best_dir = dir;
+                }
             }
         }
         if (best_dir < 0) break;
@@ -907,14 +939,14 @@ __global__ __launch_bounds__(128) void k_smvd(TzArgs, me_smvd_pu_t *out)
     const bool had = a.cost_type == 0;
     // MV = MVP in both lists (inter.c:1219-1225)
     int mv[2][2] = {{u.mvp[0][0], u.mvp[0][1]}, {u.mvp[1][0], u.mvp[1][1]}};
-    u32 cost = smvd_cost<TT>(u, mv[0][0], mv[0][1], mv[1][0], mv[1][1], had) +
+    u32 cost = smvd_cost<TT>(u, mv[0][0], mv[0][1], mv[1][0], mv[1][1], had, a.lpu) +
                u.mv_cost(u.mvbits(0) * 2 + mvr + (mvr < 4));
     // list 0's uni MV, mirrored into list 1 (inter.c:1227-1243)
     {
         const int m0x = a.mv_scale[s0 * 2], m0y = a.mv_scale[s0 * 2 + 1];
         const int dx = m0x - u.mvp[0][0], dy = m0y - u.mvp[0][1];
         const int m1x = (s16)(u.mvp[1][0] - dx), m1y = (s16)(u.mvp[1][1] - dy);
-        const u32 cb = smvd_cost<TT>(u, m0x, m0y, m1x, m1y, had) +
+        const u32 cb = smvd_cost<TT>(u, m0x, m0y, m1x, m1y, had, a.lpu) +
                        u.mv_cost(u.mvbits(dx) + u.mvbits(dy) + mvr + (mvr < 4));
         if (cb < cost) {
             cost = cb;
@@ -925,7 +957,7 @@ __global__ __launch_bounds__(128) void k_smvd(TzArgs, me_smvd_pu_t *out)
     cost = smvd_refine<TT>(u, mv, cost, 0, 1);
     if (!had) {   // SAD search: the winner's com_had (inter.c:1248-1255)
         const int dx = mv[0][0] - u.mvp[0][0], dy = mv[0][1] - u.mvp[0][1];
-        cost = smvd_cost<TT>(u, mv[0][0], mv[0][1], mv[1][0], mv[1][1], true) +
+        cost = smvd_cost<TT>(u, mv[0][0], mv[0][1], mv[1][0], mv[1][1], true, a.lpu) +
                u.mv_cost(u.mvbits(dx) + u.mvbits(dy) + mvr + (mvr < 4));
     }
     if ((threadIdx.x & (a.lpu - 1)) == 0) {
@@ -1283,8 +1315,17 @@ int me_smvd_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, con
     a.n0 = s->nref - s->nref_l1;
     a.mvr = tz->mvr;
     a.cost_type = tz->cost_type;
+    // one launch per shape, writing its own PUs' results: the launches run
+    // side by side on the search's side streams (forked from st, joined back)
+    int nshape = 0;
+    for (int k = 0; k < ME_NSHAPES; k++) nshape += s->base[k] >= 0;
+    const bool side = nshape > 1 && s->tz_side[0];
+    if (side) ME_CHECK(hipEventRecord(s->tz_ev[6], st));
+    for (int i = 0; side && i < 6; i++) ME_CHECK(hipStreamWaitEvent(s->tz_side[i], s->tz_ev[6], 0));
+    int launch = 0;
     for (int k = 0; k < ME_NSHAPES; k++) {
         if (s->base[k] < 0) continue;
+        const hipStream_t ls = side ? s->tz_side[launch++ % 6] : st;
         a.pw = ME_SHAPES_HOST[k][0];
         a.ph = ME_SHAPES_HOST[k][1];
         a.base = s->base[k];
@@ -1296,8 +1337,12 @@ int me_smvd_enqueue(const MeSearch *s, const JobSet &js, const int16_t *mvp, con
         smvd_fn fn = a.tt == 3 ? me_smvd_kernel_16x8(a.tt)
                                : (a.tt == 5 ? me_smvd_kernel_8x16(a.tt) : me_smvd_kernel_small(a.tt));
         if (!fn) { me_set_error("no SMVD kernel for shape", hipErrorInvalidValue, __FILE__, __LINE__); return -1; }
-        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs), dim3(128), 0, st, a, out);
+        hipLaunchKernelGGL(fn, dim3((a.per_ctu + upb - 1) / upb, js.n_jobs), dim3(128), 0, ls, a, out);
         ME_CHECK(hipGetLastError());
+    }
+    for (int i = 0; side && i < 6; i++) {
+        ME_CHECK(hipEventRecord(s->tz_ev[i], s->tz_side[i]));
+        ME_CHECK(hipStreamWaitEvent(st, s->tz_ev[i], 0));
     }
     return 0;
 }
