@@ -153,11 +153,13 @@ template <int NW> __device__ __forceinline__ void ldn(gbyte_t *base, u32 off, u3
 // candidate c at poff[c] from pb: both bases are the workgroup's (scalar
 // registers), so each load takes a 32-bit lane offset instead of a 64-bit
 // address.  out[c] is in every lane of the group.
-template <int NB>
+template <int NB, bool UNI>
 __device__ __forceinline__ void wave_sad_n_off(const pel *ob_, u32 ooff, int so, const pel *pb_, const u32 *poff, int sp,
                                            int w, int h, int lpu, u32 *out)
 {
-    gbyte_t *ob = (gbyte_t *)uniform_ptr(ob_), *pb = (gbyte_t *)uniform_ptr(pb_);
+    // UNI: the plane base is the workgroup's; else (bi = 1, the lanes' PUs may
+    // search different lists) each lane keeps its own 64-bit base
+    gbyte_t *ob = (gbyte_t *)uniform_ptr(ob_), *pb = (gbyte_t *)(UNI ? uniform_ptr(pb_) : pb_);
     const int li = threadIdx.x & (lpu - 1), q = w >> 2, lq = __builtin_ctz((unsigned)q);
     u32 s[NB];
 #pragma unroll
@@ -199,43 +201,6 @@ __device__ __forceinline__ void wave_sad_n_off(const pel *ob_, u32 ooff, int so,
 #endif
             }
         }
-    }
-#pragma unroll
-    for (int c = 0; c < NB; c++) out[c] = (u32)group_sum((int)s[c], lpu);
-}
-
-// The same from 64-bit lane addresses (bi = 1: the lanes' PUs may search
-// different lists, so the plane base is not the workgroup's).
-template <int NB>
-__device__ __forceinline__ void wave_sad_n(const pel *org, int so, const pel *const *pred, int sp, int w, int h,
-                                           int lpu, u32 *out)
-{
-    const int li = threadIdx.x & (lpu - 1), q = w >> 2, lq = __builtin_ctz((unsigned)q);
-    u32 s[NB];
-#pragma unroll
-    for (int c = 0; c < NB; c++) s[c] = 0;
-    for (int i = li; i < q * h; i += lpu) {
-        const int xx = (i & (q - 1)) << 2, yy = i >> lq;
-        const int oo = yy * so + xx, po = yy * sp + xx;
-#if BIT_DEPTH == 8
-        const u32 ow = *(const u32 *)(org + oo);
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-            const uintptr_t ad = (uintptr_t)(pred[c] + po);
-            const u32 *pq = (const u32 *)(ad & ~(uintptr_t)3);
-            s[c] = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(pq[1], pq[0], (u32)(ad & 3)), ow, s[c]);
-        }
-#else
-        const uint2 ow = *(const uint2 *)(org + oo);
-#pragma unroll
-        for (int c = 0; c < NB; c++) {
-            const uintptr_t ad = (uintptr_t)(pred[c] + po);
-            const u32 *pq = (const u32 *)(ad & ~(uintptr_t)3);
-            const u32 sh = (u32)(ad & 3);
-            const u32 p0 = __builtin_amdgcn_alignbyte(pq[1], pq[0], sh), p1 = __builtin_amdgcn_alignbyte(pq[2], pq[1], sh);
-            s[c] = __builtin_amdgcn_sad_u16(p0, ow.x, __builtin_amdgcn_sad_u16(p1, ow.y, s[c]));
-        }
-#endif
     }
 #pragma unroll
     for (int c = 0; c < NB; c++) out[c] = (u32)group_sum((int)s[c], lpu);
@@ -320,24 +285,14 @@ struct TzUnit {
     template <int NB>
     __device__ __forceinline__ void eval_ipel(const int *mx, const int *my, int n, u64 *cost) const
     {
-        u32 sad[NB];
-        if (!a->bi) {
-            u32 poff[NB];
+        u32 sad[NB], poff[NB];
 #pragma unroll
-            for (int c = 0; c < NB; c++) {
-                const int k = c < n ? c : 0;
-                poff[c] = (u32)(((y + my[k] + ME_PAD) * P->ref_stride + (x + mx[k] + ME_PAD)) * (int)sizeof(pel));
-            }
-            wave_sad_n_off<NB>(ob, ooff, so, pb, poff, P->ref_stride, w, h, lpu, sad);
-        } else {
-            const pel *pred[NB];
-#pragma unroll
-            for (int c = 0; c < NB; c++) {
-                const int k = c < n ? c : 0;
-                pred[c] = plane_ptr(*P, r, 0, x + mx[k], y + my[k]);
-            }
-            wave_sad_n<NB>(org, so, pred, P->ref_stride, w, h, lpu, sad);
+        for (int c = 0; c < NB; c++) {
+            const int k = c < n ? c : 0;
+            poff[c] = (u32)(((y + my[k] + ME_PAD) * P->ref_stride + (x + mx[k] + ME_PAD)) * (int)sizeof(pel));
         }
+        if (!a->bi) wave_sad_n_off<NB, true>(ob, ooff, so, pb, poff, P->ref_stride, w, h, lpu, sad);
+        else wave_sad_n_off<NB, false>(ob, ooff, so, pb, poff, P->ref_stride, w, h, lpu, sad);
 #pragma unroll
         for (int c = 0; c < NB; c++) {
             const int k = c < n ? c : 0;
