@@ -132,3 +132,23 @@ def test_lambda_table():
     o = Oracle(8)
     for qp in range(0, 64):
         assert wl.lambda_mv(qp) == o.lambda_mv(qp)
+
+
+def test_satd_norm_multiply_equals_reference_division():
+    """The kernels normalise the 32- and 128-coefficient SATD tiles by a
+    multiply with the double nearest 2 / sqrt(n) (csrc/common.h); the
+    reference divides, (int)(s / com_tbl_sqrt[i] * 2.0) in double
+    (com_cost.c:467, 586, 649, 708).  Both must give the same integer for
+    every sum a tile can reach (below 2^24 at 10 bits)."""
+    text = open(os.path.join(ROOT, "uavs3e_amd", "csrc", "common.h")).read()
+    m32 = re.search(r"me_satd_norm32\(int s\) \{ return \(int\)\(\(double\)s \* (0x[0-9a-fp.+-]+)\); \}", text)
+    m128 = re.search(r"me_satd_norm128\(int s\) \{ return \(int\)\(\(double\)s \* (0x[0-9a-fp.+-]+)\); \}", text)
+    assert m32 and m128
+    s = np.arange(1 << 24, dtype=np.float64)
+    for n, m in ((32.0, m32), (128.0, m128)):
+        a = float.fromhex(m.group(1))
+        ref = np.trunc(s / math.sqrt(n) * 2.0)
+        got = np.trunc(s * a)
+        bad = np.nonzero(ref != got)[0]
+        assert bad.size == 0, (n, bad[:5])
+        assert a == 2.0 / math.sqrt(n)   # the double nearest, as the comment says

@@ -47,6 +47,11 @@ HIP_UNITS = [("percall.hip", []), ("refs.hip", []), ("frame.hip", []), ("server.
              ("intsearch.hip", ["-DME_IS_WAVES=4"]), ("intsearch.hip", ["-DME_IS_WAVES=8"]),
              ("subpel.hip", [])]
 BUILD_DIR = os.path.join(ROOT, "_build")
+# Per-source compiler options.  subpel.hip: the matrix-core sub-pel kernel's
+# products stay in ordinary VGPRs (read straight by the VALU sums; in AGPRs
+# every coefficient costs a v_accvgpr_read, and the kernel needs 220
+# registers instead of 152).
+UNIT_FLAGS = {"subpel.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _obj_name(bd, src, defs):
@@ -101,7 +106,8 @@ def build(jobs=None, bit_depths=(8, 10)):
     os.makedirs(BUILD_DIR, exist_ok=True)
     jobs = jobs or max(1, min(16, os.cpu_count() or 1))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        units = {bd: [ex.submit(_compile_unit, bd, src, defs) for src, defs in HIP_UNITS] for bd in bit_depths}
+        units = {bd: [ex.submit(_compile_unit, bd, src, defs, UNIT_FLAGS.get(src, ())) for src, defs in HIP_UNITS]
+                 for bd in bit_depths}
         links = [ex.submit(_link_hip, bd, [f.result() for f in units[bd]]) for bd in bit_depths]
         for f in links:
             f.result()

@@ -196,6 +196,7 @@ struct SubArgs {
     int npu, nref;
     const SubShape *shp;        // device table of the searched shapes
     const u8 *blk_shape;        // workgroup -> index into shp
+    int mx;                     // 8 bits: the matrix-core kernel (its shape layout: lpu = tiles per PU, G = 32-tile batches per wave)
 };
 
 // Every shape in one launch: grid (blocks, jobs, refs) of 256 lanes.
@@ -227,6 +228,7 @@ struct MeSearch {
     SubShape *d_shp;         // sub-pel shape table of the searched shapes
     u8 *d_blk_shape;         // sub-pel workgroup -> shape
     int sub_blocks;          // sub-pel workgroups per (job, reference)
+    int sub_mx;              // the sub-pel launch runs k_subpel_mx (8 bits, one position at a time)
     // me_search_tz runs one launch per com_had tile type; the launches write
     // disjoint results, so they run side by side on these streams (default
     // priority), forked from the caller's stream by tz_ev[6] and joined back

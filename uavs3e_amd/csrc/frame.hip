@@ -135,6 +135,9 @@ static int search_layout(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nre
     // the shapes with the most samples per PU first (their workgroups run
     // longest, so they should not trail at the end of the launch).
     const int G = sp_groups >= 8 ? 8 : sp_groups >= 4 ? 4 : sp_groups >= 2 ? 2 : 1;
+    // 8 bits, one position at a time: the matrix-core kernel (no 4x4 tiles:
+    // the smallest shapes are 8x4 / 4x8)
+    const bool mx = BIT_DEPTH == 8 && G == 1;
     int order[ME_NSHAPES], no = 0;
     for (int s = 0; s < ME_NSHAPES; s++)
         if (p->base[s] >= 0) order[no++] = s;
@@ -157,14 +160,24 @@ static int search_layout(MeSearch *p, int R, u32 lambda, u32 shape_mask, int nre
         q.tpr = q.pw / q.tw;
         q.base = p->base[s];
         q.per_ctu = (128 / q.pw) * (128 / q.ph);
+        q.blk0 = blocks_total;
+        if (mx) {
+            // k_subpel_mx: one D column per tile, 128 tiles per workgroup
+            // (256 for 128x128: two 32-tile batches per wave); the tiles of
+            // a CTU's PUs of one shape fill 1, 2 or 4 workgroups
+            q.lpu = Tn;
+            q.G = Tn > 128 ? 2 : 1;
+            blocks_total += q.per_ctu * Tn / (128 * q.G);
+            continue;
+        }
         q.lpu = q.tt == 3 || q.tt == 5 ? 2 * Tn : Tn;   // 1..128, or 256 for 128x128
         // up to sp_groups search positions of a PU at a time, at most
         // 256 lanes per PU; lanes per (CTU, reference): a multiple of 256
         q.G = 256 / q.lpu < G ? 256 / q.lpu : G;
-        q.blk0 = blocks_total;
         blocks_total += q.per_ctu * q.lpu * q.G / 256;
     }
     p->sub_blocks = blocks_total;
+    p->sub_mx = mx ? 1 : 0;
     *n_shp = no;
     return 0;
 }
@@ -334,6 +347,7 @@ int me_search_finish(const MeSearch *p, const JobSet &js, const KeyLayout &kl, c
     b.npu = p->npu; b.nref = p->nref;
     b.shp = p->d_shp;
     b.blk_shape = p->d_blk_shape;
+    b.mx = p->sub_mx;
     if (me_subpel_launch(b, p->sub_blocks, js.n_jobs, p->nref, st)) return -1;
     const int npus = js.n_jobs * p->npu;
     hipLaunchKernelGGL(k_select, dim3((npus + 255) / 256), dim3(256), 0, st, res, best, npus, p->nref,

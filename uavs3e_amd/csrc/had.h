@@ -111,17 +111,34 @@ __device__ __forceinline__ void load_words(const pel *p, u32 *w)
 // One row of TW samples (TW = 4, 8) as TW/2 pairs of 16-bit lanes.  8 bits:
 // v[2j] = (p[4j], p[4j+2]), v[2j+1] = (p[4j+1], p[4j+3]); 10 bits: v[j] =
 // (p[2j], p[2j+1]), straight from the samples' dwords.
+#if BIT_DEPTH == 8
+// 8 bits: the pairs of dword j straight from the aligned dwords d[j],
+// d[j + 1] around it, for a row starting sh bytes into d[0]: v_perm picks
+// bytes sh, sh + 2 (s0) and sh + 1, sh + 3 (s1) of {d[j + 1]:d[j]} and
+// zeroes the high bytes, so one instruction aligns and widens a pair.
+__device__ __forceinline__ u32 perm_sel(u32 sh) { return sh * 0x10001u + 0x0C020C00u; }
+template <int NW>
+__device__ __forceinline__ void pairs_from(const u32 *d, u32 s0, s16x2 *v)
+{
+    const u32 s1 = s0 + 0x10001u;
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(d[j + 1], d[j], s0));
+        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(d[j + 1], d[j], s1));
+    }
+}
+#endif
+
 template <int TW>
 __device__ __forceinline__ void pairs_row(const pel *p, s16x2 *v)
 {
 #if BIT_DEPTH == 8
-    u32 w[TW / 4];
-    load_words<TW / 4>(p, w);
+    const uintptr_t ad = (uintptr_t)p;
+    const u32 *q = (const u32 *)(ad & ~(uintptr_t)3);
+    u32 d[TW / 4 + 1];
 #pragma unroll
-    for (int j = 0; j < TW / 4; j++) {
-        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C020C00u));
-        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C030C01u));
-    }
+    for (int i = 0; i <= TW / 4; i++) d[i] = q[i];
+    pairs_from<TW / 4>(d, perm_sel((u32)(ad & 3)), v);
 #else
     u32 w[TW / 2];
     load_words<TW / 2>(p, w);
@@ -138,20 +155,14 @@ template <int TW>
 __device__ __forceinline__ void pairs_row_al(gu32p q, u32 sh, s16x2 *v)
 {
     constexpr int NW = TW * (int)sizeof(pel) / 4;
-    u32 d[NW + 1], w[NW];
+    u32 d[NW + 1];
 #pragma unroll
     for (int i = 0; i <= NW; i++) d[i] = q[i];
-#pragma unroll
-    for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
 #if BIT_DEPTH == 8
-#pragma unroll
-    for (int j = 0; j < TW / 4; j++) {
-        v[2 * j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C020C00u));
-        v[2 * j + 1] = __builtin_bit_cast(s16x2, __builtin_amdgcn_perm(0u, w[j], 0x0C030C01u));
-    }
+    pairs_from<NW>(d, perm_sel(sh), v);
 #else
 #pragma unroll
-    for (int j = 0; j < TW / 2; j++) v[j] = __builtin_bit_cast(s16x2, w[j]);
+    for (int j = 0; j < NW; j++) v[j] = __builtin_bit_cast(s16x2, __builtin_amdgcn_alignbyte(d[j + 1], d[j], sh));
 #endif
 }
 
@@ -182,37 +193,46 @@ __device__ __forceinline__ void fwht_pk(s16x2 *v, int step)
 }
 
 // Sum of |coef| of the TW x TH transform whose rows are given as pairs.
-template <int TW, int TH>
+// The pair stage reads the low half of every pair with a bias of 0x8000
+// (mod 2^16).  The low halves of the pairs form a transform of their own
+// (only the pair stage mixes the halves), and input 0 enters every output of
+// a Walsh-Hadamard transform with sign +1, so 0x8000 added to the low half
+// of v[0] before the transform reaches the low half of every pair.  PRE:
+// the caller's pairs already carry it (org_pairs puts it into the current
+// block's first pair, see pk_bias); otherwise it is added here.
+template <int TW, int TH, bool PRE = false>
 __device__ __forceinline__ int had_pk_sum(s16x2 *v)
 {
     constexpr int K = TW / 2;
+    if (!PRE) v[0] += (s16x2){(short)-32768, 0};
 #pragma unroll
     for (int y = 0; y < TH; y++) fwht_pk<K>(v + y * K, 1);
 #pragma unroll
     for (int k = 0; k < K; k++) fwht_pk<TH>(v + k, K);
-#if BIT_DEPTH == 8
-    // The pair stage formed (every |lo +- hi| <= 32640): t = (lo + hi, lo - hi)
-    // in one packed multiply-add, and |t.lo| + |t.hi| summed by v_sad_u16
-    // against the sign bit (x ^ 0x8000 = x + 32768 for a signed 16-bit x).
     u32 s = 0;
+#if BIT_DEPTH == 8
+    // The pair stage formed (every |lo +- hi| <= 32640): t = (lo + hi, lo -
+    // hi), each half offset by the bias, in one packed multiply-add, and
+    // |t.lo| + |t.hi| by v_sad_u16 against 0x8000.
     const s16x2 pm = {1, -1};
 #pragma unroll
     for (int i = 0; i < K * TH; i++) {
         const s16x2 t = v[i].yy * pm + v[i].xx;
-        s = __builtin_amdgcn_sad_u16(__builtin_bit_cast(u32, t) ^ 0x80008000u, 0x80008000u, s);
+        s = __builtin_amdgcn_sad_u16(__builtin_bit_cast(u32, t), 0x80008000u, s);
     }
-    return (int)s;
 #else
-    // 10 bits: the pair stage could leave 16 bits, so 2 max(|lo|, |hi|)
-    u32 s = 0;
+    // 10 bits: lo + hi could leave 16 bits.  With L = lo + 0x8000 and both
+    // halves within +-32736, |lo + hi| + |lo - hi| (= 2 max(|lo|, |hi|)) is
+    // |L - (0x8000 - hi)| + |L - (0x8000 + hi)| on unsigned 16-bit lanes:
+    // one v_sad_u16 of (L, L) against (0x8000 - hi, 0x8000 + hi).
+    const s16x2 mp = {-1, 1}, bias = {(short)-32768, (short)-32768};
 #pragma unroll
     for (int i = 0; i < K * TH; i++) {
-        const u32 a = __builtin_bit_cast(u32, __builtin_elementwise_max(v[i], (s16x2)0 - v[i]));
-        const u32 lo = a & 0xFFFFu, hi = a >> 16;   // both in [0, 32767]
-        s += lo > hi ? lo : hi;
+        const s16x2 b = v[i].yy * mp + bias;
+        s = __builtin_amdgcn_sad_u16(__builtin_bit_cast(u32, v[i].xx), __builtin_bit_cast(u32, b), s);
     }
-    return (int)(2 * s);
 #endif
+    return (int)s;
 }
 
 // had_pk_sum<8, 8> of the half of a 16x8 / 8x16 tile.  At 10 bits the half
@@ -221,10 +241,11 @@ __device__ __forceinline__ int had_pk_sum(s16x2 *v)
 // pairs P, Q of rows y and y + 4, the four coefficients of their last two
 // stages sum to 2 max(|p.lo + q.lo|, |p.hi + q.hi|) +
 // 2 max(|p.lo - q.lo|, |p.hi - q.hi|).
+template <bool PRE = false>
 __device__ __forceinline__ int had_pk_sum_half(s16x2 *v)
 {
 #if BIT_DEPTH == 8
-    return had_pk_sum<8, 8>(v);
+    return had_pk_sum<8, 8, PRE>(v);
 #else
 #pragma unroll
     for (int y = 0; y < 8; y++) fwht_pk<4>(v + y * 4, 1);
@@ -295,6 +316,9 @@ __device__ __forceinline__ int satd_t(const pel *org, int so, const pel *cur, in
 // lane, from org_pairs<TT>, then satd_pre<TT> / had_pre<TT> per prediction.
 template <int TT> struct tile_dims { static constexpr int W = TT == 0 || TT == 4 ? 4 : 8, H = TT == 0 || TT == 2 ? 4 : 8; };
 template <int TT> constexpr int org_pairs_n() { return TT == 3 || TT == 5 ? 32 : tile_dims<TT>::W * tile_dims<TT>::H / 2; }
+// whether the pair stage of TT's tiles takes the low-half bias (the 10-bit
+// 16x8 / 8x16 halves end on 32-bit lanes without it)
+template <int TT> constexpr bool pk_bias() { return BIT_DEPTH == 8 || !(TT == 3 || TT == 5); }
 
 // For TT 3/5 the pairs of the tile's half sum (half 0) or difference (half 1)
 // of its two 8x8 halves (at most 2046 at 10 bits).
@@ -317,6 +341,7 @@ __device__ __forceinline__ void org_pairs(const pel *org, int so, int half, s16x
 #pragma unroll
         for (int y = 0; y < H; y++) pairs_row<W>(org + y * so, o + y * (W / 2));
     }
+    if (pk_bias<TT>()) o[0] += (s16x2){(short)-32768, 0};
 }
 
 template <int TT>
@@ -340,7 +365,7 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 #pragma unroll
                 for (int k = 0; k < 4; k++) v[y * 4 + k] = o[y * 4 + k] - (a[k] + b[k] * sgv);
             }
-            return had_pk_sum_half(v);
+            return had_pk_sum_half<pk_bias<TT>()>(v);
         } else {
             constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
 #pragma unroll
@@ -350,7 +375,7 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 #pragma unroll
                 for (int k = 0; k < W / 2; k++) v[y * (W / 2) + k] = o[y * (W / 2) + k] - c[k];
             }
-            return had_pk_sum<W, H>(v);
+            return had_pk_sum<W, H, true>(v);
         }
     }
     if (TT == 3 || TT == 5) {
@@ -364,7 +389,7 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 #pragma unroll
             for (int k = 0; k < 4; k++) v[y * 4 + k] = o[y * 4 + k] - (a[k] + b[k] * sgv);
         }
-        return had_pk_sum_half(v);
+        return had_pk_sum_half<pk_bias<TT>()>(v);
     } else {
         constexpr int W = tile_dims<TT>::W, H = tile_dims<TT>::H;
 #pragma unroll
@@ -374,7 +399,7 @@ __device__ __forceinline__ int had_pre(const s16x2 *o, const pel *cur, int sc, i
 #pragma unroll
             for (int k = 0; k < W / 2; k++) v[y * (W / 2) + k] = o[y * (W / 2) + k] - c[k];
         }
-        return had_pk_sum<W, H>(v);
+        return had_pk_sum<W, H, true>(v);
     }
 }
 
